@@ -1,0 +1,528 @@
+// C ABI of the exmarg HIP library: context, handles, argument checks and
+// the entry points declared in include/exmarg_hip.h.
+#include <cstdarg>
+#include <cmath>
+
+#include "exm_common.h"
+
+namespace exm {
+
+// ---------------------------------------------------------------- errors
+static thread_local char g_err[1024] = {0};
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+const char* last_error() { return g_err; }
+
+// ---------------------------------------------------------------- context
+static Context g_ctx;
+static int64_t g_mem_in_use = 0;
+
+Context& ctx() { return g_ctx; }
+
+int ensure_init() {
+    EXM_CHECK(g_ctx.device >= 0, EXM_ERR_STATE, "exm_init has not been called");
+    EXM_HIP(hipSetDevice(g_ctx.device));
+    return EXM_OK;
+}
+
+static int dev_alloc(void** p, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+        *p = nullptr;
+        return EXM_ERR_NOMEM;
+    }
+    g_mem_in_use += (int64_t)bytes;
+    return EXM_OK;
+}
+
+static void dev_free(void* p, size_t bytes) {
+    if (p) {
+        (void)hipFree(p);
+        g_mem_in_use -= (int64_t)(bytes ? bytes : 256);
+    }
+}
+
+int scratch(int slot, size_t bytes, void** out) {
+    Context& c = g_ctx;
+    if (c.scratch_size[slot] < bytes) {
+        EXM_HIP(hipStreamSynchronize(c.stream));
+        dev_free(c.scratch[slot], c.scratch_size[slot]);
+        c.scratch[slot] = nullptr;
+        c.scratch_size[slot] = 0;
+        EXM_TRY(dev_alloc(&c.scratch[slot], bytes));
+        c.scratch_size[slot] = bytes;
+    }
+    *out = c.scratch[slot];
+    return EXM_OK;
+}
+
+Store::~Store() {
+    dev_free(d_T, sizeof(float) * (size_t)tstride * ntab);
+    dev_free(d_tau, sizeof(int32_t) * ntau);
+}
+Bank::~Bank() { dev_free(d_W, sizeof(float2) * (size_t)n_tmpl * n_freq); }
+Traj::~Traj() { dev_free(d_buf, sizeof(float) * 4 * (size_t)n_tmpl * n_time); }
+Times::~Times() {
+    dev_free(d_sel, sizeof(int32_t) * (size_t)n_tmpl * n_walk);
+    dev_free(d_lw, sizeof(float) * (size_t)n_tmpl * n_walk);
+}
+Plan::~Plan() {
+    dev_free(d_m, sizeof(int32_t) * njobs);
+    dev_free(d_tab, sizeof(int32_t) * njobs);
+    dev_free(d_offb, sizeof(double) * njobs);
+}
+
+// ---------------------------------------------------------------- registry
+static std::unordered_map<exm_handle, Object*> g_objects;
+static uint64_t g_next = 1;
+
+exm_handle register_object(Object* o) {
+    exm_handle h = (g_next++ << 8) | o->kind;
+    g_objects[h] = o;
+    return h;
+}
+
+Object* lookup(exm_handle h, uint32_t kind) {
+    auto it = g_objects.find(h);
+    if (it == g_objects.end() || it->second->kind != kind) return nullptr;
+    return it->second;
+}
+
+static int get_plain(exm_handle h, Object** o) {
+    auto it = g_objects.find(h);
+    EXM_CHECK(it != g_objects.end(), EXM_ERR_HANDLE, "unknown handle %llu", (unsigned long long)h);
+    *o = it->second;
+    return EXM_OK;
+}
+
+}  // namespace exm
+
+using namespace exm;
+
+#define LOCK std::lock_guard<std::recursive_mutex> lock_(exm::ctx().mu)
+#define NEED(type, var, h, kind)                                                                 \
+    type* var = exm::get<type>((h), (kind));                                                     \
+    EXM_CHECK(var != nullptr, EXM_ERR_HANDLE, "handle %llu is not a " #type, (unsigned long long)(h))
+
+extern "C" {
+
+int exm_init(int device) {
+    LOCK;
+    Context& c = ctx();
+    if (c.device >= 0) {
+        EXM_CHECK(c.device == device, EXM_ERR_ARG, "library already initialised on device %d", c.device);
+        return EXM_OK;
+    }
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    EXM_CHECK(e == hipSuccess && n > 0, EXM_ERR_STATE, "no HIP device visible (%s)", hipGetErrorString(e));
+    EXM_CHECK(device >= 0 && device < n, EXM_ERR_ARG, "device %d out of range [0, %d)", device, n);
+    EXM_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    EXM_HIP(hipGetDeviceProperties(&prop, device));
+    EXM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    c.num_cu = prop.multiProcessorCount;
+    c.total_mem = (int64_t)prop.totalGlobalMem;
+    snprintf(c.name, sizeof(c.name), "%s", prop.gcnArchName);
+    c.device = device;
+    return EXM_OK;
+}
+
+int exm_last_error(char* buf, size_t n) {
+    if (buf && n) {
+        snprintf(buf, n, "%s", last_error());
+    }
+    return EXM_OK;
+}
+
+int exm_device_info(char* name, size_t n, int64_t* total_mem, int* num_cu) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    if (name && n) snprintf(name, n, "%s", ctx().name);
+    if (total_mem) *total_mem = ctx().total_mem;
+    if (num_cu) *num_cu = ctx().num_cu;
+    return EXM_OK;
+}
+
+int exm_free(exm_handle h) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    Object* o = nullptr;
+    EXM_TRY(get_plain(h, &o));
+    EXM_HIP(hipStreamSynchronize(ctx().stream));
+    g_objects.erase(h);
+    delete o;
+    return EXM_OK;
+}
+
+int exm_mem_in_use(int64_t* bytes) {
+    LOCK;
+    *bytes = g_mem_in_use;
+    return EXM_OK;
+}
+
+// ---------------------------------------------------------------- tables
+int exm_store_create(int ntab, int ntau, int nphif, int nb, int nd, const int64_t* tau_idx, double beta0,
+                     double dbeta, double delta0, double ddelta, exm_handle* out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    EXM_CHECK(ntab > 0 && ntau > 0 && nb >= 2 && nd >= 2, EXM_ERR_ARG,
+              "bad table geometry ntab=%d ntau=%d nb=%d nd=%d", ntab, ntau, nb, nd);
+    EXM_CHECK(nphif == NPHIF, EXM_ERR_ARG, "tables must hold %d phase bins, got %d", NPHIF, nphif);
+    EXM_CHECK(dbeta > 0 && ddelta > 0, EXM_ERR_ARG, "grid steps must be positive");
+    EXM_CHECK(tau_idx != nullptr, EXM_ERR_ARG, "tau_idx is NULL");
+    Store* st = new Store();
+    st->kind = K_STORE;
+    st->ntab = ntab;
+    st->ntau = ntau;
+    st->nphif = nphif;
+    st->nb = nb;
+    st->nd = nd;
+    st->tstride = (int64_t)ntau * nphif * nb * nd;
+    st->tau.assign(tau_idx, tau_idx + ntau);
+    st->beta0 = beta0;
+    st->dbeta = dbeta;
+    st->delta0 = delta0;
+    st->ddelta = ddelta;
+    std::vector<int32_t> t32(ntau);
+    for (int i = 0; i < ntau; ++i) {
+        if (tau_idx[i] > INT32_MAX / 2 || tau_idx[i] < -INT32_MAX / 2) {
+            delete st;
+            set_error("lag %lld out of range", (long long)tau_idx[i]);
+            return EXM_ERR_ARG;
+        }
+        t32[i] = (int32_t)tau_idx[i];
+    }
+    int rc = dev_alloc((void**)&st->d_T, sizeof(float) * (size_t)st->tstride * ntab);
+    if (rc == EXM_OK) rc = dev_alloc((void**)&st->d_tau, sizeof(int32_t) * ntau);
+    if (rc != EXM_OK) {
+        delete st;
+        return rc;
+    }
+    EXM_HIP(hipMemcpyAsync(st->d_tau, t32.data(), sizeof(int32_t) * ntau, hipMemcpyHostToDevice, ctx().stream));
+    EXM_HIP(hipStreamSynchronize(ctx().stream));
+    *out = register_object(st);
+    return EXM_OK;
+}
+
+int exm_store_set_table(exm_handle h, int k, const float* table) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Store, st, h, K_STORE);
+    EXM_CHECK(k >= 0 && k < st->ntab, EXM_ERR_ARG, "table index %d out of range [0, %d)", k, st->ntab);
+    EXM_CHECK(table != nullptr, EXM_ERR_ARG, "table is NULL");
+    // stage in slabs of one lag through scratch, then reorder on the device
+    size_t lag_floats = (size_t)st->nphif * st->nb * st->nd;
+    float* stage = nullptr;
+    EXM_TRY(scratch(3, sizeof(float) * (size_t)st->tstride, (void**)&stage));
+    EXM_HIP(hipMemcpyAsync(stage, table, sizeof(float) * lag_floats * st->ntau, hipMemcpyHostToDevice,
+                           ctx().stream));
+    EXM_TRY(launch_store_transpose_in(st, k, stage));
+    EXM_HIP(hipStreamSynchronize(ctx().stream));
+    return EXM_OK;
+}
+
+int exm_store_fill_synthetic(exm_handle h, int k, uint64_t seed, int64_t node, int kind) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Store, st, h, K_STORE);
+    EXM_CHECK(k >= 0 && k < st->ntab, EXM_ERR_ARG, "table index %d out of range [0, %d)", k, st->ntab);
+    EXM_CHECK(kind == 0 || kind == 1, EXM_ERR_ARG, "kind must be 0 (smooth) or 1 (rough)");
+    EXM_TRY(launch_fill_synthetic(st, k, seed, node, kind));
+    EXM_HIP(hipStreamSynchronize(ctx().stream));
+    return EXM_OK;
+}
+
+int exm_store_read_table(exm_handle h, int k, float* table) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Store, st, h, K_STORE);
+    EXM_CHECK(k >= 0 && k < st->ntab, EXM_ERR_ARG, "table index %d out of range [0, %d)", k, st->ntab);
+    float* stage = nullptr;
+    EXM_TRY(scratch(3, sizeof(float) * (size_t)st->tstride, (void**)&stage));
+    EXM_TRY(launch_store_transpose_out(st, k, stage));
+    EXM_HIP(hipMemcpyAsync(table, stage, sizeof(float) * (size_t)st->tstride, hipMemcpyDeviceToHost,
+                           ctx().stream));
+    EXM_HIP(hipStreamSynchronize(ctx().stream));
+    return EXM_OK;
+}
+
+// ---------------------------------------------------------------- bank, filter, trajectories
+int exm_bank_create(const float* W, int n_tmpl, int n_freq, int lo, int64_t n_time, exm_handle* out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    EXM_CHECK(n_tmpl > 0 && n_freq > 0 && lo >= 0, EXM_ERR_ARG, "bad bank shape n_tmpl=%d n_freq=%d lo=%d",
+              n_tmpl, n_freq, lo);
+    EXM_CHECK(n_time >= (int64_t)lo + n_freq, EXM_ERR_ARG, "n_time %lld shorter than the band end %d",
+              (long long)n_time, lo + n_freq);
+    EXM_CHECK(n_time <= (1ll << 30), EXM_ERR_ARG, "n_time %lld too large", (long long)n_time);
+    Bank* b = new Bank();
+    b->kind = K_BANK;
+    b->n_tmpl = n_tmpl;
+    b->n_freq = n_freq;
+    b->lo = lo;
+    b->n_time = n_time;
+    int rc = dev_alloc((void**)&b->d_W, sizeof(float2) * (size_t)n_tmpl * n_freq);
+    if (rc != EXM_OK) {
+        delete b;
+        return rc;
+    }
+    EXM_HIP(hipMemcpyAsync(b->d_W, W, sizeof(float2) * (size_t)n_tmpl * n_freq, hipMemcpyHostToDevice,
+                           ctx().stream));
+    EXM_HIP(hipStreamSynchronize(ctx().stream));
+    *out = register_object(b);
+    return EXM_OK;
+}
+
+static int new_traj(int n_tmpl, int64_t n_time, Traj** out) {
+    Traj* t = new Traj();
+    t->kind = K_TRAJ;
+    t->n_tmpl = n_tmpl;
+    t->n_time = n_time;
+    int rc = dev_alloc((void**)&t->d_buf, sizeof(float) * 4 * (size_t)n_tmpl * n_time);
+    if (rc != EXM_OK) {
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return EXM_OK;
+}
+
+int exm_filter(exm_handle hb, const double* dH, const double* dL, const double* s2H, const double* s2L,
+               exm_handle* traj_out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Bank, bank, hb, K_BANK);
+    EXM_CHECK(dH && dL && s2H && s2L, EXM_ERR_ARG, "segment arrays must not be NULL");
+    Traj* t = nullptr;
+    EXM_TRY(new_traj(bank->n_tmpl, bank->n_time, &t));
+    int rc = launch_filter(bank, dH, dL, s2H, s2L, t);
+    if (rc == EXM_OK) {
+        hipError_t e = hipStreamSynchronize(ctx().stream);
+        if (e != hipSuccess) {
+            set_error("filter: %s", hipGetErrorString(e));
+            rc = EXM_ERR_HIP;
+        }
+    }
+    if (rc != EXM_OK) {
+        delete t;
+        return rc;
+    }
+    *traj_out = register_object(t);
+    return EXM_OK;
+}
+
+int exm_traj_create(int n_tmpl, int64_t n_time, const float* lqH, const float* phH, const float* lqL,
+                    const float* phL, exm_handle* out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    EXM_CHECK(n_tmpl > 0 && n_time > 0, EXM_ERR_ARG, "bad trajectory shape");
+    Traj* t = nullptr;
+    EXM_TRY(new_traj(n_tmpl, n_time, &t));
+    size_t nb = sizeof(float) * (size_t)n_tmpl * n_time;
+    const float* src[4] = {lqH, phH, lqL, phL};
+    float* dst[4] = {t->lqH(), t->phH(), t->lqL(), t->phL()};
+    for (int i = 0; i < 4; ++i) {
+        hipError_t e = hipMemcpyAsync(dst[i], src[i], nb, hipMemcpyHostToDevice, ctx().stream);
+        if (e != hipSuccess) {
+            delete t;
+            set_error("trajectory upload: %s", hipGetErrorString(e));
+            return EXM_ERR_HIP;
+        }
+    }
+    EXM_HIP(hipStreamSynchronize(ctx().stream));
+    *out = register_object(t);
+    return EXM_OK;
+}
+
+int exm_traj_read(exm_handle h, float* lqH, float* phH, float* lqL, float* phL) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Traj, t, h, K_TRAJ);
+    size_t nb = sizeof(float) * (size_t)t->n_tmpl * t->n_time;
+    float* dst[4] = {lqH, phH, lqL, phL};
+    const float* src[4] = {t->lqH(), t->phH(), t->lqL(), t->phL()};
+    for (int i = 0; i < 4; ++i)
+        if (dst[i]) EXM_HIP(hipMemcpyAsync(dst[i], src[i], nb, hipMemcpyDeviceToHost, ctx().stream));
+    EXM_HIP(hipStreamSynchronize(ctx().stream));
+    return EXM_OK;
+}
+
+// ---------------------------------------------------------------- times
+int exm_select_times(exm_handle ht, int lag_lo, int lag_hi, int K, int S, const int64_t* pick, float lw_rand,
+                     exm_handle* times_out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Traj, traj, ht, K_TRAJ);
+    EXM_CHECK(lag_lo <= lag_hi, EXM_ERR_ARG, "lag range [%d, %d] is empty", lag_lo, lag_hi);
+    EXM_CHECK((int64_t)(lag_hi - lag_lo) < traj->n_time, EXM_ERR_ARG, "lag range wider than the segment");
+    EXM_CHECK(K > 0 && K < traj->n_time, EXM_ERR_ARG, "n_loud %d must lie in (0, n_time=%lld)", K,
+              (long long)traj->n_time);
+    EXM_CHECK(S > 0, EXM_ERR_ARG, "n_random must be positive, got %d", S);
+    EXM_CHECK(K + S <= 16384, EXM_ERR_ARG, "n_loud + n_random = %d exceeds 16384", K + S);
+    EXM_CHECK(pick != nullptr, EXM_ERR_ARG, "pick is NULL");
+    const int64_t nrest = traj->n_time - K;
+    for (int64_t i = 0; i < (int64_t)traj->n_tmpl * S; ++i)
+        EXM_CHECK(pick[i] >= 0 && pick[i] < nrest, EXM_ERR_ARG, "pick[%lld] = %lld outside [0, %lld)",
+                  (long long)i, (long long)pick[i], (long long)nrest);
+    Times* tm = new Times();
+    tm->kind = K_TIMES;
+    tm->n_tmpl = traj->n_tmpl;
+    tm->n_walk = K + S;
+    int rc = dev_alloc((void**)&tm->d_sel, sizeof(int32_t) * (size_t)tm->n_tmpl * tm->n_walk);
+    if (rc == EXM_OK) rc = dev_alloc((void**)&tm->d_lw, sizeof(float) * (size_t)tm->n_tmpl * tm->n_walk);
+    if (rc == EXM_OK) rc = launch_select(traj, lag_lo, lag_hi, K, S, pick, lw_rand, tm);
+    if (rc == EXM_OK) {
+        hipError_t e = hipStreamSynchronize(ctx().stream);
+        if (e != hipSuccess) {
+            set_error("select_times: %s", hipGetErrorString(e));
+            rc = EXM_ERR_HIP;
+        }
+    }
+    if (rc != EXM_OK) {
+        delete tm;
+        return rc;
+    }
+    *times_out = register_object(tm);
+    return EXM_OK;
+}
+
+int exm_times_read(exm_handle h, int32_t* sel, float* lw) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Times, tm, h, K_TIMES);
+    size_t n = (size_t)tm->n_tmpl * tm->n_walk;
+    if (sel) EXM_HIP(hipMemcpyAsync(sel, tm->d_sel, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx().stream));
+    if (lw) EXM_HIP(hipMemcpyAsync(lw, tm->d_lw, sizeof(float) * n, hipMemcpyDeviceToHost, ctx().stream));
+    EXM_HIP(hipStreamSynchronize(ctx().stream));
+    return EXM_OK;
+}
+
+// ---------------------------------------------------------------- gather
+static int check_times(Traj* traj, exm_handle htimes, Times** tm) {
+    *tm = nullptr;
+    if (htimes == 0) return EXM_OK;
+    Times* t = get<Times>(htimes, K_TIMES);
+    EXM_CHECK(t != nullptr, EXM_ERR_HANDLE, "handle %llu is not a Times", (unsigned long long)htimes);
+    EXM_CHECK(t->n_tmpl == traj->n_tmpl, EXM_ERR_ARG, "times are for %d templates, trajectories for %d",
+              t->n_tmpl, traj->n_tmpl);
+    *tm = t;
+    return EXM_OK;
+}
+
+static int upload_jobs(int64_t njobs, const int32_t* m, const int32_t* tab, const double* offb, int32_t** dm,
+                       int32_t** dtab, double** doffb) {
+    void* p = nullptr;
+    size_t need = (size_t)njobs * (4 + 4 + 8) + 512;
+    EXM_TRY(scratch(2, need, &p));
+    char* base = (char*)p;
+    *doffb = (double*)base;
+    *dm = (int32_t*)(base + (((size_t)njobs * 8 + 255) & ~(size_t)255));
+    *dtab = (int32_t*)((char*)*dm + (((size_t)njobs * 4 + 255) & ~(size_t)255));
+    EXM_HIP(hipMemcpyAsync(*doffb, offb, 8 * (size_t)njobs, hipMemcpyHostToDevice, ctx().stream));
+    EXM_HIP(hipMemcpyAsync(*dm, m, 4 * (size_t)njobs, hipMemcpyHostToDevice, ctx().stream));
+    EXM_HIP(hipMemcpyAsync(*dtab, tab, 4 * (size_t)njobs, hipMemcpyHostToDevice, ctx().stream));
+    return EXM_OK;
+}
+
+int exm_gather(exm_handle htraj, exm_handle hstore, const int64_t* mass_ids, int64_t nsel, const double* lna,
+               const int64_t* tab_ids, int ns, exm_handle htimes, double* out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Traj, traj, htraj, K_TRAJ);
+    NEED(Store, st, hstore, K_STORE);
+    Times* tm = nullptr;
+    EXM_TRY(check_times(traj, htimes, &tm));
+    EXM_CHECK(nsel > 0 && ns > 0, EXM_ERR_ARG, "empty gather (nsel=%lld ns=%d)", (long long)nsel, ns);
+    EXM_CHECK(mass_ids && lna && tab_ids && out, EXM_ERR_ARG, "NULL argument");
+    int64_t njobs = nsel * ns;
+    // job e = n * nsel + j  ->  out[n, j]
+    std::vector<int32_t> m(njobs), tab(njobs);
+    std::vector<double> offb(njobs);
+    for (int64_t j = 0; j < nsel; ++j) {
+        EXM_CHECK(mass_ids[j] >= 0 && mass_ids[j] < traj->n_tmpl, EXM_ERR_ARG,
+                  "mass_ids[%lld] = %lld outside [0, %d)", (long long)j, (long long)mass_ids[j], traj->n_tmpl);
+        for (int n = 0; n < ns; ++n) {
+            int64_t t = tab_ids[j * ns + n];
+            EXM_CHECK(t >= 0 && t < st->ntab, EXM_ERR_ARG, "tab_ids[%lld, %d] = %lld outside [0, %d)",
+                      (long long)j, n, (long long)t, st->ntab);
+            int64_t e = (int64_t)n * nsel + j;
+            m[e] = (int32_t)mass_ids[j];
+            tab[e] = (int32_t)t;
+            offb[e] = (lna[j * ns + n] - st->beta0) / st->dbeta;
+        }
+    }
+    int32_t *dm, *dtab;
+    double* doffb;
+    EXM_TRY(upload_jobs(njobs, m.data(), tab.data(), offb.data(), &dm, &dtab, &doffb));
+    double* dout = nullptr;
+    EXM_TRY(scratch(1, sizeof(double) * (size_t)njobs, (void**)&dout));
+    EXM_TRY(launch_gather(traj, st, dm, dtab, doffb, njobs, tm, dout));
+    EXM_HIP(hipMemcpyAsync(out, dout, sizeof(double) * njobs, hipMemcpyDeviceToHost, ctx().stream));
+    EXM_HIP(hipStreamSynchronize(ctx().stream));
+    return EXM_OK;
+}
+
+int exm_plan_create(exm_handle hstore, int n_tmpl, int64_t njobs, const int32_t* job_m, const int32_t* job_tab,
+                    const double* job_lna, exm_handle* out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Store, st, hstore, K_STORE);
+    EXM_CHECK(njobs > 0 && n_tmpl > 0, EXM_ERR_ARG, "empty plan");
+    std::vector<double> offb(njobs);
+    for (int64_t e = 0; e < njobs; ++e) {
+        EXM_CHECK(job_m[e] >= 0 && job_m[e] < n_tmpl, EXM_ERR_ARG, "job %lld: template %d outside [0, %d)",
+                  (long long)e, job_m[e], n_tmpl);
+        EXM_CHECK(job_tab[e] >= 0 && job_tab[e] < st->ntab, EXM_ERR_ARG, "job %lld: table %d outside [0, %d)",
+                  (long long)e, job_tab[e], st->ntab);
+        offb[e] = (job_lna[e] - st->beta0) / st->dbeta;
+    }
+    Plan* p = new Plan();
+    p->kind = K_PLAN;
+    p->njobs = njobs;
+    p->n_tmpl = n_tmpl;
+    p->h_m.assign(job_m, job_m + njobs);
+    p->h_tab.assign(job_tab, job_tab + njobs);
+    int rc = dev_alloc((void**)&p->d_m, 4 * (size_t)njobs);
+    if (rc == EXM_OK) rc = dev_alloc((void**)&p->d_tab, 4 * (size_t)njobs);
+    if (rc == EXM_OK) rc = dev_alloc((void**)&p->d_offb, 8 * (size_t)njobs);
+    if (rc != EXM_OK) {
+        delete p;
+        return rc;
+    }
+    EXM_HIP(hipMemcpyAsync(p->d_m, job_m, 4 * (size_t)njobs, hipMemcpyHostToDevice, ctx().stream));
+    EXM_HIP(hipMemcpyAsync(p->d_tab, job_tab, 4 * (size_t)njobs, hipMemcpyHostToDevice, ctx().stream));
+    EXM_HIP(hipMemcpyAsync(p->d_offb, offb.data(), 8 * (size_t)njobs, hipMemcpyHostToDevice, ctx().stream));
+    EXM_HIP(hipStreamSynchronize(ctx().stream));
+    *out = register_object(p);
+    return EXM_OK;
+}
+
+int exm_plan_gather(exm_handle hplan, exm_handle htraj, exm_handle hstore, exm_handle htimes, double* out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Plan, plan, hplan, K_PLAN);
+    NEED(Traj, traj, htraj, K_TRAJ);
+    NEED(Store, st, hstore, K_STORE);
+    Times* tm = nullptr;
+    EXM_TRY(check_times(traj, htimes, &tm));
+    EXM_CHECK(plan->n_tmpl == traj->n_tmpl, EXM_ERR_ARG, "plan is for %d templates, trajectories for %d",
+              plan->n_tmpl, traj->n_tmpl);
+    double* dout = nullptr;
+    EXM_TRY(scratch(1, sizeof(double) * (size_t)plan->njobs, (void**)&dout));
+    EXM_TRY(launch_gather(traj, st, plan->d_m, plan->d_tab, plan->d_offb, plan->njobs, tm, dout));
+    EXM_HIP(hipMemcpyAsync(out, dout, sizeof(double) * plan->njobs, hipMemcpyDeviceToHost, ctx().stream));
+    EXM_HIP(hipStreamSynchronize(ctx().stream));
+    return EXM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+// Shared definitions of the exmarg HIP library: error state, the device
+// context, object types behind the opaque handles, and device helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/exmarg_hip.h"
+
+namespace exm {
+
+constexpr int NPHI = 96;
+constexpr int NPHIF = NPHI / 2 + 1;
+
+// ---------------------------------------------------------------- errors
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define EXM_HIP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            exm::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                           __LINE__);                                                       \
+            return EXM_ERR_HIP;                                                             \
+        }                                                                                   \
+    } while (0)
+
+#define EXM_FFT(call)                                                              \
+    do {                                                                           \
+        hipfftResult r_ = (call);                                                  \
+        if (r_ != HIPFFT_SUCCESS) {                                                \
+            exm::set_error("%s failed: hipfft status %d (%s:%d)", #call, (int)r_, \
+                           __FILE__, __LINE__);                                    \
+            return EXM_ERR_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+
+#define EXM_CHECK(cond, code, ...)     \
+    do {                               \
+        if (!(cond)) {                 \
+            exm::set_error(__VA_ARGS__); \
+            return code;               \
+        }                              \
+    } while (0)
+
+#define EXM_TRY(expr)            \
+    do {                         \
+        int rc_ = (expr);        \
+        if (rc_ != EXM_OK) return rc_; \
+    } while (0)
+
+// ---------------------------------------------------------------- objects
+enum Kind : uint32_t { K_STORE = 1, K_BANK = 2, K_TRAJ = 3, K_TIMES = 4, K_PLAN = 5, K_BATCH = 6 };
+
+struct Object {
+    uint32_t kind;
+    virtual ~Object() {}
+};
+
+// Tables of every node in one device allocation, layout [table][lag][beta][phase][Delta]
+// (float32): one beta row of all phases is contiguous, nphif * nd floats.
+struct Store : Object {
+    int ntab = 0, ntau = 0, nphif = 0, nb = 0, nd = 0;
+    int64_t tstride = 0;   // floats per table
+    std::vector<int64_t> tau;
+    double beta0 = 0, dbeta = 0, delta0 = 0, ddelta = 0;
+    float* d_T = nullptr;
+    int32_t* d_tau = nullptr;  // lags (samples)
+    ~Store();
+};
+
+struct Bank : Object {
+    int n_tmpl = 0, n_freq = 0, lo = 0;
+    int64_t n_time = 0;
+    float2* d_W = nullptr;  // (n_tmpl, n_freq) complex64
+    ~Bank();
+};
+
+// Matched-filter outputs of one segment: four (n_tmpl, n_time) float32 arrays.
+struct Traj : Object {
+    int n_tmpl = 0;
+    int64_t n_time = 0;
+    float* d_buf = nullptr;  // lqH | phH | lqL | phL
+    float* lqH() const { return d_buf; }
+    float* phH() const { return d_buf + (int64_t)n_tmpl * n_time; }
+    float* lqL() const { return d_buf + 2 * (int64_t)n_tmpl * n_time; }
+    float* phL() const { return d_buf + 3 * (int64_t)n_tmpl * n_time; }
+    ~Traj();
+};
+
+// Arrival times walked per template (sorted) and their log weights.
+struct Times : Object {
+    int n_tmpl = 0, n_walk = 0;
+    int32_t* d_sel = nullptr;
+    float* d_lw = nullptr;
+    ~Times();
+};
+
+// A flat list of (template, table, ln a) reads.
+struct Plan : Object {
+    int64_t njobs = 0;
+    int n_tmpl = 0;
+    int32_t* d_m = nullptr;
+    int32_t* d_tab = nullptr;
+    double* d_offb = nullptr;
+    std::vector<int32_t> h_m, h_tab;
+    ~Plan();
+};
+
+// ---------------------------------------------------------------- context
+struct Context {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    // grow-only scratch buffers
+    void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t scratch_size[4] = {0, 0, 0, 0};
+    std::unordered_map<uint64_t, hipfftHandle> fft_plans;  // key: (n_time << 20) | batch
+    int num_cu = 0;
+    int64_t total_mem = 0;
+    char name[256] = {0};
+};
+
+Context& ctx();
+int ensure_init();
+int scratch(int slot, size_t bytes, void** out);
+
+// handle registry
+exm_handle register_object(Object* o);
+Object* lookup(exm_handle h, uint32_t kind);
+template <class T>
+T* get(exm_handle h, uint32_t kind) {
+    return static_cast<T*>(lookup(h, kind));
+}
+
+// ---------------------------------------------------------------- kernels (host launchers)
+int launch_filter(Bank* bank, const double* dH, const double* dL, const double* s2H, const double* s2L, Traj* traj);
+int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_t* pick, float lw_rand,
+                  Times* times);
+int launch_gather(Traj* traj, Store* store, const int32_t* d_m, const int32_t* d_tab, const double* d_offb,
+                  int64_t njobs, Times* times, double* d_out);
+int launch_fill_synthetic(Store* st, int k, uint64_t seed, int64_t node, int kind);
+int launch_store_transpose_in(Store* st, int k, const float* d_src);
+int launch_store_transpose_out(Store* st, int k, float* d_dst);
+
+// ---------------------------------------------------------------- device helpers
+// e^x by Cody-Waite reduction and a degree-13 Taylor polynomial, every operation separately
+// rounded: the same operation sequence as the CPU oracle, so arrival-time ranks agree bitwise.
+__device__ __forceinline__ double pexp(double x) {
+    const double k = __builtin_rint(__dmul_rn(x, 1.4426950408889634));
+    const double r = __dsub_rn(__dsub_rn(x, __dmul_rn(k, 6.93147180369123816490e-01)),
+                               __dmul_rn(k, 1.90821492927058770002e-10));
+    const double c[14] = {1.0, 1.0, 0.5, 0.16666666666666666, 0.041666666666666664, 0.008333333333333333,
+                          0.001388888888888889, 0.0001984126984126984, 2.48015873015873e-05,
+                          2.7557319223985893e-06, 2.755731922398589e-07, 2.505210838544172e-08,
+                          2.08767569878681e-09, 1.6059043836821613e-10};
+    double p = c[13];
+#pragma unroll
+    for (int i = 12; i >= 0; --i) p = __dadd_rn(__dmul_rn(p, r), c[i]);
+    return ldexp(p, (int)k);
+}
+
+}  // namespace exm

@@ -1,0 +1,126 @@
+// The table read of Eq. (C7) (src/exmarg/search/gather.py:1-24): for every job (template m,
+// table k, ln a) and lag tau, the running (max, sum) of exp(lnS) over the walked arrival times,
+// then the log-sum-exp over the lags, minus ln n_time (gather.py:103-107).
+//
+// k_gather_ref evaluates one lookup exactly as the reference CPU kernel does
+// (src/exmarg/search/gather.py:45-98): float64 arithmetic on float32 inputs, phase difference
+// formed in float32, every operation separately rounded.  One thread per (job, lag), the jobs of a
+// lag adjacent so a wavefront shares one lag slab of the tables.
+#include <cmath>
+
+#include "exm_common.h"
+
+#pragma clang fp contract(off)
+
+namespace exm {
+
+__global__ __launch_bounds__(256) void k_gather_ref(
+    const float* __restrict__ lqH, const float* __restrict__ phH, const float* __restrict__ lqL,
+    const float* __restrict__ phL, int64_t n_time, const float* __restrict__ T, int64_t tstride, int nphif, int nb,
+    int nd, const int32_t* __restrict__ tau, int ntau, const int32_t* __restrict__ jm,
+    const int32_t* __restrict__ jtab, const double* __restrict__ joffb, int64_t njobs, double rdb, double offd,
+    double rdd, const int32_t* __restrict__ sel, const float* __restrict__ lw, int n_walk_sel,
+    double* __restrict__ om, double* __restrict__ os) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (e >= njobs) return;
+    const int m = jm[e];
+    const double offb = joffb[e];
+    const float* Tn = T + (int64_t)jtab[e] * tstride + (int64_t)r * nb * nphif * nd;
+    const int64_t base = (int64_t)m * n_time;
+    int64_t tau0 = ((int64_t)tau[r] % n_time + n_time) % n_time;
+    const double rph = NPHI / 6.283185307179586;
+    const int64_t n_walk = sel ? n_walk_sel : n_time;
+    const int64_t row = (int64_t)nphif * nd;   // floats per beta row
+    double mx = -1e300, s = 0.0;
+    for (int64_t k = 0; k < n_walk; ++k) {
+        const int64_t t = sel ? (int64_t)sel[(int64_t)m * n_walk_sel + k] : k;
+        int64_t i1 = t + tau0;
+        if (i1 >= n_time) i1 -= n_time;
+        const float dph = phH[base + i1] - phL[base + t];
+        double x3 = (double)dph * rph;
+        if (x3 < 0.0) x3 += NPHI;
+        if (x3 >= NPHI) x3 -= NPHI;
+        int c0 = (int)x3;
+        if (c0 >= NPHI) c0 = NPHI - 1;
+        else if (c0 < 0) c0 = 0;
+        const double fc = x3 - c0;
+        const int c1 = (c0 + 1) % NPHI;
+        const double aH = lqH[base + i1], aL = lqL[base + t];
+        const double xbr = aL * rdb + offb;
+        double xb = xbr;
+        if (xb < 0.0) xb = 0.0;
+        else if (xb > nb - 1 - 1e-6) xb = nb - 1 - 1e-6;
+        const double aLc = aL + (xb - xbr) / rdb;
+        double xd = (aH - aLc) * rdd + offd;
+        if (xd < 0.0) xd = 0.0;
+        else if (xd > nd - 1 - 1e-6) xd = nd - 1 - 1e-6;
+        int d0 = (int)xd;
+        d0 = d0 < 0 ? 0 : (d0 > nd - 1 ? nd - 1 : d0);   // NaN inputs stay inside the table
+        const double fdf = xd - d0, fd0 = 1.0 - fdf;
+        const int d1 = d0 + 1 < nd ? d0 + 1 : nd - 1;
+        int b0 = (int)xb;
+        b0 = b0 < 0 ? 0 : (b0 > nb - 1 ? nb - 1 : b0);
+        const double fbf = xb - b0, fb0 = 1.0 - fbf;
+        const int b1 = b0 + 1 < nb ? b0 + 1 : nb - 1;
+        double acc = 0.0;
+#pragma unroll
+        for (int cc = 0; cc < 2; ++cc) {
+            int ci = cc ? c1 : c0;
+            if (ci > NPHI / 2) ci = NPHI - ci;
+            const double fci = cc ? fc : 1.0 - fc;
+            const float* r0 = Tn + b0 * row + (int64_t)ci * nd;
+            const float* r1 = Tn + b1 * row + (int64_t)ci * nd;
+            const double v = fb0 * (fd0 * (double)r0[d0] + fdf * (double)r0[d1]) +
+                             fbf * (fd0 * (double)r1[d0] + fdf * (double)r1[d1]);
+            acc += fci * v;
+        }
+        if (sel) acc += (double)lw[(int64_t)m * n_walk_sel + k];
+        if (acc > mx) {
+            s = s * exp(mx - acc) + 1.0;
+            mx = acc;
+        } else {
+            s += exp(acc - mx);
+        }
+    }
+    om[e * ntau + r] = mx;
+    os[e * ntau + r] = s;
+}
+
+// ln Lambda[e] = max_r om + ln sum_r os e^{om - max} - ln n_time
+__global__ void k_lse_lags(const double* __restrict__ om, const double* __restrict__ os, int64_t njobs, int ntau,
+                           double ln_n_time, double* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= njobs) return;
+    const double* a = om + e * ntau;
+    const double* b = os + e * ntau;
+    double mx = a[0];
+    for (int r = 1; r < ntau; ++r) mx = a[r] > mx ? a[r] : mx;
+    double tot = 0.0;
+    for (int r = 0; r < ntau; ++r) tot += b[r] * exp(a[r] - mx);
+    out[e] = mx + log(tot) - ln_n_time;
+}
+
+int launch_gather(Traj* traj, Store* st, const int32_t* d_m, const int32_t* d_tab, const double* d_offb,
+                  int64_t njobs, Times* times, double* d_out) {
+    Context& c = ctx();
+    double* pairs = nullptr;
+    EXM_TRY(scratch(0, sizeof(double) * 2 * (size_t)njobs * st->ntau, (void**)&pairs));
+    double* om = pairs;
+    double* os = pairs + (size_t)njobs * st->ntau;
+    const double rdb = 2.0 / st->dbeta;
+    const double offd = -st->delta0 / st->ddelta;
+    const double rdd = 1.0 / st->ddelta;
+    dim3 grid((unsigned)((njobs + 255) / 256), (unsigned)st->ntau);
+    hipLaunchKernelGGL(k_gather_ref, grid, dim3(256), 0, c.stream, traj->lqH(), traj->phH(), traj->lqL(),
+                       traj->phL(), traj->n_time, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->d_tau,
+                       st->ntau, d_m, d_tab, d_offb, njobs, rdb, offd, rdd, times ? times->d_sel : nullptr,
+                       times ? times->d_lw : nullptr, times ? times->n_walk : 0, om, os);
+    EXM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_lse_lags, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0, c.stream, om, os, njobs,
+                       st->ntau, std::log((double)traj->n_time), d_out);
+    EXM_HIP(hipGetLastError());
+    return EXM_OK;
+}
+
+}  // namespace exm

@@ -1,0 +1,283 @@
+"""
+Parameter estimation of single segments with the stored tables on the GPU
+(src/exmarg/search/pe.py:1-346).
+
+pe_intrinsic  (M_det, q, chi) on a template bank, the distance integrated inside the tables
+pe_redshift   (m1_source, z) at equal mass and zero spin, the distance a parameter through y = D0 / D_L(z)
+
+Per segment the device runs the matched filter, the arrival-time selection and one gather of
+every (template, node) pair plus the boundary points; the host assembles the node values with the
+distance-prior weights (float64) and averages over the bank.
+"""
+
+import numpy as np
+
+from .. import cosmology
+from ..priors import distance_density
+from . import gather as _gather, store as _store
+from .bank import check_device
+from .segments import D0, read_segments
+
+
+def _dl_of_z(z):
+    return cosmology.luminosity_distance(z)
+
+
+def _node_geometry(bank, st, ymin):
+    """which (node, template) pairs the store covers, their ln a and y, and the boundary points
+    (src/exmarg/search/pe.py:58-84): a = s/u is valid for sqrt(a) >= ymin and
+    min(alpha_lo, ymin^2) < a < alpha_hi; a template whose a crosses that lower limit between two
+    nodes also gets a point at a = alpha_lo' (1 + 1e-9) read from the upper node's table (row nn)."""
+    s = np.asarray(st.s, float)
+    u = np.asarray(bank.u, float)
+    nn, nm = s.size, u.size
+    alo = np.minimum(np.asarray(st.alpha_lo, float), ymin * ymin)
+    a = s[:, None] / u[None, :]
+    valid = (np.sqrt(a) >= ymin) & (a > alo[:, None]) & (a < np.asarray(st.alpha_hi, float)[:, None])
+    lna = np.where(valid, np.log(np.where(valid, a, 1.0)), 0.0)
+    y = np.zeros((nn + 1, nm))
+    y[:nn] = np.where(valid, np.sqrt(a), 0.0)
+    s_prev = np.concatenate([[0.0], s[:-1]])
+    low = valid & (s_prev[:, None] / u[None, :] <= alo[:, None])
+    bnode = np.where(low, np.arange(nn)[:, None], -1).max(axis=0)
+    blna = np.zeros(nm)
+    has = bnode >= 0
+    a_b = alo[bnode[has]] * (1.0 + 1e-9)
+    blna[has] = np.log(a_b)
+    y[nn, has] = np.sqrt(a_b)
+    return valid, lna, y, bnode, blna
+
+
+def _sorted_points(yN):
+    """per template: the rows with y > 0 sorted by y, as (order, count) with order (n_rows, nm)"""
+    Y = np.where(yN > 0, yN, np.inf)
+    order = np.argsort(Y, axis=0, kind="stable")
+    count = (yN > 0).sum(axis=0)
+    return order, count
+
+
+def _interval_weights(x, ok, density, ymin):
+    """weights of the piecewise-linear integration on consecutive points x[i], x[i+1] (columns) where
+    ok[i] marks a valid interval; returns w with the shape of x"""
+    a, b = x[:-1], x[1:]
+    h = np.where(ok, b - a, 1.0)
+    a = np.where(ok, a, 0.0)
+    b = np.where(ok, b, 1.0)
+    if density is None:
+        c, k = 3.0 * ymin ** 3, 3.0
+        ea, eb = np.exp(-k * a), np.exp(-k * b)
+        I0 = (ea - eb) / k
+        I1 = (a * ea - b * eb) / k + (ea - eb) / k ** 2
+        left = c * (b * I0 - I1) / h
+        right = c * (I1 - a * I0) / h
+    else:
+        lny, p = density
+        step = (b - a) / 400.0
+        t = a[..., None] + np.arange(401) * step[..., None]
+        t[..., -1] = b
+        f = np.interp(t.ravel(), lny, p, left=0.0, right=0.0).reshape(t.shape)
+        left = np.trapezoid(f * (b[..., None] - t) / h[..., None], t, axis=-1)
+        right = np.trapezoid(f * (t - a[..., None]) / h[..., None], t, axis=-1)
+    w = np.zeros(x.shape)
+    w[1:] += np.where(ok, right, 0.0)
+    w[:-1] += np.where(ok, left, 0.0)
+    return w
+
+
+def distance_weights(y, prior, ymin):
+    """ln of the weights that integrate the distance prior against a likelihood ratio linear in x = ln y
+    between the points y (ascending): sum_j w_j Lambda(y_j) = integral dy p(y) Lambda(y) over
+    [y_0, y_last] (src/exmarg/search/pe.py:260-286).
+
+    prior  'volume': p(y) = 3 ymin^3 y^-4, integrated exactly on every interval;
+           'madau-dickinson': the density in ln y of exmarg_amd.priors, trapezoid rule on 401 points
+           inside every interval"""
+    density = distance_density(prior)
+    x = np.log(np.asarray(y, float))
+    ok = np.ones(max(x.size - 1, 0), bool)
+    w = _interval_weights(x, ok, density, ymin)
+    with np.errstate(divide="ignore"):
+        return np.log(w)
+
+
+def _bank_weights(yN, prior, ymin, bank):
+    """lnW (nn+1, nm): distance weights of every template over the points it reaches
+    (src/exmarg/search/pe.py:321-327), all templates at once"""
+    density = distance_density(prior)
+    order, count = _sorted_points(yN)
+    bad = np.nonzero(count < 2)[0]
+    if bad.size:
+        m = int(bad[0])
+        raise ValueError("template %d (M %.2f, u %.3f) reaches fewer than two store nodes" % (m, bank.M[m], bank.u[m]))
+    Ys = np.take_along_axis(yN, order, axis=0)
+    rank = np.arange(yN.shape[0])[:, None]
+    x = np.log(np.where(rank < count[None, :], Ys, 1.0))
+    ok = (rank[1:] < count[None, :])
+    w = _interval_weights(x, ok, density, ymin)
+    lnW = np.full(yN.shape, -np.inf)
+    with np.errstate(divide="ignore"):
+        lw = np.where(rank < count[None, :], np.log(np.where(rank < count[None, :], w, 1.0)), -np.inf)
+    np.put_along_axis(lnW, order, lw, axis=0)
+    return lnW
+
+
+def _log_mean(lam, w):
+    """ln sum_i w_i exp(lam_i) for normalized weights w, lam = -inf allowed (src/exmarg/search/pe.py:135-142)"""
+    lam, w = np.asarray(lam), np.asarray(w)
+    fin = np.isfinite(lam) & (w > 0)
+    if not bool(fin.any()):
+        return -np.inf
+    mx = np.where(fin, lam, -np.inf).max()
+    return float(mx + np.log(np.sum(np.where(fin, w * np.exp(np.where(fin, lam, mx) - mx), 0.0))))
+
+
+def _cell_weights(bank):
+    """product of the spacings of the distinct values of every varying axis (src/exmarg/search/pe.py:338-343)"""
+    cell = np.ones(bank.u.size)
+    for ax in (bank.M, bank.q, bank.chi):
+        vals = np.unique(ax)
+        if vals.size > 1:
+            cell *= np.gradient(vals)[np.searchsorted(vals, ax)]
+    return cell / cell.sum()
+
+
+def _jobs(geometry):
+    """the flat read list: every valid (node, template) pair, then every boundary point
+    (src/exmarg/search/pe.py:178-194); returns (job_m, job_tab, job_lna, row) with row the lamN row"""
+    valid, lna, _, bnode, blna = geometry
+    nn = valid.shape[0]
+    k, m = np.nonzero(valid)
+    bm = np.nonzero(bnode >= 0)[0]
+    job_m = np.concatenate([m, bm]).astype(np.int32)
+    job_tab = np.concatenate([k, bnode[bm]]).astype(np.int32)
+    job_lna = np.concatenate([lna[k, m], blna[bm]])
+    row = np.concatenate([k, np.full(bm.size, nn)])
+    return job_m, job_tab, job_lna, row
+
+
+class _Analysis:
+    """device state of one (bank, store, ymin): the uploaded tables and the read plan"""
+
+    def __init__(self, bank, store, ymin, resident):
+        check_device(bank.device)
+        self.bank, self.store, self.resident = bank, store, resident
+        self.geometry = _node_geometry(bank, store, ymin)
+        self.was_resident = store.device is not None
+        self.dev = _store.upload(store)
+        key = (id(store), float(ymin))
+        cache = bank.__dict__.setdefault("_plans", {})
+        if key not in cache:
+            job_m, job_tab, job_lna, row = _jobs(self.geometry)
+            cache[key] = (_gather.Plan(self.dev, bank.u.size, job_m, job_tab, job_lna), row)
+        self.plan, self.row = cache[key]
+
+    def node_values(self, seg, j, n_loud, n_random, seed):
+        """lamN (nn+1, nm) of segment j of seg (src/exmarg/search/pe.py:168-200)"""
+        bank = self.bank
+        traj = bank.trajectories({I: seg["d_" + I][j] for I in "HL"}, {I: seg["sig2_" + I] for I in "HL"})
+        times = None
+        if n_loud > 0:
+            times = _gather.select_times(traj, self.store.tau_idx, n_loud, n_random, seed + int(seg["rows"][j]))
+        vals = self.plan.run(traj, self.dev, times)
+        if times is not None:
+            times.free()
+        traj.free()
+        nn = self.geometry[0].shape[0]
+        lamN = np.full((nn + 1, bank.u.size), -np.inf)
+        lamN[self.row, self.plan.job_m] = vals
+        return lamN
+
+    def close(self):
+        if not self.resident and not self.was_resident:
+            _store.free(self.store)
+
+
+def pe_intrinsic(segments, bank, store, rows=None, prior="volume", ymin=0.1267, chunk=4, nodes_per_launch=6,
+                 block=256, resident=False, n_loud=1024, n_random=1024, seed=0):
+    """ln Lambda of each template of the bank, the distance integrated over its prior
+    (src/exmarg/search/pe.py:289-346).
+
+    segments          anything read_segments takes
+    bank              mass_bank built on the same segments, rows (M_det, q, chi), device 'hip'
+    store             load_redshift_store (or store.synthetic_store)
+    rows              segment indices (default all)
+    prior             'volume' or 'madau-dickinson'
+    ymin              smallest y = D0 / D_L of the prior
+    resident          keep the tables on the device after the call
+    n_loud, n_random  arrival times visited per template (select_times); n_loud = 0 visits all of them
+    seed              seed of the random draw (plus the segment index)
+    chunk, nodes_per_launch, block are accepted for the reference signature: every node of a segment
+    is read in one launch.
+
+    returns dict: M, q, chi, u (n_template,), ln_lambda (n_segment, n_template), ln_b (n_segment,), rows."""
+    seg = read_segments(segments, rows)
+    an = _Analysis(bank, store, ymin, resident)
+    try:
+        lnW = _bank_weights(an.geometry[2], prior, ymin, bank)
+        out = np.full((seg["rows"].size, bank.u.size), -np.inf)
+        for j in range(seg["rows"].size):
+            v = lnW + an.node_values(seg, j, n_loud, n_random, seed)
+            mx = v.max(axis=0)
+            out[j] = mx + np.log(np.exp(v - mx[None, :]).sum(axis=0))
+    finally:
+        an.close()
+    w = _cell_weights(bank)
+    ln_b = np.array([_log_mean(L, w) for L in out])
+    return dict(M=bank.M, q=bank.q, chi=bank.chi, u=bank.u, ln_lambda=out, ln_b=ln_b, rows=seg["rows"])
+
+
+def _to_source_frame(lamN, yN, Mdet, yq, m1, z):
+    """node values (n_node + 1, n_template) -> ln Lambda on (m1, z) (src/exmarg/search/pe.py:87-132):
+    linear in y per template between its finite points, then linear in M_det = 2 m1 (1 + z) across the
+    templates finite at that z; -inf outside the points"""
+    nm = lamN.shape[1]
+    surf = np.full((yq.size, nm), -np.inf)
+    ok = np.isfinite(lamN) & (yN > 0)
+    for m in np.nonzero(ok.sum(axis=0) >= 2)[0]:
+        ys, ls = yN[ok[:, m], m], lamN[ok[:, m], m]
+        o = np.argsort(ys, kind="stable")
+        ys, ls = ys[o], ls[o]
+        inside = (yq >= ys[0]) & (yq <= ys[-1])
+        surf[inside, m] = np.interp(yq[inside], ys, ls)
+    out = np.full((m1.size, z.size), -np.inf)
+    good = np.isfinite(surf)
+    for iz in np.nonzero(good.sum(axis=1) >= 2)[0]:
+        Mg, Lg = Mdet[good[iz]], surf[iz, good[iz]]
+        Md = 2.0 * m1 * (1.0 + z[iz])
+        inb = (Md >= Mg[0]) & (Md <= Mg[-1])
+        out[inb, iz] = np.interp(Md[inb], Mg, Lg)
+    return out
+
+
+def pe_redshift(segments, bank, store, rows=None, chunk=8, zmax=8.0, nz=1600, m1_range=(3.0, 150.0), nm1=1600,
+                ymin=0.1267, nodes_per_launch=6, block=256, mdet_range=(10.0, 500.0), resident=False, n_loud=1024,
+                n_random=1024, seed=0):
+    """ln Lambda(m1_source, z) of each segment (src/exmarg/search/pe.py:203-257).
+
+    bank is a mass_bank of a one-column bank of detector-frame total masses (ascending, equal mass, zero
+    spin) on the same segments.  ln B averages Lambda over the prior uniform in M_det = 2 m1 (1+z) on
+    mdet_range and proportional to D_L^2 in luminosity distance, p(m1, z) ~ (1+z) D_L^2 dD_L/dz.
+
+    returns dict: m1 (nm1,), z (nz,), ln_lambda (n_segment, nm1, nz), ln_b (n_segment,), rows."""
+    if np.any(np.diff(bank.M) <= 0) or np.any(bank.q != 1.0) or np.any(bank.chi != 0.0):
+        raise ValueError("pe_redshift needs an ascending equal-mass zero-spin bank")
+    seg = read_segments(segments, rows)
+    z = np.linspace(0.02, zmax, int(nz))
+    m1 = np.geomspace(m1_range[0], m1_range[1], int(nm1))
+    dl = _dl_of_z(z)
+    yq = D0 / dl
+    an = _Analysis(bank, store, ymin, resident)
+    out = np.full((seg["rows"].size, m1.size, z.size), -np.inf)
+    try:
+        for j in range(seg["rows"].size):
+            lamN = an.node_values(seg, j, n_loud, n_random, seed)
+            out[j] = _to_source_frame(lamN, an.geometry[2], bank.M, yq, m1, z)
+    finally:
+        an.close()
+    lnp = np.log(np.maximum((1.0 + z) * dl ** 2 * np.gradient(dl, z), 1e-300))
+    Md = 2.0 * m1[:, None] * (1.0 + z[None, :])
+    inside = (Md >= mdet_range[0]) & (Md <= mdet_range[1])
+    w = np.where(inside, np.exp(lnp)[None, :], 0.0) * np.gradient(m1)[:, None] * np.gradient(z)[None, :]
+    w /= w.sum()
+    ln_b = np.array([_log_mean(L, w) for L in out])
+    return dict(m1=m1, z=z, ln_lambda=out, ln_b=ln_b, rows=seg["rows"])

@@ -1,0 +1,201 @@
+"""
+The (beta, Delta) tables of the redshift store (src/exmarg/search/store.py:1-127).
+
+Host side: bd_*.npz files read into host memory (optionally pinned) or mapped
+from disk, sorted by signal power s.  Device side: one HBM-resident stack of
+every node, uploaded once and kept for the life of the store object (288 GB of
+HBM holds the 40 GB store whole).
+"""
+
+import ctypes
+import glob
+import os
+import zipfile
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .. import _hip, synth
+
+NPHI = 96                 # phase bins of the full circle; the tables hold NPHI/2 + 1 (store.py:20)
+
+
+def _member_offset(path, name):
+    with zipfile.ZipFile(path) as zf:
+        info = zf.getinfo(name)
+    if info.compress_type != 0:
+        raise ValueError("member %s of %s is compressed" % (name, path))
+    with open(path, "rb") as fh:
+        fh.seek(info.header_offset)
+        head = fh.read(30)
+        nlen = int.from_bytes(head[26:28], "little")
+        elen = int.from_bytes(head[28:30], "little")
+        fh.seek(info.header_offset + 30 + nlen + elen)
+        ver = np.lib.format.read_magic(fh)
+        rd = np.lib.format.read_array_header_1_0 if ver == (1, 0) else np.lib.format.read_array_header_2_0
+        shape, fortran, dtype = rd(fh)
+        return fh.tell(), shape, fortran, dtype
+
+
+def read_npz_member(path, name):
+    """one uncompressed .npy member of an npz, read sequentially (src/exmarg/search/store.py:23-36)"""
+    offset, shape, fortran, dtype = _member_offset(path, name)
+    with open(path, "rb") as fh:
+        fh.seek(offset)
+        arr = np.fromfile(fh, dtype=dtype, count=int(np.prod(shape)))
+    return arr.reshape(shape, order="F" if fortran else "C")
+
+
+def map_npz_member(path, name):
+    """read-only memory map of one uncompressed .npy member (src/exmarg/search/store.py:39-52)"""
+    offset, shape, fortran, dtype = _member_offset(path, name)
+    return np.asarray(np.memmap(path, dtype=dtype, mode="r", offset=offset, shape=shape,
+                                order="F" if fortran else "C"))
+
+
+@dataclass
+class RedshiftStore:
+    """signal power nodes s (sorted), their amplitude limits, grids, host tables and the truncation order
+    (src/exmarg/search/store.py:76-87).  tables may be None for a store that lives on the device only;
+    device holds the DeviceStore once uploaded."""
+    s: np.ndarray
+    alpha_lo: np.ndarray
+    alpha_hi: np.ndarray
+    beta: list
+    delta: list
+    tau_idx: np.ndarray
+    tables: list
+    n_max: int
+    device: object = field(default=None, repr=False)
+    shape: tuple = None
+    synthetic: dict = None
+
+    def table_shape(self):
+        if self.tables is not None:
+            return tuple(self.tables[0].shape)
+        return tuple(self.shape)
+
+
+def load_redshift_store(store_dir, pinned=False, readers=6, mmap=False):
+    """every node of the redshift store in host memory (src/exmarg/search/store.py:90-107).
+    pinned is accepted for the reference signature: the upload stages through the library's own
+    buffers.  mmap=True maps the tables from disk instead of reading them."""
+    files = glob.glob(os.path.join(store_dir, "bd_*.npz"))
+    if not files:
+        raise FileNotFoundError("no bd_*.npz in %s" % store_dir)
+    head = []
+    for f in files:
+        with np.load(f) as h:
+            head.append((float(h["s_node"]), float(h["alpha"][0]), float(h["alpha"][-1]),
+                         h["beta"], h["delta"], h["tau_idx"], f))
+    head.sort(key=lambda r: r[0])
+    s, alo, ahi, beta, delta, tau, files = zip(*head)
+    if mmap:
+        tables = [map_npz_member(f, "T.npy") for f in files]
+    else:
+        with ThreadPoolExecutor(max_workers=max(1, int(readers))) as ex:
+            tables = list(ex.map(lambda f: read_npz_member(f, "T.npy"), files))
+    return RedshiftStore(s=np.array(s), alpha_lo=np.array(alo), alpha_hi=np.array(ahi), beta=list(beta),
+                         delta=list(delta), tau_idx=tau[0], tables=tables,
+                         n_max=int(os.path.basename(files[0]).split("_n")[1].split("_")[0]))
+
+
+class DeviceStore:
+    """the tables of every node stacked on the device (src/exmarg/search/store.py:110-120)"""
+
+    def __init__(self, handle, shape, n_tables, beta, delta, tau_idx):
+        self.handle = handle
+        self.shape = tuple(shape)
+        self.n_tables = n_tables
+        self.beta = np.asarray(beta, float)
+        self.delta = np.asarray(delta, float)
+        self.tau_idx = np.asarray(tau_idx, np.int64)
+
+    def read_table(self, k):
+        out = np.empty(self.shape, np.float32)
+        _hip.call("exm_store_read_table", int(self.handle), int(k), _hip.ptr(out))
+        return out
+
+    def free(self):
+        self.handle.free()
+
+    @property
+    def nbytes(self):
+        return 4 * int(np.prod(self.shape)) * self.n_tables
+
+
+def _check_grids(st):
+    b0, d0 = np.asarray(st.beta[0]), np.asarray(st.delta[0])
+    for b, d in zip(st.beta, st.delta):
+        if not (np.array_equal(np.asarray(b), b0) and np.array_equal(np.asarray(d), d0)):
+            raise ValueError("every node of a store must share its beta and Delta grids")
+
+
+def _create(shape, n_tables, beta, delta, tau_idx):
+    _hip.init()
+    ntau, nphif, nb, nd = shape
+    if len(tau_idx) != ntau:
+        raise ValueError("tau_idx has %d lags, tables %d" % (len(tau_idx), ntau))
+    beta, delta = np.asarray(beta, float), np.asarray(delta, float)
+    tau = _hip.contiguous(tau_idx, np.int64)
+    h = _hip.new_handle("exm_store_create", int(n_tables), int(ntau), int(nphif), int(nb), int(nd), _hip.ptr(tau),
+                        float(beta[0]), float(beta[1] - beta[0]), float(delta[0]), float(delta[1] - delta[0]),
+                        kind="store")
+    return DeviceStore(h, shape, n_tables, beta, delta, tau)
+
+
+def upload(store):
+    """host tables -> one device stack, cached on store.device (src/exmarg/search/pe.py:146-154 with
+    resident=True)"""
+    if store.device is not None:
+        return store.device
+    _check_grids(store)
+    if store.tables is None:
+        raise ValueError("store has no host tables to upload")
+    shape = tuple(store.tables[0].shape)
+    dev = _create(shape, len(store.tables), store.beta[0], store.delta[0], store.tau_idx)
+    for k, T in enumerate(store.tables):
+        if tuple(T.shape) != shape:
+            raise ValueError("table %d has shape %s, expected %s" % (k, T.shape, shape))
+        a = _hip.contiguous(T, np.float32)
+        _hip.call("exm_store_set_table", int(dev.handle), k, _hip.ptr(a))
+    store.device = dev
+    return dev
+
+
+def free(store):
+    """release the device copy of a store (src/exmarg/search/store.py:123-127)"""
+    if store is not None and store.device is not None:
+        store.device.free()
+        store.device = None
+
+
+def synthetic_store(n_nodes=1, seed=0, kind="smooth", s_nodes=None, shape=None, max_lag=43, host=False):
+    """a store of synthetic tables generated on the device (and on the host when host=True), with the
+    grids and lags of the reference store files (exmarg_amd.synth.synthetic_store_arrays gives the same
+    values on the host)"""
+    beta, delta, _ = synth.store_grids()
+    tau = synth.lag_indices(max_lag)
+    if shape is not None:
+        ntau, nphif, nb, nd = shape
+        tau = synth.lag_indices((ntau - 1) // 2)
+        beta = beta[0] + (beta[1] - beta[0]) * np.arange(nb)
+        delta = (delta[1] - delta[0]) * (np.arange(nd) - nd // 2)
+    shp = (tau.size, synth.NPHIF, beta.size, delta.size)
+    if s_nodes is None:
+        s_nodes = synth.S_NODES[-n_nodes:] if n_nodes < synth.S_NODES.size else synth.S_NODES
+    s_nodes = np.asarray(s_nodes, float)
+    nn = s_nodes.size
+    tables = None
+    if host:
+        tables = [synth.table_values(seed, k, shp, tau, kind).reshape(shp) for k in range(nn)]
+    st = RedshiftStore(s=s_nodes, alpha_lo=np.full(nn, synth.ALPHA[0]), alpha_hi=np.full(nn, synth.ALPHA[-1]),
+                       beta=[beta] * nn, delta=[delta] * nn, tau_idx=tau, tables=tables, n_max=120, shape=shp,
+                       synthetic=dict(seed=seed, kind=kind, nodes=np.arange(nn)))
+    dev = _create(shp, nn, beta, delta, tau)
+    for k in range(nn):
+        _hip.call("exm_store_fill_synthetic", int(dev.handle), k, ctypes.c_uint64(seed), k,
+                  1 if kind == "rough" else 0)
+    st.device = dev
+    return st

@@ -1,0 +1,128 @@
+/*
+ * exmarg_hip — C ABI of the MI355X (gfx950) implementation of the exmarg
+ * segment analysis (package exmarg.search of the reference).
+ *
+ * Plain pointers and sizes only.  Host arrays are owned by the caller and
+ * are not retained after a call returns.  Device memory is owned by the
+ * library behind opaque 64-bit handles and released with exm_free.
+ * Every call returns EXM_OK (0) or a negative code and then sets a
+ * thread-local message readable with exm_last_error.  Calls are
+ * synchronous: results written to host pointers are complete on return.
+ *
+ * Each entry point names the reference interface it replaces
+ * (path:line in the reference repository).
+ */
+#ifndef EXMARG_HIP_H
+#define EXMARG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef uint64_t exm_handle;
+
+enum {
+    EXM_OK = 0,
+    EXM_ERR_ARG = -1,     /* bad argument: the Python layer raises ValueError */
+    EXM_ERR_HIP = -2,     /* HIP or hipFFT runtime failure: RuntimeError */
+    EXM_ERR_NOMEM = -3,   /* device allocation refused or over budget: MemoryError */
+    EXM_ERR_HANDLE = -4,  /* unknown or wrong-kind handle: ValueError */
+    EXM_ERR_STATE = -5    /* library not initialised / no device: RuntimeError */
+};
+
+/* ---------------------------------------------------------------- library */
+
+/* Select device `device` and create the library's stream.  Idempotent. */
+int exm_init(int device);
+
+/* Copy the last error message of this thread into buf (NUL-terminated). */
+int exm_last_error(char* buf, size_t n);
+
+/* Device name, HBM bytes and compute-unit count of the initialised device. */
+int exm_device_info(char* name, size_t n, int64_t* total_mem, int* num_cu);
+
+/* Release any object.  Replaces src/exmarg/search/store.py:123-127 (free) for tables
+ * and the implicit cupy frees of trajectories (src/exmarg/search/pe.py:198-199). */
+int exm_free(exm_handle h);
+
+/* Bytes of device memory the library currently holds. */
+int exm_mem_in_use(int64_t* bytes);
+
+/* ---------------------------------------------------------------- tables */
+
+/* A device stack of ntab tables of shape (ntau, nphif, nb, nd) and their grids.
+ * Replaces src/exmarg/search/store.py:110-120 (upload): the library allocates the stack;
+ * tables are then set one by one.  beta0, dbeta, delta0, ddelta are beta[0], beta[1]-beta[0],
+ * delta[0], delta[1]-delta[0] of the store grids (src/exmarg/search/gather.py:110-111). */
+int exm_store_create(int ntab, int ntau, int nphif, int nb, int nd, const int64_t* tau_idx,
+                     double beta0, double dbeta, double delta0, double ddelta, exm_handle* out);
+
+/* Copy table k (host float32, reference layout (ntau, nphif, nb, nd), C order) to the device.
+ * Replaces one copy of src/exmarg/search/store.py:117-118. */
+int exm_store_set_table(exm_handle store, int k, const float* table);
+
+/* Fill table k on the device with synthetic store values (seed, node, kind 0 smooth / 1 rough),
+ * bit-identical to the host writer of the synthetic store files.  No reference counterpart:
+ * it stands in for reading 0.75 GB files per node (src/exmarg/search/store.py:55-67). */
+int exm_store_fill_synthetic(exm_handle store, int k, uint64_t seed, int64_t node, int kind);
+
+/* Copy table k back to the host in the reference layout (for checks). */
+int exm_store_read_table(exm_handle store, int k, float* table);
+
+/* ---------------------------------------------------------------- bank and matched filter */
+
+/* Template factors W (n_tmpl, n_freq) complex64 as interleaved float pairs, first bin lo,
+ * filter length n_time.  Replaces the device copy of src/exmarg/search/bank.py:54 (MassBank.W). */
+int exm_bank_create(const float* W, int n_tmpl, int n_freq, int lo, int64_t n_time, exm_handle* out);
+
+/* Matched filter of one segment against every template of the bank: trajectories lqH, phH,
+ * lqL, phL, each (n_tmpl, n_time) float32.  dH, dL are (n_freq) complex128 as interleaved
+ * double pairs, s2H, s2L (n_freq) float64.  Weighting and transform in float64, one rounding
+ * to float32 at the output.  Replaces src/exmarg/search/bank.py:56-78 (MassBank.trajectories). */
+int exm_filter(exm_handle bank, const double* dH, const double* dL, const double* s2H, const double* s2L,
+               exm_handle* traj_out);
+
+/* Trajectories from host arrays (n_tmpl, n_time) float32, and back. */
+int exm_traj_create(int n_tmpl, int64_t n_time, const float* lqH, const float* phH, const float* lqL,
+                    const float* phL, exm_handle* out);
+int exm_traj_read(exm_handle traj, float* lqH, float* phH, float* lqL, float* phL);
+
+/* ---------------------------------------------------------------- arrival times */
+
+/* Arrival times of every template: the K loudest by e^{2 lqL} + e^{2 max_{lag_lo..lag_hi} lqH}
+ * (float64 score, ties to the smaller time) plus S drawn ones, pick (n_tmpl, S) int64 indexes
+ * into the remaining times in time order, log weight lw_rand; all sorted by time.
+ * Replaces src/exmarg/search/gather.py:374-420 (select_times). */
+int exm_select_times(exm_handle traj, int lag_lo, int lag_hi, int K, int S, const int64_t* pick, float lw_rand,
+                     exm_handle* times_out);
+
+/* sel (n_tmpl, K+S) int32 and lw (n_tmpl, K+S) float32 of a times object. */
+int exm_times_read(exm_handle times, int32_t* sel, float* lw);
+
+/* ---------------------------------------------------------------- table read */
+
+/* ln Lambda of (template, table) pairs: out (ns, nsel) float64 on the host.
+ * mass_ids (nsel) int64, lna and tab_ids (nsel, ns); times 0 walks every arrival time.
+ * Replaces src/exmarg/search/gather.py:340-371 (gather_gpu) and :114-140 (gather_cpu),
+ * including _lse_lags (:103-107). */
+int exm_gather(exm_handle traj, exm_handle store, const int64_t* mass_ids, int64_t nsel, const double* lna,
+               const int64_t* tab_ids, int ns, exm_handle times, double* out);
+
+/* A reusable list of njobs (template, table, ln a) reads for exm_plan_gather.  Built once per
+ * bank, store and distance bound from the node geometry (src/exmarg/search/pe.py:58-84,
+ * 181-185). */
+int exm_plan_create(exm_handle store, int n_tmpl, int64_t njobs, const int32_t* job_m, const int32_t* job_tab,
+                    const double* job_lna, exm_handle* out);
+
+/* ln Lambda of every job of a plan for one segment, out (njobs) float64: every node group and
+ * the boundary reads of src/exmarg/search/pe.py:178-194 in one launch. */
+int exm_plan_gather(exm_handle plan, exm_handle traj, exm_handle store, exm_handle times, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,185 @@
+"""Parity of each HIP stage with the oracle on the same inputs (MI355X)."""
+import numpy as np
+import pytest
+
+from exmarg_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+# ---------------------------------------------------------------- store
+def test_synthetic_tables_match_host_recipe(hip, oracle):
+    from exmarg_amd.search import store as st
+    say("store: synthetic smooth+rough, reduced geometry")
+    for kind in ("smooth", "rough"):
+        s = st.synthetic_store(n_nodes=2, seed=5, kind=kind, shape=(7, 49, 12, 15), host=True)
+        for k in range(2):
+            np.testing.assert_array_equal(s.device.read_table(k), s.tables[k])
+        st.free(s)
+
+
+def test_synthetic_full_geometry_sample(hip):
+    from exmarg_amd.search import store as st
+    say("store: one node at full geometry (87, 49, 200, 219)")
+    s = st.synthetic_store(n_nodes=1, seed=0)
+    T = s.device.read_table(0)
+    idx = np.random.default_rng(0).integers(0, T.size, 20000)
+    ref = synth.table_values(0, 0, T.shape, s.tau_idx, "smooth", flat=idx)
+    np.testing.assert_array_equal(T.ravel()[idx], ref)
+    st.free(s)
+
+
+def test_set_table_roundtrip(hip):
+    from exmarg_amd.search import store as st
+    say("store: upload and read back")
+    arr = synth.synthetic_store_arrays(n_nodes=3, seed=2, kind="rough", shape=(5, 49, 6, 7))
+    rs = st.RedshiftStore(s=arr["s"], alpha_lo=np.full(3, 0.16), alpha_hi=np.full(3, 4e4), beta=[arr["beta"]] * 3,
+                          delta=[arr["delta"]] * 3, tau_idx=arr["tau_idx"], tables=arr["tables"], n_max=120)
+    dev = st.upload(rs)
+    for k in range(3):
+        np.testing.assert_array_equal(dev.read_table(k), arr["tables"][k])
+    st.free(rs)
+
+
+# ---------------------------------------------------------------- filter
+def _segment(T=4.0, fmin=20.0, fmax=256.0, snr=10.0, seed=3):
+    seg, truth, _ = synth.simulate_at_snr(1, seed, (30.0, 0.8, 0.1), snr, T, fmin, fmax)
+    seg = dict(seg, sig2_H=seg["sig2"], sig2_L=seg["sig2"])
+    return seg
+
+
+def test_filter_matches_fp64_oracle(hip, oracle):
+    from exmarg_amd.search.bank import MassBank
+    say("filter: 24 templates, T=4 s, 20-256 Hz")
+    seg = _segment()
+    bank_rows = synth.chirp_mass_grid_bank(30.0, 0.8, 0.1, 24)
+    bank = MassBank(bank_rows[:, 0], bank_rows[:, 1], bank_rows[:, 2], seg)
+    d = {I: seg["d_" + I][0] for I in "HL"}
+    s2 = {I: seg["sig2"] for I in "HL"}
+    got = bank.trajectories(d, s2).to_host()
+    ref = oracle.filter_fp64(bank.W, bank.lo, bank.n_time, d, s2)
+    for k in ("lqH", "lqL"):
+        diff = np.abs(got[k].astype(np.float64) - ref[k])
+        ulp = np.spacing(np.abs(ref[k])).astype(np.float64)
+        assert (diff <= ulp).all(), k
+        assert np.mean(got[k] == ref[k]) > 0.999, k
+    for k in ("phH", "phL"):
+        dph = np.angle(np.exp(1j * (got[k].astype(np.float64) - ref[k])))
+        assert np.abs(dph).max() < 1e-6, k
+
+
+def test_filter_single_bin_closed_form(hip):
+    from exmarg_amd.search.bank import MassBank
+    say("filter: single-bin closed form")
+    nu = np.arange(40, 72)
+    seg = dict(f=nu / 2.0, nu=nu, T=2.0, fmin=20.0, fmax=36.0, sig2_H=np.full(nu.size, 4.0))
+    W = np.zeros((1, nu.size), np.complex64)
+    W[0, 5] = 0.5 - 0.25j
+    bank = MassBank([30.0], [1.0], [0.0], seg, factors=(W, np.ones(1)))
+    d = np.full(nu.size, 2.0 - 1.0j)
+    got = bank.trajectories({"H": d, "L": d}, {"H": seg["sig2_H"], "L": seg["sig2_H"]}).to_host()
+    w = complex(W[0, 5]) * np.conj(2.0 - 1.0j) / 4.0
+    np.testing.assert_allclose(got["lqH"][0], np.log(abs(w)), atol=1e-6)
+    t = np.arange(bank.n_time)
+    ph = np.angle(w) - 2 * np.pi * 45 * t / bank.n_time
+    assert np.abs(np.angle(np.exp(1j * (got["phH"][0] - ph)))).max() < 2e-6
+
+
+# ---------------------------------------------------------------- select_times
+def _rand_traj(nm, nt, seed):
+    rng = np.random.default_rng(seed)
+    return {"lqH": rng.standard_normal((nm, nt)).astype(np.float32),
+            "phH": rng.uniform(-np.pi, np.pi, (nm, nt)).astype(np.float32),
+            "lqL": rng.standard_normal((nm, nt)).astype(np.float32),
+            "phL": rng.uniform(-np.pi, np.pi, (nm, nt)).astype(np.float32)}
+
+
+@pytest.mark.parametrize("nm,nt,K,S,lag", [(5, 4096, 64, 100, 43), (3, 20000, 1024, 1024, 43), (4, 300, 299, 5, 3)])
+def test_select_matches_oracle(hip, oracle, nm, nt, K, S, lag):
+    from exmarg_amd.search import gather as g
+    say("select: nm=%d nt=%d K=%d S=%d" % (nm, nt, K, S))
+    traj = _rand_traj(nm, nt, seed=nt)
+    tau = synth.lag_indices(lag)
+    sel, lw = g.select_times(traj, tau, K, S, seed=7).to_host()
+    rsel, rlw = oracle.select_times(traj, tau, K, S, seed=7)
+    np.testing.assert_array_equal(sel, rsel)
+    np.testing.assert_array_equal(lw, rlw)
+
+
+def test_select_ties_take_earliest(hip, oracle):
+    from exmarg_amd.search import gather as g
+    say("select: all-equal scores")
+    traj = {k: np.zeros((2, 1000), np.float32) for k in ("lqH", "phH", "lqL", "phL")}
+    sel, lw = g.select_times(traj, synth.lag_indices(5), 100, 50, seed=1).to_host()
+    np.testing.assert_array_equal(sel[:, :100][lw[:, :100] == 0].reshape(2, -1)[:, :10], np.tile(np.arange(10), (2, 1)))
+    rsel, rlw = oracle.select_times(traj, synth.lag_indices(5), 100, 50, seed=1)
+    np.testing.assert_array_equal(sel, rsel)
+
+
+def test_select_rejects_bad_arguments(hip):
+    from exmarg_amd.search import gather as g
+    traj = _rand_traj(1, 100, 0)
+    with pytest.raises(ValueError):
+        g.select_times(traj, synth.lag_indices(3), 100, 5, seed=0)
+    with pytest.raises(ValueError):
+        g.select_times(traj, np.array([0, 1, 3]), 10, 5, seed=0)
+
+
+# ---------------------------------------------------------------- gather
+def _tiny_store(kind, shape, n_nodes=2, seed=1):
+    from exmarg_amd.search import store as st
+    return st.synthetic_store(n_nodes=n_nodes, seed=seed, kind=kind, shape=shape, host=True)
+
+
+@pytest.mark.parametrize("kind", ["smooth", "rough"])
+@pytest.mark.parametrize("subsampled", [False, True])
+def test_gather_matches_oracle(hip, oracle, kind, subsampled):
+    from exmarg_amd.search import gather as g
+    say("gather: kind=%s subsampled=%s" % (kind, subsampled))
+    shape = (9, 49, 40, 31)
+    s = _tiny_store(kind, shape)
+    nm, nt = 6, 2048
+    traj = _rand_traj(nm, nt, seed=21)
+    traj["lqH"] *= 2.0
+    traj["lqL"] *= 2.0
+    tau = s.tau_idx
+    times_dev = g.select_times(traj, tau, 64, 96, seed=3) if subsampled else None
+    times = times_dev.to_host() if subsampled else None
+    ids = np.array([0, 2, 3, 5])
+    lna = np.array([[0.5, -2.0], [1.0, 3.0], [-4.0, 0.1], [2.5, 2.5]])
+    tab = np.array([[0, 1], [1, 0], [0, 0], [1, 1]])
+    beta, delta = s.beta[0], s.delta[0]
+    got = g.gather(traj, s, ids, lna, tab, times=times_dev)
+    tabs = oracle.Tables(tables=s.tables, tau_idx=tau)
+    ref = oracle.gather(traj, tabs, beta, delta, ids, lna, tab, times)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
+
+
+def test_gather_full_geometry_one_node(hip, oracle):
+    from exmarg_amd.search import gather as g
+    from exmarg_amd.search import store as st
+    say("gather: full table geometry, synthetic node, 4 templates x 8192 times")
+    s = st.synthetic_store(n_nodes=1, seed=0)
+    traj = _rand_traj(4, 8192, seed=2)
+    ids = np.arange(4)
+    lna = np.array([[-2.0], [0.0], [3.0], [6.0]])
+    tab = np.zeros((4, 1), np.int64)
+    got = g.gather(traj, s, ids, lna, tab)
+    tabs = oracle.Tables(synth=dict(seed=0, kind="smooth", nodes=[0]), shape=s.table_shape(), tau_idx=s.tau_idx)
+    ref = oracle.gather(traj, tabs, s.beta[0], s.delta[0], ids, lna, tab)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
+    st.free(s)
+
+
+def test_gather_rejects_bad_ids(hip):
+    from exmarg_amd.search import gather as g
+    s = _tiny_store("smooth", (5, 49, 8, 9), n_nodes=1)
+    traj = _rand_traj(2, 64, 0)
+    with pytest.raises(ValueError):
+        g.gather(traj, s, np.array([5]), np.zeros((1, 1)), np.zeros((1, 1), np.int64))
+    with pytest.raises(ValueError):
+        g.gather(traj, s, np.array([0]), np.zeros((1, 1)), np.ones((1, 1), np.int64))

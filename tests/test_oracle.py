@@ -1,0 +1,275 @@
+"""Pins of the CPU oracle (oracle/) against what the mathematics fixes: closed forms, library routines,
+brute force on tiny inputs and the golden vectors held in the reference's data files.  Each test is built
+so that a plausible mistake in the oracle (a dropped term, a wrong sign or index, a transposed operand)
+fails it."""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from exmarg_amd import synth
+
+NT, NM = 64, 3
+SHAPE = (5, 49, 8, 9)          # lags -2..2, 49 phases, 8 beta rows, 9 Delta columns
+
+
+def tiny_grids(nb=8, nd=9):
+    beta, delta, _ = synth.store_grids()
+    db, dd = beta[1] - beta[0], delta[1] - delta[0]
+    return beta[0] + db * np.arange(nb), dd * (np.arange(nd) - nd // 2)
+
+
+def tiny_traj(seed=1, nm=NM, nt=NT, spread=1.0):
+    rng = np.random.default_rng(seed)
+    return {"lqH": (spread * rng.standard_normal((nm, nt))).astype(np.float32),
+            "phH": rng.uniform(-np.pi, np.pi, (nm, nt)).astype(np.float32),
+            "lqL": (spread * rng.standard_normal((nm, nt))).astype(np.float32),
+            "phL": rng.uniform(-np.pi, np.pi, (nm, nt)).astype(np.float32)}
+
+
+def table_from(fn, shape=SHAPE):
+    r, c, b, d = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    return fn(r, c, b, d).astype(np.float32)
+
+
+def run(oracle, traj, T, lna, times=None, ntab_ids=None):
+    beta, delta = tiny_grids(T.shape[2], T.shape[3])
+    tau = synth.lag_indices((T.shape[0] - 1) // 2)
+    tabs = oracle.Tables(tables=[T], tau_idx=tau)
+    ids = np.arange(NM)
+    val = oracle.gather(traj, tabs, beta, delta, ids, np.asarray(lna, float).reshape(NM, 1),
+                        np.zeros((NM, 1), np.int64), times)
+    return val[0], beta, delta, tau
+
+
+def index_coords(traj, lna, beta, delta, tau):
+    """xb (nm, nt), xd (nm, ntau, nt), x3 (nm, ntau, nt) of every lookup, from their definitions"""
+    nb, nd = beta.size, delta.size
+    db, dd = beta[1] - beta[0], delta[1] - delta[0]
+    aL = traj["lqL"].astype(np.float64)
+    xbr = aL * (2.0 / db) + ((np.asarray(lna)[:, None] - beta[0]) / db)
+    xb = np.clip(xbr, 0.0, nb - 1 - 1e-6)
+    aLc = aL + (xb - xbr) / (2.0 / db)
+    t = np.arange(NT)
+    xd = np.empty((NM, tau.size, NT))
+    x3 = np.empty((NM, tau.size, NT))
+    for r, tv in enumerate(tau):
+        i1 = (t + tv) % NT
+        xd[:, r] = np.clip((traj["lqH"][:, i1] - aLc) * (1.0 / dd) + (-delta[0] / dd), 0.0, nd - 1 - 1e-6)
+        dph = (traj["phH"][:, i1] - traj["phL"]).astype(np.float32).astype(np.float64)
+        x3[:, r] = np.mod(dph * (96 / (2 * np.pi)), 96.0)
+    return xb, xd, x3
+
+
+def lse(v, axis=None):
+    mx = np.max(v, axis=axis, keepdims=True)
+    return np.squeeze(mx, axis=axis) + np.log(np.sum(np.exp(v - mx), axis=axis))
+
+
+LNA = [0.3, -1.2, 2.0]
+
+
+def test_constant_table(oracle):
+    traj = tiny_traj()
+    val, *_ = run(oracle, traj, np.full(SHAPE, 1.75, np.float32), LNA)
+    np.testing.assert_allclose(val, 1.75 + np.log(SHAPE[0]), rtol=0, atol=1e-12)
+
+
+def test_constant_table_subsampled_weights_restore_the_sum(oracle):
+    traj = tiny_traj()
+    times = oracle.select_times(traj, synth.lag_indices(2), 10, 12, seed=3)
+    val, *_ = run(oracle, traj, np.full(SHAPE, -0.5, np.float32), LNA, times)
+    np.testing.assert_allclose(val, -0.5 + np.log(SHAPE[0]), atol=2e-7)
+
+
+def test_trilinear_table_is_reproduced_exactly(oracle):
+    # T = alpha b d c' + d/128 - b/32 (dyadic, so exact in float32): bilinear interpolation in
+    # (beta, Delta) and linear interpolation between the two (mirrored) phase bins reproduce it exactly
+    traj = tiny_traj(spread=1.4)
+    alpha = 0.0625
+    T = table_from(lambda r, c, b, d: alpha * b * d * c + d / 128.0 - b / 32.0)
+    val, beta, delta, tau = run(oracle, traj, T, LNA)
+    xb, xd, x3 = index_coords(traj, LNA, beta, delta, tau)
+    tri = np.minimum(x3, 96.0 - x3)
+    acc = alpha * xb[:, None, :] * xd * tri + xd / 128.0 - xb[:, None, :] / 32.0
+    expect = lse(acc.reshape(NM, -1), axis=1) - np.log(NT)
+    np.testing.assert_allclose(val, expect, rtol=0, atol=1e-9)
+    # clamps were exercised on both axes
+    assert (xb == 0).any() or (xb >= 6.999).any()
+
+
+def test_lag_slab_goes_with_its_lag(oracle):
+    # slab r carries an offset delta_r and a Delta slope: a slab read at the wrong lag changes the sum
+    traj = tiny_traj(seed=5)
+    offs = np.array([0.0, 0.875, -0.375, 1.25, 0.25])
+    T = table_from(lambda r, c, b, d: offs[r] + 0.375 * d)
+    val, beta, delta, tau = run(oracle, traj, T, LNA)
+    _, xd, _ = index_coords(traj, LNA, beta, delta, tau)
+    acc = offs[None, :, None] + 0.375 * xd
+    expect = lse(acc.reshape(NM, -1), axis=1) - np.log(NT)
+    np.testing.assert_allclose(val, expect, rtol=0, atol=1e-9)
+
+
+def test_lag_wraps_around_the_segment(oracle):
+    # shifting every trajectory in time by a full period of lags leaves ln Lambda unchanged
+    traj = tiny_traj(seed=7)
+    T = table_from(lambda r, c, b, d: np.sin(r + 0.3 * c) + 0.2 * b - 0.1 * d)
+    v0, *_ = run(oracle, traj, T, LNA)
+    rolled = {k: np.roll(v, 17, axis=1) for k, v in traj.items()}
+    v1, *_ = run(oracle, rolled, T, LNA)
+    np.testing.assert_allclose(v0, v1, rtol=0, atol=1e-12)
+
+
+def test_phase_is_read_evenly(oracle):
+    # swapping the phases of the two detectors flips dphi; the tables are even in dphi
+    traj = tiny_traj(seed=9)
+    T = table_from(lambda r, c, b, d: np.cos(0.2 * c) * (1 + 0.1 * b) + 0.05 * d)
+    swapped = dict(traj, phH=(-traj["phH"]).astype(np.float32), phL=(-traj["phL"]).astype(np.float32))
+    v0, *_ = run(oracle, traj, T, LNA)
+    v1, *_ = run(oracle, swapped, T, LNA)
+    np.testing.assert_allclose(v0, v1, rtol=0, atol=1e-5)
+
+
+def test_pexp_accuracy(oracle):
+    xs = np.linspace(-100.0, 60.0, 20001)
+    got = np.array([oracle.lib().oracle_pexp(float(x)) for x in xs[::50]])
+    ref = np.array([math.exp(x) for x in xs[::50]])
+    assert np.max(np.abs(got / ref - 1)) < 5e-15
+    # the numpy evaluation of the same operation sequence is bit-identical
+    np.testing.assert_array_equal(oracle.pexp_np(xs[::50]), got)
+
+
+def test_select_c_matches_numpy_lexsort(oracle):
+    traj = tiny_traj(seed=11, nt=256)
+    traj = {k: v[:, :256] for k, v in traj.items()}
+    tau = synth.lag_indices(4)
+    a = oracle.select_times(traj, tau, 20, 30, seed=4)
+    b = oracle.select_times_np(traj, tau, 20, 30, seed=4)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_select_top_times_outrank_the_rest(oracle):
+    nt, K, S = 300, 25, 40
+    rng = np.random.default_rng(2)
+    traj = {"lqH": rng.standard_normal((2, nt)).astype(np.float32),
+            "lqL": rng.standard_normal((2, nt)).astype(np.float32)}
+    sel, lw = oracle.select_times(traj, synth.lag_indices(3), K, S, seed=8)
+    assert (np.diff(sel, axis=1) >= 0).all()
+    for m in range(2):
+        mx = np.array([max(traj["lqH"][m, (t + k) % nt] for k in range(-3, 4)) for t in range(nt)])
+        score = np.exp(2.0 * traj["lqL"][m].astype(float)) + np.exp(2.0 * mx.astype(float))
+        top = sel[m][lw[m] == 0]
+        others = np.setdiff1d(np.arange(nt), top)
+        assert top.size == K and score[top].min() > score[others].max()
+        drawn = sel[m][lw[m] != 0]
+        pick = np.random.default_rng(8).integers(0, nt - K, size=(2, S))[m]
+        np.testing.assert_array_equal(np.sort(others[pick]), drawn)
+    np.testing.assert_allclose(lw[lw != 0], np.float32(np.log((nt - K) / S)))
+
+
+def test_select_ties_go_to_earlier_times(oracle):
+    traj = {"lqH": np.zeros((1, 128), np.float32), "lqL": np.zeros((1, 128), np.float32)}
+    sel, lw = oracle.select_times(traj, synth.lag_indices(2), 16, 8, seed=0)
+    np.testing.assert_array_equal(sel[0][lw[0] == 0], np.arange(16))
+
+
+def test_filter_single_bin_closed_form(oracle):
+    # one nonzero bin nu0: z(t) = W w e^{-2 pi i nu0 t / N}, constant modulus, linear phase
+    n_time, lo, nf, nu0 = 256, 10, 20, 13
+    W = np.zeros((1, nf), np.complex64)
+    W[0, nu0 - lo] = 0.5 - 0.25j
+    d = np.full(nf, 2.0 - 1.0j)
+    s2 = np.full(nf, 4.0)
+    traj = oracle.filter_fp64(W, lo, n_time, {"H": d, "L": d}, {"H": s2, "L": s2})
+    w = complex(W[0, nu0 - lo]) * np.conj(2.0 - 1.0j) / 4.0
+    t = np.arange(n_time)
+    np.testing.assert_allclose(traj["lqH"][0], np.float32(np.log(abs(w))), rtol=0, atol=1e-6)
+    ph = np.angle(w) - 2 * np.pi * nu0 * t / n_time
+    dphi = np.angle(np.exp(1j * (traj["phL"][0].astype(float) - ph)))
+    assert np.abs(dphi).max() < 1e-6
+
+
+def test_filter_matches_scipy_fft(oracle):
+    from scipy import fft
+    rng = np.random.default_rng(0)
+    n_time, lo, nf = 512, 30, 33
+    W = (rng.standard_normal((4, nf)) + 1j * rng.standard_normal((4, nf))).astype(np.complex64)
+    d = {I: rng.standard_normal(nf) + 1j * rng.standard_normal(nf) for I in "HL"}
+    s2 = {I: rng.uniform(0.5, 2, nf) for I in "HL"}
+    traj = oracle.filter_fp64(W, lo, n_time, d, s2)
+    pad = np.zeros((4, n_time), complex)
+    pad[:, lo:lo + nf] = W.astype(complex) * (np.conj(d["L"]) / s2["L"])
+    z = fft.fft(pad, axis=1)
+    np.testing.assert_allclose(traj["lqL"], np.log(np.abs(z)), rtol=0, atol=2e-6)
+
+
+def test_volume_weights_closed_form(oracle):
+    ymin = 0.1267
+    y = np.sort(np.random.default_rng(3).uniform(0.13, 3.0, 9))
+    w = np.exp(oracle.distance_weights(y, "volume", ymin))
+    np.testing.assert_allclose(w.sum(), ymin ** 3 * (y[0] ** -3 - y[-1] ** -3), rtol=1e-13)
+    lam = 0.4 + 1.7 * np.log(y)
+    ref = integrate.quad(lambda v: 3 * ymin ** 3 * v ** -4 * (0.4 + 1.7 * np.log(v)), y[0], y[-1],
+                         epsabs=0, epsrel=1e-13, limit=200)[0]
+    np.testing.assert_allclose(np.sum(w * lam), ref, rtol=1e-11)
+
+
+def test_madau_dickinson_weights_against_quad(oracle):
+    lny, p = oracle.distance_density("madau-dickinson")
+    y = np.exp(np.linspace(-2.0, 1.5, 7))
+    w = np.exp(oracle.distance_weights(y, "madau-dickinson", 0.0982))
+    lam = 2.0 - 0.8 * np.log(y)
+    f = lambda x: np.interp(x, lny, p, left=0.0, right=0.0) * (2.0 - 0.8 * x)
+    ref = integrate.quad(f, np.log(y[0]), np.log(y[-1]), points=list(np.log(y[1:-1])), limit=500)[0]
+    np.testing.assert_allclose(np.sum(w * lam), ref, rtol=2e-5)
+
+
+def test_log_mean_closed_form(oracle):
+    vals = np.array([0.5, 2.0, 3.5, 1.0])
+    assert abs(oracle.log_mean(np.log(vals), np.full(4, 0.25)) - np.log(vals.mean())) < 1e-14
+    assert oracle.log_mean(np.array([-np.inf, np.log(2.0)]), np.array([0.5, 0.5])) == pytest.approx(np.log(1.0))
+    assert oracle.log_mean(np.array([-np.inf]), np.array([1.0])) == -np.inf
+
+
+def test_node_geometry_inequalities(oracle):
+    s = synth.S_NODES
+    u = np.array([0.9, 5.0, 60.0])
+    ymin = 0.1267
+    valid, lna, y, bnode, blna = oracle.node_geometry(s, np.full(53, 0.16), np.full(53, 4e4), u, ymin)
+    a = s[:, None] / u[None, :]
+    np.testing.assert_array_equal(valid, (a >= ymin ** 2) & (a > ymin ** 2) & (a < 4e4))
+    for m in range(3):
+        k = int(np.nonzero(valid[:, m])[0][0])
+        assert bnode[m] == k and s[k - 1] / u[m] <= ymin ** 2
+        assert blna[m] == pytest.approx(np.log(ymin ** 2 * (1 + 1e-9)), abs=1e-15)
+
+
+def test_dl_matches_reference_planck18_pairs(oracle):
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "planck18_pairs.npz"))
+    np.testing.assert_allclose(oracle.dl_of_z(z["z"]), z["dl"], rtol=1e-7)
+
+
+def test_to_source_frame_brute_force(oracle):
+    rng = np.random.default_rng(4)
+    nn, nm = 6, 5
+    yN = np.sort(rng.uniform(0.2, 2.0, (nn, nm)), axis=0)
+    lamN = rng.standard_normal((nn, nm))
+    lamN[2, 1] = -np.inf
+    Mdet = np.linspace(20, 60, nm)
+    yq, m1, z = np.array([0.3, 0.9, 1.5]), np.array([9.0, 12.0, 20.0]), np.array([0.1, 0.5, 0.8])
+    out = oracle.to_source_frame(lamN, yN, Mdet, yq, m1, z)
+    for iz in range(3):
+        col = []
+        for m in range(nm):
+            ok = np.isfinite(lamN[:, m])
+            ys, ls = yN[ok, m], lamN[ok, m]
+            col.append(np.interp(yq[iz], ys, ls) if ys[0] <= yq[iz] <= ys[-1] else -np.inf)
+        col = np.array(col)
+        g = np.isfinite(col)
+        for i in range(3):
+            Md = 2 * m1[i] * (1 + z[iz])
+            exp_ = np.interp(Md, Mdet[g], col[g]) if g.sum() >= 2 and Mdet[g][0] <= Md <= Mdet[g][-1] else -np.inf
+            assert out[i, iz] == pytest.approx(exp_) if np.isfinite(exp_) else out[i, iz] == -np.inf
