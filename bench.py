@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark of the exmarg segment analysis on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): one two-detector segment, T = 16 s at
+4096 Hz, 20-1024 Hz (n_time = 131072, 16065 bins), Gaussian noise colored by the analytic aLIGO fit with
+one TaylorF2 injection (M_det = 30, q = 0.8, chi = 0.1, network SNR 12); a 2000-template bank on a
+20 x 10 x 10 grid over M_det in [10, 80], q in [0.2, 1], chi in [-0.5, 0.5]; all 53 nodes of synthetic
+float32 tables in the reference store layout (39.6 GB, HBM-resident); pe_intrinsic with the 'volume'
+prior and the reference defaults n_loud = n_random = 1024.
+
+A step is one pe_intrinsic call on the segment (matched filter, time selection, every table read, the
+node and bank assembly).  value = segments/s over the timed steps.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+TABLE_BYTES = 746_877_600        # one (87, 49, 200, 219) float32 table
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n-loud", type=int, default=1024)
+    p.add_argument("--n-random", type=int, default=1024)
+    p.add_argument("--grid", default="20,10,10", help="bank grid (M, q, chi)")
+    p.add_argument("--nodes", type=int, default=53)
+    p.add_argument("--cpu-templates", type=int, default=32, help="templates in the CPU-baseline sample")
+    p.add_argument("--check-templates", type=int, default=8, help="templates checked against the oracle")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle check")
+    p.add_argument("--gather-kernel", type=int, default=0, help="0 automatic, 1 reference-order kernel, 2 band kernel")
+    return p.parse_args()
+
+
+def workload(args):
+    from exmarg_amd import synth
+    seg, truth, snr = synth.simulate_at_snr(1, 20261203, (30.0, 0.8, 0.1), 12.0, 16.0, 20.0, 1024.0)
+    shape = tuple(int(v) for v in args.grid.split(","))
+    rows = synth.grid_bank(shape=shape)
+    return seg, rows, float(snr[0])
+
+
+def segment_bytes(F, n_tmpl, n_time, n_nodes):
+    """algorithmic bytes of one segment: data, template factors, filter output written and read once,
+    every table read once"""
+    return 32 * F + 8 * n_tmpl * F + 32 * n_tmpl * n_time + TABLE_BYTES * n_nodes
+
+
+def cpu_leg(args, seg, bank, store, res, an_geometry, seed_row):
+    """CPU baseline (oracle restatement, OpenMP) on a bounded sample, and the oracle check of a few
+    templates' ln Lambda"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import exm_oracle as O
+    from exmarg_amd.search import read_segments
+    from exmarg_amd.search.pe import _bank_weights
+    O.lib()
+    rseg = read_segments(seg)
+    d = {I: rseg["d_" + I][0] for I in "HL"}
+    s2 = {I: rseg["sig2_" + I] for I in "HL"}
+    nm, nt = bank.u.size, bank.n_time
+    K, S = args.n_loud, args.n_random
+    valid, lna, yN, bnode, blna = an_geometry
+    nn = valid.shape[0]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    full_pick = O.draw_picks(nm, nt, K, S, seed_row)
+    tabs_synth = O.Tables(synth=dict(seed=0, kind="smooth", nodes=np.arange(nn)), shape=store.table_shape(),
+                          tau_idx=store.tau_idx)
+    beta, delta = store.beta[0], store.delta[0]
+
+    def jobs_for(sub, nodes=None):
+        ids, tabs, lnas, rows = [], [], [], []
+        for k in range(nn):
+            if nodes is not None and k not in nodes:
+                continue
+            for j, m in enumerate(sub):
+                if valid[k, m]:
+                    ids.append(j); tabs.append(k); lnas.append(lna[k, m]); rows.append((k, j))
+        for j, m in enumerate(sub):
+            if bnode[m] >= 0 and (nodes is None or bnode[m] in nodes):
+                ids.append(j); tabs.append(bnode[m]); lnas.append(blna[m]); rows.append((nn, j))
+        return np.array(ids), np.array(tabs)[:, None], np.array(lnas)[:, None], rows
+
+    # -- oracle check of a few templates (tables by the synthetic recipe, every node)
+    chk = np.linspace(0, nm - 1, args.check_templates).astype(int)
+    trc = O.filter_fp64(bank.W[chk], bank.lo, nt, d, s2)
+    tmc = O.select_times(trc, store.tau_idx, K, S, 0, pick=full_pick[chk])
+    ids, tabs, lnas, rows = jobs_for(chk)
+    vals = O.gather(trc, tabs_synth, beta, delta, ids, lnas, tabs, tmc)[0]
+    lamN = np.full((nn + 1, chk.size), -np.inf)
+    for v, (k, j) in zip(vals, rows):
+        lamN[k, j] = v
+    lnW = _bank_weights(yN, "volume", 0.1267, bank)[:, chk]
+    v = lnW + lamN
+    mx = v.max(axis=0)
+    lnl_ref = mx + np.log(np.exp(v - mx).sum(axis=0))
+    err = float(np.max(np.abs(lnl_ref - res["ln_lambda"][0][chk])))
+
+    # -- CPU baseline: filter + selection of a template sample, gather rate on the tables of 4 nodes in RAM
+    samp = np.linspace(0, nm - 1, args.cpu_templates).astype(int)
+    t0 = time.perf_counter()
+    trs = O.filter_fp64(bank.W[samp], bank.lo, nt, d, s2)
+    t_filter = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tms = O.select_times(trs, store.tau_idx, K, S, 0, pick=full_pick[samp])
+    t_select = time.perf_counter() - t0
+    counts = valid.sum(axis=1)
+    nodes = list(np.argsort(-counts)[:4])
+    host_tables = []
+    for k in range(nn):
+        host_tables.append(O.synth_table(0, k, store.table_shape(), store.tau_idx) if k in nodes else None)
+    ids, tabs, lnas, _ = jobs_for(samp, nodes=set(nodes))
+    tabs_mem = O.Tables(tables=[t if t is not None else host_tables[nodes[0]] for t in host_tables],
+                        tau_idx=store.tau_idx)
+    t0 = time.perf_counter()
+    O.gather(trs, tabs_mem, beta, delta, ids, lnas, tabs, tms)
+    t_gather = time.perf_counter() - t0
+    lookups_sample = ids.size * store.tau_idx.size * (K + S)
+    cpu_lookup_rate = lookups_sample / t_gather
+    n_jobs_all = int(valid.sum() + (bnode >= 0).sum())
+    lookups_all = n_jobs_all * store.tau_idx.size * (K + S)
+    t_seg = (t_filter + t_select) * nm / samp.size + lookups_all / cpu_lookup_rate
+    sample = ("filter+selection of %d of %d templates (scaled x%.1f) and the gather of those templates on the "
+              "4 most-used nodes (%d reads x 87 lags x %d times, tables in RAM), scaled to all %d reads"
+              % (samp.size, nm, nm / samp.size, ids.size, K + S, n_jobs_all))
+    cpu = dict(value=1.0 / t_seg, unit="segments/s", cores=threads, kind="port", sample=sample,
+               seconds_per_segment=t_seg, gather_lookups_per_s=cpu_lookup_rate)
+    return cpu, err, chk.size
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the path runs one process per GPU and this build targets a single "
+              "MI355X (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    from exmarg_amd import _hip
+    from exmarg_amd.search import mass_bank, pe_intrinsic, store as stm
+    from exmarg_amd.search.pe import _node_geometry
+
+    seg, rows, snr = workload(args)
+    _hip.init(0)
+    _hip.call("exm_set_gather_kernel", args.gather_kernel)
+    info = _hip.device_info()
+    st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
+    bank = mass_bank(rows, seg, "hip")
+    kw = dict(prior="volume", ymin=0.1267, n_loud=args.n_loud, n_random=args.n_random, seed=0, resident=True)
+    for _ in range(args.warmup):
+        pe_intrinsic(seg, bank, st, **kw)
+    _hip.profile(True)
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        res = pe_intrinsic(seg, bank, st, **kw)
+        lat.append(time.perf_counter() - t0)
+    wall = time.perf_counter() - t_start
+    g_ms, g_n = _hip.profile_get("gather")
+    f_ms, f_n = _hip.profile_get("filter")
+    s_ms, s_n = _hip.profile_get("select")
+    w_ms, w_n = _hip.profile_get("walk")
+    _hip.profile(False)
+
+    geometry = _node_geometry(bank, st, 0.1267)
+    valid, bnode = geometry[0], geometry[3]
+    n_nodes_used = int(valid.any(axis=1).sum())
+    F = bank.nu.size
+    nm, nt = bank.u.size, bank.n_time
+    seg_bytes = segment_bytes(F, nm, nt, n_nodes_used)
+    g_s = g_ms / 1e3 / max(g_n, 1)
+    n_jobs = int(valid.sum() + (bnode >= 0).sum())
+    walk = args.n_loud + args.n_random if args.n_loud > 0 else nt
+    lookups = n_jobs * st.tau_idx.size * walk
+    achieved = seg_bytes / g_s / 1e9
+    value = args.steps / wall
+    out = {
+        "metric": "segments/sec at ln B err <=1e-5 vs FP64 CPU; p50/p99 latency; HBM GB/s",
+        "value": value, "unit": "segments/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 tables, f64 gather arithmetic", "data": "synthetic",
+        "config": {"workload": "one 16 s segment, 20-1024 Hz, 2000-template (M_det, q, chi) bank, 53 HBM-resident "
+                               "table nodes, pe_intrinsic 'volume' prior, n_loud=n_random=1024",
+                   "n_templates": nm, "n_time": nt, "n_freq": F, "nodes": n_nodes_used, "reads": n_jobs,
+                   "injection_snr": round(snr, 3), "device": info["name"], "num_cu": info["num_cu"]},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "gather",
+                     "bytes_per_launch": seg_bytes, "kernel_ms": 1e3 * g_s},
+        "stages_ms": {"filter": f_ms / max(f_n, 1), "select": s_ms / max(s_n, 1),
+                      "walk": w_ms / max(w_n, 1), "gather": g_ms / max(g_n, 1)},
+        "lookups_per_s": lookups / g_s,
+        "path_hbm_frac": seg_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS,
+        "ln_b": float(res["ln_b"][0]),
+    }
+    if not args.no_cpu:
+        an_geometry = geometry
+        cpu, err, nchk = cpu_leg(args, seg, bank, st, res, an_geometry, 0)
+        out["cpu_baseline"] = cpu
+        out["oracle_check"] = {"templates": nchk, "max_abs_dln_lambda": err, "bound_on_dln_b": err}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

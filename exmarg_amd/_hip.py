@@ -40,6 +40,10 @@ SIGNATURES = {
     "exm_gather": [H, H, P, I64, P, P, I32, H, P],
     "exm_plan_create": [H, I32, I64, P, P, P, HP],
     "exm_plan_gather": [H, H, H, H, P],
+    "exm_set_gather_kernel": [I32],
+    "exm_profile_enable": [I32],
+    "exm_profile_reset": [],
+    "exm_profile_get": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)],
 }
 
 
@@ -110,6 +114,18 @@ def mem_in_use():
     v = ctypes.c_int64()
     call("exm_mem_in_use", ctypes.byref(v))
     return v.value
+
+
+def profile(on=True):
+    call("exm_profile_enable", 1 if on else 0)
+    call("exm_profile_reset")
+
+
+def profile_get(name):
+    ms = ctypes.c_double()
+    n = ctypes.c_int64()
+    call("exm_profile_get", name.encode(), ctypes.byref(ms), ctypes.byref(n))
+    return ms.value, n.value
 
 
 def ptr(a):

@@ -1,5 +1,6 @@
 // C ABI of the exmarg HIP library: context, handles, argument checks and
 // the entry points declared in include/exmarg_hip.h.
+#include <algorithm>
 #include <cstdarg>
 #include <cmath>
 
@@ -75,10 +76,104 @@ Times::~Times() {
     dev_free(d_sel, sizeof(int32_t) * (size_t)n_tmpl * n_walk);
     dev_free(d_lw, sizeof(float) * (size_t)n_tmpl * n_walk);
 }
-Plan::~Plan() {
-    dev_free(d_m, sizeof(int32_t) * njobs);
-    dev_free(d_tab, sizeof(int32_t) * njobs);
-    dev_free(d_offb, sizeof(double) * njobs);
+Plan::~Plan() { jobset_free(&js); }
+
+int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const double* offb, JobSet* js) {
+    js->njobs = njobs;
+    js->order.resize(njobs);
+    for (int64_t i = 0; i < njobs; ++i) js->order[i] = i;
+    std::stable_sort(js->order.begin(), js->order.end(), [&](int64_t a, int64_t b) { return tab[a] < tab[b]; });
+    std::vector<int32_t> sm(njobs), st(njobs), tiles;
+    std::vector<double> so(njobs);
+    for (int64_t i = 0; i < njobs; ++i) {
+        sm[i] = m[js->order[i]];
+        st[i] = tab[js->order[i]];
+        so[i] = offb[js->order[i]];
+    }
+    // tiles: runs of one table split evenly into pieces of at most TILE_JOBS jobs
+    for (int64_t a = 0; a < njobs;) {
+        int64_t b = a;
+        while (b < njobs && st[b] == st[a]) ++b;
+        const int64_t n = b - a;
+        const int64_t pieces = (n + TILE_JOBS - 1) / TILE_JOBS;
+        for (int64_t q = 0; q < pieces; ++q) {
+            const int64_t lo = a + n * q / pieces, hi = a + n * (q + 1) / pieces;
+            tiles.push_back((int32_t)lo);
+            tiles.push_back((int32_t)(hi - lo));
+        }
+        a = b;
+    }
+    js->ntiles = (int)(tiles.size() / 2);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_m = 0, o_t = al(4 * (size_t)njobs), o_o = o_t + al(4 * (size_t)njobs),
+                 o_tiles = o_o + al(8 * (size_t)njobs), total = o_tiles + al(4 * tiles.size());
+    EXM_TRY(dev_alloc(&js->d_block, total));
+    js->bytes = total;
+    char* base = (char*)js->d_block;
+    js->d_m = (int32_t*)(base + o_m);
+    js->d_tab = (int32_t*)(base + o_t);
+    js->d_offb = (double*)(base + o_o);
+    js->d_tiles = (int32_t*)(base + o_tiles);
+    EXM_HIP(hipMemcpyAsync(js->d_m, sm.data(), 4 * (size_t)njobs, hipMemcpyHostToDevice, g_ctx.stream));
+    EXM_HIP(hipMemcpyAsync(js->d_tab, st.data(), 4 * (size_t)njobs, hipMemcpyHostToDevice, g_ctx.stream));
+    EXM_HIP(hipMemcpyAsync(js->d_offb, so.data(), 8 * (size_t)njobs, hipMemcpyHostToDevice, g_ctx.stream));
+    EXM_HIP(hipMemcpyAsync(js->d_tiles, tiles.data(), 4 * tiles.size(), hipMemcpyHostToDevice, g_ctx.stream));
+    EXM_HIP(hipStreamSynchronize(g_ctx.stream));
+    return EXM_OK;
+}
+
+void jobset_free(JobSet* js) {
+    if (js->d_block) {
+        (void)hipStreamSynchronize(g_ctx.stream);
+        dev_free(js->d_block, js->bytes);
+    }
+    js->d_block = nullptr;
+}
+
+static int g_gather_mode = 0;
+int gather_mode() { return g_gather_mode; }
+
+// ---------------------------------------------------------------- profiling
+struct ProfileEntry {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    double done_ms = 0.0;
+    int64_t count = 0;
+};
+static bool g_profile = false;
+static std::unordered_map<std::string, ProfileEntry> g_prof;
+
+ProfileScope::ProfileScope(const char* n) : name(n) {
+    if (!g_profile) return;
+    if (hipEventCreate(&start) != hipSuccess) {
+        start = nullptr;
+        return;
+    }
+    (void)hipEventRecord(start, g_ctx.stream);
+}
+
+ProfileScope::~ProfileScope() {
+    if (!start) return;
+    hipEvent_t stop;
+    if (hipEventCreate(&stop) != hipSuccess) {
+        (void)hipEventDestroy(start);
+        return;
+    }
+    (void)hipEventRecord(stop, g_ctx.stream);
+    g_prof[name].events.emplace_back(start, stop);
+}
+
+static void profile_collect() {
+    for (auto& kv : g_prof) {
+        for (auto& ev : kv.second.events) {
+            float ms = 0.0f;
+            (void)hipEventSynchronize(ev.second);
+            if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) kv.second.done_ms += ms;
+            kv.second.count += 1;
+            (void)hipEventDestroy(ev.first);
+            (void)hipEventDestroy(ev.second);
+        }
+        kv.second.events.clear();
+    }
 }
 
 // ---------------------------------------------------------------- registry
@@ -378,6 +473,7 @@ int exm_select_times(exm_handle ht, int lag_lo, int lag_hi, int K, int S, const 
     tm->kind = K_TIMES;
     tm->n_tmpl = traj->n_tmpl;
     tm->n_walk = K + S;
+    tm->lw_rand = lw_rand;
     int rc = dev_alloc((void**)&tm->d_sel, sizeof(int32_t) * (size_t)tm->n_tmpl * tm->n_walk);
     if (rc == EXM_OK) rc = dev_alloc((void**)&tm->d_lw, sizeof(float) * (size_t)tm->n_tmpl * tm->n_walk);
     if (rc == EXM_OK) rc = launch_select(traj, lag_lo, lag_hi, K, S, pick, lw_rand, tm);
@@ -419,21 +515,6 @@ static int check_times(Traj* traj, exm_handle htimes, Times** tm) {
     return EXM_OK;
 }
 
-static int upload_jobs(int64_t njobs, const int32_t* m, const int32_t* tab, const double* offb, int32_t** dm,
-                       int32_t** dtab, double** doffb) {
-    void* p = nullptr;
-    size_t need = (size_t)njobs * (4 + 4 + 8) + 512;
-    EXM_TRY(scratch(2, need, &p));
-    char* base = (char*)p;
-    *doffb = (double*)base;
-    *dm = (int32_t*)(base + (((size_t)njobs * 8 + 255) & ~(size_t)255));
-    *dtab = (int32_t*)((char*)*dm + (((size_t)njobs * 4 + 255) & ~(size_t)255));
-    EXM_HIP(hipMemcpyAsync(*doffb, offb, 8 * (size_t)njobs, hipMemcpyHostToDevice, ctx().stream));
-    EXM_HIP(hipMemcpyAsync(*dm, m, 4 * (size_t)njobs, hipMemcpyHostToDevice, ctx().stream));
-    EXM_HIP(hipMemcpyAsync(*dtab, tab, 4 * (size_t)njobs, hipMemcpyHostToDevice, ctx().stream));
-    return EXM_OK;
-}
-
 int exm_gather(exm_handle htraj, exm_handle hstore, const int64_t* mass_ids, int64_t nsel, const double* lna,
                const int64_t* tab_ids, int ns, exm_handle htimes, double* out) {
     LOCK;
@@ -461,15 +542,25 @@ int exm_gather(exm_handle htraj, exm_handle hstore, const int64_t* mass_ids, int
             offb[e] = (lna[j * ns + n] - st->beta0) / st->dbeta;
         }
     }
-    int32_t *dm, *dtab;
-    double* doffb;
-    EXM_TRY(upload_jobs(njobs, m.data(), tab.data(), offb.data(), &dm, &dtab, &doffb));
+    JobSet js;
+    EXM_TRY(jobset_build(njobs, m.data(), tab.data(), offb.data(), &js));
     double* dout = nullptr;
-    EXM_TRY(scratch(1, sizeof(double) * (size_t)njobs, (void**)&dout));
-    EXM_TRY(launch_gather(traj, st, dm, dtab, doffb, njobs, tm, dout));
-    EXM_HIP(hipMemcpyAsync(out, dout, sizeof(double) * njobs, hipMemcpyDeviceToHost, ctx().stream));
-    EXM_HIP(hipStreamSynchronize(ctx().stream));
-    return EXM_OK;
+    int rc = scratch(1, sizeof(double) * (size_t)njobs, (void**)&dout);
+    if (rc == EXM_OK) rc = launch_gather(traj, st, js, tm, dout);
+    std::vector<double> sorted(njobs);
+    if (rc == EXM_OK && hipMemcpyAsync(sorted.data(), dout, sizeof(double) * njobs, hipMemcpyDeviceToHost,
+                                       ctx().stream) != hipSuccess) {
+        set_error("gather: copy of the results failed");
+        rc = EXM_ERR_HIP;
+    }
+    if (rc == EXM_OK && hipStreamSynchronize(ctx().stream) != hipSuccess) {
+        set_error("gather: kernel failed");
+        rc = EXM_ERR_HIP;
+    }
+    if (rc == EXM_OK)
+        for (int64_t i = 0; i < njobs; ++i) out[js.order[i]] = sorted[i];
+    jobset_free(&js);
+    return rc;
 }
 
 int exm_plan_create(exm_handle hstore, int n_tmpl, int64_t njobs, const int32_t* job_m, const int32_t* job_tab,
@@ -478,6 +569,7 @@ int exm_plan_create(exm_handle hstore, int n_tmpl, int64_t njobs, const int32_t*
     EXM_TRY(ensure_init());
     NEED(Store, st, hstore, K_STORE);
     EXM_CHECK(njobs > 0 && n_tmpl > 0, EXM_ERR_ARG, "empty plan");
+    EXM_CHECK(job_m && job_tab && job_lna, EXM_ERR_ARG, "NULL argument");
     std::vector<double> offb(njobs);
     for (int64_t e = 0; e < njobs; ++e) {
         EXM_CHECK(job_m[e] >= 0 && job_m[e] < n_tmpl, EXM_ERR_ARG, "job %lld: template %d outside [0, %d)",
@@ -488,21 +580,12 @@ int exm_plan_create(exm_handle hstore, int n_tmpl, int64_t njobs, const int32_t*
     }
     Plan* p = new Plan();
     p->kind = K_PLAN;
-    p->njobs = njobs;
     p->n_tmpl = n_tmpl;
-    p->h_m.assign(job_m, job_m + njobs);
-    p->h_tab.assign(job_tab, job_tab + njobs);
-    int rc = dev_alloc((void**)&p->d_m, 4 * (size_t)njobs);
-    if (rc == EXM_OK) rc = dev_alloc((void**)&p->d_tab, 4 * (size_t)njobs);
-    if (rc == EXM_OK) rc = dev_alloc((void**)&p->d_offb, 8 * (size_t)njobs);
+    int rc = jobset_build(njobs, job_m, job_tab, offb.data(), &p->js);
     if (rc != EXM_OK) {
         delete p;
         return rc;
     }
-    EXM_HIP(hipMemcpyAsync(p->d_m, job_m, 4 * (size_t)njobs, hipMemcpyHostToDevice, ctx().stream));
-    EXM_HIP(hipMemcpyAsync(p->d_tab, job_tab, 4 * (size_t)njobs, hipMemcpyHostToDevice, ctx().stream));
-    EXM_HIP(hipMemcpyAsync(p->d_offb, offb.data(), 8 * (size_t)njobs, hipMemcpyHostToDevice, ctx().stream));
-    EXM_HIP(hipStreamSynchronize(ctx().stream));
     *out = register_object(p);
     return EXM_OK;
 }
@@ -517,11 +600,46 @@ int exm_plan_gather(exm_handle hplan, exm_handle htraj, exm_handle hstore, exm_h
     EXM_TRY(check_times(traj, htimes, &tm));
     EXM_CHECK(plan->n_tmpl == traj->n_tmpl, EXM_ERR_ARG, "plan is for %d templates, trajectories for %d",
               plan->n_tmpl, traj->n_tmpl);
+    const int64_t njobs = plan->js.njobs;
     double* dout = nullptr;
-    EXM_TRY(scratch(1, sizeof(double) * (size_t)plan->njobs, (void**)&dout));
-    EXM_TRY(launch_gather(traj, st, plan->d_m, plan->d_tab, plan->d_offb, plan->njobs, tm, dout));
-    EXM_HIP(hipMemcpyAsync(out, dout, sizeof(double) * plan->njobs, hipMemcpyDeviceToHost, ctx().stream));
+    EXM_TRY(scratch(1, sizeof(double) * (size_t)njobs, (void**)&dout));
+    EXM_TRY(launch_gather(traj, st, plan->js, tm, dout));
+    std::vector<double> sorted(njobs);
+    EXM_HIP(hipMemcpyAsync(sorted.data(), dout, sizeof(double) * njobs, hipMemcpyDeviceToHost, ctx().stream));
     EXM_HIP(hipStreamSynchronize(ctx().stream));
+    for (int64_t i = 0; i < njobs; ++i) out[plan->js.order[i]] = sorted[i];
+    return EXM_OK;
+}
+
+int exm_set_gather_kernel(int which) {
+    LOCK;
+    EXM_CHECK(which >= 0 && which <= 2, EXM_ERR_ARG, "gather kernel must be 0 (auto), 1 (reference) or 2 (band)");
+    g_gather_mode = which;
+    return EXM_OK;
+}
+
+int exm_profile_enable(int on) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    g_profile = on != 0;
+    return EXM_OK;
+}
+
+int exm_profile_reset(void) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    profile_collect();
+    g_prof.clear();
+    return EXM_OK;
+}
+
+int exm_profile_get(const char* name, double* total_ms, int64_t* count) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    profile_collect();
+    auto it = g_prof.find(name ? name : "");
+    *total_ms = it == g_prof.end() ? 0.0 : it->second.done_ms;
+    *count = it == g_prof.end() ? 0 : it->second.count;
     return EXM_OK;
 }
 

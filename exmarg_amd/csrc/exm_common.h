@@ -100,21 +100,38 @@ struct Traj : Object {
 // Arrival times walked per template (sorted) and their log weights.
 struct Times : Object {
     int n_tmpl = 0, n_walk = 0;
+    float lw_rand = 0.0f;   // log weight of every drawn time (the loud times carry 0)
     int32_t* d_sel = nullptr;
     float* d_lw = nullptr;
     ~Times();
 };
 
-// A flat list of (template, table, ln a) reads.
-struct Plan : Object {
+// A list of (template, table, ln a) reads grouped by table: jobs sorted by table, tiles of at most
+// TILE_JOBS jobs of one table, and for each sorted job its index in the caller's order.
+constexpr int TILE_JOBS = 1024;
+struct JobSet {
     int64_t njobs = 0;
+    int ntiles = 0;
+    int32_t* d_m = nullptr;       // (njobs) template row, table-sorted
+    int32_t* d_tab = nullptr;     // (njobs) table index
+    double* d_offb = nullptr;     // (njobs) (ln a - beta0) / dbeta
+    int32_t* d_tiles = nullptr;   // (ntiles, 2) first job, job count
+    std::vector<int64_t> order;   // sorted position -> caller index
+    void* d_block = nullptr;
+    size_t bytes = 0;
+};
+int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const double* offb, JobSet* js);
+void jobset_free(JobSet* js);
+
+// A reusable job set for exm_plan_gather.
+struct Plan : Object {
     int n_tmpl = 0;
-    int32_t* d_m = nullptr;
-    int32_t* d_tab = nullptr;
-    double* d_offb = nullptr;
-    std::vector<int32_t> h_m, h_tab;
+    JobSet js;
     ~Plan();
 };
+
+// gather kernel choice: 0 automatic, 1 reference-order kernel, 2 band kernel
+int gather_mode();
 
 // ---------------------------------------------------------------- context
 struct Context {
@@ -122,8 +139,9 @@ struct Context {
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
     // grow-only scratch buffers
-    void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};
-    size_t scratch_size[4] = {0, 0, 0, 0};
+    static constexpr int NSCRATCH = 8;
+    void* scratch[NSCRATCH] = {};
+    size_t scratch_size[NSCRATCH] = {};
     std::unordered_map<uint64_t, hipfftHandle> fft_plans;  // key: (n_time << 20) | batch
     int num_cu = 0;
     int64_t total_mem = 0;
@@ -132,6 +150,14 @@ struct Context {
 
 Context& ctx();
 int ensure_init();
+
+// kernel timing: events around named stages on the library stream (enabled by exm_profile_enable)
+struct ProfileScope {
+    hipEvent_t start = nullptr;
+    const char* name;
+    explicit ProfileScope(const char* n);
+    ~ProfileScope();
+};
 int scratch(int slot, size_t bytes, void** out);
 
 // handle registry
@@ -146,8 +172,10 @@ T* get(exm_handle h, uint32_t kind) {
 int launch_filter(Bank* bank, const double* dH, const double* dL, const double* s2H, const double* s2L, Traj* traj);
 int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_t* pick, float lw_rand,
                   Times* times);
-int launch_gather(Traj* traj, Store* store, const int32_t* d_m, const int32_t* d_tab, const double* d_offb,
-                  int64_t njobs, Times* times, double* d_out);
+// ln Lambda of every job of js into d_out (table-sorted order)
+int launch_gather(Traj* traj, Store* store, const JobSet& js, Times* times, double* d_out);
+int launch_gather_ref(Traj* traj, Store* store, const JobSet& js, Times* times, double* om, double* os);
+int launch_gather_band(Traj* traj, Store* store, const JobSet& js, Times* times, double* om, double* os);
 int launch_fill_synthetic(Store* st, int k, uint64_t seed, int64_t node, int kind);
 int launch_store_transpose_in(Store* st, int k, const float* d_src);
 int launch_store_transpose_out(Store* st, int k, float* d_dst);

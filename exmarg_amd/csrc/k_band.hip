@@ -1,0 +1,372 @@
+// The table read for a selection of walked times (src/exmarg/search/gather.py:32-100 for
+// times = select_times(...)), organised around the beta rows of the tables.
+//
+// For every template, beta = ln a + 2 ln|z_L(t)| of a walked time t is increasing in ln|z_L(t)| at
+// every node and lag.  So, once per segment:
+//   k_walk_sort    sorts each template's walked times by ln|z_L|;
+//   k_walk_stream  writes, for every (template, lag) and walked time in that order, the phase bin and
+//                  fraction of phi_H(t + tau) - phi_L(t), a drawn-time flag and ln|z_H(t + tau)|:
+//                  8 bytes per lookup, read contiguously by the gather;
+//   k_read_bounds  finds, for every read (template, table, ln a), where each beta row starts in the
+//                  sorted walk.
+// k_gather_piece then runs one workgroup per (table, lag, tile of at most 1024 reads of that table).
+// It sweeps the beta rows in increasing order with a ring of three rows (every phase bin and Delta
+// column, 3 x 43 KB at the reference geometry) in LDS.  At row b the lookups of the tile whose beta
+// row is b are cut into pieces of PIECE consecutive walked times of one read; the pieces are dealt
+// evenly to the lanes, each lane sums its pieces and merges the (max, sum) pair of each piece into the
+// read's accumulator in LDS (64-bit compare-and-swap).  Every table value a lookup reads comes from LDS
+// and every table row is read from HBM once per workgroup.
+//
+// Arithmetic: index and fraction arithmetic and the interpolation in float64 (the MI355X runs float64
+// FMAs at the float32 rate), exp of the non-positive difference in float32, sums in float64 within a
+// piece; the accumulator keeps (max, sum) as float32 with the sum rescaled to the rounded max.
+#include <climits>
+#include <cmath>
+
+#include "exm_common.h"
+
+namespace exm {
+
+constexpr int WALK_THREADS = 1024;
+constexpr int BAND_THREADS = 1024;
+constexpr double FC_SCALE = 16777216.0;   // phase fraction in 24-bit fixed point
+constexpr uint32_t FC_MASK = 0xFFFFFFu;
+constexpr uint32_t DRAWN = 0x80000000u;   // walked time from the random draw (log weight lw_rand)
+constexpr int PIECE = 8;
+
+__device__ __forceinline__ uint32_t float_order_key(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ __launch_bounds__(WALK_THREADS) void k_walk_sort(const float* __restrict__ lqL, int64_t nt,
+                                                            const int32_t* __restrict__ sel,
+                                                            const float* __restrict__ lw, int nw, int nwp,
+                                                            float* __restrict__ wa, float* __restrict__ wl,
+                                                            int32_t* __restrict__ wt) {
+    extern __shared__ uint64_t skeys[];
+    const int m = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int32_t* s = sel + (int64_t)m * nw;
+    const float* q = lqL + (int64_t)m * nt;
+    for (int i = tid; i < nwp; i += WALK_THREADS)
+        skeys[i] = i < nw ? (((uint64_t)float_order_key(q[s[i]]) << 32) | (uint32_t)i) : ~0ull;
+    __syncthreads();
+    for (int size = 2; size <= nwp; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = tid; i < nwp; i += WALK_THREADS) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const uint64_t a = skeys[i], b = skeys[j];
+                    if ((a > b) == up) {
+                        skeys[i] = b;
+                        skeys[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int p = tid; p < nw; p += WALK_THREADS) {
+        const int i = (int)(uint32_t)skeys[p];
+        const int t = s[i];
+        wa[(int64_t)m * nw + p] = q[t];
+        wl[(int64_t)m * nw + p] = lw[(int64_t)m * nw + i];
+        wt[(int64_t)m * nw + p] = t;
+    }
+}
+
+// one block per (template, 64 walked times); 4 groups of lanes cover the lags
+__global__ __launch_bounds__(256) void k_walk_stream(const float* __restrict__ lqH, const float* __restrict__ phH,
+                                                     const float* __restrict__ phL, int64_t nt,
+                                                     const int32_t* __restrict__ tau, int ntau,
+                                                     const int32_t* __restrict__ wt, const float* __restrict__ wl,
+                                                     int nw, uint2* __restrict__ stream) {
+    const int m = blockIdx.x;
+    const int p = blockIdx.y * 64 + (threadIdx.x & 63);
+    if (p >= nw) return;
+    const int64_t base = (int64_t)m * nt;
+    const int64_t t = wt[(int64_t)m * nw + p];
+    const float pL = phL[base + t];
+    const uint32_t drawn = wl[(int64_t)m * nw + p] != 0.0f ? DRAWN : 0u;
+    const double rph = NPHI / 6.283185307179586;
+    for (int r = threadIdx.x >> 6; r < ntau; r += 4) {
+        const int64_t tau0 = ((int64_t)tau[r] % nt + nt) % nt;
+        int64_t i1 = t + tau0;
+        if (i1 >= nt) i1 -= nt;
+        const float dph = __fsub_rn(phH[base + i1], pL);
+        double x3 = (double)dph * rph;
+        if (x3 < 0.0) x3 += NPHI;
+        if (x3 >= NPHI) x3 -= NPHI;
+        int c0 = (int)x3;
+        c0 = c0 < 0 ? 0 : (c0 > NPHI - 1 ? NPHI - 1 : c0);
+        const double fc = x3 - c0;
+        uint32_t fq = (uint32_t)__double2uint_rn(fc * FC_SCALE);
+        fq = fq > FC_MASK ? FC_MASK : fq;
+        stream[((int64_t)m * ntau + r) * nw + p] = make_uint2(drawn | ((uint32_t)c0 << 24) | fq,
+                                                              __float_as_uint(lqH[base + i1]));
+    }
+}
+
+// beta row of one walked time of one read: the clamped coordinate xb, the unclamped xbr and the row
+__device__ __forceinline__ int beta_row(float aL, double offb, double rdb, int nb, double& xb, double& xbr) {
+    xbr = (double)aL * rdb + offb;
+    const double xbmax = nb - 1 - 1e-6;
+    xb = xbr < 0.0 ? 0.0 : (xbr > xbmax ? xbmax : xbr);
+    const int b0 = (int)xb;
+    return b0 < 0 ? 0 : (b0 > nb - 2 ? nb - 2 : b0);   // NaN inputs stay inside the table
+}
+
+// per read: first and last beta row of its walk and the start of every row in the sorted walk,
+// bounds[j * (nb + 1) + (b - first)] for b in [first, last + 1]
+__global__ void k_read_bounds(const float* __restrict__ wa, int nw, const int32_t* __restrict__ jm,
+                              const double* __restrict__ joffb, int64_t njobs, double rdb, int nb,
+                              int32_t* __restrict__ first_last, uint16_t* __restrict__ bounds) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    const float* a = wa + (int64_t)jm[j] * nw;
+    const double offb = joffb[j];
+    double xb, xbr;
+    const int first = beta_row(a[0], offb, rdb, nb, xb, xbr);
+    int last = beta_row(a[nw - 1], offb, rdb, nb, xb, xbr);
+    last = last < first ? first : last;                 // only NaN input breaks the order
+    first_last[2 * j] = first;
+    first_last[2 * j + 1] = last;
+    uint16_t* bj = bounds + j * (int64_t)(nb + 1);
+    int row = first;
+    bj[0] = 0;
+    for (int p = 1; p < nw; ++p) {
+        const int b = beta_row(a[p], offb, rdb, nb, xb, xbr);
+        while (row < b) bj[++row - first] = (uint16_t)p;
+    }
+    bj[last + 1 - first] = (uint16_t)nw;
+}
+
+__device__ __forceinline__ uint64_t pack_pair(float m, float s) {
+    return ((uint64_t)__float_as_uint(s) << 32) | (uint64_t)__float_as_uint(m);
+}
+
+// byte offset of the accumulators after the three table rows, 16-byte aligned
+__host__ __device__ constexpr size_t piece_acc_offset(int rowlen) {
+    return ((size_t)12 * rowlen + 15) & ~(size_t)15;
+}
+
+// one workgroup per (tile of reads of one table, lag)
+__global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
+    const uint2* __restrict__ stream, const float* __restrict__ wa, int nw, float lw_rand,
+    const float* __restrict__ T, int64_t tstride, int nphif, int nb, int nd, int ntau,
+    const int32_t* __restrict__ jm, const int32_t* __restrict__ jtab, const double* __restrict__ joffb,
+    const int32_t* __restrict__ tiles, const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds,
+    double rdb, double inv_rdb, double offd, double rdd, double* __restrict__ om, double* __restrict__ os) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int rowlen = nphif * nd;
+    float* rows = (float*)lds;                                              // 3 rows
+    // (m, s) per read of the tile; 64-bit LDS atomics need an 8-byte aligned address
+    uint64_t* acc = (uint64_t*)(lds + piece_acc_offset(rowlen));
+    int* qinc = (int*)(acc + BAND_THREADS);                                 // inclusive piece counts
+    int* wsum = qinc + BAND_THREADS;                                        // 16 wave sums
+    int* misc = wsum + 16;                                                  // band range
+    const int r = blockIdx.y;
+    const int j0 = tiles[2 * blockIdx.x], nj = tiles[2 * blockIdx.x + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const float* Tr = T + (int64_t)jtab[j0] * tstride + (int64_t)r * nb * rowlen;
+    const double xdmax = nd - 1 - 1e-6;
+
+    int my_first = INT_MAX, my_last = INT_MIN;
+    if (tid < nj) {
+        my_first = first_last[2 * (j0 + tid)];
+        my_last = first_last[2 * (j0 + tid) + 1];
+    }
+    acc[tid] = pack_pair(-3.0e38f, 0.0f);
+    if (tid == 0) {
+        misc[0] = INT_MAX;
+        misc[1] = INT_MIN;
+    }
+    __syncthreads();
+    if (tid < nj) {
+        atomicMin(&misc[0], my_first);
+        atomicMax(&misc[1], my_last);
+    }
+    __syncthreads();
+    const int b_start = misc[0], b_end = misc[1];
+    for (int q = 0; q < 3; ++q) {
+        const int b = b_start + q;
+        if (b > nb - 1 || b > b_end + 1) break;
+        float* dst = rows + (b % 3) * rowlen;
+        const float* src = Tr + (int64_t)b * rowlen;
+        for (int i = tid; i < rowlen; i += BAND_THREADS) dst[i] = src[i];
+    }
+    __syncthreads();
+
+    for (int b = b_start; b <= b_end; ++b) {
+        // pieces of every read in this row
+        int p_lo = 0, p_hi = 0;
+        if (tid < nj && b >= my_first && b <= my_last) {
+            const uint16_t* bj = bounds + (int64_t)(j0 + tid) * (nb + 1);
+            p_lo = bj[b - my_first];
+            p_hi = bj[b + 1 - my_first];
+        }
+        const int q = (p_hi - p_lo + PIECE - 1) / PIECE;
+        // inclusive block scan of q
+        int x = q;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wsum[wid] = x;
+        __syncthreads();
+        if (wid == 0) {
+            int sw = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) {
+                const int y = __shfl_up(sw, d, 64);
+                if (lane >= d) sw += y;
+            }
+            if (lane < 16) wsum[lane] = sw;
+        }
+        __syncthreads();
+        x += wid ? wsum[wid - 1] : 0;
+        qinc[tid] = x;
+        const int Q = wsum[15];
+        __syncthreads();
+
+        const int sb = b % 3;
+        const int sb1 = sb + 1 == 3 ? 0 : sb + 1;
+        const float* R0 = rows + sb * rowlen;
+        const float* R1 = rows + sb1 * rowlen;
+        for (int i = tid; i < Q; i += BAND_THREADS) {
+            // read of piece i: first jj with qinc[jj] > i
+            int lo = 0, hi = nj - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (qinc[mid] > i) hi = mid;
+                else lo = mid + 1;
+            }
+            const int jj = lo;
+            const int64_t job = (int64_t)j0 + jj;
+            const int before = jj ? qinc[jj - 1] : 0;
+            const uint16_t* bj = bounds + job * (nb + 1);
+            const int f = first_last[2 * job];
+            const int pa = min(nw, (int)bj[b - f] + (i - before) * PIECE);
+            const int pe = min(min(pa + PIECE, (int)bj[b + 1 - f]), nw);
+            const int m = jm[job];
+            const double offb = joffb[job];
+            const float* wam = wa + (int64_t)m * nw;
+            const uint2* sm = stream + ((int64_t)m * ntau + r) * nw;
+            double mx = -1e300, s = 0.0;
+            for (int p = pa; p < pe; ++p) {
+                double xb, xbr;
+                const float aL = wam[p];
+                beta_row(aL, offb, rdb, nb, xb, xbr);
+                const uint2 e = sm[p];
+                const uint32_t code = e.x;
+                const double aH = (double)__uint_as_float(e.y);
+                const int c0 = (int)((code >> 24) & 0x7F);
+                const double fc = (double)(code & FC_MASK) * (1.0 / FC_SCALE);
+                const int c1 = c0 + 1 == NPHI ? 0 : c0 + 1;
+                const int ci0 = c0 > NPHI / 2 ? NPHI - c0 : c0;
+                const int ci1 = c1 > NPHI / 2 ? NPHI - c1 : c1;
+                const double aLc = (double)aL + (xb - xbr) * inv_rdb;
+                double xd = (aH - aLc) * rdd + offd;
+                xd = xd < 0.0 ? 0.0 : (xd > xdmax ? xdmax : xd);
+                int d0 = (int)xd;
+                d0 = d0 < 0 ? 0 : (d0 > nd - 2 ? nd - 2 : d0);
+                const double fd = xd - d0, gd = 1.0 - fd;
+                const double fb = xb - b, gb = 1.0 - fb;
+                const int o0 = ci0 * nd + d0, o1 = ci1 * nd + d0;
+                const double v0 = gb * (gd * (double)R0[o0] + fd * (double)R0[o0 + 1]) +
+                                  fb * (gd * (double)R1[o0] + fd * (double)R1[o0 + 1]);
+                const double v1 = gb * (gd * (double)R0[o1] + fd * (double)R0[o1 + 1]) +
+                                  fb * (gd * (double)R1[o1] + fd * (double)R1[o1 + 1]);
+                const double a = (1.0 - fc) * v0 + fc * v1 + ((code & DRAWN) ? (double)lw_rand : 0.0);
+                if (a > mx) {
+                    s = s * (double)__expf((float)(mx - a)) + 1.0;
+                    mx = a;
+                } else {
+                    s += (double)__expf((float)(a - mx));
+                }
+            }
+            // merge the piece into the read's accumulator: (max, sum) as float32, sum rescaled to the
+            // rounded max
+            uint64_t old = acc[jj], seen;
+            do {
+                seen = old;
+                const float mo = __uint_as_float((uint32_t)seen);
+                const float so = __uint_as_float((uint32_t)(seen >> 32));
+                const double M = mx > (double)mo ? mx : (double)mo;
+                const float Mf = (float)M;
+                const double snew = (double)so * (double)__expf((float)((double)mo - (double)Mf)) +
+                                    s * (double)__expf((float)(mx - (double)Mf));
+                old = atomicCAS((unsigned long long*)&acc[jj], (unsigned long long)seen,
+                                (unsigned long long)pack_pair(Mf, (float)snew));
+            } while (old != seen);
+        }
+        __syncthreads();
+        const int nrow = b + 3;
+        if (nrow <= b_end + 1 && nrow <= nb - 1) {
+            float* dst = rows + sb * rowlen;
+            const float* src = Tr + (int64_t)nrow * rowlen;
+            for (int i = tid; i < rowlen; i += BAND_THREADS) dst[i] = src[i];
+        }
+        __syncthreads();
+    }
+    if (tid < nj) {
+        const uint64_t v = acc[tid];
+        om[(int64_t)(j0 + tid) * ntau + r] = (double)__uint_as_float((uint32_t)v);
+        os[(int64_t)(j0 + tid) * ntau + r] = (double)__uint_as_float((uint32_t)(v >> 32));
+    }
+}
+
+int launch_gather_band(Traj* traj, Store* st, const JobSet& js, Times* times, double* om, double* os) {
+    Context& c = ctx();
+    const int nm = traj->n_tmpl, nw = times->n_walk;
+    EXM_CHECK(nw <= 65535, EXM_ERR_ARG, "walk of %d times exceeds the 16-bit row bounds", nw);
+    int nwp = 1;
+    while (nwp < nw) nwp <<= 1;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t walk_bytes = al((size_t)nm * nw * 4);
+    const size_t fl_bytes = al(sizeof(int32_t) * 2 * (size_t)js.njobs);
+    const size_t bnd_bytes = al(sizeof(uint16_t) * (size_t)js.njobs * (st->nb + 1));
+    void* wbuf = nullptr;
+    EXM_TRY(scratch(5, 3 * walk_bytes + fl_bytes + bnd_bytes, &wbuf));
+    float* wa = (float*)wbuf;
+    float* wl = (float*)((char*)wbuf + walk_bytes);
+    int32_t* wt = (int32_t*)((char*)wbuf + 2 * walk_bytes);
+    int32_t* first_last = (int32_t*)((char*)wbuf + 3 * walk_bytes);
+    uint16_t* bounds = (uint16_t*)((char*)wbuf + 3 * walk_bytes + fl_bytes);
+    uint2* stream = nullptr;
+    EXM_TRY(scratch(4, sizeof(uint2) * (size_t)nm * st->ntau * nw, (void**)&stream));
+    const double rdb = 2.0 / st->dbeta;
+    {
+        ProfileScope prof("walk");
+        const size_t lds = sizeof(uint64_t) * nwp;
+        EXM_HIP(hipFuncSetAttribute((const void*)k_walk_sort, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_walk_sort, dim3(nm), dim3(WALK_THREADS), lds, c.stream, traj->lqL(), traj->n_time,
+                           times->d_sel, times->d_lw, nw, nwp, wa, wl, wt);
+        EXM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_walk_stream, dim3(nm, (nw + 63) / 64), dim3(256), 0, c.stream, traj->lqH(), traj->phH(),
+                           traj->phL(), traj->n_time, st->d_tau, st->ntau, wt, wl, nw, stream);
+        EXM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_read_bounds, dim3((unsigned)((js.njobs + 255) / 256)), dim3(256), 0, c.stream, wa, nw,
+                           js.d_m, js.d_offb, js.njobs, rdb, st->nb, first_last, bounds);
+        EXM_HIP(hipGetLastError());
+    }
+    const size_t acc_off = piece_acc_offset(st->nphif * st->nd);
+    EXM_CHECK(acc_off % 8 == 0, EXM_ERR_ARG, "accumulator offset %zu is not 8-byte aligned", acc_off);
+    const size_t lds = acc_off + sizeof(uint64_t) * BAND_THREADS + sizeof(int) * (BAND_THREADS + 16 + 4);
+    EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "three table rows and the accumulators (%zu bytes) exceed the LDS", lds);
+    EXM_HIP(hipFuncSetAttribute((const void*)k_gather_piece, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    {
+        ProfileScope prof("gather");
+        hipLaunchKernelGGL(k_gather_piece, dim3(js.ntiles, st->ntau), dim3(BAND_THREADS), lds, c.stream, stream, wa,
+                           nw, times->lw_rand, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m,
+                           js.d_tab, js.d_offb, js.d_tiles, first_last, bounds, rdb, 1.0 / rdb,
+                           -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os);
+        EXM_HIP(hipGetLastError());
+    }
+    return EXM_OK;
+}
+
+}  // namespace exm

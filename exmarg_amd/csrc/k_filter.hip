@@ -85,6 +85,7 @@ int launch_filter(Bank* bank, const double* dH, const double* dL, const double* 
     EXM_HIP(hipMemcpyAsync(ddL, dL, sizeof(double2) * nf, hipMemcpyHostToDevice, c.stream));
     EXM_HIP(hipMemcpyAsync(ds2H, s2H, sizeof(double) * nf, hipMemcpyHostToDevice, c.stream));
     EXM_HIP(hipMemcpyAsync(ds2L, s2L, sizeof(double) * nf, hipMemcpyHostToDevice, c.stream));
+    ProfileScope prof("filter");
     for (int det = 0; det < 2; ++det) {
         const double2* d = det == 0 ? ddH : ddL;
         const double* s2 = det == 0 ? ds2H : ds2L;

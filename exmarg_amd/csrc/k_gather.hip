@@ -101,24 +101,37 @@ __global__ void k_lse_lags(const double* __restrict__ om, const double* __restri
     out[e] = mx + log(tot) - ln_n_time;
 }
 
-int launch_gather(Traj* traj, Store* st, const int32_t* d_m, const int32_t* d_tab, const double* d_offb,
-                  int64_t njobs, Times* times, double* d_out) {
+int launch_gather_ref(Traj* traj, Store* st, const JobSet& js, Times* times, double* om, double* os) {
     Context& c = ctx();
-    double* pairs = nullptr;
-    EXM_TRY(scratch(0, sizeof(double) * 2 * (size_t)njobs * st->ntau, (void**)&pairs));
-    double* om = pairs;
-    double* os = pairs + (size_t)njobs * st->ntau;
     const double rdb = 2.0 / st->dbeta;
     const double offd = -st->delta0 / st->ddelta;
     const double rdd = 1.0 / st->ddelta;
-    dim3 grid((unsigned)((njobs + 255) / 256), (unsigned)st->ntau);
+    dim3 grid((unsigned)((js.njobs + 255) / 256), (unsigned)st->ntau);
     hipLaunchKernelGGL(k_gather_ref, grid, dim3(256), 0, c.stream, traj->lqH(), traj->phH(), traj->lqL(),
                        traj->phL(), traj->n_time, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->d_tau,
-                       st->ntau, d_m, d_tab, d_offb, njobs, rdb, offd, rdd, times ? times->d_sel : nullptr,
-                       times ? times->d_lw : nullptr, times ? times->n_walk : 0, om, os);
+                       st->ntau, js.d_m, js.d_tab, js.d_offb, js.njobs, rdb, offd, rdd,
+                       times ? times->d_sel : nullptr, times ? times->d_lw : nullptr, times ? times->n_walk : 0, om,
+                       os);
     EXM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_lse_lags, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0, c.stream, om, os, njobs,
-                       st->ntau, std::log((double)traj->n_time), d_out);
+    return EXM_OK;
+}
+
+int launch_gather(Traj* traj, Store* st, const JobSet& js, Times* times, double* d_out) {
+    Context& c = ctx();
+    double* pairs = nullptr;
+    EXM_TRY(scratch(0, sizeof(double) * 2 * (size_t)js.njobs * st->ntau, (void**)&pairs));
+    double* om = pairs;
+    double* os = pairs + (size_t)js.njobs * st->ntau;
+    const int mode = gather_mode();
+    const bool band_ok = times != nullptr && times->n_walk <= 8192;
+    if (band_ok && mode != 1) {
+        EXM_TRY(launch_gather_band(traj, st, js, times, om, os));
+    } else {
+        ProfileScope prof("gather");
+        EXM_TRY(launch_gather_ref(traj, st, js, times, om, os));
+    }
+    hipLaunchKernelGGL(k_lse_lags, dim3((unsigned)((js.njobs + 255) / 256)), dim3(256), 0, c.stream, om, os,
+                       js.njobs, st->ntau, std::log((double)traj->n_time), d_out);
     EXM_HIP(hipGetLastError());
     return EXM_OK;
 }

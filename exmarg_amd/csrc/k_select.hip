@@ -217,6 +217,7 @@ int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_
     EXM_HIP(hipMemcpyAsync(dpick, pick, sizeof(int64_t) * (size_t)traj->n_tmpl * S, hipMemcpyHostToDevice,
                            c.stream));
     EXM_HIP(hipFuncSetAttribute((const void*)k_select, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    ProfileScope prof("select");
     for (int m0 = 0; m0 < traj->n_tmpl; m0 += chunk) {
         const int nb = std::min(chunk, traj->n_tmpl - m0);
         hipLaunchKernelGGL(k_select, dim3(nb), dim3(SEL_THREADS), lds, c.stream, traj->lqH(), traj->lqL(), nt,

@@ -160,16 +160,24 @@ class _Analysis:
 
     def __init__(self, bank, store, ymin, resident):
         check_device(bank.device)
-        self.bank, self.store, self.resident = bank, store, resident
+        self.bank, self.store, self.resident, self.ymin = bank, store, resident, float(ymin)
         self.geometry = _node_geometry(bank, store, ymin)
         self.was_resident = store.device is not None
+        self.dev = None
+
+    def start(self):
+        """upload the tables and build the read plan (after the host-side checks have passed)"""
+        bank, store = self.bank, self.store
         self.dev = _store.upload(store)
-        key = (id(store), float(ymin))
+        key = (id(store), self.ymin)
         cache = bank.__dict__.setdefault("_plans", {})
         if key not in cache:
             job_m, job_tab, job_lna, row = _jobs(self.geometry)
+            if job_m.size == 0:
+                raise ValueError("no template of the bank reaches a node of the store")
             cache[key] = (_gather.Plan(self.dev, bank.u.size, job_m, job_tab, job_lna), row)
         self.plan, self.row = cache[key]
+        return self
 
     def node_values(self, seg, j, n_loud, n_random, seed):
         """lamN (nn+1, nm) of segment j of seg (src/exmarg/search/pe.py:168-200)"""
@@ -188,7 +196,7 @@ class _Analysis:
         return lamN
 
     def close(self):
-        if not self.resident and not self.was_resident:
+        if self.dev is not None and not self.resident and not self.was_resident:
             _store.free(self.store)
 
 
@@ -212,8 +220,9 @@ def pe_intrinsic(segments, bank, store, rows=None, prior="volume", ymin=0.1267, 
     returns dict: M, q, chi, u (n_template,), ln_lambda (n_segment, n_template), ln_b (n_segment,), rows."""
     seg = read_segments(segments, rows)
     an = _Analysis(bank, store, ymin, resident)
+    lnW = _bank_weights(an.geometry[2], prior, ymin, bank)
+    an.start()
     try:
-        lnW = _bank_weights(an.geometry[2], prior, ymin, bank)
         out = np.full((seg["rows"].size, bank.u.size), -np.inf)
         for j in range(seg["rows"].size):
             v = lnW + an.node_values(seg, j, n_loud, n_random, seed)
@@ -266,7 +275,7 @@ def pe_redshift(segments, bank, store, rows=None, chunk=8, zmax=8.0, nz=1600, m1
     m1 = np.geomspace(m1_range[0], m1_range[1], int(nm1))
     dl = _dl_of_z(z)
     yq = D0 / dl
-    an = _Analysis(bank, store, ymin, resident)
+    an = _Analysis(bank, store, ymin, resident).start()
     out = np.full((seg["rows"].size, m1.size, z.size), -np.inf)
     try:
         for j in range(seg["rows"].size):

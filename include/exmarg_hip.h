@@ -121,6 +121,19 @@ int exm_plan_create(exm_handle store, int n_tmpl, int64_t njobs, const int32_t* 
  * the boundary reads of src/exmarg/search/pe.py:178-194 in one launch. */
 int exm_plan_gather(exm_handle plan, exm_handle traj, exm_handle store, exm_handle times, double* out);
 
+/* Choose the gather kernel: 0 automatic (the band kernel for a selection of at most 8192 walked
+ * times per template, the reference-order kernel otherwise), 1 the reference-order kernel, 2 the
+ * band kernel where it applies.  No reference counterpart (the reference has one kernel per device). */
+int exm_set_gather_kernel(int which);
+
+/* ---------------------------------------------------------------- measurement
+ * No reference counterpart: kernel timing with HIP events recorded on the library's stream around
+ * every launch of a named stage ("filter", "select", "gather"), for the benchmark. */
+int exm_profile_enable(int on);
+int exm_profile_reset(void);
+/* total milliseconds and launch count of stage `name` since the last reset (synchronizes). */
+int exm_profile_get(const char* name, double* total_ms, int64_t* count);
+
 #ifdef __cplusplus
 }
 #endif

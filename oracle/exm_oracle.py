@@ -117,14 +117,17 @@ def random_log_weight(n_time, K, S):
     return np.float32(np.log((n_time - K) / S))
 
 
-def select_times(traj, tau_idx, n_loud, n_random, seed):
-    """(sel (n_template, K+S) int32 sorted, lw float32) by the C oracle"""
+def select_times(traj, tau_idx, n_loud, n_random, seed, pick=None):
+    """(sel (n_template, K+S) int32 sorted, lw float32) by the C oracle; pick overrides the draw
+    (rows of the draw of a larger bank)"""
     K, S = int(n_loud), int(n_random)
     lqH = np.ascontiguousarray(traj["lqH"], np.float32)
     lqL = np.ascontiguousarray(traj["lqL"], np.float32)
     nm, nt = lqH.shape
     lo, hi = lag_range(tau_idx)
-    pick = np.ascontiguousarray(draw_picks(nm, nt, K, S, seed), np.int64)
+    if pick is None:
+        pick = draw_picks(nm, nt, K, S, seed)
+    pick = np.ascontiguousarray(pick, np.int64)
     sel = np.empty((nm, K + S), np.int32)
     lw = np.empty((nm, K + S), np.float32)
     lib().oracle_select(_p(lqH), _p(lqL), nm, nt, lo, hi, K, S, _p(pick), float(random_log_weight(nt, K, S)),
@@ -169,6 +172,15 @@ def select_times_np(traj, tau_idx, n_loud, n_random, seed):
 
 
 # ------------------------------------------------------------------ gather
+def synth_table(seed, node, shape, tau_idx, kind="smooth"):
+    """one synthetic table in host memory (reference layout), by the C recipe"""
+    out = np.empty(shape, np.float32)
+    tau = np.ascontiguousarray(tau_idx, np.int64)
+    lib().oracle_synth_fill(int(seed), int(node), 1 if kind == "rough" else 0, *[int(v) for v in shape],
+                            _p(tau), _p(out))
+    return out
+
+
 class Tables:
     """the tables a gather reads: host arrays (reference layout) or a synthetic recipe"""
 

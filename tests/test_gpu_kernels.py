@@ -135,11 +135,22 @@ def _tiny_store(kind, shape, n_nodes=2, seed=1):
     return st.synthetic_store(n_nodes=n_nodes, seed=seed, kind=kind, shape=shape, host=True)
 
 
+KERNELS = {1: 1e-12, 2: 1e-6}     # reference-order kernel: same operation order as the oracle; band kernel
+
+
+@pytest.fixture
+def kernel(hip, request):
+    hip.call("exm_set_gather_kernel", request.param)
+    yield request.param
+    hip.call("exm_set_gather_kernel", 0)
+
+
+@pytest.mark.parametrize("kernel", [1, 2], indirect=True)
 @pytest.mark.parametrize("kind", ["smooth", "rough"])
 @pytest.mark.parametrize("subsampled", [False, True])
-def test_gather_matches_oracle(hip, oracle, kind, subsampled):
+def test_gather_matches_oracle(hip, oracle, kernel, kind, subsampled):
     from exmarg_amd.search import gather as g
-    say("gather: kind=%s subsampled=%s" % (kind, subsampled))
+    say("gather: kernel=%d kind=%s subsampled=%s" % (kernel, kind, subsampled))
     shape = (9, 49, 40, 31)
     s = _tiny_store(kind, shape)
     nm, nt = 6, 2048
@@ -156,7 +167,29 @@ def test_gather_matches_oracle(hip, oracle, kind, subsampled):
     got = g.gather(traj, s, ids, lna, tab, times=times_dev)
     tabs = oracle.Tables(tables=s.tables, tau_idx=tau)
     ref = oracle.gather(traj, tabs, beta, delta, ids, lna, tab, times)
-    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=KERNELS[kernel] if subsampled else 1e-12)
+
+
+@pytest.mark.parametrize("kernel", [2], indirect=True)
+def test_band_gather_clamps_and_many_reads(hip, oracle, kernel):
+    # beta clamped at both table edges, Delta clamped, more reads of one table than one tile holds
+    from exmarg_amd.search import gather as g
+    say("gather: band kernel, clamped rows, 1500 reads of one table")
+    s = _tiny_store("rough", (5, 49, 24, 17), n_nodes=1, seed=4)
+    nm, nt = 3, 4096
+    traj = _rand_traj(nm, nt, seed=8)
+    traj["lqH"] *= 4.0
+    traj["lqL"] *= 4.0
+    times_dev = g.select_times(traj, s.tau_idx, 300, 700, seed=9)
+    times = times_dev.to_host()
+    rng = np.random.default_rng(1)
+    ids = rng.integers(0, nm, 1500)
+    lna = rng.uniform(-12.0, 12.0, (1500, 1))
+    tab = np.zeros((1500, 1), np.int64)
+    got = g.gather(traj, s, ids, lna, tab, times=times_dev)
+    ref = oracle.gather(traj, oracle.Tables(tables=s.tables, tau_idx=s.tau_idx), s.beta[0], s.delta[0], ids, lna,
+                        tab, times)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
 
 
 def test_gather_full_geometry_one_node(hip, oracle):

@@ -1,0 +1,102 @@
+"""End-to-end parity of pe_intrinsic / pe_redshift (device='hip') with the FP64 oracle on the same
+segments, banks and tables."""
+import numpy as np
+import pytest
+
+from exmarg_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+LNB_TOL = 1e-5          # |ln B - ln B_oracle| bar of the analysis
+LNL_TOL = 1e-5          # per-template ln Lambda
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def small_problem(n_templates=12, T=4.0, fmax=256.0, nodes=slice(None, None, 4), seed=11):
+    seg1, _, snr = synth.simulate_at_snr(1, seed, (30.0, 0.8, 0.1), 12.0, T, 20.0, fmax)
+    seg0, _, _ = synth.simulate(1, seed + 50, lambda r: (30.0, 0.8, 0.1), lambda r: 0.0, T, 20.0, fmax)
+    seg = dict(seg1)
+    seg["d_H"] = np.concatenate([seg1["d_H"], seg0["d_H"]])
+    seg["d_L"] = np.concatenate([seg1["d_L"], seg0["d_L"]])
+    bank = synth.chirp_mass_grid_bank(30.0, 0.8, 0.1, n_templates)
+    return seg, bank, synth.S_NODES[nodes]
+
+
+def oracle_inputs(bank, st, tabs_seed=0):
+    b = dict(W=bank.W, u=bank.u, M=bank.M, q=bank.q, chi=bank.chi, lo=bank.lo, n_time=bank.n_time)
+    store = dict(s=st.s, alpha_lo=st.alpha_lo, alpha_hi=st.alpha_hi, beta=st.beta[0], delta=st.delta[0],
+                 tau_idx=st.tau_idx)
+    return b, store
+
+
+@pytest.fixture(scope="module")
+def problem(hip):
+    from exmarg_amd.search import mass_bank, read_segments, store as stm
+    seg, bankrows, s_nodes = small_problem()
+    st = stm.synthetic_store(seed=0, s_nodes=s_nodes)
+    bank = mass_bank(bankrows, seg, "hip")
+    yield seg, bank, st
+    stm.free(st)
+
+
+@pytest.mark.parametrize("prior,n_loud", [("volume", 1024), ("volume", 0), ("madau-dickinson", 256)])
+def test_pe_intrinsic_matches_oracle(problem, oracle, prior, n_loud):
+    from exmarg_amd.search import pe_intrinsic, read_segments
+    seg, bank, st = problem
+    say("pe_intrinsic prior=%s n_loud=%d n_tmpl=%d nodes=%d" % (prior, n_loud, bank.u.size, st.s.size))
+    ymin = 0.1267 if prior == "volume" else 0.0982
+    res = pe_intrinsic(seg, bank, st, prior=prior, ymin=ymin, n_loud=n_loud, n_random=n_loud or 1024, seed=3,
+                       resident=True)
+    b, store = oracle_inputs(bank, st)
+    tabs = oracle.Tables(synth=dict(seed=0, kind="smooth", nodes=np.arange(st.s.size)), shape=st.table_shape(),
+                         tau_idx=st.tau_idx)
+    ref = oracle.pe_intrinsic(read_segments(seg), b, store, tabs, prior=prior, ymin=ymin, n_loud=n_loud,
+                              n_random=n_loud or 1024, seed=3)
+    say("ln_b hip", res["ln_b"], "oracle", ref["ln_b"])
+    np.testing.assert_allclose(res["ln_lambda"], ref["ln_lambda"], rtol=0, atol=LNL_TOL)
+    np.testing.assert_allclose(res["ln_b"], ref["ln_b"], rtol=0, atol=LNB_TOL)
+    assert res["ln_b"][0] > res["ln_b"][1]          # the injection outweighs noise
+
+
+def test_pe_redshift_matches_oracle(hip, oracle):
+    from exmarg_amd.search import mass_bank, pe_redshift, read_segments, store as stm
+    say("pe_redshift 10 templates, 80x90 grid")
+    seg, _, s_nodes = small_problem()
+    masses = np.geomspace(20.0, 60.0, 10)
+    bank = mass_bank(masses, seg, "hip")
+    st = stm.synthetic_store(seed=0, s_nodes=s_nodes)
+    kw = dict(zmax=3.0, nz=90, m1_range=(5.0, 40.0), nm1=80, mdet_range=(20.0, 60.0), seed=5)
+    res = pe_redshift(seg, bank, st, resident=False, **kw)
+    b, store = oracle_inputs(bank, st)
+    tabs = oracle.Tables(synth=dict(seed=0, kind="smooth", nodes=np.arange(st.s.size)), shape=st.table_shape(),
+                         tau_idx=st.tau_idx)
+    rseg = read_segments(seg)
+    geometry = oracle.node_geometry(store["s"], store["alpha_lo"], store["alpha_hi"], bank.u, 0.1267)
+    z = np.linspace(0.02, 3.0, 90)
+    m1 = np.geomspace(5.0, 40.0, 80)
+    yq = oracle.D0 / oracle.dl_of_z(z)
+    w = oracle.pe_redshift_weights(m1, z, (20.0, 60.0))
+    for e in range(2):
+        row = dict(d_H=rseg["d_H"][e], d_L=rseg["d_L"][e], sig2_H=rseg["sig2_H"], sig2_L=rseg["sig2_L"], row=e)
+        _, _, lamN = oracle.analyze_segment(row, b, store, tabs, geometry, 1024, 1024, 5)
+        L = oracle.to_source_frame(lamN, geometry[2], bank.M, yq, m1, z)
+        fin = np.isfinite(L)
+        np.testing.assert_array_equal(np.isfinite(res["ln_lambda"][e]), fin)
+        np.testing.assert_allclose(res["ln_lambda"][e][fin], L[fin], rtol=0, atol=LNL_TOL)
+        assert abs(res["ln_b"][e] - oracle.log_mean(L, w)) < LNB_TOL
+    # a store generated on the device has no host copy to upload again: resident=False keeps it
+    assert st.device is not None
+    stm.free(st)
+
+
+def test_pe_intrinsic_rejects_templates_outside_the_store(hip):
+    from exmarg_amd.search import mass_bank, pe_intrinsic, store as stm
+    seg, bankrows, _ = small_problem(n_templates=3)
+    st = stm.synthetic_store(seed=0, s_nodes=synth.S_NODES[:2])
+    bank = mass_bank(bankrows, seg, "hip")
+    with pytest.raises(ValueError, match="fewer than two"):
+        pe_intrinsic(seg, bank, st)
+    stm.free(st)
