@@ -4,9 +4,10 @@
 // For every template, beta = ln a + 2 ln|z_L(t)| of a walked time t is increasing in ln|z_L(t)| at
 // every node and lag.  So, once per segment:
 //   k_walk_sort    sorts each template's walked times by ln|z_L|;
-//   k_walk_stream  writes, for every (template, lag) and walked time in that order, the phase bin and
-//                  fraction of phi_H(t + tau) - phi_L(t), a drawn-time flag and ln|z_H(t + tau)|:
-//                  8 bytes per lookup, read contiguously by the gather;
+//   k_walk_stream  writes, for every (template, lag) and walked time in that order, the stored phase bin
+//                  of phi_H(t + tau) - phi_L(t) (the bin mirrored to [0, 48]), the side of its
+//                  neighbour, the fraction, a drawn-time flag and ln|z_H(t + tau)|: 8 bytes per lookup,
+//                  read contiguously by the gather;
 //   k_read_bounds  finds, for every read (template, table, ln a), where each beta row starts in the
 //                  sorted walk.
 // k_gather_piece then runs one workgroup per (table, lag, tile of at most 1024 reads of that table).
@@ -32,7 +33,11 @@ constexpr int BAND_THREADS = 1024;
 constexpr double FC_SCALE = 16777216.0;   // phase fraction in 24-bit fixed point
 constexpr uint32_t FC_MASK = 0xFFFFFFu;
 constexpr uint32_t DRAWN = 0x80000000u;   // walked time from the random draw (log weight lw_rand)
+constexpr uint32_t DOWN = 0x01000000u;    // the second phase bin is the stored bin below the first
+constexpr int BIN_SHIFT = 25;             // 6 bits of stored bin (0..48)
 constexpr int PIECE = 8;
+constexpr int OWNER_CAP = 4096;           // pieces dealt per round
+constexpr int PREFETCH = 11;              // floats of the next row per thread (rows up to 11264 floats)
 
 __device__ __forceinline__ uint32_t float_order_key(float f) {
     const uint32_t u = __float_as_uint(f);
@@ -104,7 +109,10 @@ __global__ __launch_bounds__(256) void k_walk_stream(const float* __restrict__ l
         const double fc = x3 - c0;
         uint32_t fq = (uint32_t)__double2uint_rn(fc * FC_SCALE);
         fq = fq > FC_MASK ? FC_MASK : fq;
-        stream[((int64_t)m * ntau + r) * nw + p] = make_uint2(drawn | ((uint32_t)c0 << 24) | fq,
+        // stored bins of c0 and c0 + 1 (mod 96) are adjacent: ci1 = ci0 + 1 below the mirror, ci0 - 1 from it on
+        const uint32_t ci0 = (uint32_t)(c0 > NPHI / 2 ? NPHI - c0 : c0);
+        const uint32_t down = c0 >= NPHI / 2 ? DOWN : 0u;
+        stream[((int64_t)m * ntau + r) * nw + p] = make_uint2(drawn | (ci0 << BIN_SHIFT) | down | fq,
                                                               __float_as_uint(lqH[base + i1]));
     }
 }
@@ -158,20 +166,23 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     const float* __restrict__ T, int64_t tstride, int nphif, int nb, int nd, int ntau,
     const int32_t* __restrict__ jm, const int32_t* __restrict__ jtab, const double* __restrict__ joffb,
     const int32_t* __restrict__ tiles, const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds,
-    double rdb, double inv_rdb, double offd, double rdd, double* __restrict__ om, double* __restrict__ os) {
+    double rdb, double offd, double rdd, double* __restrict__ om, double* __restrict__ os) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int rowlen = nphif * nd;
     float* rows = (float*)lds;                                              // 3 rows
     // (m, s) per read of the tile; 64-bit LDS atomics need an 8-byte aligned address
     uint64_t* acc = (uint64_t*)(lds + piece_acc_offset(rowlen));
-    int* qinc = (int*)(acc + BAND_THREADS);                                 // inclusive piece counts
-    int* wsum = qinc + BAND_THREADS;                                        // 16 wave sums
-    int* misc = wsum + 16;                                                  // band range
+    int* qex = (int*)(acc + BAND_THREADS);                                  // first piece of each read
+    uint16_t* plo = (uint16_t*)(qex + BAND_THREADS);                        // walk range of each read in row b
+    uint16_t* phi = plo + BAND_THREADS;
+    uint16_t* owner = phi + BAND_THREADS;                                   // read of each dealt piece
+    int* wsum = (int*)(owner + OWNER_CAP);                                  // 16 wave sums + band range
     const int r = blockIdx.y;
     const int j0 = tiles[2 * blockIdx.x], nj = tiles[2 * blockIdx.x + 1];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const float* Tr = T + (int64_t)jtab[j0] * tstride + (int64_t)r * nb * rowlen;
-    const double xdmax = nd - 1 - 1e-6;
+    const double xbmax = nb - 1 - 1e-6, xdmax = nd - 1 - 1e-6;
+    const double qbd = rdd / rdb;     // Delta coordinate per beta coordinate
 
     int my_first = INT_MAX, my_last = INT_MIN;
     if (tid < nj) {
@@ -180,16 +191,16 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     }
     acc[tid] = pack_pair(-3.0e38f, 0.0f);
     if (tid == 0) {
-        misc[0] = INT_MAX;
-        misc[1] = INT_MIN;
+        wsum[16] = INT_MAX;
+        wsum[17] = INT_MIN;
     }
     __syncthreads();
     if (tid < nj) {
-        atomicMin(&misc[0], my_first);
-        atomicMax(&misc[1], my_last);
+        atomicMin(&wsum[16], my_first);
+        atomicMax(&wsum[17], my_last);
     }
     __syncthreads();
-    const int b_start = misc[0], b_end = misc[1];
+    const int b_start = wsum[16], b_end = wsum[17];
     for (int q = 0; q < 3; ++q) {
         const int b = b_start + q;
         if (b > nb - 1 || b > b_end + 1) break;
@@ -200,15 +211,26 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     __syncthreads();
 
     for (int b = b_start; b <= b_end; ++b) {
-        // pieces of every read in this row
-        int p_lo = 0, p_hi = 0;
+        // next row: loads issued now, written to LDS after this row's lookups
+        const int nrow = b + 3;
+        const bool pf = nrow <= b_end + 1 && nrow <= nb - 1;
+        float pre[PREFETCH];
+        if (pf) {
+            const float* src = Tr + (int64_t)nrow * rowlen;
+#pragma unroll
+            for (int k = 0; k < PREFETCH; ++k) {
+                const int i = tid + k * BAND_THREADS;
+                pre[k] = i < rowlen ? src[i] : 0.0f;
+            }
+        }
+        // walk range and pieces of every read in this row
+        int lo_p = 0, hi_p = 0;
         if (tid < nj && b >= my_first && b <= my_last) {
             const uint16_t* bj = bounds + (int64_t)(j0 + tid) * (nb + 1);
-            p_lo = bj[b - my_first];
-            p_hi = bj[b + 1 - my_first];
+            lo_p = bj[b - my_first];
+            hi_p = bj[b + 1 - my_first];
         }
-        const int q = (p_hi - p_lo + PIECE - 1) / PIECE;
-        // inclusive block scan of q
+        const int q = (hi_p - lo_p + PIECE - 1) / PIECE;
         int x = q;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -227,88 +249,94 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
             if (lane < 16) wsum[lane] = sw;
         }
         __syncthreads();
-        x += wid ? wsum[wid - 1] : 0;
-        qinc[tid] = x;
+        const int my_qex = x - q + (wid ? wsum[wid - 1] : 0);
         const int Q = wsum[15];
-        __syncthreads();
+        qex[tid] = my_qex;
+        plo[tid] = (uint16_t)lo_p;
+        phi[tid] = (uint16_t)hi_p;
 
         const int sb = b % 3;
-        const int sb1 = sb + 1 == 3 ? 0 : sb + 1;
         const float* R0 = rows + sb * rowlen;
-        const float* R1 = rows + sb1 * rowlen;
-        for (int i = tid; i < Q; i += BAND_THREADS) {
-            // read of piece i: first jj with qinc[jj] > i
-            int lo = 0, hi = nj - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (qinc[mid] > i) hi = mid;
-                else lo = mid + 1;
-            }
-            const int jj = lo;
-            const int64_t job = (int64_t)j0 + jj;
-            const int before = jj ? qinc[jj - 1] : 0;
-            const uint16_t* bj = bounds + job * (nb + 1);
-            const int f = first_last[2 * job];
-            const int pa = min(nw, (int)bj[b - f] + (i - before) * PIECE);
-            const int pe = min(min(pa + PIECE, (int)bj[b + 1 - f]), nw);
-            const int m = jm[job];
-            const double offb = joffb[job];
-            const float* wam = wa + (int64_t)m * nw;
-            const uint2* sm = stream + ((int64_t)m * ntau + r) * nw;
-            double mx = -1e300, s = 0.0;
-            for (int p = pa; p < pe; ++p) {
-                double xb, xbr;
-                const float aL = wam[p];
-                beta_row(aL, offb, rdb, nb, xb, xbr);
-                const uint2 e = sm[p];
-                const uint32_t code = e.x;
-                const double aH = (double)__uint_as_float(e.y);
-                const int c0 = (int)((code >> 24) & 0x7F);
-                const double fc = (double)(code & FC_MASK) * (1.0 / FC_SCALE);
-                const int c1 = c0 + 1 == NPHI ? 0 : c0 + 1;
-                const int ci0 = c0 > NPHI / 2 ? NPHI - c0 : c0;
-                const int ci1 = c1 > NPHI / 2 ? NPHI - c1 : c1;
-                const double aLc = (double)aL + (xb - xbr) * inv_rdb;
-                double xd = (aH - aLc) * rdd + offd;
-                xd = xd < 0.0 ? 0.0 : (xd > xdmax ? xdmax : xd);
-                int d0 = (int)xd;
-                d0 = d0 < 0 ? 0 : (d0 > nd - 2 ? nd - 2 : d0);
-                const double fd = xd - d0, gd = 1.0 - fd;
-                const double fb = xb - b, gb = 1.0 - fb;
-                const int o0 = ci0 * nd + d0, o1 = ci1 * nd + d0;
-                const double v0 = gb * (gd * (double)R0[o0] + fd * (double)R0[o0 + 1]) +
-                                  fb * (gd * (double)R1[o0] + fd * (double)R1[o0 + 1]);
-                const double v1 = gb * (gd * (double)R0[o1] + fd * (double)R0[o1 + 1]) +
-                                  fb * (gd * (double)R1[o1] + fd * (double)R1[o1 + 1]);
-                const double a = (1.0 - fc) * v0 + fc * v1 + ((code & DRAWN) ? (double)lw_rand : 0.0);
-                if (a > mx) {
-                    s = s * (double)__expf((float)(mx - a)) + 1.0;
-                    mx = a;
-                } else {
-                    s += (double)__expf((float)(a - mx));
+        const float* R1 = rows + (sb + 1 == 3 ? 0 : sb + 1) * rowlen;
+        const double bd = (double)b;
+        for (int base = 0; base < Q; base += OWNER_CAP) {
+            const int lim = min(Q - base, OWNER_CAP);
+            for (int i = max(my_qex, base); i < min(my_qex + q, base + lim); ++i) owner[i - base] = (uint16_t)tid;
+            __syncthreads();
+            for (int i = tid; i < lim; i += BAND_THREADS) {
+                const int jj = owner[i];
+                const int64_t job = (int64_t)j0 + jj;
+                const int pa = (int)plo[jj] + (base + i - qex[jj]) * PIECE;
+                const int pe = min(pa + PIECE, (int)phi[jj]);
+                const int m = jm[job];
+                const double offb = joffb[job];
+                const double cj = offd + offb * qbd;
+                const float* wam = wa + (int64_t)m * nw;
+                const uint2* sm = stream + ((int64_t)m * ntau + r) * nw;
+                double mx = -1e300, s = 0.0;
+                // all records of the piece are requested before the first lookup
+                float aLv[PIECE];
+                uint2 ev[PIECE];
+#pragma unroll
+                for (int k = 0; k < PIECE; ++k) {
+                    if (pa + k < pe) {
+                        aLv[k] = wam[pa + k];
+                        ev[k] = sm[pa + k];
+                    }
                 }
+#pragma unroll
+                for (int k = 0; k < PIECE; ++k) {
+                    if (pa + k >= pe) break;
+                    const double aL = (double)aLv[k];
+                    const uint2 e = ev[k];
+                    const uint32_t code = e.x;
+                    // beta coordinate (clamped) and its fraction in row b
+                    const double xb = fmin(fmax(fma(aL, rdb, offb), 0.0), xbmax);
+                    const double fb = xb - bd;
+                    // Delta coordinate: ln q_L moves with the beta clamp, ln q_H stays
+                    const double xd = fmin(fmax(fma(-qbd, xb, fma((double)__uint_as_float(e.y), rdd, cj)), 0.0),
+                                           xdmax);
+                    int d0 = (int)xd;
+                    d0 = min(max(d0, 0), nd - 2);
+                    const double fd = xd - (double)d0;
+                    const double fc = (double)(code & FC_MASK) * (1.0 / FC_SCALE);
+                    const int o0 = (int)__umul24((code >> BIN_SHIFT) & 0x3F, (uint32_t)nd) + d0;
+                    const int o1 = (code & DOWN) ? o0 - nd : o0 + nd;
+                    const double t00 = R0[o0], t01 = R0[o0 + 1], t10 = R1[o0], t11 = R1[o0 + 1];
+                    const double u00 = R0[o1], u01 = R0[o1 + 1], u10 = R1[o1], u11 = R1[o1 + 1];
+                    const double v0a = fma(fd, t01 - t00, t00), v0b = fma(fd, t11 - t10, t10);
+                    const double v1a = fma(fd, u01 - u00, u00), v1b = fma(fd, u11 - u10, u10);
+                    const double v0 = fma(fb, v0b - v0a, v0a), v1 = fma(fb, v1b - v1a, v1a);
+                    const double a = fma(fc, v1 - v0, v0) + ((code & DRAWN) ? (double)lw_rand : 0.0);
+                    // running (max, sum): one exp of the non-positive difference
+                    const double dlt = a - mx;
+                    const double ex = (double)__expf(-(float)fabs(dlt));
+                    s = dlt > 0.0 ? fma(s, ex, 1.0) : s + ex;
+                    mx = fmax(mx, a);
+                }
+                // merge the piece into the read's accumulator: (max, sum) as float32, the sum rescaled to
+                // the rounded max
+                uint64_t old = acc[jj], seen;
+                do {
+                    seen = old;
+                    const float mo = __uint_as_float((uint32_t)seen);
+                    const float so = __uint_as_float((uint32_t)(seen >> 32));
+                    const float Mf = (float)fmax(mx, (double)mo);
+                    const double snew = (double)so * (double)__expf(mo - Mf) +
+                                        s * (double)__expf((float)(mx - (double)Mf));
+                    old = atomicCAS((unsigned long long*)&acc[jj], (unsigned long long)seen,
+                                    (unsigned long long)pack_pair(Mf, (float)snew));
+                } while (old != seen);
             }
-            // merge the piece into the read's accumulator: (max, sum) as float32, sum rescaled to the
-            // rounded max
-            uint64_t old = acc[jj], seen;
-            do {
-                seen = old;
-                const float mo = __uint_as_float((uint32_t)seen);
-                const float so = __uint_as_float((uint32_t)(seen >> 32));
-                const double M = mx > (double)mo ? mx : (double)mo;
-                const float Mf = (float)M;
-                const double snew = (double)so * (double)__expf((float)((double)mo - (double)Mf)) +
-                                    s * (double)__expf((float)(mx - (double)Mf));
-                old = atomicCAS((unsigned long long*)&acc[jj], (unsigned long long)seen,
-                                (unsigned long long)pack_pair(Mf, (float)snew));
-            } while (old != seen);
+            __syncthreads();
         }
-        __syncthreads();
-        const int nrow = b + 3;
-        if (nrow <= b_end + 1 && nrow <= nb - 1) {
+        if (pf) {
             float* dst = rows + sb * rowlen;
-            const float* src = Tr + (int64_t)nrow * rowlen;
-            for (int i = tid; i < rowlen; i += BAND_THREADS) dst[i] = src[i];
+#pragma unroll
+            for (int k = 0; k < PREFETCH; ++k) {
+                const int i = tid + k * BAND_THREADS;
+                if (i < rowlen) dst[i] = pre[k];
+            }
         }
         __syncthreads();
     }
@@ -355,15 +383,18 @@ int launch_gather_band(Traj* traj, Store* st, const JobSet& js, Times* times, do
     }
     const size_t acc_off = piece_acc_offset(st->nphif * st->nd);
     EXM_CHECK(acc_off % 8 == 0, EXM_ERR_ARG, "accumulator offset %zu is not 8-byte aligned", acc_off);
-    const size_t lds = acc_off + sizeof(uint64_t) * BAND_THREADS + sizeof(int) * (BAND_THREADS + 16 + 4);
+    EXM_CHECK(st->nphif * st->nd <= PREFETCH * BAND_THREADS, EXM_ERR_ARG, "table row of %d floats exceeds %d",
+              st->nphif * st->nd, PREFETCH * BAND_THREADS);
+    const size_t lds = acc_off + sizeof(uint64_t) * BAND_THREADS + sizeof(int) * BAND_THREADS +
+                       2 * sizeof(uint16_t) * BAND_THREADS + sizeof(uint16_t) * OWNER_CAP + sizeof(int) * 20;
     EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "three table rows and the accumulators (%zu bytes) exceed the LDS", lds);
     EXM_HIP(hipFuncSetAttribute((const void*)k_gather_piece, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     {
         ProfileScope prof("gather");
         hipLaunchKernelGGL(k_gather_piece, dim3(js.ntiles, st->ntau), dim3(BAND_THREADS), lds, c.stream, stream, wa,
                            nw, times->lw_rand, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m,
-                           js.d_tab, js.d_offb, js.d_tiles, first_last, bounds, rdb, 1.0 / rdb,
-                           -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os);
+                           js.d_tab, js.d_offb, js.d_tiles, first_last, bounds, rdb, -st->delta0 / st->ddelta,
+                           1.0 / st->ddelta, om, os);
         EXM_HIP(hipGetLastError());
     }
     return EXM_OK;
