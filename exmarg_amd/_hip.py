@@ -27,6 +27,7 @@ SIGNATURES = {
     "exm_device_info": [ctypes.c_char_p, SZ, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)],
     "exm_free": [H],
     "exm_mem_in_use": [ctypes.POINTER(ctypes.c_int64)],
+    "exm_mem_info": [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
     "exm_store_create": [I32, I32, I32, I32, I32, P, F64, F64, F64, F64, HP],
     "exm_store_set_table": [H, I32, P],
     "exm_store_fill_synthetic": [H, I32, U64, I64, I32],
@@ -40,6 +41,11 @@ SIGNATURES = {
     "exm_gather": [H, H, P, I64, P, P, I32, H, P],
     "exm_plan_create": [H, I32, I64, P, P, P, HP],
     "exm_plan_gather": [H, H, H, H, P],
+    "exm_batch_create": [H, I64, I32, HP],
+    "exm_batch_add": [H, H, H, P, P, P, P, I32, I32, I32, I32, P, F32, ctypes.POINTER(ctypes.c_int)],
+    "exm_batch_run": [H, P],
+    "exm_batch_reset": [H],
+    "exm_batch_info": [H, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)],
     "exm_set_gather_kernel": [I32],
     "exm_profile_enable": [I32],
     "exm_profile_reset": [],
@@ -114,6 +120,12 @@ def mem_in_use():
     v = ctypes.c_int64()
     call("exm_mem_in_use", ctypes.byref(v))
     return v.value
+
+
+def mem_info():
+    f, t = ctypes.c_int64(), ctypes.c_int64()
+    call("exm_mem_info", ctypes.byref(f), ctypes.byref(t))
+    return f.value, t.value
 
 
 def profile(on=True):

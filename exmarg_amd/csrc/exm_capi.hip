@@ -77,6 +77,7 @@ Times::~Times() {
     dev_free(d_lw, sizeof(float) * (size_t)n_tmpl * n_walk);
 }
 Plan::~Plan() { jobset_free(&js); }
+Batch::~Batch() { dev_free(d_stream, bytes); }
 
 int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const double* offb, JobSet* js) {
     js->njobs = njobs;
@@ -256,6 +257,16 @@ int exm_free(exm_handle h) {
     EXM_HIP(hipStreamSynchronize(ctx().stream));
     g_objects.erase(h);
     delete o;
+    return EXM_OK;
+}
+
+int exm_mem_info(int64_t* free_bytes, int64_t* total_bytes) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    size_t f = 0, t = 0;
+    EXM_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = (int64_t)f;
+    if (total_bytes) *total_bytes = (int64_t)t;
     return EXM_OK;
 }
 
@@ -581,6 +592,9 @@ int exm_plan_create(exm_handle hstore, int n_tmpl, int64_t njobs, const int32_t*
     Plan* p = new Plan();
     p->kind = K_PLAN;
     p->n_tmpl = n_tmpl;
+    p->h_m.assign(job_m, job_m + njobs);
+    p->h_tab.assign(job_tab, job_tab + njobs);
+    p->h_offb = offb;
     int rc = jobset_build(njobs, job_m, job_tab, offb.data(), &p->js);
     if (rc != EXM_OK) {
         delete p;
@@ -608,6 +622,151 @@ int exm_plan_gather(exm_handle hplan, exm_handle htraj, exm_handle hstore, exm_h
     EXM_HIP(hipMemcpyAsync(sorted.data(), dout, sizeof(double) * njobs, hipMemcpyDeviceToHost, ctx().stream));
     EXM_HIP(hipStreamSynchronize(ctx().stream));
     for (int64_t i = 0; i < njobs; ++i) out[plan->js.order[i]] = sorted[i];
+    return EXM_OK;
+}
+
+// ---------------------------------------------------------------- batches
+int exm_batch_create(exm_handle hstore, int64_t max_templates, int n_walk, exm_handle* out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Store, st, hstore, K_STORE);
+    EXM_CHECK(max_templates > 0 && n_walk > 0 && n_walk <= 8192, EXM_ERR_ARG,
+              "batch needs max_templates > 0 and 0 < n_walk <= 8192 (got %lld, %d)", (long long)max_templates, n_walk);
+    Batch* b = new Batch();
+    b->kind = K_BATCH;
+    b->store = hstore;
+    b->nw = n_walk;
+    b->ntau = st->ntau;
+    b->cap = max_templates;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t s_bytes = al(sizeof(uint2) * (size_t)max_templates * st->ntau * n_walk);
+    const size_t wa_bytes = al(sizeof(float) * (size_t)max_templates * n_walk);
+    const size_t lw_bytes = al(sizeof(float) * (size_t)max_templates);
+    const size_t nt_bytes = al(sizeof(double) * (size_t)max_templates);
+    b->bytes = s_bytes + wa_bytes + lw_bytes + nt_bytes;
+    int rc = dev_alloc((void**)&b->d_stream, b->bytes);
+    if (rc != EXM_OK) {
+        delete b;
+        return rc;
+    }
+    char* base = (char*)b->d_stream;
+    b->d_wa = (float*)(base + s_bytes);
+    b->d_lw = (float*)(base + s_bytes + wa_bytes);
+    b->d_lnnt = (double*)(base + s_bytes + wa_bytes + lw_bytes);
+    *out = register_object(b);
+    return EXM_OK;
+}
+
+int exm_batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, const double* dH, const double* dL,
+                  const double* s2H, const double* s2L, int lag_lo, int lag_hi, int K, int S, const int64_t* pick,
+                  float lw_rand, int* slot) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Batch, b, hbatch, K_BATCH);
+    NEED(Bank, bank, hbank, K_BANK);
+    NEED(Plan, plan, hplan, K_PLAN);
+    NEED(Store, st, b->store, K_STORE);
+    EXM_CHECK(dH && dL && s2H && s2L && pick, EXM_ERR_ARG, "NULL argument");
+    EXM_CHECK(K + S == b->nw, EXM_ERR_ARG, "n_loud + n_random = %d, the batch walks %d times", K + S, b->nw);
+    EXM_CHECK(plan->n_tmpl == bank->n_tmpl, EXM_ERR_ARG, "plan is for %d templates, bank has %d", plan->n_tmpl,
+              bank->n_tmpl);
+    EXM_CHECK(b->used + bank->n_tmpl <= b->cap, EXM_ERR_NOMEM, "batch full: %lld + %d templates exceed %lld",
+              (long long)b->used, bank->n_tmpl, (long long)b->cap);
+    EXM_CHECK(lag_lo <= lag_hi && (int64_t)(lag_hi - lag_lo) < bank->n_time, EXM_ERR_ARG, "bad lag range");
+    EXM_CHECK(K > 0 && K < bank->n_time && S > 0, EXM_ERR_ARG, "n_loud %d / n_random %d invalid for n_time %lld", K,
+              S, (long long)bank->n_time);
+    const int64_t nrest = bank->n_time - K;
+    for (int64_t i = 0; i < (int64_t)bank->n_tmpl * S; ++i)
+        EXM_CHECK(pick[i] >= 0 && pick[i] < nrest, EXM_ERR_ARG, "pick[%lld] outside [0, %lld)", (long long)i,
+                  (long long)nrest);
+    // transient trajectories and times in scratch slot 7
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t tr_bytes = al(sizeof(float) * 4 * (size_t)bank->n_tmpl * bank->n_time);
+    const size_t sel_bytes = al(sizeof(int32_t) * (size_t)bank->n_tmpl * b->nw);
+    void* tbuf = nullptr;
+    EXM_TRY(scratch(7, tr_bytes + 2 * sel_bytes, &tbuf));
+    Traj tr;
+    tr.n_tmpl = bank->n_tmpl;
+    tr.n_time = bank->n_time;
+    tr.d_buf = (float*)tbuf;
+    Times tm;
+    tm.n_tmpl = bank->n_tmpl;
+    tm.n_walk = b->nw;
+    tm.lw_rand = lw_rand;
+    tm.d_sel = (int32_t*)((char*)tbuf + tr_bytes);
+    tm.d_lw = (float*)((char*)tbuf + tr_bytes + sel_bytes);
+    int rc = launch_filter(bank, dH, dL, s2H, s2L, &tr);
+    if (rc == EXM_OK) rc = launch_select(&tr, lag_lo, lag_hi, K, S, pick, lw_rand, &tm);
+    if (rc == EXM_OK)
+        rc = launch_walk(&tr, &tm, st, b->d_wa + b->used * b->nw, b->d_stream + b->used * b->ntau * b->nw);
+    if (rc == EXM_OK) rc = launch_fill(b->d_lw + b->used, bank->n_tmpl, lw_rand);
+    if (rc == EXM_OK) rc = launch_fill(b->d_lnnt + b->used, bank->n_tmpl, std::log((double)bank->n_time));
+    // the host-side trajectory and times objects only borrow scratch memory
+    tr.d_buf = nullptr;
+    tm.d_sel = nullptr;
+    tm.d_lw = nullptr;
+    if (rc != EXM_OK) return rc;
+    EXM_HIP(hipStreamSynchronize(ctx().stream));
+    for (size_t e = 0; e < plan->h_m.size(); ++e) {
+        b->h_m.push_back((int32_t)(plan->h_m[e] + b->used));
+        b->h_tab.push_back(plan->h_tab[e]);
+        b->h_offb.push_back(plan->h_offb[e]);
+    }
+    b->seg_reads.push_back((int64_t)plan->h_m.size());
+    b->used += bank->n_tmpl;
+    if (slot) *slot = (int)b->seg_reads.size() - 1;
+    return EXM_OK;
+}
+
+int exm_batch_run(exm_handle hbatch, double* out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Batch, b, hbatch, K_BATCH);
+    NEED(Store, st, b->store, K_STORE);
+    EXM_CHECK(out != nullptr, EXM_ERR_ARG, "out is NULL");
+    const int64_t njobs = (int64_t)b->h_m.size();
+    if (njobs == 0) return EXM_OK;
+    JobSet js;
+    EXM_TRY(jobset_build(njobs, b->h_m.data(), b->h_tab.data(), b->h_offb.data(), &js));
+    double *pairs = nullptr, *dout = nullptr;
+    int rc = scratch(0, sizeof(double) * 2 * (size_t)njobs * st->ntau, (void**)&pairs);
+    if (rc == EXM_OK) rc = scratch(1, sizeof(double) * (size_t)njobs, (void**)&dout);
+    if (rc == EXM_OK)
+        rc = launch_piece(st, b->d_stream, b->d_wa, b->nw, b->d_lw, js, pairs, pairs + (size_t)njobs * st->ntau);
+    if (rc == EXM_OK)
+        rc = launch_lse(js, st->ntau, pairs, pairs + (size_t)njobs * st->ntau, b->d_lnnt, 0.0, dout);
+    std::vector<double> sorted(njobs);
+    if (rc == EXM_OK && (hipMemcpyAsync(sorted.data(), dout, sizeof(double) * njobs, hipMemcpyDeviceToHost,
+                                        ctx().stream) != hipSuccess ||
+                         hipStreamSynchronize(ctx().stream) != hipSuccess)) {
+        set_error("batch gather failed");
+        rc = EXM_ERR_HIP;
+    }
+    if (rc == EXM_OK)
+        for (int64_t i = 0; i < njobs; ++i) out[js.order[i]] = sorted[i];
+    jobset_free(&js);
+    return rc;
+}
+
+int exm_batch_reset(exm_handle hbatch) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Batch, b, hbatch, K_BATCH);
+    b->used = 0;
+    b->h_m.clear();
+    b->h_tab.clear();
+    b->h_offb.clear();
+    b->seg_reads.clear();
+    return EXM_OK;
+}
+
+int exm_batch_info(exm_handle hbatch, int64_t* templates_used, int64_t* reads, int* segments) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Batch, b, hbatch, K_BATCH);
+    if (templates_used) *templates_used = b->used;
+    if (reads) *reads = (int64_t)b->h_m.size();
+    if (segments) *segments = (int)b->seg_reads.size();
     return EXM_OK;
 }
 

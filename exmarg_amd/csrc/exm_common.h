@@ -123,11 +123,30 @@ struct JobSet {
 int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const double* offb, JobSet* js);
 void jobset_free(JobSet* js);
 
-// A reusable job set for exm_plan_gather.
+// A reusable job set for exm_plan_gather, with the reads in the caller's order kept on the host.
 struct Plan : Object {
     int n_tmpl = 0;
     JobSet js;
+    std::vector<int32_t> h_m, h_tab;
+    std::vector<double> h_offb;
     ~Plan();
+};
+
+// Segments analysed together: walk records of every admitted segment in device buffers, the reads of
+// every segment gathered by one launch.
+struct Batch : Object {
+    exm_handle store = 0;
+    int nw = 0, ntau = 0;
+    int64_t cap = 0, used = 0;          // templates (over all admitted segments)
+    uint2* d_stream = nullptr;           // (cap, ntau, nw)
+    float* d_wa = nullptr;               // (cap, nw)
+    float* d_lw = nullptr;               // (cap) drawn-time log weight per template
+    double* d_lnnt = nullptr;            // (cap) ln n_time per template
+    size_t bytes = 0;
+    std::vector<int32_t> h_m, h_tab;
+    std::vector<double> h_offb;
+    std::vector<int64_t> seg_reads;
+    ~Batch();
 };
 
 // gather kernel choice: 0 automatic, 1 reference-order kernel, 2 band kernel
@@ -176,6 +195,14 @@ int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_
 int launch_gather(Traj* traj, Store* store, const JobSet& js, Times* times, double* d_out);
 int launch_gather_ref(Traj* traj, Store* store, const JobSet& js, Times* times, double* om, double* os);
 int launch_gather_band(Traj* traj, Store* store, const JobSet& js, Times* times, double* om, double* os);
+int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* stream_dst);
+int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const float* lw_g, const JobSet& js,
+                 double* om, double* os);
+// LSE over the lags of every read; ln n_time per template (ln_nt_g) or one value for all (ln_nt)
+int launch_lse(const JobSet& js, int ntau, const double* om, const double* os, const double* ln_nt_g, double ln_nt,
+               double* d_out);
+int launch_fill(float* dst, int64_t n, float v);
+int launch_fill(double* dst, int64_t n, double v);
 int launch_fill_synthetic(Store* st, int k, uint64_t seed, int64_t node, int kind);
 int launch_store_transpose_in(Store* st, int k, const float* d_src);
 int launch_store_transpose_out(Store* st, int k, float* d_dst);

@@ -13,14 +13,14 @@
 // k_gather_piece then runs one workgroup per (table, lag, tile of at most 1024 reads of that table).
 // It sweeps the beta rows in increasing order with a ring of three rows (every phase bin and Delta
 // column, 3 x 43 KB at the reference geometry) in LDS.  At row b the lookups of the tile whose beta
-// row is b are cut into pieces of PIECE consecutive walked times of one read; the pieces are dealt
-// evenly to the lanes, each lane sums its pieces and merges the (max, sum) pair of each piece into the
-// read's accumulator in LDS (64-bit compare-and-swap).  Every table value a lookup reads comes from LDS
-// and every table row is read from HBM once per workgroup.
+// row is b are cut into pieces of PIECE consecutive walked times of one read; the pieces are dealt to
+// the lanes in rounds of one piece per lane, each lane writes its piece's (max, sum) to LDS, and the
+// lane that owns a read merges that read's pieces in piece order.  Every table value a lookup reads
+// comes from LDS, every table row is read from HBM once per workgroup, and a read's result does not
+// depend on which other reads share its workgroup (the merge order is fixed by the read alone).
 //
-// Arithmetic: index and fraction arithmetic and the interpolation in float64 (the MI355X runs float64
-// FMAs at the float32 rate), exp of the non-positive difference in float32, sums in float64 within a
-// piece; the accumulator keeps (max, sum) as float32 with the sum rescaled to the rounded max.
+// Arithmetic: index and fraction arithmetic, the interpolation and every sum in float64 (the MI355X
+// runs float64 FMAs at the float32 rate); exp of non-positive differences in float32.
 #include <climits>
 #include <cmath>
 
@@ -36,7 +36,6 @@ constexpr uint32_t DRAWN = 0x80000000u;   // walked time from the random draw (l
 constexpr uint32_t DOWN = 0x01000000u;    // the second phase bin is the stored bin below the first
 constexpr int BIN_SHIFT = 25;             // 6 bits of stored bin (0..48)
 constexpr int PIECE = 8;
-constexpr int OWNER_CAP = 4096;           // pieces dealt per round
 constexpr int PREFETCH = 11;              // floats of the next row per thread (rows up to 11264 floats)
 
 __device__ __forceinline__ uint32_t float_order_key(float f) {
@@ -151,18 +150,20 @@ __global__ void k_read_bounds(const float* __restrict__ wa, int nw, const int32_
     bj[last + 1 - first] = (uint16_t)nw;
 }
 
-__device__ __forceinline__ uint64_t pack_pair(float m, float s) {
-    return ((uint64_t)__float_as_uint(s) << 32) | (uint64_t)__float_as_uint(m);
+// byte offset of the piece results after the three table rows, 16-byte aligned
+__host__ __device__ constexpr size_t piece_res_offset(int rowlen) {
+    return ((size_t)12 * rowlen + 15) & ~(size_t)15;
 }
 
-// byte offset of the accumulators after the three table rows, 16-byte aligned
-__host__ __device__ constexpr size_t piece_acc_offset(int rowlen) {
-    return ((size_t)12 * rowlen + 15) & ~(size_t)15;
+// bytes of LDS the piece kernel uses for a row of rowlen floats
+__host__ __device__ constexpr size_t piece_lds_bytes(int rowlen) {
+    return piece_res_offset(rowlen) + 2 * sizeof(double) * BAND_THREADS + sizeof(int) * BAND_THREADS +
+           3 * sizeof(uint16_t) * BAND_THREADS + sizeof(int) * 20;
 }
 
 // one workgroup per (tile of reads of one table, lag)
 __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
-    const uint2* __restrict__ stream, const float* __restrict__ wa, int nw, float lw_rand,
+    const uint2* __restrict__ stream, const float* __restrict__ wa, int nw, const float* __restrict__ lw_g,
     const float* __restrict__ T, int64_t tstride, int nphif, int nb, int nd, int ntau,
     const int32_t* __restrict__ jm, const int32_t* __restrict__ jtab, const double* __restrict__ joffb,
     const int32_t* __restrict__ tiles, const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds,
@@ -170,13 +171,13 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     extern __shared__ __align__(16) unsigned char lds[];
     const int rowlen = nphif * nd;
     float* rows = (float*)lds;                                              // 3 rows
-    // (m, s) per read of the tile; 64-bit LDS atomics need an 8-byte aligned address
-    uint64_t* acc = (uint64_t*)(lds + piece_acc_offset(rowlen));
-    int* qex = (int*)(acc + BAND_THREADS);                                  // first piece of each read
+    double* pm = (double*)(lds + piece_res_offset(rowlen));                 // (max, sum) of each dealt piece
+    double* ps = pm + BAND_THREADS;
+    int* qex = (int*)(ps + BAND_THREADS);                                   // first piece of each read
     uint16_t* plo = (uint16_t*)(qex + BAND_THREADS);                        // walk range of each read in row b
     uint16_t* phi = plo + BAND_THREADS;
     uint16_t* owner = phi + BAND_THREADS;                                   // read of each dealt piece
-    int* wsum = (int*)(owner + OWNER_CAP);                                  // 16 wave sums + band range
+    int* wsum = (int*)(owner + BAND_THREADS);                               // 16 wave sums + band range
     const int r = blockIdx.y;
     const int j0 = tiles[2 * blockIdx.x], nj = tiles[2 * blockIdx.x + 1];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -189,7 +190,8 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
         my_first = first_last[2 * (j0 + tid)];
         my_last = first_last[2 * (j0 + tid) + 1];
     }
-    acc[tid] = pack_pair(-3.0e38f, 0.0f);
+    // running (max, sum) of this lane's read, merged piece by piece in piece order
+    double amx = -1e300, asum = 0.0;
     if (tid == 0) {
         wsum[16] = INT_MAX;
         wsum[17] = INT_MIN;
@@ -259,18 +261,21 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
         const float* R0 = rows + sb * rowlen;
         const float* R1 = rows + (sb + 1 == 3 ? 0 : sb + 1) * rowlen;
         const double bd = (double)b;
-        for (int base = 0; base < Q; base += OWNER_CAP) {
-            const int lim = min(Q - base, OWNER_CAP);
-            for (int i = max(my_qex, base); i < min(my_qex + q, base + lim); ++i) owner[i - base] = (uint16_t)tid;
+        // rounds of one piece per lane
+        for (int base = 0; base < Q; base += BAND_THREADS) {
+            const int lim = min(Q - base, BAND_THREADS);
+            const int my_lo = max(my_qex, base), my_hi = min(my_qex + q, base + lim);
+            for (int i = my_lo; i < my_hi; ++i) owner[i - base] = (uint16_t)tid;
             __syncthreads();
-            for (int i = tid; i < lim; i += BAND_THREADS) {
-                const int jj = owner[i];
+            if (tid < lim) {
+                const int jj = owner[tid];
                 const int64_t job = (int64_t)j0 + jj;
-                const int pa = (int)plo[jj] + (base + i - qex[jj]) * PIECE;
+                const int pa = (int)plo[jj] + (base + tid - qex[jj]) * PIECE;
                 const int pe = min(pa + PIECE, (int)phi[jj]);
                 const int m = jm[job];
                 const double offb = joffb[job];
                 const double cj = offd + offb * qbd;
+                const double lw_rand = (double)lw_g[m];
                 const float* wam = wa + (int64_t)m * nw;
                 const uint2* sm = stream + ((int64_t)m * ntau + r) * nw;
                 double mx = -1e300, s = 0.0;
@@ -307,26 +312,26 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
                     const double v0a = fma(fd, t01 - t00, t00), v0b = fma(fd, t11 - t10, t10);
                     const double v1a = fma(fd, u01 - u00, u00), v1b = fma(fd, u11 - u10, u10);
                     const double v0 = fma(fb, v0b - v0a, v0a), v1 = fma(fb, v1b - v1a, v1a);
-                    const double a = fma(fc, v1 - v0, v0) + ((code & DRAWN) ? (double)lw_rand : 0.0);
+                    const double a = fma(fc, v1 - v0, v0) + ((code & DRAWN) ? lw_rand : 0.0);
                     // running (max, sum): one exp of the non-positive difference
                     const double dlt = a - mx;
                     const double ex = (double)__expf(-(float)fabs(dlt));
                     s = dlt > 0.0 ? fma(s, ex, 1.0) : s + ex;
                     mx = fmax(mx, a);
                 }
-                // merge the piece into the read's accumulator: (max, sum) as float32, the sum rescaled to
-                // the rounded max
-                uint64_t old = acc[jj], seen;
-                do {
-                    seen = old;
-                    const float mo = __uint_as_float((uint32_t)seen);
-                    const float so = __uint_as_float((uint32_t)(seen >> 32));
-                    const float Mf = (float)fmax(mx, (double)mo);
-                    const double snew = (double)so * (double)__expf(mo - Mf) +
-                                        s * (double)__expf((float)(mx - (double)Mf));
-                    old = atomicCAS((unsigned long long*)&acc[jj], (unsigned long long)seen,
-                                    (unsigned long long)pack_pair(Mf, (float)snew));
-                } while (old != seen);
+                pm[tid] = mx;
+                ps[tid] = s;
+            }
+            __syncthreads();
+            // the read's lane merges its pieces of this round in piece order
+            for (int i = my_lo; i < my_hi; ++i) {
+                const double m2 = pm[i - base], s2 = ps[i - base];
+                if (m2 > amx) {
+                    asum = fma(asum, (double)__expf((float)(amx - m2)), s2);
+                    amx = m2;
+                } else {
+                    asum = fma(s2, (double)__expf((float)(m2 - amx)), asum);
+                }
             }
             __syncthreads();
         }
@@ -341,63 +346,104 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
         __syncthreads();
     }
     if (tid < nj) {
-        const uint64_t v = acc[tid];
-        om[(int64_t)(j0 + tid) * ntau + r] = (double)__uint_as_float((uint32_t)v);
-        os[(int64_t)(j0 + tid) * ntau + r] = (double)__uint_as_float((uint32_t)(v >> 32));
+        om[(int64_t)(j0 + tid) * ntau + r] = amx;
+        os[(int64_t)(j0 + tid) * ntau + r] = asum;
     }
 }
 
-int launch_gather_band(Traj* traj, Store* st, const JobSet& js, Times* times, double* om, double* os) {
+// walk records of one segment into wa_dst (n_tmpl, nw) and stream_dst (n_tmpl, ntau, nw)
+int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* stream_dst) {
     Context& c = ctx();
     const int nm = traj->n_tmpl, nw = times->n_walk;
     EXM_CHECK(nw <= 65535, EXM_ERR_ARG, "walk of %d times exceeds the 16-bit row bounds", nw);
     int nwp = 1;
     while (nwp < nw) nwp <<= 1;
+    const size_t walk_bytes = ((size_t)nm * nw * 4 + 255) & ~(size_t)255;
+    void* wbuf = nullptr;
+    EXM_TRY(scratch(5, 2 * walk_bytes, &wbuf));
+    float* wl = (float*)wbuf;
+    int32_t* wt = (int32_t*)((char*)wbuf + walk_bytes);
+    ProfileScope prof("walk");
+    const size_t lds = sizeof(uint64_t) * nwp;
+    EXM_HIP(hipFuncSetAttribute((const void*)k_walk_sort, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_walk_sort, dim3(nm), dim3(WALK_THREADS), lds, c.stream, traj->lqL(), traj->n_time,
+                       times->d_sel, times->d_lw, nw, nwp, wa_dst, wl, wt);
+    EXM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_walk_stream, dim3(nm, (nw + 63) / 64), dim3(256), 0, c.stream, traj->lqH(), traj->phH(),
+                       traj->phL(), traj->n_time, st->d_tau, st->ntau, wt, wl, nw, stream_dst);
+    EXM_HIP(hipGetLastError());
+    return EXM_OK;
+}
+
+// piece gather of every read of js; template ids index wa, stream and lw_g
+int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const float* lw_g, const JobSet& js,
+                 double* om, double* os) {
+    Context& c = ctx();
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t walk_bytes = al((size_t)nm * nw * 4);
     const size_t fl_bytes = al(sizeof(int32_t) * 2 * (size_t)js.njobs);
     const size_t bnd_bytes = al(sizeof(uint16_t) * (size_t)js.njobs * (st->nb + 1));
-    void* wbuf = nullptr;
-    EXM_TRY(scratch(5, 3 * walk_bytes + fl_bytes + bnd_bytes, &wbuf));
-    float* wa = (float*)wbuf;
-    float* wl = (float*)((char*)wbuf + walk_bytes);
-    int32_t* wt = (int32_t*)((char*)wbuf + 2 * walk_bytes);
-    int32_t* first_last = (int32_t*)((char*)wbuf + 3 * walk_bytes);
-    uint16_t* bounds = (uint16_t*)((char*)wbuf + 3 * walk_bytes + fl_bytes);
-    uint2* stream = nullptr;
-    EXM_TRY(scratch(4, sizeof(uint2) * (size_t)nm * st->ntau * nw, (void**)&stream));
+    void* bbuf = nullptr;
+    EXM_TRY(scratch(6, fl_bytes + bnd_bytes, &bbuf));
+    int32_t* first_last = (int32_t*)bbuf;
+    uint16_t* bounds = (uint16_t*)((char*)bbuf + fl_bytes);
     const double rdb = 2.0 / st->dbeta;
     {
-        ProfileScope prof("walk");
-        const size_t lds = sizeof(uint64_t) * nwp;
-        EXM_HIP(hipFuncSetAttribute((const void*)k_walk_sort, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_walk_sort, dim3(nm), dim3(WALK_THREADS), lds, c.stream, traj->lqL(), traj->n_time,
-                           times->d_sel, times->d_lw, nw, nwp, wa, wl, wt);
-        EXM_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_walk_stream, dim3(nm, (nw + 63) / 64), dim3(256), 0, c.stream, traj->lqH(), traj->phH(),
-                           traj->phL(), traj->n_time, st->d_tau, st->ntau, wt, wl, nw, stream);
-        EXM_HIP(hipGetLastError());
+        ProfileScope prof("bounds");
         hipLaunchKernelGGL(k_read_bounds, dim3((unsigned)((js.njobs + 255) / 256)), dim3(256), 0, c.stream, wa, nw,
                            js.d_m, js.d_offb, js.njobs, rdb, st->nb, first_last, bounds);
         EXM_HIP(hipGetLastError());
     }
-    const size_t acc_off = piece_acc_offset(st->nphif * st->nd);
-    EXM_CHECK(acc_off % 8 == 0, EXM_ERR_ARG, "accumulator offset %zu is not 8-byte aligned", acc_off);
+    EXM_CHECK(piece_res_offset(st->nphif * st->nd) % 16 == 0, EXM_ERR_ARG, "piece results misaligned");
     EXM_CHECK(st->nphif * st->nd <= PREFETCH * BAND_THREADS, EXM_ERR_ARG, "table row of %d floats exceeds %d",
               st->nphif * st->nd, PREFETCH * BAND_THREADS);
-    const size_t lds = acc_off + sizeof(uint64_t) * BAND_THREADS + sizeof(int) * BAND_THREADS +
-                       2 * sizeof(uint16_t) * BAND_THREADS + sizeof(uint16_t) * OWNER_CAP + sizeof(int) * 20;
-    EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "three table rows and the accumulators (%zu bytes) exceed the LDS", lds);
+    const size_t lds = piece_lds_bytes(st->nphif * st->nd);
+    EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "three table rows and the piece results (%zu bytes) exceed the LDS", lds);
     EXM_HIP(hipFuncSetAttribute((const void*)k_gather_piece, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    {
-        ProfileScope prof("gather");
-        hipLaunchKernelGGL(k_gather_piece, dim3(js.ntiles, st->ntau), dim3(BAND_THREADS), lds, c.stream, stream, wa,
-                           nw, times->lw_rand, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m,
-                           js.d_tab, js.d_offb, js.d_tiles, first_last, bounds, rdb, -st->delta0 / st->ddelta,
-                           1.0 / st->ddelta, om, os);
-        EXM_HIP(hipGetLastError());
-    }
+    ProfileScope prof("gather");
+    hipLaunchKernelGGL(k_gather_piece, dim3(js.ntiles, st->ntau), dim3(BAND_THREADS), lds, c.stream, stream, wa, nw,
+                       lw_g, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab, js.d_offb,
+                       js.d_tiles, first_last, bounds, rdb, -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os);
+    EXM_HIP(hipGetLastError());
     return EXM_OK;
+}
+
+__global__ void k_fill_f32(float* __restrict__ dst, int64_t n, float v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+__global__ void k_fill_f64(double* __restrict__ dst, int64_t n, double v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+int launch_fill(float* dst, int64_t n, float v) {
+    if (n <= 0) return EXM_OK;
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx().stream, dst, n, v);
+    EXM_HIP(hipGetLastError());
+    return EXM_OK;
+}
+
+int launch_fill(double* dst, int64_t n, double v) {
+    if (n <= 0) return EXM_OK;
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx().stream, dst, n, v);
+    EXM_HIP(hipGetLastError());
+    return EXM_OK;
+}
+
+int launch_gather_band(Traj* traj, Store* st, const JobSet& js, Times* times, double* om, double* os) {
+    const int nm = traj->n_tmpl, nw = times->n_walk;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t wa_bytes = al((size_t)nm * nw * 4), lw_bytes = al((size_t)nm * 4);
+    const size_t st_bytes = sizeof(uint2) * (size_t)nm * st->ntau * nw;
+    void* buf = nullptr;
+    EXM_TRY(scratch(4, wa_bytes + lw_bytes + st_bytes, &buf));
+    float* wa = (float*)buf;
+    float* lw_g = (float*)((char*)buf + wa_bytes);
+    uint2* stream = (uint2*)((char*)buf + wa_bytes + lw_bytes);
+    EXM_TRY(launch_fill(lw_g, nm, times->lw_rand));
+    EXM_TRY(launch_walk(traj, times, st, wa, stream));
+    return launch_piece(st, stream, wa, nw, lw_g, js, om, os);
 }
 
 }  // namespace exm

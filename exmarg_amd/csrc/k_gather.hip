@@ -89,9 +89,11 @@ __global__ __launch_bounds__(256) void k_gather_ref(
 
 // ln Lambda[e] = max_r om + ln sum_r os e^{om - max} - ln n_time
 __global__ void k_lse_lags(const double* __restrict__ om, const double* __restrict__ os, int64_t njobs, int ntau,
-                           double ln_n_time, double* __restrict__ out) {
+                           const int32_t* __restrict__ jm, const double* __restrict__ ln_nt_g, double ln_nt,
+                           double* __restrict__ out) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= njobs) return;
+    const double ln_n_time = ln_nt_g ? ln_nt_g[jm[e]] : ln_nt;
     const double* a = om + e * ntau;
     const double* b = os + e * ntau;
     double mx = a[0];
@@ -117,7 +119,6 @@ int launch_gather_ref(Traj* traj, Store* st, const JobSet& js, Times* times, dou
 }
 
 int launch_gather(Traj* traj, Store* st, const JobSet& js, Times* times, double* d_out) {
-    Context& c = ctx();
     double* pairs = nullptr;
     EXM_TRY(scratch(0, sizeof(double) * 2 * (size_t)js.njobs * st->ntau, (void**)&pairs));
     double* om = pairs;
@@ -130,8 +131,13 @@ int launch_gather(Traj* traj, Store* st, const JobSet& js, Times* times, double*
         ProfileScope prof("gather");
         EXM_TRY(launch_gather_ref(traj, st, js, times, om, os));
     }
-    hipLaunchKernelGGL(k_lse_lags, dim3((unsigned)((js.njobs + 255) / 256)), dim3(256), 0, c.stream, om, os,
-                       js.njobs, st->ntau, std::log((double)traj->n_time), d_out);
+    return launch_lse(js, st->ntau, om, os, nullptr, std::log((double)traj->n_time), d_out);
+}
+
+int launch_lse(const JobSet& js, int ntau, const double* om, const double* os, const double* ln_nt_g, double ln_nt,
+               double* d_out) {
+    hipLaunchKernelGGL(k_lse_lags, dim3((unsigned)((js.njobs + 255) / 256)), dim3(256), 0, ctx().stream, om, os,
+                       js.njobs, ntau, js.d_m, ln_nt_g, ln_nt, d_out);
     EXM_HIP(hipGetLastError());
     return EXM_OK;
 }

@@ -1,0 +1,154 @@
+"""
+Batched analysis: many segments, possibly of different lengths and banks, read in one gather launch.
+
+The reference analyses segments chunk by chunk with one gather launch and one device-to-host copy per
+node group and segment (src/exmarg/search/pe.py:178-194).  Here every admitted segment is filtered and
+its arrival times selected and sorted on the device; its walk records (8 bytes per template, lag and
+walked time) stay in a batch buffer sized by the free HBM, and exm_batch_run gathers every
+(segment, template, node) of the batch at once.
+
+    res = pe_intrinsic_batch([(segments_16s, bank_16s), (segments_64s, bank_64s)], store)
+    res[0]["ln_b"], res[1]["ln_lambda"]
+"""
+
+import numpy as np
+
+from .. import _hip
+from . import pe as _pe
+from .gather import lag_range, random_draw
+from .segments import read_segments
+
+RECORD_BYTES = 8          # bytes of walk record per template, lag and walked time
+
+
+def template_bytes(ntau, n_walk):
+    """device bytes one template of one segment holds in a batch"""
+    return ntau * n_walk * RECORD_BYTES + n_walk * 4 + 12
+
+
+class SegmentBatch:
+    """walk records of up to max_templates templates over all admitted segments"""
+
+    def __init__(self, store, n_loud=1024, n_random=1024, max_templates=None, reserve_bytes=8 << 30,
+                 max_bytes=None):
+        self.store = store
+        self.dev = store.device
+        if self.dev is None:
+            raise ValueError("upload the store first (exmarg_amd.search.store.upload)")
+        self.K, self.S = int(n_loud), int(n_random)
+        if self.K <= 0 or self.S <= 0:
+            raise ValueError("batched analysis walks a selection: n_loud and n_random must be positive")
+        self.nw = self.K + self.S
+        self.ntau = int(np.asarray(store.tau_idx).size)
+        self.lag_lo, self.lag_hi = lag_range(store.tau_idx)
+        per = template_bytes(self.ntau, self.nw)
+        if max_templates is None:
+            free, _ = _hip.mem_info()
+            budget = free - reserve_bytes
+            if max_bytes is not None:
+                budget = min(budget, max_bytes)
+            max_templates = int(budget // per)
+        if max_templates <= 0:
+            raise MemoryError("no device memory for a batch (%d bytes per template)" % per)
+        self.cap = int(max_templates)
+        self.handle = _hip.new_handle("exm_batch_create", int(self.dev.handle), self.cap, self.nw, kind="batch")
+        self.used = 0
+        self.entries = []          # (analysis, reads) per admitted segment
+
+    def fits(self, bank):
+        return self.used + bank.u.size <= self.cap
+
+    def add(self, analysis, d, sig2, seed):
+        """admit one segment (d, sig2: dicts 'H', 'L' of (n_freq,) arrays) analysed with analysis.bank"""
+        bank = analysis.bank
+        nm, nt = bank.u.size, bank.n_time
+        if not 0 < self.K < nt:
+            raise ValueError("n_loud must lie in (0, n_time), got %d" % self.K)
+        pick = np.ascontiguousarray(random_draw(nm, nt, self.K, self.S, seed), np.int64)
+        lw = np.float32(np.log((nt - self.K) / self.S))
+        arrs = [_hip.contiguous(d["H"], np.complex128), _hip.contiguous(d["L"], np.complex128),
+                _hip.contiguous(sig2["H"], np.float64), _hip.contiguous(sig2["L"], np.float64)]
+        slot = _hip.ctypes.c_int()
+        _hip.call("exm_batch_add", int(self.handle), int(bank.handle), int(analysis.plan.handle),
+                  *[_hip.ptr(a) for a in arrs], self.lag_lo, self.lag_hi, self.K, self.S, _hip.ptr(pick), float(lw),
+                  _hip.ctypes.byref(slot))
+        self.used += nm
+        self.entries.append(analysis)
+        return slot.value
+
+    def run(self):
+        """lamN (n_node + 1, n_template) of every admitted segment, in admission order"""
+        nreads = sum(a.plan.job_m.size for a in self.entries)
+        out = np.empty(nreads, np.float64)
+        if nreads:
+            _hip.call("exm_batch_run", int(self.handle), _hip.ptr(out))
+        res, o = [], 0
+        for a in self.entries:
+            n = a.plan.job_m.size
+            nn = a.geometry[0].shape[0]
+            lamN = np.full((nn + 1, a.bank.u.size), -np.inf)
+            lamN[a.row, a.plan.job_m] = out[o:o + n]
+            res.append(lamN)
+            o += n
+        return res
+
+    def reset(self):
+        _hip.call("exm_batch_reset", int(self.handle))
+        self.used = 0
+        self.entries = []
+
+    def free(self):
+        self.handle.free()
+
+
+def pe_intrinsic_batch(items, store, prior="volume", ymin=0.1267, n_loud=1024, n_random=1024, seed=0,
+                       max_templates=None, max_bytes=None):
+    """pe_intrinsic over many segment sets in batches that share one gather launch per batch.
+
+    items  list of (segments, bank): anything read_segments takes and a mass_bank built on those
+           segments (device 'hip'); banks may differ in length, band and templates
+    other arguments as pe_intrinsic (src/exmarg/search/pe.py:289-346).
+
+    returns a list with one pe_intrinsic result dict per item."""
+    from . import store as _store
+    _store.upload(store)
+    prepared = []
+    for segments, bank in items:
+        seg = read_segments(segments)
+        an = _pe._Analysis(bank, store, ymin, True)
+        lnW = _pe._bank_weights(an.geometry[2], prior, ymin, bank)
+        an.start()
+        prepared.append((seg, bank, an, lnW))
+    batch = SegmentBatch(store, n_loud, n_random, max_templates=max_templates, max_bytes=max_bytes)
+    outs = [np.full((seg["rows"].size, bank.u.size), -np.inf) for seg, bank, _, _ in prepared]
+    pending = []
+
+    def flush():
+        for (i, j), lamN in zip(pending, batch.run()):
+            lnW = prepared[i][3]
+            v = lnW + lamN
+            mx = v.max(axis=0)
+            outs[i][j] = mx + np.log(np.exp(v - mx[None, :]).sum(axis=0))
+        pending.clear()
+        batch.reset()
+
+    try:
+        for i, (seg, bank, an, lnW) in enumerate(prepared):
+            if bank.u.size > batch.cap:
+                raise MemoryError("a bank of %d templates exceeds the batch capacity %d" % (bank.u.size, batch.cap))
+            for j in range(seg["rows"].size):
+                if not batch.fits(bank):
+                    flush()
+                batch.add(an, {I: seg["d_" + I][j] for I in "HL"}, {I: seg["sig2_" + I] for I in "HL"},
+                          seed + int(seg["rows"][j]))
+                pending.append((i, j))
+        if pending:
+            flush()
+    finally:
+        batch.free()
+    results = []
+    for (seg, bank, an, lnW), out in zip(prepared, outs):
+        w = _pe._cell_weights(bank)
+        results.append(dict(M=bank.M, q=bank.q, chi=bank.chi, u=bank.u, ln_lambda=out,
+                            ln_b=np.array([_pe._log_mean(L, w) for L in out]), rows=seg["rows"]))
+    return results

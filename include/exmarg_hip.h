@@ -51,6 +51,10 @@ int exm_free(exm_handle h);
 /* Bytes of device memory the library currently holds. */
 int exm_mem_in_use(int64_t* bytes);
 
+/* Free and total device memory (hipMemGetInfo); replaces the cupy memGetInfo of
+ * src/exmarg/search/pe.py:43 that sizes segment chunks. */
+int exm_mem_info(int64_t* free_bytes, int64_t* total_bytes);
+
 /* ---------------------------------------------------------------- tables */
 
 /* A device stack of ntab tables of shape (ntau, nphif, nb, nd) and their grids.
@@ -120,6 +124,31 @@ int exm_plan_create(exm_handle store, int n_tmpl, int64_t njobs, const int32_t* 
 /* ln Lambda of every job of a plan for one segment, out (njobs) float64: every node group and
  * the boundary reads of src/exmarg/search/pe.py:178-194 in one launch. */
 int exm_plan_gather(exm_handle plan, exm_handle traj, exm_handle store, exm_handle times, double* out);
+
+/* ---------------------------------------------------------------- batches
+ * Many segments analysed together (no reference counterpart: the reference loops over segments in
+ * src/exmarg/search/pe.py:186-194 with one launch and one device-to-host copy per node group and
+ * segment).  A batch holds the walk records (8 bytes per template, lag and walked time) of every
+ * admitted segment; exm_batch_run reads every (segment, template, node) in one gather launch. */
+
+/* A batch over store for at most max_templates templates in total, each walking n_walk times. */
+int exm_batch_create(exm_handle store, int64_t max_templates, int n_walk, exm_handle* out);
+
+/* Admit one segment: matched filter with bank, arrival-time selection (as exm_select_times), walk
+ * records into the batch; the reads of plan (built for this bank) join the batch.  *slot receives
+ * the segment's index in the batch.  EXM_ERR_NOMEM when the batch is full. */
+int exm_batch_add(exm_handle batch, exm_handle bank, exm_handle plan, const double* dH, const double* dL,
+                  const double* s2H, const double* s2L, int lag_lo, int lag_hi, int K, int S, const int64_t* pick,
+                  float lw_rand, int* slot);
+
+/* ln Lambda of every read of the batch: out holds, segment after segment, the reads of each segment's
+ * plan in plan order (float64). */
+int exm_batch_run(exm_handle batch, double* out);
+
+/* Empty the batch (keeps its memory). */
+int exm_batch_reset(exm_handle batch);
+
+int exm_batch_info(exm_handle batch, int64_t* templates_used, int64_t* reads, int* segments);
 
 /* Choose the gather kernel: 0 automatic (the band kernel for a selection of at most 8192 walked
  * times per template, the reference-order kernel otherwise), 1 the reference-order kernel, 2 the

@@ -100,3 +100,29 @@ def test_pe_intrinsic_rejects_templates_outside_the_store(hip):
     with pytest.raises(ValueError, match="fewer than two"):
         pe_intrinsic(seg, bank, st)
     stm.free(st)
+
+
+def test_batch_of_uneven_segments_matches_single_calls(hip, oracle):
+    """segments of 2 s and 4 s with their own banks in one batch (capacity forcing two gather launches)
+    give the same ln Lambda as pe_intrinsic one set at a time, and match the oracle"""
+    from exmarg_amd.search import mass_bank, pe_intrinsic, pe_intrinsic_batch, read_segments, store as stm
+    say("batch: 2 s and 4 s segments, two banks, capacity of 3 segments")
+    st = stm.synthetic_store(seed=0, s_nodes=synth.S_NODES[::4])
+    items = []
+    for T, seed in ((2.0, 31), (4.0, 32)):
+        seg1, _, _ = synth.simulate_at_snr(2, seed, (30.0, 0.8, 0.1), 14.0, T, 20.0, 256.0)
+        seg0, _, _ = synth.simulate(1, seed + 9, lambda r: (30.0, 0.8, 0.1), lambda r: 0.0, T, 20.0, 256.0)
+        seg = dict(seg1, d_H=np.concatenate([seg1["d_H"], seg0["d_H"]]), d_L=np.concatenate([seg1["d_L"], seg0["d_L"]]))
+        bank = mass_bank(synth.chirp_mass_grid_bank(30.0, 0.8, 0.1, 8), seg, "hip")
+        items.append((seg, bank))
+    res = pe_intrinsic_batch(items, st, n_loud=128, n_random=128, seed=4, max_templates=3 * 8)
+    for (seg, bank), r in zip(items, res):
+        single = pe_intrinsic(seg, bank, st, n_loud=128, n_random=128, seed=4, resident=True)
+        np.testing.assert_allclose(r["ln_lambda"], single["ln_lambda"], rtol=0, atol=1e-9)
+        b, store = oracle_inputs(bank, st)
+        tabs = oracle.Tables(synth=dict(seed=0, kind="smooth", nodes=np.arange(st.s.size)), shape=st.table_shape(),
+                             tau_idx=st.tau_idx)
+        ref = oracle.pe_intrinsic(read_segments(seg), b, store, tabs, n_loud=128, n_random=128, seed=4)
+        np.testing.assert_allclose(r["ln_lambda"], ref["ln_lambda"], rtol=0, atol=LNL_TOL)
+        np.testing.assert_allclose(r["ln_b"], ref["ln_b"], rtol=0, atol=LNB_TOL)
+    stm.free(st)
