@@ -41,6 +41,9 @@ def parse():
     p.add_argument("--check-templates", type=int, default=8, help="templates checked against the oracle")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle check")
     p.add_argument("--gather-kernel", type=int, default=0, help="0 automatic, 1 reference-order kernel, 2 band kernel")
+    p.add_argument("--piece", type=int, default=0, help="walked times per piece of the band kernel (0 default)")
+    p.add_argument("--precision", choices=["f64", "f32"], default="f64",
+                   help="table interpolation of the band kernel (index arithmetic and log-sum-exp stay float64)")
     return p.parse_args()
 
 
@@ -152,6 +155,8 @@ def main():
     seg, rows, snr = workload(args)
     _hip.init(0)
     _hip.call("exm_set_gather_kernel", args.gather_kernel)
+    _hip.call("exm_set_gather_piece", args.piece)
+    _hip.call("exm_set_gather_precision", 1 if args.precision == "f32" else 0)
     info = _hip.device_info()
     st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
     bank = mass_bank(rows, seg, "hip")
@@ -170,7 +175,16 @@ def main():
     f_ms, f_n = _hip.profile_get("filter")
     s_ms, s_n = _hip.profile_get("select")
     w_ms, w_n = _hip.profile_get("walk")
+    b_ms, b_n = _hip.profile_get("bounds")
     _hip.profile(False)
+    f64_check = None
+    if args.precision == "f32":
+        # the same segment through the float64 form: the size-independent check of the float32 form
+        _hip.call("exm_set_gather_precision", 0)
+        ref64 = pe_intrinsic(seg, bank, st, **kw)
+        _hip.call("exm_set_gather_precision", 1)
+        f64_check = {"max_abs_dln_lambda": float(np.nanmax(np.abs(res["ln_lambda"] - ref64["ln_lambda"]))),
+                     "dln_b": float(abs(res["ln_b"][0] - ref64["ln_b"][0]))}
 
     geometry = _node_geometry(bank, st, 0.1267)
     valid, bnode = geometry[0], geometry[3]
@@ -188,7 +202,8 @@ def main():
         "metric": "segments/sec at ln B err <=1e-5 vs FP64 CPU; p50/p99 latency; HBM GB/s",
         "value": value, "unit": "segments/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 tables, f64 gather arithmetic", "data": "synthetic",
+        "dtype": "f64" if args.precision == "f64" else "f32 interpolation, f64 index arithmetic and log-sum-exp",
+        "data": "synthetic",
         "config": {"workload": "one 16 s segment, 20-1024 Hz, 2000-template (M_det, q, chi) bank, 53 HBM-resident "
                                "table nodes, pe_intrinsic 'volume' prior, n_loud=n_random=1024",
                    "n_templates": nm, "n_time": nt, "n_freq": F, "nodes": n_nodes_used, "reads": n_jobs,
@@ -198,11 +213,13 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "gather",
                      "bytes_per_launch": seg_bytes, "kernel_ms": 1e3 * g_s},
         "stages_ms": {"filter": f_ms / max(f_n, 1), "select": s_ms / max(s_n, 1),
-                      "walk": w_ms / max(w_n, 1), "gather": g_ms / max(g_n, 1)},
+                      "walk": w_ms / max(w_n, 1), "bounds": b_ms / max(b_n, 1), "gather": g_ms / max(g_n, 1)},
         "lookups_per_s": lookups / g_s,
         "path_hbm_frac": seg_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS,
         "ln_b": float(res["ln_b"][0]),
     }
+    if f64_check is not None:
+        out["float32_form_vs_float64"] = f64_check
     if not args.no_cpu:
         an_geometry = geometry
         cpu, err, nchk = cpu_leg(args, seg, bank, st, res, an_geometry, 0)

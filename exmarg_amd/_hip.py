@@ -12,7 +12,8 @@ import threading
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_lib", "libexmarg_hip.so")
+# EXMARG_HIP_LIB names another build of the same library (for instance one compiled with other tuning)
+LIB_PATH = os.environ.get("EXMARG_HIP_LIB") or os.path.join(HERE, "_lib", "libexmarg_hip.so")
 
 EXM_OK, EXM_ERR_ARG, EXM_ERR_HIP, EXM_ERR_NOMEM, EXM_ERR_HANDLE, EXM_ERR_STATE = 0, -1, -2, -3, -4, -5
 
@@ -47,6 +48,8 @@ SIGNATURES = {
     "exm_batch_reset": [H],
     "exm_batch_info": [H, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)],
     "exm_set_gather_kernel": [I32],
+    "exm_set_gather_piece": [I32],
+    "exm_set_gather_precision": [I32],
     "exm_profile_enable": [I32],
     "exm_profile_reset": [],
     "exm_profile_get": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)],

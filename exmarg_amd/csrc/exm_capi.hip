@@ -67,7 +67,7 @@ int scratch(int slot, size_t bytes, void** out) {
 }
 
 Store::~Store() {
-    dev_free(d_T, sizeof(float) * (size_t)tstride * ntab);
+    dev_free(d_T, table_bytes());
     dev_free(d_tau, sizeof(int32_t) * ntau);
 }
 Bank::~Bank() { dev_free(d_W, sizeof(float2) * (size_t)n_tmpl * n_freq); }
@@ -308,7 +308,7 @@ int exm_store_create(int ntab, int ntau, int nphif, int nb, int nd, const int64_
         }
         t32[i] = (int32_t)tau_idx[i];
     }
-    int rc = dev_alloc((void**)&st->d_T, sizeof(float) * (size_t)st->tstride * ntab);
+    int rc = dev_alloc((void**)&st->d_T, st->table_bytes());
     if (rc == EXM_OK) rc = dev_alloc((void**)&st->d_tau, sizeof(int32_t) * ntau);
     if (rc != EXM_OK) {
         delete st;
@@ -774,6 +774,20 @@ int exm_set_gather_kernel(int which) {
     LOCK;
     EXM_CHECK(which >= 0 && which <= 2, EXM_ERR_ARG, "gather kernel must be 0 (auto), 1 (reference) or 2 (band)");
     g_gather_mode = which;
+    return EXM_OK;
+}
+
+int exm_set_gather_piece(int n) {
+    LOCK;
+    EXM_CHECK(n == 0 || n == 4 || n == 8 || n == 16, EXM_ERR_ARG, "piece length must be 0 (default), 4, 8 or 16");
+    g_ctx.piece = n;
+    return EXM_OK;
+}
+
+int exm_set_gather_precision(int p) {
+    LOCK;
+    EXM_CHECK(p == 0 || p == 1, EXM_ERR_ARG, "gather precision must be 0 (float64) or 1 (float32 interpolation)");
+    g_ctx.f32_interp = p;
     return EXM_OK;
 }
 

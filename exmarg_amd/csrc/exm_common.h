@@ -67,14 +67,17 @@ struct Object {
 };
 
 // Tables of every node in one device allocation, layout [table][lag][beta][phase][Delta]
-// (float32): one beta row of all phases is contiguous, nphif * nd floats.
+// (float32): one beta row of all phases is contiguous, nphif * nd floats.  The allocation ends with
+// TABLE_PAD_FLOATS spare floats so that row copies in whole 64-float blocks stay inside it.
+constexpr int TABLE_PAD_FLOATS = 256;
 struct Store : Object {
     int ntab = 0, ntau = 0, nphif = 0, nb = 0, nd = 0;
     int64_t tstride = 0;   // floats per table
     std::vector<int64_t> tau;
     double beta0 = 0, dbeta = 0, delta0 = 0, ddelta = 0;
-    float* d_T = nullptr;
+    float* d_T = nullptr;      // ntab * tstride floats + TABLE_PAD_FLOATS
     int32_t* d_tau = nullptr;  // lags (samples)
+    size_t table_bytes() const { return sizeof(float) * ((size_t)tstride * ntab + TABLE_PAD_FLOATS); }
     ~Store();
 };
 
@@ -162,6 +165,8 @@ struct Context {
     void* scratch[NSCRATCH] = {};
     size_t scratch_size[NSCRATCH] = {};
     std::unordered_map<uint64_t, hipfftHandle> fft_plans;  // key: (n_time << 20) | batch
+    int piece = 0;                                          // walked times per piece (0: default)
+    int f32_interp = 0;                                     // band kernel interpolates in float32
     int num_cu = 0;
     int64_t total_mem = 0;
     char name[256] = {0};

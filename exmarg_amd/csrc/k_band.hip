@@ -35,8 +35,7 @@ constexpr uint32_t FC_MASK = 0xFFFFFFu;
 constexpr uint32_t DRAWN = 0x80000000u;   // walked time from the random draw (log weight lw_rand)
 constexpr uint32_t DOWN = 0x01000000u;    // the second phase bin is the stored bin below the first
 constexpr int BIN_SHIFT = 25;             // 6 bits of stored bin (0..48)
-constexpr int PIECE = 8;
-constexpr int PREFETCH = 11;              // floats of the next row per thread (rows up to 11264 floats)
+constexpr int PIECE = 8;                  // default walked times per piece
 
 __device__ __forceinline__ uint32_t float_order_key(float f) {
     const uint32_t u = __float_as_uint(f);
@@ -150,9 +149,12 @@ __global__ void k_read_bounds(const float* __restrict__ wa, int nw, const int32_
     bj[last + 1 - first] = (uint16_t)nw;
 }
 
-// byte offset of the piece results after the three table rows, 16-byte aligned
+// floats of one ring slot: a table row rounded up to whole 64-float blocks (one block per wave copy)
+__host__ __device__ constexpr int row_slot(int rowlen) { return (rowlen + 63) & ~63; }
+
+// byte offset of the piece results after the three ring slots, 16-byte aligned
 __host__ __device__ constexpr size_t piece_res_offset(int rowlen) {
-    return ((size_t)12 * rowlen + 15) & ~(size_t)15;
+    return ((size_t)12 * row_slot(rowlen) + 15) & ~(size_t)15;
 }
 
 // bytes of LDS the piece kernel uses for a row of rowlen floats
@@ -161,7 +163,22 @@ __host__ __device__ constexpr size_t piece_lds_bytes(int rowlen) {
            3 * sizeof(uint16_t) * BAND_THREADS + sizeof(int) * 20;
 }
 
-// one workgroup per (tile of reads of one table, lag)
+// copy one table row (slot floats, whole 64-float blocks) from global memory straight into LDS:
+// wave w moves blocks w, w + 16, ...; each lane one float of a block, no register staging.  The
+// loads complete asynchronously; the caller waits (s_waitcnt) before the barrier that publishes them.
+__device__ __forceinline__ void row_to_lds(const float* __restrict__ src, float* dst, int slot, int wid, int lane) {
+    for (int k = wid * 64; k < slot; k += BAND_THREADS) {
+        __builtin_amdgcn_global_load_lds((const void*)(src + k + lane),
+                                         (__attribute__((address_space(3))) void*)(dst + k), 4, 0, 0);
+    }
+}
+
+__device__ __forceinline__ double vfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float vfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// one workgroup per (tile of reads of one table, lag); PN walked times per piece; V the type of the
+// table interpolation (double, or float for the float32 form)
+template <int PN, typename V>
 __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     const uint2* __restrict__ stream, const float* __restrict__ wa, int nw, const float* __restrict__ lw_g,
     const float* __restrict__ T, int64_t tstride, int nphif, int nb, int nd, int ntau,
@@ -169,8 +186,8 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     const int32_t* __restrict__ tiles, const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds,
     double rdb, double offd, double rdd, double* __restrict__ om, double* __restrict__ os) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int rowlen = nphif * nd;
-    float* rows = (float*)lds;                                              // 3 rows
+    const int rowlen = nphif * nd, slot = row_slot(rowlen);
+    float* rows = (float*)lds;                                              // 3 ring slots
     double* pm = (double*)(lds + piece_res_offset(rowlen));                 // (max, sum) of each dealt piece
     double* ps = pm + BAND_THREADS;
     int* qex = (int*)(ps + BAND_THREADS);                                   // first piece of each read
@@ -203,28 +220,17 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     }
     __syncthreads();
     const int b_start = wsum[16], b_end = wsum[17];
-    for (int q = 0; q < 3; ++q) {
-        const int b = b_start + q;
-        if (b > nb - 1 || b > b_end + 1) break;
-        float* dst = rows + (b % 3) * rowlen;
-        const float* src = Tr + (int64_t)b * rowlen;
-        for (int i = tid; i < rowlen; i += BAND_THREADS) dst[i] = src[i];
-    }
+    // rows b_start .. b_start + 2 (as far as the band and the table reach)
+    const int b_top = min(b_end + 1, nb - 1);
+    for (int q = 0; q < 3 && b_start + q <= b_top; ++q)
+        row_to_lds(Tr + (int64_t)(b_start + q) * rowlen, rows + ((b_start + q) % 3) * slot, slot, wid, lane);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
     for (int b = b_start; b <= b_end; ++b) {
-        // next row: loads issued now, written to LDS after this row's lookups
-        const int nrow = b + 3;
-        const bool pf = nrow <= b_end + 1 && nrow <= nb - 1;
-        float pre[PREFETCH];
-        if (pf) {
-            const float* src = Tr + (int64_t)nrow * rowlen;
-#pragma unroll
-            for (int k = 0; k < PREFETCH; ++k) {
-                const int i = tid + k * BAND_THREADS;
-                pre[k] = i < rowlen ? src[i] : 0.0f;
-            }
-        }
+        // row b + 2 into the slot row b - 1 left; it lands while row b is read
+        if (b > b_start && b + 2 <= b_top)
+            row_to_lds(Tr + (int64_t)(b + 2) * rowlen, rows + ((b + 2) % 3) * slot, slot, wid, lane);
         // walk range and pieces of every read in this row
         int lo_p = 0, hi_p = 0;
         if (tid < nj && b >= my_first && b <= my_last) {
@@ -232,7 +238,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
             lo_p = bj[b - my_first];
             hi_p = bj[b + 1 - my_first];
         }
-        const int q = (hi_p - lo_p + PIECE - 1) / PIECE;
+        const int q = (hi_p - lo_p + PN - 1) / PN;
         int x = q;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -258,8 +264,8 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
         phi[tid] = (uint16_t)hi_p;
 
         const int sb = b % 3;
-        const float* R0 = rows + sb * rowlen;
-        const float* R1 = rows + (sb + 1 == 3 ? 0 : sb + 1) * rowlen;
+        const float* R0 = rows + sb * slot;
+        const float* R1 = rows + (sb + 1 == 3 ? 0 : sb + 1) * slot;
         const double bd = (double)b;
         // rounds of one piece per lane
         for (int base = 0; base < Q; base += BAND_THREADS) {
@@ -270,8 +276,8 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
             if (tid < lim) {
                 const int jj = owner[tid];
                 const int64_t job = (int64_t)j0 + jj;
-                const int pa = (int)plo[jj] + (base + tid - qex[jj]) * PIECE;
-                const int pe = min(pa + PIECE, (int)phi[jj]);
+                const int pa = (int)plo[jj] + (base + tid - qex[jj]) * PN;
+                const int pe = min(pa + PN, (int)phi[jj]);
                 const int m = jm[job];
                 const double offb = joffb[job];
                 const double cj = offd + offb * qbd;
@@ -279,45 +285,53 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
                 const float* wam = wa + (int64_t)m * nw;
                 const uint2* sm = stream + ((int64_t)m * ntau + r) * nw;
                 double mx = -1e300, s = 0.0;
-                // all records of the piece are requested before the first lookup
-                float aLv[PIECE];
-                uint2 ev[PIECE];
+                // records are requested in chunks of up to 8 before the chunk's first lookup; the slots
+                // past the piece's end read its last record and add nothing (a = -1e300 after a real
+                // first value), so the unrolled lookups carry no branch and their LDS reads overlap
+                constexpr int CH = PN < 8 ? PN : 8;
 #pragma unroll
-                for (int k = 0; k < PIECE; ++k) {
-                    if (pa + k < pe) {
-                        aLv[k] = wam[pa + k];
-                        ev[k] = sm[pa + k];
+                for (int c0 = 0; c0 < PN; c0 += CH) {
+                    if (c0 > 0 && pa + c0 >= pe) break;
+                    float aLv[CH];
+                    uint2 ev[CH];
+#pragma unroll
+                    for (int k = 0; k < CH; ++k) {
+                        const int i = min(pa + c0 + k, pe - 1);
+                        aLv[k] = wam[i];
+                        ev[k] = sm[i];
                     }
-                }
 #pragma unroll
-                for (int k = 0; k < PIECE; ++k) {
-                    if (pa + k >= pe) break;
-                    const double aL = (double)aLv[k];
-                    const uint2 e = ev[k];
-                    const uint32_t code = e.x;
-                    // beta coordinate (clamped) and its fraction in row b
-                    const double xb = fmin(fmax(fma(aL, rdb, offb), 0.0), xbmax);
-                    const double fb = xb - bd;
-                    // Delta coordinate: ln q_L moves with the beta clamp, ln q_H stays
-                    const double xd = fmin(fmax(fma(-qbd, xb, fma((double)__uint_as_float(e.y), rdd, cj)), 0.0),
-                                           xdmax);
-                    int d0 = (int)xd;
-                    d0 = min(max(d0, 0), nd - 2);
-                    const double fd = xd - (double)d0;
-                    const double fc = (double)(code & FC_MASK) * (1.0 / FC_SCALE);
-                    const int o0 = (int)__umul24((code >> BIN_SHIFT) & 0x3F, (uint32_t)nd) + d0;
-                    const int o1 = (code & DOWN) ? o0 - nd : o0 + nd;
-                    const double t00 = R0[o0], t01 = R0[o0 + 1], t10 = R1[o0], t11 = R1[o0 + 1];
-                    const double u00 = R0[o1], u01 = R0[o1 + 1], u10 = R1[o1], u11 = R1[o1 + 1];
-                    const double v0a = fma(fd, t01 - t00, t00), v0b = fma(fd, t11 - t10, t10);
-                    const double v1a = fma(fd, u01 - u00, u00), v1b = fma(fd, u11 - u10, u10);
-                    const double v0 = fma(fb, v0b - v0a, v0a), v1 = fma(fb, v1b - v1a, v1a);
-                    const double a = fma(fc, v1 - v0, v0) + ((code & DRAWN) ? lw_rand : 0.0);
-                    // running (max, sum): one exp of the non-positive difference
-                    const double dlt = a - mx;
-                    const double ex = (double)__expf(-(float)fabs(dlt));
-                    s = dlt > 0.0 ? fma(s, ex, 1.0) : s + ex;
-                    mx = fmax(mx, a);
+                    for (int k = 0; k < CH; ++k) {
+                        const double aL = (double)aLv[k];
+                        const uint2 e = ev[k];
+                        const uint32_t code = e.x;
+                        // beta coordinate (clamped) and its fraction in row b
+                        const double xb = fmin(fmax(fma(aL, rdb, offb), 0.0), xbmax);
+                        const double fb = xb - bd;
+                        // Delta coordinate: ln q_L moves with the beta clamp, ln q_H stays
+                        const double xd = fmin(fmax(fma(-qbd, xb, fma((double)__uint_as_float(e.y), rdd, cj)), 0.0),
+                                               xdmax);
+                        int d0 = (int)xd;
+                        d0 = min(max(d0, 0), nd - 2);
+                        const double fd = xd - (double)d0;
+                        const V fc = (V)(code & FC_MASK) * (V)(1.0 / FC_SCALE);
+                        const int o0 = (int)__umul24((code >> BIN_SHIFT) & 0x3F, (uint32_t)nd) + d0;
+                        const int o1 = (code & DOWN) ? o0 - nd : o0 + nd;
+                        const V fdv = (V)fd, fbv = (V)fb;
+                        const V t00 = R0[o0], t01 = R0[o0 + 1], t10 = R1[o0], t11 = R1[o0 + 1];
+                        const V u00 = R0[o1], u01 = R0[o1 + 1], u10 = R1[o1], u11 = R1[o1 + 1];
+                        const V v0a = vfma(fdv, t01 - t00, t00), v0b = vfma(fdv, t11 - t10, t10);
+                        const V v1a = vfma(fdv, u01 - u00, u00), v1b = vfma(fdv, u11 - u10, u10);
+                        const V v0 = vfma(fbv, v0b - v0a, v0a), v1 = vfma(fbv, v1b - v1a, v1a);
+                        const double a = pa + c0 + k < pe
+                                             ? (double)vfma(fc, v1 - v0, v0) + ((code & DRAWN) ? lw_rand : 0.0)
+                                             : -1e300;
+                        // running (max, sum): one exp of the non-positive difference
+                        const double dlt = a - mx;
+                        const double ex = (double)__expf(-(float)fabs(dlt));
+                        s = dlt > 0.0 ? fma(s, ex, 1.0) : s + ex;
+                        mx = fmax(mx, a);
+                    }
                 }
                 pm[tid] = mx;
                 ps[tid] = s;
@@ -335,14 +349,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
             }
             __syncthreads();
         }
-        if (pf) {
-            float* dst = rows + sb * rowlen;
-#pragma unroll
-            for (int k = 0; k < PREFETCH; ++k) {
-                const int i = tid + k * BAND_THREADS;
-                if (i < rowlen) dst[i] = pre[k];
-            }
-        }
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
     if (tid < nj) {
@@ -394,14 +401,19 @@ int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const 
         EXM_HIP(hipGetLastError());
     }
     EXM_CHECK(piece_res_offset(st->nphif * st->nd) % 16 == 0, EXM_ERR_ARG, "piece results misaligned");
-    EXM_CHECK(st->nphif * st->nd <= PREFETCH * BAND_THREADS, EXM_ERR_ARG, "table row of %d floats exceeds %d",
-              st->nphif * st->nd, PREFETCH * BAND_THREADS);
+    EXM_CHECK(row_slot(st->nphif * st->nd) - st->nphif * st->nd <= TABLE_PAD_FLOATS, EXM_ERR_ARG,
+              "row copy overruns the table padding");
     const size_t lds = piece_lds_bytes(st->nphif * st->nd);
     EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "three table rows and the piece results (%zu bytes) exceed the LDS", lds);
-    EXM_HIP(hipFuncSetAttribute((const void*)k_gather_piece, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int pn = c.piece ? c.piece : PIECE;
+    auto kern = c.f32_interp ? (pn == 4 ? k_gather_piece<4, float> : pn == 16 ? k_gather_piece<16, float>
+                                                                               : k_gather_piece<8, float>)
+                             : (pn == 4 ? k_gather_piece<4, double> : pn == 16 ? k_gather_piece<16, double>
+                                                                                : k_gather_piece<8, double>);
+    EXM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     ProfileScope prof("gather");
-    hipLaunchKernelGGL(k_gather_piece, dim3(js.ntiles, st->ntau), dim3(BAND_THREADS), lds, c.stream, stream, wa, nw,
-                       lw_g, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab, js.d_offb,
+    hipLaunchKernelGGL(kern, dim3(js.ntiles, st->ntau), dim3(BAND_THREADS), lds, c.stream, stream, wa, nw, lw_g,
+                       st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab, js.d_offb,
                        js.d_tiles, first_last, bounds, rdb, -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os);
     EXM_HIP(hipGetLastError());
     return EXM_OK;

@@ -155,6 +155,16 @@ int exm_batch_info(exm_handle batch, int64_t* templates_used, int64_t* reads, in
  * band kernel where it applies.  No reference counterpart (the reference has one kernel per device). */
 int exm_set_gather_kernel(int which);
 
+/* Walked times per piece of the band kernel: 4, 8 or 16 (0 restores the default, 8).  Any length
+ * gives the same results to rounding; the choice is a tuning knob.  No reference counterpart. */
+int exm_set_gather_piece(int n);
+
+/* Arithmetic of the band kernel's table interpolation: 0 float64 (default), 1 float32.  Index and
+ * fraction arithmetic and every log-sum-exp stay float64 in both; the float32 form is the precision
+ * form of the batched analysis (the reference's CUDA kernel interpolates in float32 throughout,
+ * src/exmarg/search/gather.py:143-306).  Applies to the band kernel only. */
+int exm_set_gather_precision(int p);
+
 /* ---------------------------------------------------------------- measurement
  * No reference counterpart: kernel timing with HIP events recorded on the library's stream around
  * every launch of a named stage ("filter", "select", "gather"), for the benchmark. */

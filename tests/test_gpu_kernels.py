@@ -170,11 +170,19 @@ def test_gather_matches_oracle(hip, oracle, kernel, kind, subsampled):
     np.testing.assert_allclose(got, ref, rtol=0, atol=KERNELS[kernel] if subsampled else 1e-12)
 
 
+@pytest.fixture
+def piece(hip, request):
+    hip.call("exm_set_gather_piece", request.param)
+    yield request.param
+    hip.call("exm_set_gather_piece", 0)
+
+
 @pytest.mark.parametrize("kernel", [2], indirect=True)
-def test_band_gather_clamps_and_many_reads(hip, oracle, kernel):
+@pytest.mark.parametrize("piece", [4, 8, 16], indirect=True)
+def test_band_gather_clamps_and_many_reads(hip, oracle, kernel, piece):
     # beta clamped at both table edges, Delta clamped, more reads of one table than one tile holds
     from exmarg_amd.search import gather as g
-    say("gather: band kernel, clamped rows, 1500 reads of one table")
+    say("gather: band kernel, piece %d, clamped rows, 1500 reads of one table" % piece)
     s = _tiny_store("rough", (5, 49, 24, 17), n_nodes=1, seed=4)
     nm, nt = 3, 4096
     traj = _rand_traj(nm, nt, seed=8)
@@ -190,6 +198,39 @@ def test_band_gather_clamps_and_many_reads(hip, oracle, kernel):
     ref = oracle.gather(traj, oracle.Tables(tables=s.tables, tau_idx=s.tau_idx), s.beta[0], s.delta[0], ids, lna,
                         tab, times)
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("kind", ["smooth", "rough"])
+def test_band_gather_float32_form(hip, oracle, kind):
+    """float32 table interpolation (index arithmetic and log-sum-exp in float64) stays within 2e-6
+    of the float64 oracle; the float64 form of the same reads (float32 exp inside the log-sum-exp)
+    stays within 1e-7 of it"""
+    from exmarg_amd.search import gather as g
+    say("gather: band kernel, float32 interpolation, %s table" % kind)
+    s = _tiny_store(kind, (7, 49, 30, 23), n_nodes=2, seed=6)
+    traj = _rand_traj(4, 4096, seed=12)
+    traj["lqH"] *= 3.0
+    traj["lqL"] *= 3.0
+    times_dev = g.select_times(traj, s.tau_idx, 200, 300, seed=2)
+    times = times_dev.to_host()
+    rng = np.random.default_rng(3)
+    ids = rng.integers(0, 4, 300)
+    lna = rng.uniform(-6.0, 6.0, (300, 2))
+    tab = rng.integers(0, 2, (300, 2))
+    ref = oracle.gather(traj, oracle.Tables(tables=s.tables, tau_idx=s.tau_idx), s.beta[0], s.delta[0], ids, lna,
+                        tab, times)
+    hip.call("exm_set_gather_kernel", 2)
+    try:
+        hip.call("exm_set_gather_precision", 1)
+        got32 = g.gather(traj, s, ids, lna, tab, times=times_dev)
+        hip.call("exm_set_gather_precision", 0)
+        got64 = g.gather(traj, s, ids, lna, tab, times=times_dev)
+    finally:
+        hip.call("exm_set_gather_precision", 0)
+        hip.call("exm_set_gather_kernel", 0)
+    np.testing.assert_allclose(got64, ref, rtol=0, atol=1e-7)
+    np.testing.assert_allclose(got32, ref, rtol=0, atol=2e-6)
+    assert np.abs(got32 - ref).max() > 0.0          # the float32 form is a different computation
 
 
 def test_gather_full_geometry_one_node(hip, oracle):
