@@ -53,6 +53,7 @@ SIGNATURES = {
     "exm_profile_enable": [I32],
     "exm_profile_reset": [],
     "exm_profile_get": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)],
+    "exm_copy_calibration": [I64, ctypes.POINTER(ctypes.c_double)],
 }
 
 

@@ -816,4 +816,32 @@ int exm_profile_get(const char* name, double* total_ms, int64_t* count) {
     return EXM_OK;
 }
 
+int exm_copy_calibration(int64_t bytes, double* ms) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    EXM_CHECK(bytes > 0 && bytes % 16 == 0, EXM_ERR_ARG, "copy size must be a positive multiple of 16 bytes");
+    void *a = nullptr, *b = nullptr;
+    EXM_TRY(dev_alloc(&a, (size_t)bytes));
+    int rc = dev_alloc(&b, (size_t)bytes);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == EXM_OK) rc = launch_fill((float*)a, bytes / 4, 1.0f);
+    if (rc == EXM_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
+        set_error("hipEventCreate failed");
+        rc = EXM_ERR_HIP;
+    }
+    if (rc == EXM_OK) rc = hipEventRecord(e0, g_ctx.stream) == hipSuccess ? EXM_OK : EXM_ERR_HIP;
+    if (rc == EXM_OK) rc = launch_copy_calibration((float4*)b, (const float4*)a, bytes / 16);
+    if (rc == EXM_OK) rc = hipEventRecord(e1, g_ctx.stream) == hipSuccess ? EXM_OK : EXM_ERR_HIP;
+    if (rc == EXM_OK) rc = hipEventSynchronize(e1) == hipSuccess ? EXM_OK : EXM_ERR_HIP;
+    float t = 0.0f;
+    if (rc == EXM_OK) rc = hipEventElapsedTime(&t, e0, e1) == hipSuccess ? EXM_OK : EXM_ERR_HIP;
+    if (ms) *ms = t;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(g_ctx.stream);
+    dev_free(a, (size_t)bytes);
+    if (b) dev_free(b, (size_t)bytes);
+    return rc;
+}
+
 }  // extern "C"

@@ -207,6 +207,7 @@ int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const 
 int launch_lse(const JobSet& js, int ntau, const double* om, const double* os, const double* ln_nt_g, double ln_nt,
                double* d_out);
 int launch_fill(float* dst, int64_t n, float v);
+int launch_copy_calibration(float4* dst, const float4* src, int64_t n);
 int launch_fill(double* dst, int64_t n, double v);
 int launch_fill_synthetic(Store* st, int k, uint64_t seed, int64_t node, int kind);
 int launch_store_transpose_in(Store* st, int k, const float* d_src);

@@ -286,8 +286,8 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
                 const uint2* sm = stream + ((int64_t)m * ntau + r) * nw;
                 double mx = -1e300, s = 0.0;
                 // records are requested in chunks of up to 8 before the chunk's first lookup; the slots
-                // past the piece's end read its last record and add nothing (a = -1e300 after a real
-                // first value), so the unrolled lookups carry no branch and their LDS reads overlap
+                // past the piece's end read its last record and add nothing (a = -1e300, the first slot
+                // is always real).
                 constexpr int CH = PN < 8 ? PN : 8;
 #pragma unroll
                 for (int c0 = 0; c0 < PN; c0 += CH) {
@@ -300,8 +300,8 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
                         aLv[k] = wam[i];
                         ev[k] = sm[i];
                     }
-#pragma unroll
-                    for (int k = 0; k < CH; ++k) {
+                    // value of lookup k of the chunk (-1e300 past the piece's end)
+                    auto lookup = [&](int k) -> double {
                         const double aL = (double)aLv[k];
                         const uint2 e = ev[k];
                         const uint32_t code = e.x;
@@ -323,14 +323,41 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
                         const V v0a = vfma(fdv, t01 - t00, t00), v0b = vfma(fdv, t11 - t10, t10);
                         const V v1a = vfma(fdv, u01 - u00, u00), v1b = vfma(fdv, u11 - u10, u10);
                         const V v0 = vfma(fbv, v0b - v0a, v0a), v1 = vfma(fbv, v1b - v1a, v1a);
-                        const double a = pa + c0 + k < pe
-                                             ? (double)vfma(fc, v1 - v0, v0) + ((code & DRAWN) ? lw_rand : 0.0)
-                                             : -1e300;
-                        // running (max, sum): one exp of the non-positive difference
-                        const double dlt = a - mx;
-                        const double ex = (double)__expf(-(float)fabs(dlt));
-                        s = dlt > 0.0 ? fma(s, ex, 1.0) : s + ex;
-                        mx = fmax(mx, a);
+                        return pa + c0 + k < pe ? (double)vfma(fc, v1 - v0, v0) + ((code & DRAWN) ? lw_rand : 0.0)
+                                   : -1e300;
+                    };
+                    if constexpr (sizeof(V) == 4) {
+                        // independent lookups (their LDS reads and interpolations overlap), then the
+                        // chunk's (max, sum): exps of non-positive differences
+                        double av[CH];
+#pragma unroll
+                        for (int k = 0; k < CH; ++k) av[k] = lookup(k);
+                        double cm = av[0];
+#pragma unroll
+                        for (int k = 1; k < CH; ++k) cm = fmax(cm, av[k]);
+                        double cs = 0.0;
+#pragma unroll
+                        for (int k = 0; k < CH; ++k) cs += (double)__expf((float)(av[k] - cm));
+                        if (c0 == 0) {
+                            mx = cm;
+                            s = cs;
+                        } else {
+                            const double dlt = cm - mx;
+                            const double ex = (double)__expf(-(float)fabs(dlt));
+                            s = dlt > 0.0 ? fma(s, ex, cs) : fma(cs, ex, s);
+                            mx = fmax(mx, cm);
+                        }
+                    } else {
+                        // running (max, sum), one exp of the non-positive difference per lookup (the
+                        // float64 interpolation leaves no registers for overlapping whole lookups)
+#pragma unroll
+                        for (int k = 0; k < CH; ++k) {
+                            const double a = lookup(k);
+                            const double dlt = a - mx;
+                            const double ex = (double)__expf(-(float)fabs(dlt));
+                            s = dlt > 0.0 ? fma(s, ex, 1.0) : s + ex;
+                            mx = fmax(mx, a);
+                        }
                     }
                 }
                 pm[tid] = mx;

@@ -94,4 +94,17 @@ int launch_fill_synthetic(Store* st, int k, uint64_t seed, int64_t node, int kin
     return EXM_OK;
 }
 
+// plain copy of n float4 values: the kernel of known traffic (16 n bytes read, 16 n written) that
+// calibrates the memory-traffic counters
+__global__ void k_copy_calibration(float4* __restrict__ dst, const float4* __restrict__ src, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int launch_copy_calibration(float4* dst, const float4* src, int64_t n) {
+    hipLaunchKernelGGL(k_copy_calibration, dim3(4096), dim3(256), 0, ctx().stream, dst, src, n);
+    EXM_HIP(hipGetLastError());
+    return EXM_OK;
+}
+
 }  // namespace exm

@@ -173,6 +173,11 @@ int exm_profile_reset(void);
 /* total milliseconds and launch count of stage `name` since the last reset (synchronizes). */
 int exm_profile_get(const char* name, double* total_ms, int64_t* count);
 
+/* Copy `bytes` (a multiple of 16) between two fresh device buffers with a plain copy kernel and
+ * return its time: the kernel of known traffic that calibrates the memory-traffic counters of a
+ * profiler.  No reference counterpart. */
+int exm_copy_calibration(int64_t bytes, double* ms);
+
 #ifdef __cplusplus
 }
 #endif
