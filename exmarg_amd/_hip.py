@@ -34,6 +34,8 @@ SIGNATURES = {
     "exm_store_fill_synthetic": [H, I32, U64, I64, I32],
     "exm_store_read_table": [H, I32, P],
     "exm_bank_create": [P, I32, I32, I32, I64, HP],
+    "exm_bank_taylorf2": [P, P, P, I32, F64, I32, I32, I64, P, F64, P, HP],
+    "exm_bank_read": [H, P],
     "exm_filter": [H, P, P, P, P, HP],
     "exm_traj_create": [I32, I64, P, P, P, P, HP],
     "exm_traj_read": [H, P, P, P, P],

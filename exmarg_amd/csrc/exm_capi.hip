@@ -389,6 +389,66 @@ int exm_bank_create(const float* W, int n_tmpl, int n_freq, int lo, int64_t n_ti
     return EXM_OK;
 }
 
+int exm_bank_taylorf2(const double* M, const double* q, const double* chi, int n_tmpl, double T, int lo, int n_freq,
+                      int64_t n_time, const double* sig2_H, double d0_mpc, double* u, exm_handle* out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    EXM_CHECK(M && q && chi && sig2_H && u && out, EXM_ERR_ARG, "NULL argument");
+    EXM_CHECK(n_tmpl > 0 && n_freq > 0 && lo >= 0 && T > 0 && d0_mpc > 0, EXM_ERR_ARG,
+              "bad bank n_tmpl=%d n_freq=%d lo=%d T=%g D0=%g", n_tmpl, n_freq, lo, T, d0_mpc);
+    EXM_CHECK(n_time >= (int64_t)lo + n_freq && n_time <= (1ll << 30), EXM_ERR_ARG, "bad n_time %lld",
+              (long long)n_time);
+    for (int i = 0; i < n_tmpl; ++i)
+        EXM_CHECK(M[i] > 0 && q[i] > 0 && q[i] <= 1 && fabs(chi[i]) < 1, EXM_ERR_ARG,
+                  "template %d: M=%g q=%g chi=%g outside M > 0, 0 < q <= 1, |chi| < 1", i, M[i], q[i], chi[i]);
+    Bank* b = new Bank();
+    b->kind = K_BANK;
+    b->n_tmpl = n_tmpl;
+    b->n_freq = n_freq;
+    b->lo = lo;
+    b->n_time = n_time;
+    int rc = dev_alloc((void**)&b->d_W, sizeof(float2) * (size_t)n_tmpl * n_freq);
+    if (rc != EXM_OK) {
+        delete b;
+        return rc;
+    }
+    // inputs and u through the small-upload scratch slot: (M, q, chi, u) per template, then sig2_H
+    const size_t nb = sizeof(double) * (4 * (size_t)n_tmpl + n_freq);
+    void* buf = nullptr;
+    rc = scratch(2, nb, &buf);
+    if (rc != EXM_OK) {
+        delete b;
+        return rc;
+    }
+    double* d = (double*)buf;
+    hipStream_t s = g_ctx.stream;
+    EXM_HIP(hipMemcpyAsync(d, M, sizeof(double) * n_tmpl, hipMemcpyHostToDevice, s));
+    EXM_HIP(hipMemcpyAsync(d + n_tmpl, q, sizeof(double) * n_tmpl, hipMemcpyHostToDevice, s));
+    EXM_HIP(hipMemcpyAsync(d + 2 * (size_t)n_tmpl, chi, sizeof(double) * n_tmpl, hipMemcpyHostToDevice, s));
+    EXM_HIP(hipMemcpyAsync(d + 4 * (size_t)n_tmpl, sig2_H, sizeof(double) * n_freq, hipMemcpyHostToDevice, s));
+    rc = launch_taylorf2(d, d + n_tmpl, d + 2 * (size_t)n_tmpl, n_tmpl, n_freq, lo, T, d0_mpc, d + 4 * (size_t)n_tmpl,
+                         b->d_W, d + 3 * (size_t)n_tmpl);
+    if (rc != EXM_OK) {
+        delete b;
+        return rc;
+    }
+    EXM_HIP(hipMemcpyAsync(u, d + 3 * (size_t)n_tmpl, sizeof(double) * n_tmpl, hipMemcpyDeviceToHost, s));
+    EXM_HIP(hipStreamSynchronize(s));
+    *out = register_object(b);
+    return EXM_OK;
+}
+
+int exm_bank_read(exm_handle h, float* W) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Bank, b, h, K_BANK);
+    EXM_CHECK(W != nullptr, EXM_ERR_ARG, "W is NULL");
+    EXM_HIP(hipMemcpyAsync(W, b->d_W, sizeof(float2) * (size_t)b->n_tmpl * b->n_freq, hipMemcpyDeviceToHost,
+                           g_ctx.stream));
+    EXM_HIP(hipStreamSynchronize(g_ctx.stream));
+    return EXM_OK;
+}
+
 static int new_traj(int n_tmpl, int64_t n_time, Traj** out) {
     Traj* t = new Traj();
     t->kind = K_TRAJ;

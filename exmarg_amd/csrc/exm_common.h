@@ -208,6 +208,8 @@ int launch_lse(const JobSet& js, int ntau, const double* om, const double* os, c
                double* d_out);
 int launch_fill(float* dst, int64_t n, float v);
 int launch_copy_calibration(float4* dst, const float4* src, int64_t n);
+int launch_taylorf2(const double* d_M, const double* d_q, const double* d_chi, int n_tmpl, int n_freq, int lo,
+                    double T, double d0_mpc, const double* d_sig2H, float2* d_W, double* d_u);
 int launch_fill(double* dst, int64_t n, double v);
 int launch_fill_synthetic(Store* st, int k, uint64_t seed, int64_t node, int kind);
 int launch_store_transpose_in(Store* st, int k, const float* d_src);

@@ -66,8 +66,9 @@ class MassBank:
     W          (n_template, n_freq) complex64 (a / D0) e^{i psi0}, host copy; the device copy is
                behind .handle
 
-    W comes from `factors` when given (any waveform model), else from the TaylorF2 model of
-    exmarg_amd.synth."""
+    W comes from `factors` when given (any waveform model, uploaded from the host), else from
+    TaylorF2 computed on the device (exm_bank_taylorf2, the model of exmarg_amd.synth.taylorf2);
+    the host copy of a device-computed W is read back on first access."""
 
     def __init__(self, M, q, chi, seg, device="hip", factors=None):
         check_device(device)
@@ -79,17 +80,33 @@ class MassBank:
             raise ValueError("frequency bins must be contiguous")
         self.lo, self.hi = int(self.nu[0]), int(self.nu[-1]) + 1
         self.n_time = 8 * int(self.nu.max())
+        _hip.init()
         if factors is None:
-            W, u = synth.template_factors(self.M, self.q, self.chi, seg)
+            sig2 = _hip.contiguous(seg["sig2_H"], np.float64)
+            if sig2.shape != (self.nu.size,):
+                raise ValueError("sig2_H must have %d bins, got %s" % (self.nu.size, sig2.shape))
+            self.u = np.empty(self.M.size)
+            self._W = None
+            self.handle = _hip.new_handle("exm_bank_taylorf2", _hip.ptr(self.M), _hip.ptr(self.q), _hip.ptr(self.chi),
+                                          self.M.size, float(self.T), self.lo, self.nu.size, self.n_time,
+                                          _hip.ptr(sig2), float(synth.D0), _hip.ptr(self.u), kind="bank")
         else:
             W, u = factors
-        self.W = np.ascontiguousarray(W, np.complex64)
-        self.u = np.ascontiguousarray(u, float)
-        if self.W.shape != (self.M.size, self.nu.size):
-            raise ValueError("W has shape %s, expected %s" % (self.W.shape, (self.M.size, self.nu.size)))
-        _hip.init()
-        self.handle = _hip.new_handle("exm_bank_create", _hip.ptr(self.W.view(np.float32)), self.M.size,
-                                      self.nu.size, self.lo, self.n_time, kind="bank")
+            self._W = np.ascontiguousarray(W, np.complex64)
+            self.u = np.ascontiguousarray(u, float)
+            if self._W.shape != (self.M.size, self.nu.size):
+                raise ValueError("W has shape %s, expected %s" % (self._W.shape, (self.M.size, self.nu.size)))
+            self.handle = _hip.new_handle("exm_bank_create", _hip.ptr(self._W.view(np.float32)), self.M.size,
+                                          self.nu.size, self.lo, self.n_time, kind="bank")
+
+    @property
+    def W(self):
+        """(n_template, n_freq) complex64 host copy of the template factors"""
+        if self._W is None:
+            W = np.empty((self.M.size, self.nu.size), np.complex64)
+            _hip.call("exm_bank_read", int(self.handle), _hip.ptr(W.view(np.float32)))
+            self._W = W
+        return self._W
 
     def trajectories(self, d, sig2, chunk=1024):
         """matched filter of one segment against every template (src/exmarg/search/bank.py:56-78).

@@ -82,6 +82,18 @@ int exm_store_read_table(exm_handle store, int k, float* table);
  * filter length n_time.  Replaces the device copy of src/exmarg/search/bank.py:54 (MassBank.W). */
 int exm_bank_create(const float* W, int n_tmpl, int n_freq, int lo, int64_t n_time, exm_handle* out);
 
+/* Template factors computed on the device from TaylorF2 (detector-frame total mass M, mass ratio
+ * 0 < q <= 1, common aligned spin chi per template) on the bins lo .. lo + n_freq - 1 of a
+ * segment of T seconds: W = (a / D0) e^{-i psi} (D0 = d0_mpc) and u = sum a^2 / (D0^2 sig2_H)
+ * (float64, written to u).  The device form of the bank construction of
+ * src/exmarg/search/bank.py:35-54 with TaylorF2 in place of IMRPhenomXAS (no device
+ * lalsimulation exists); the host model exmarg_amd.synth.taylorf2 computes the same factors. */
+int exm_bank_taylorf2(const double* M, const double* q, const double* chi, int n_tmpl, double T, int lo, int n_freq,
+                      int64_t n_time, const double* sig2_H, double d0_mpc, double* u, exm_handle* out);
+
+/* Copy W (n_tmpl, n_freq) complex64 of a bank back to the host. */
+int exm_bank_read(exm_handle bank, float* W);
+
 /* Matched filter of one segment against every template of the bank: trajectories lqH, phH,
  * lqL, phL, each (n_tmpl, n_time) float32.  dH, dL are (n_freq) complex128 as interleaved
  * double pairs, s2H, s2L (n_freq) float64.  Weighting and transform in float64, one rounding
