@@ -47,6 +47,10 @@ SIGNATURES = {
     "exm_batch_create": [H, I64, I32, HP],
     "exm_batch_add": [H, H, H, P, P, P, P, I32, I32, I32, I32, P, F32, ctypes.POINTER(ctypes.c_int)],
     "exm_batch_run": [H, P],
+    "exm_batch_launch": [H],
+    "exm_reserve": [I32, I64, I32, I32],
+    "exm_batch_poll": [H, ctypes.POINTER(ctypes.c_int)],
+    "exm_batch_collect": [H, P],
     "exm_batch_reset": [H],
     "exm_batch_info": [H, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)],
     "exm_set_gather_kernel": [I32],

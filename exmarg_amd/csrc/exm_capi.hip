@@ -77,7 +77,19 @@ Times::~Times() {
     dev_free(d_lw, sizeof(float) * (size_t)n_tmpl * n_walk);
 }
 Plan::~Plan() { jobset_free(&js); }
-Batch::~Batch() { dev_free(d_stream, bytes); }
+Batch::~Batch() {
+    if (done) {
+        (void)hipEventSynchronize(done);
+        (void)hipEventDestroy(done);
+    }
+    if (stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+    }
+    if (js.d_block) dev_free(js.d_block, js.bytes);
+    if (d_work) dev_free(d_work, work_bytes);
+    dev_free(d_stream, bytes);
+}
 
 int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const double* offb, JobSet* js) {
     js->njobs = njobs;
@@ -115,11 +127,11 @@ int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const doub
     js->d_tab = (int32_t*)(base + o_t);
     js->d_offb = (double*)(base + o_o);
     js->d_tiles = (int32_t*)(base + o_tiles);
-    EXM_HIP(hipMemcpyAsync(js->d_m, sm.data(), 4 * (size_t)njobs, hipMemcpyHostToDevice, g_ctx.stream));
-    EXM_HIP(hipMemcpyAsync(js->d_tab, st.data(), 4 * (size_t)njobs, hipMemcpyHostToDevice, g_ctx.stream));
-    EXM_HIP(hipMemcpyAsync(js->d_offb, so.data(), 8 * (size_t)njobs, hipMemcpyHostToDevice, g_ctx.stream));
-    EXM_HIP(hipMemcpyAsync(js->d_tiles, tiles.data(), 4 * tiles.size(), hipMemcpyHostToDevice, g_ctx.stream));
-    EXM_HIP(hipStreamSynchronize(g_ctx.stream));
+    EXM_HIP(hipMemcpyAsync(js->d_m, sm.data(), 4 * (size_t)njobs, hipMemcpyHostToDevice, cur_stream()));
+    EXM_HIP(hipMemcpyAsync(js->d_tab, st.data(), 4 * (size_t)njobs, hipMemcpyHostToDevice, cur_stream()));
+    EXM_HIP(hipMemcpyAsync(js->d_offb, so.data(), 8 * (size_t)njobs, hipMemcpyHostToDevice, cur_stream()));
+    EXM_HIP(hipMemcpyAsync(js->d_tiles, tiles.data(), 4 * tiles.size(), hipMemcpyHostToDevice, cur_stream()));
+    EXM_HIP(hipStreamSynchronize(cur_stream()));
     return EXM_OK;
 }
 
@@ -149,7 +161,7 @@ ProfileScope::ProfileScope(const char* n) : name(n) {
         start = nullptr;
         return;
     }
-    (void)hipEventRecord(start, g_ctx.stream);
+    (void)hipEventRecord(start, cur_stream());
 }
 
 ProfileScope::~ProfileScope() {
@@ -159,7 +171,7 @@ ProfileScope::~ProfileScope() {
         (void)hipEventDestroy(start);
         return;
     }
-    (void)hipEventRecord(stop, g_ctx.stream);
+    (void)hipEventRecord(stop, cur_stream());
     g_prof[name].events.emplace_back(start, stop);
 }
 
@@ -727,6 +739,7 @@ int exm_batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, const d
     NEED(Plan, plan, hplan, K_PLAN);
     NEED(Store, st, b->store, K_STORE);
     EXM_CHECK(dH && dL && s2H && s2L && pick, EXM_ERR_ARG, "NULL argument");
+    EXM_CHECK(!b->in_flight, EXM_ERR_STATE, "batch in flight; collect it before adding segments");
     EXM_CHECK(K + S == b->nw, EXM_ERR_ARG, "n_loud + n_random = %d, the batch walks %d times", K + S, b->nw);
     EXM_CHECK(plan->n_tmpl == bank->n_tmpl, EXM_ERR_ARG, "plan is for %d templates, bank has %d", plan->n_tmpl,
               bank->n_tmpl);
@@ -778,40 +791,125 @@ int exm_batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, const d
     return EXM_OK;
 }
 
-int exm_batch_run(exm_handle hbatch, double* out) {
+static void batch_release_jobs(Batch* b) {
+    if (b->js.d_block) dev_free(b->js.d_block, b->js.bytes);
+    b->js = JobSet();
+}
+
+int exm_reserve(int n_tmpl, int64_t n_time, int n_freq, int n_walk) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    EXM_CHECK(n_tmpl > 0 && n_time > 0 && n_freq > 0 && n_walk > 0, EXM_ERR_ARG, "bad reserve shape");
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t nm = (size_t)n_tmpl, nt = (size_t)n_time, nw = (size_t)n_walk, nf = (size_t)n_freq;
+    void* p = nullptr;
+    // filter and selection work (<= 1 GiB plus one row), uploads, walk temporaries, batch admission
+    EXM_TRY(scratch(0, std::max<size_t>((size_t)1 << 30, 16 * nt) + 16 * nt, &p));
+    EXM_TRY(scratch(2, 8 * std::max(std::max(6 * nf + 32, nm * nw), 4 * nm + nf), &p));
+    EXM_TRY(scratch(5, 2 * al(4 * nm * nw), &p));
+    EXM_TRY(scratch(7, al(16 * nm * nt) + 2 * al(4 * nm * nw), &p));
+    return EXM_OK;
+}
+
+int exm_batch_launch(exm_handle hbatch) {
     LOCK;
     EXM_TRY(ensure_init());
     NEED(Batch, b, hbatch, K_BATCH);
     NEED(Store, st, b->store, K_STORE);
-    EXM_CHECK(out != nullptr, EXM_ERR_ARG, "out is NULL");
+    EXM_CHECK(!b->in_flight, EXM_ERR_STATE, "batch already launched; collect it first");
+    if (!b->stream) EXM_HIP(hipStreamCreate(&b->stream));
+    if (!b->done) EXM_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
     const int64_t njobs = (int64_t)b->h_m.size();
-    if (njobs == 0) return EXM_OK;
-    JobSet js;
-    EXM_TRY(jobset_build(njobs, b->h_m.data(), b->h_tab.data(), b->h_offb.data(), &js));
-    double *pairs = nullptr, *dout = nullptr;
-    int rc = scratch(0, sizeof(double) * 2 * (size_t)njobs * st->ntau, (void**)&pairs);
-    if (rc == EXM_OK) rc = scratch(1, sizeof(double) * (size_t)njobs, (void**)&dout);
-    if (rc == EXM_OK)
-        rc = launch_piece(st, b->d_stream, b->d_wa, b->nw, b->d_lw, js, pairs, pairs + (size_t)njobs * st->ntau);
-    if (rc == EXM_OK)
-        rc = launch_lse(js, st->ntau, pairs, pairs + (size_t)njobs * st->ntau, b->d_lnnt, 0.0, dout);
-    std::vector<double> sorted(njobs);
-    if (rc == EXM_OK && (hipMemcpyAsync(sorted.data(), dout, sizeof(double) * njobs, hipMemcpyDeviceToHost,
-                                        ctx().stream) != hipSuccess ||
-                         hipStreamSynchronize(ctx().stream) != hipSuccess)) {
-        set_error("batch gather failed");
-        rc = EXM_ERR_HIP;
+    if (njobs > 0) {
+        StreamScope scope(b->stream);
+        EXM_TRY(jobset_build(njobs, b->h_m.data(), b->h_tab.data(), b->h_offb.data(), &b->js));
+        auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        const size_t bnd = al(piece_bounds_bytes(st, njobs));
+        const size_t pairs = al(sizeof(double) * 2 * (size_t)njobs * st->ntau);
+        const size_t need = bnd + pairs + sizeof(double) * (size_t)njobs;
+        if (need > b->work_bytes) {
+            if (b->d_work) dev_free(b->d_work, b->work_bytes);
+            b->d_work = nullptr;
+            b->work_bytes = 0;
+            int rc = dev_alloc(&b->d_work, need);
+            if (rc != EXM_OK) {
+                batch_release_jobs(b);
+                return rc;
+            }
+            b->work_bytes = need;
+        }
+        char* w = (char*)b->d_work;
+        double* om = (double*)(w + bnd);
+        double* os = om + (size_t)njobs * st->ntau;
+        double* dout = (double*)(w + bnd + pairs);
+        int rc = launch_piece(st, b->d_stream, b->d_wa, b->nw, b->d_lw, b->js, om, os, w);
+        if (rc == EXM_OK) rc = launch_lse(b->js, st->ntau, om, os, b->d_lnnt, 0.0, dout);
+        if (rc != EXM_OK) {
+            (void)hipStreamSynchronize(b->stream);
+            batch_release_jobs(b);
+            return rc;
+        }
     }
-    if (rc == EXM_OK)
-        for (int64_t i = 0; i < njobs; ++i) out[js.order[i]] = sorted[i];
-    jobset_free(&js);
+    EXM_HIP(hipEventRecord(b->done, b->stream));
+    b->in_flight = true;
+    return EXM_OK;
+}
+
+int exm_batch_poll(exm_handle hbatch, int* done) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Batch, b, hbatch, K_BATCH);
+    EXM_CHECK(done != nullptr, EXM_ERR_ARG, "done is NULL");
+    EXM_CHECK(b->in_flight, EXM_ERR_STATE, "batch not launched");
+    const hipError_t e = hipEventQuery(b->done);
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return EXM_OK;
+    }
+    EXM_HIP(e);
+    *done = 1;
+    return EXM_OK;
+}
+
+int exm_batch_collect(exm_handle hbatch, double* out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Batch, b, hbatch, K_BATCH);
+    NEED(Store, st, b->store, K_STORE);
+    EXM_CHECK(b->in_flight, EXM_ERR_STATE, "batch not launched");
+    const int64_t njobs = (int64_t)b->h_m.size();
+    EXM_CHECK(out != nullptr || njobs == 0, EXM_ERR_ARG, "out is NULL");
+    b->in_flight = false;
+    int rc = hipEventSynchronize(b->done) == hipSuccess ? EXM_OK : EXM_ERR_HIP;
+    if (rc != EXM_OK) set_error("batch gather failed");
+    if (rc == EXM_OK && njobs > 0) {
+        auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        const size_t off = al(piece_bounds_bytes(st, njobs)) + al(sizeof(double) * 2 * (size_t)njobs * st->ntau);
+        std::vector<double> sorted(njobs);
+        if (hipMemcpyAsync(sorted.data(), (char*)b->d_work + off, sizeof(double) * njobs, hipMemcpyDeviceToHost,
+                           b->stream) != hipSuccess ||
+            hipStreamSynchronize(b->stream) != hipSuccess) {
+            set_error("batch result copy failed");
+            rc = EXM_ERR_HIP;
+        } else {
+            for (int64_t i = 0; i < njobs; ++i) out[b->js.order[i]] = sorted[i];
+        }
+    }
+    batch_release_jobs(b);
     return rc;
+}
+
+int exm_batch_run(exm_handle hbatch, double* out) {
+    LOCK;
+    EXM_TRY(exm_batch_launch(hbatch));
+    return exm_batch_collect(hbatch, out);
 }
 
 int exm_batch_reset(exm_handle hbatch) {
     LOCK;
     EXM_TRY(ensure_init());
     NEED(Batch, b, hbatch, K_BATCH);
+    EXM_CHECK(!b->in_flight, EXM_ERR_STATE, "batch in flight; collect it first");
     b->used = 0;
     b->h_m.clear();
     b->h_tab.clear();

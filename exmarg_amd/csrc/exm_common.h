@@ -149,6 +149,13 @@ struct Batch : Object {
     std::vector<int32_t> h_m, h_tab;
     std::vector<double> h_offb;
     std::vector<int64_t> seg_reads;
+    // asynchronous gather (exm_batch_launch): own stream, completion event, jobs and work memory
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool in_flight = false;
+    JobSet js;
+    void* d_work = nullptr;              // first/last rows, row bounds, (max, sum) pairs, results
+    size_t work_bytes = 0;
     ~Batch();
 };
 
@@ -158,7 +165,8 @@ int gather_mode();
 // ---------------------------------------------------------------- context
 struct Context {
     int device = -1;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;         // the library's stream
+    hipStream_t launch_stream = nullptr;  // when set, gather-side launches go here (a batch's own stream)
     std::recursive_mutex mu;
     // grow-only scratch buffers
     static constexpr int NSCRATCH = 8;
@@ -174,6 +182,18 @@ struct Context {
 
 Context& ctx();
 int ensure_init();
+
+// stream of the gather-side launches: a batch's own stream inside exm_batch_launch, else the library's
+inline hipStream_t cur_stream() {
+    Context& c = ctx();
+    return c.launch_stream ? c.launch_stream : c.stream;
+}
+
+// routes gather-side launches to stream s for the lifetime of the scope
+struct StreamScope {
+    explicit StreamScope(hipStream_t s) { ctx().launch_stream = s; }
+    ~StreamScope() { ctx().launch_stream = nullptr; }
+};
 
 // kernel timing: events around named stages on the library stream (enabled by exm_profile_enable)
 struct ProfileScope {
@@ -201,8 +221,10 @@ int launch_gather(Traj* traj, Store* store, const JobSet& js, Times* times, doub
 int launch_gather_ref(Traj* traj, Store* store, const JobSet& js, Times* times, double* om, double* os);
 int launch_gather_band(Traj* traj, Store* store, const JobSet& js, Times* times, double* om, double* os);
 int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* stream_dst);
+// bounds_buf: piece_bounds_bytes() of device memory for the row bounds (scratch slot 6 when null)
+size_t piece_bounds_bytes(const Store* st, int64_t njobs);
 int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const float* lw_g, const JobSet& js,
-                 double* om, double* os);
+                 double* om, double* os, void* bounds_buf = nullptr);
 // LSE over the lags of every read; ln n_time per template (ln_nt_g) or one value for all (ln_nt)
 int launch_lse(const JobSet& js, int ntau, const double* om, const double* os, const double* ln_nt_g, double ln_nt,
                double* d_out);

@@ -410,20 +410,25 @@ int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* strea
 }
 
 // piece gather of every read of js; template ids index wa, stream and lw_g
+size_t piece_bounds_bytes(const Store* st, int64_t njobs) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    return al(sizeof(int32_t) * 2 * (size_t)njobs) + al(sizeof(uint16_t) * (size_t)njobs * (st->nb + 1));
+}
+
 int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const float* lw_g, const JobSet& js,
-                 double* om, double* os) {
+                 double* om, double* os, void* bounds_buf) {
     Context& c = ctx();
+    const hipStream_t cs = cur_stream();
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t fl_bytes = al(sizeof(int32_t) * 2 * (size_t)js.njobs);
-    const size_t bnd_bytes = al(sizeof(uint16_t) * (size_t)js.njobs * (st->nb + 1));
-    void* bbuf = nullptr;
-    EXM_TRY(scratch(6, fl_bytes + bnd_bytes, &bbuf));
+    void* bbuf = bounds_buf;
+    if (!bbuf) EXM_TRY(scratch(6, piece_bounds_bytes(st, js.njobs), &bbuf));
     int32_t* first_last = (int32_t*)bbuf;
     uint16_t* bounds = (uint16_t*)((char*)bbuf + fl_bytes);
     const double rdb = 2.0 / st->dbeta;
     {
         ProfileScope prof("bounds");
-        hipLaunchKernelGGL(k_read_bounds, dim3((unsigned)((js.njobs + 255) / 256)), dim3(256), 0, c.stream, wa, nw,
+        hipLaunchKernelGGL(k_read_bounds, dim3((unsigned)((js.njobs + 255) / 256)), dim3(256), 0, cs, wa, nw,
                            js.d_m, js.d_offb, js.njobs, rdb, st->nb, first_last, bounds);
         EXM_HIP(hipGetLastError());
     }
@@ -439,7 +444,7 @@ int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const 
                                                                                 : k_gather_piece<8, double>);
     EXM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     ProfileScope prof("gather");
-    hipLaunchKernelGGL(kern, dim3(js.ntiles, st->ntau), dim3(BAND_THREADS), lds, c.stream, stream, wa, nw, lw_g,
+    hipLaunchKernelGGL(kern, dim3(js.ntiles, st->ntau), dim3(BAND_THREADS), lds, cs, stream, wa, nw, lw_g,
                        st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab, js.d_offb,
                        js.d_tiles, first_last, bounds, rdb, -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os);
     EXM_HIP(hipGetLastError());
@@ -458,14 +463,14 @@ __global__ void k_fill_f64(double* __restrict__ dst, int64_t n, double v) {
 
 int launch_fill(float* dst, int64_t n, float v) {
     if (n <= 0) return EXM_OK;
-    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx().stream, dst, n, v);
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, cur_stream(), dst, n, v);
     EXM_HIP(hipGetLastError());
     return EXM_OK;
 }
 
 int launch_fill(double* dst, int64_t n, double v) {
     if (n <= 0) return EXM_OK;
-    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx().stream, dst, n, v);
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, cur_stream(), dst, n, v);
     EXM_HIP(hipGetLastError());
     return EXM_OK;
 }

@@ -109,7 +109,7 @@ int launch_gather_ref(Traj* traj, Store* st, const JobSet& js, Times* times, dou
     const double offd = -st->delta0 / st->ddelta;
     const double rdd = 1.0 / st->ddelta;
     dim3 grid((unsigned)((js.njobs + 255) / 256), (unsigned)st->ntau);
-    hipLaunchKernelGGL(k_gather_ref, grid, dim3(256), 0, c.stream, traj->lqH(), traj->phH(), traj->lqL(),
+    hipLaunchKernelGGL(k_gather_ref, grid, dim3(256), 0, cur_stream(), traj->lqH(), traj->phH(), traj->lqL(),
                        traj->phL(), traj->n_time, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->d_tau,
                        st->ntau, js.d_m, js.d_tab, js.d_offb, js.njobs, rdb, offd, rdd,
                        times ? times->d_sel : nullptr, times ? times->d_lw : nullptr, times ? times->n_walk : 0, om,
@@ -136,7 +136,7 @@ int launch_gather(Traj* traj, Store* st, const JobSet& js, Times* times, double*
 
 int launch_lse(const JobSet& js, int ntau, const double* om, const double* os, const double* ln_nt_g, double ln_nt,
                double* d_out) {
-    hipLaunchKernelGGL(k_lse_lags, dim3((unsigned)((js.njobs + 255) / 256)), dim3(256), 0, ctx().stream, om, os,
+    hipLaunchKernelGGL(k_lse_lags, dim3((unsigned)((js.njobs + 255) / 256)), dim3(256), 0, cur_stream(), om, os,
                        js.njobs, ntau, js.d_m, ln_nt_g, ln_nt, d_out);
     EXM_HIP(hipGetLastError());
     return EXM_OK;

@@ -15,3 +15,4 @@ from .bank import mass_bank, MassBank
 from .store import load_redshift_store, synthetic_store
 from .pe import pe_redshift, pe_intrinsic, distance_weights
 from .batch import pe_intrinsic_batch, SegmentBatch
+from .serve import Server

@@ -78,10 +78,24 @@ class SegmentBatch:
 
     def run(self):
         """lamN (n_node + 1, n_template) of every admitted segment, in admission order"""
+        self.launch()
+        return self.collect()
+
+    def launch(self):
+        """enqueue the gather of every admitted segment on the batch's own stream and return"""
+        _hip.call("exm_batch_launch", int(self.handle))
+
+    def done(self):
+        """whether the launched gather has finished"""
+        flag = _hip.ctypes.c_int()
+        _hip.call("exm_batch_poll", int(self.handle), _hip.ctypes.byref(flag))
+        return bool(flag.value)
+
+    def collect(self):
+        """wait for the launched gather; lamN of every admitted segment, in admission order"""
         nreads = sum(a.plan.job_m.size for a in self.entries)
-        out = np.empty(nreads, np.float64)
-        if nreads:
-            _hip.call("exm_batch_run", int(self.handle), _hip.ptr(out))
+        out = np.empty(max(nreads, 1), np.float64)
+        _hip.call("exm_batch_collect", int(self.handle), _hip.ptr(out))
         res, o = [], 0
         for a in self.entries:
             n = a.plan.job_m.size

@@ -1,0 +1,194 @@
+"""
+Serving: a request loop with continuous batching over the HBM-resident tables.
+
+A request is one segment and its own template bank (any length, band and size).  The server keeps
+two batch slots.  While the gather of one slot runs on its own stream, arriving requests are
+admitted into the other: their bank is built on the device (TaylorF2), their segment filtered, its
+arrival times selected and walked, and the walk records (8 bytes per template, lag and walked time)
+stay in the slot's HBM buffer.  When the running gather ends, its requests are finished on the host
+(node log-sum-exp with the distance-prior weights, ln B) and the other slot is launched at once, so
+admission and gather overlap on the device and a request never waits for a full batch.
+
+Admission is by HBM budget: a slot holds at most `slot_templates` templates of walk records
+(template_bytes each); a request larger than a slot is refused.  The reference has no serving
+loop; it analyses one segment group at a time (src/exmarg/search/pe.py:289-346, pp.py:96-97).
+
+    srv = Server(store)
+    r = srv.submit(segment, bank_rows)          # bank rows (M_det, q, chi)
+    while not r.done: srv.step()
+    r.ln_b, r.ln_lambda, r.latency
+"""
+
+import collections
+import time
+
+import numpy as np
+
+from .. import _hip
+from . import pe as _pe, store as _store
+from .bank import mass_bank
+from .batch import SegmentBatch, template_bytes
+from .segments import read_segments
+
+
+class Request:
+    """one segment (a single row) analysed with its own bank"""
+
+    def __init__(self, rid, seg, rows, seed, t_submit):
+        self.id, self.seg, self.rows, self.seed = rid, seg, rows, int(seed)
+        self.t_submit = t_submit
+        self.t_admit = self.t_done = None
+        self.ln_lambda = self.ln_b = None
+        self.bank = self.an = self.lnW = None
+        self.error = None
+
+    @property
+    def done(self):
+        return self.t_done is not None
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.t_submit
+
+
+class Server:
+    """continuous-batching request loop (two batch slots on their own streams)
+
+    store           RedshiftStore (uploaded on construction)
+    n_loud/n_random arrival times walked per template (select_times)
+    slot_templates  templates of walk records per slot (default: half of the HBM left after
+                    reserve_bytes, split over the two slots)
+    max_request     (n_tmpl, n_time, n_freq) of the largest request expected: the work memory for
+                    admitting it is reserved up front so that no admission allocates while a gather
+                    runs
+    prior, ymin     distance prior of the node weights (pe_intrinsic)"""
+
+    def __init__(self, store, n_loud=1024, n_random=1024, slot_templates=None, reserve_bytes=16 << 30,
+                 max_request=None, prior="volume", ymin=0.1267, clock=time.perf_counter):
+        self.store, self.prior, self.ymin, self.clock = store, prior, float(ymin), clock
+        _store.upload(store)
+        self.K, self.S = int(n_loud), int(n_random)
+        if max_request is not None:
+            nm, nt, nf = (int(v) for v in max_request)
+            _hip.call("exm_reserve", nm, nt, nf, self.K + self.S)
+        per = template_bytes(np.asarray(store.tau_idx).size, self.K + self.S)
+        if slot_templates is None:
+            free, _ = _hip.mem_info()
+            slot_templates = int((free - reserve_bytes) // (2 * per))
+        if slot_templates <= 0:
+            raise MemoryError("no device memory for a batch slot (%d bytes per template)" % per)
+        self.slots = [SegmentBatch(store, self.K, self.S, max_templates=slot_templates) for _ in range(2)]
+        self.filling, self.flying = 0, None           # slot indices
+        self.members = {0: [], 1: []}                   # admitted requests per slot
+        self.queue = collections.deque()
+        self.release = []                               # device objects freed once nothing is in flight
+        self.next_id = 0
+        self.stats = dict(batches=0, admitted=0, admitted_during_gather=0, finished=0, refused=0)
+
+    # ------------------------------------------------------------ client side
+    def submit(self, segments, bank_rows, seed=0, t_submit=None):
+        """queue one segment (the first row of `segments`) with its bank rows (M_det, q, chi)"""
+        seg = read_segments(segments, rows=[0])
+        rows = np.asarray(bank_rows, float)
+        req = Request(self.next_id, seg, rows, seed, self.clock() if t_submit is None else t_submit)
+        self.next_id += 1
+        self.queue.append(req)
+        return req
+
+    @property
+    def idle(self):
+        return not self.queue and self.flying is None and not self.members[self.filling]
+
+    # ------------------------------------------------------------ server loop
+    def step(self):
+        """one pass of the loop: finish a completed gather, launch the filling slot when the device
+        has no gather, admit one queued request; returns whether anything happened"""
+        progress = False
+        if self.flying is not None and self.slots[self.flying].done():
+            self._finish(self.flying)
+            progress = True
+        if self.flying is None and self.members[self.filling]:
+            self.slots[self.filling].launch()
+            self.flying, self.filling = self.filling, 1 - self.filling
+            self.stats["batches"] += 1
+            progress = True
+        if self.queue and self._admit(self.queue[0]):
+            self.queue.popleft()
+            progress = True
+        return progress
+
+    def run(self, until=None, poll=2e-4):
+        """step until every submitted request has finished (or until() is true)"""
+        while not self.idle and not (until and until()):
+            if not self.step():
+                time.sleep(poll)
+
+    def close(self):
+        if self.flying is not None:
+            self._finish(self.flying)
+        for s in self.slots:
+            s.free()
+        self._release()
+
+    # ------------------------------------------------------------ internals
+    def _admit(self, req):
+        """admit the request into the filling slot if it fits; False leaves it queued"""
+        batch = self.slots[self.filling]
+        n = req.rows.shape[0] if req.rows.ndim == 2 else req.rows.size
+        if n > batch.cap:
+            req.error = MemoryError("request of %d templates exceeds a batch slot of %d" % (n, batch.cap))
+            req.t_done = self.clock()
+            self.stats["refused"] += 1
+            return True
+        if batch.used + n > batch.cap:
+            return False
+        seg = req.seg
+        bank = mass_bank(req.rows, seg, "hip")
+        an = _pe._Analysis(bank, self.store, self.ymin, True)
+        try:
+            lnW = _pe._bank_weights(an.geometry[2], self.prior, self.ymin, bank)
+            an.start()
+        except ValueError as e:
+            self.release.append(bank.handle)
+            req.bank = None
+            req.error = e
+            req.t_done = self.clock()
+            self.stats["refused"] += 1
+            return True
+        batch.add(an, {I: seg["d_" + I][0] for I in "HL"}, {I: seg["sig2_" + I] for I in "HL"},
+                  req.seed + int(seg["rows"][0]))
+        req.bank, req.an, req.lnW = bank, an, lnW
+        req.t_admit = self.clock()
+        self.members[self.filling].append(req)
+        self.stats["admitted"] += 1
+        if self.flying is not None:
+            self.stats["admitted_during_gather"] += 1
+        return True
+
+    def _finish(self, k):
+        batch = self.slots[k]
+        lamNs = batch.collect()
+        t = self.clock()
+        for req, lamN in zip(self.members[k], lamNs):
+            v = req.lnW + lamN
+            mx = v.max(axis=0)
+            with np.errstate(invalid="ignore"):
+                L = mx + np.log(np.exp(v - mx[None, :]).sum(axis=0))
+            req.ln_lambda = L
+            req.ln_b = _pe._log_mean(L, _pe._cell_weights(req.bank))
+            req.t_done = t
+            self.release.append(req.bank.handle)
+            self.release.append(req.an.plan.handle)
+            req.an = None
+            self.stats["finished"] += 1
+        self.members[k] = []
+        batch.reset()
+        self.flying = None
+        self._release()
+
+    def _release(self):
+        """free banks and read plans of finished requests (a free may wait for the device, so it is
+        done right after a gather has been collected, before the next launch)"""
+        for h in self.release:
+            h.free()
+        self.release = []

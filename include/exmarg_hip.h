@@ -157,6 +157,21 @@ int exm_batch_add(exm_handle batch, exm_handle bank, exm_handle plan, const doub
  * plan in plan order (float64). */
 int exm_batch_run(exm_handle batch, double* out);
 
+/* Asynchronous form of exm_batch_run for continuous batching: exm_batch_launch enqueues the gather of
+ * every admitted segment on the batch's own stream and returns; exm_batch_poll reports whether it
+ * has finished; exm_batch_collect waits for it and writes the results as exm_batch_run does.  While a
+ * batch is in flight, segments can be admitted to another batch (filter, selection and walk run on
+ * the library stream and overlap the gather on the device).  An in-flight batch refuses
+ * exm_batch_add and exm_batch_reset.  No reference counterpart (the reference analyses one segment
+ * group at a time, src/exmarg/search/pe.py:173-194). */
+int exm_batch_launch(exm_handle batch);
+/* Grow the library's work memory to what admitting a segment of n_tmpl templates, n_time samples,
+ * n_freq bins and n_walk walked times needs, so that later admissions allocate nothing (a growth
+ * while another batch is in flight would wait for it).  No reference counterpart. */
+int exm_reserve(int n_tmpl, int64_t n_time, int n_freq, int n_walk);
+int exm_batch_poll(exm_handle batch, int* done);
+int exm_batch_collect(exm_handle batch, double* out);
+
 /* Empty the batch (keeps its memory). */
 int exm_batch_reset(exm_handle batch);
 
