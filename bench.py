@@ -44,6 +44,12 @@ def parse():
     p.add_argument("--piece", type=int, default=0, help="walked times per piece of the band kernel (0 default)")
     p.add_argument("--precision", choices=["f64", "f32"], default="f64",
                    help="table interpolation of the band kernel (index arithmetic and log-sum-exp stay float64)")
+    p.add_argument("--mode", choices=["plain", "serve"], default="plain",
+                   help="plain: pe_intrinsic on one segment per step (the headline); serve: the request loop "
+                        "under Poisson arrivals")
+    p.add_argument("--rate", type=float, default=50.0, help="serve: mean arrival rate (requests/s)")
+    p.add_argument("--requests", type=int, default=40, help="serve: requests in the timed run")
+    p.add_argument("--seed", type=int, default=7, help="serve: seed of the request stream")
     return p.parse_args()
 
 
@@ -142,12 +148,95 @@ def cpu_leg(args, seg, bank, store, res, an_geometry, seed_row):
     return cpu, err, chk.size
 
 
+def serve_requests(n, seed, rate):
+    """the serving workload: Poisson arrivals at `rate`; T uniform over {4, 8, 16, 32, 64} s at 20-1024 Hz;
+    banks of n_tmpl log-uniform in [500, 8000] on (n_tmpl / 20) x 5 x 4 grids over M_det in [10, 80],
+    q in [0.2, 1], chi in [-0.5, 0.5]; 30 % with one injection of network SNR log-uniform in [6, 40]
+    drawn inside the bank range, the rest noise only"""
+    from exmarg_amd import synth
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    out = []
+    for i in range(n):
+        T = float(rng.choice([4.0, 8.0, 16.0, 32.0, 64.0]))
+        n_tmpl = int(np.exp(rng.uniform(np.log(500.0), np.log(8000.0))))
+        nM = max(2, int(round(n_tmpl / 20.0)))
+        rows = synth.grid_bank(shape=(nM, 5, 4))
+        if rng.uniform() < 0.3:
+            params = (float(rng.uniform(12.0, 75.0)), float(rng.uniform(0.25, 1.0)), float(rng.uniform(-0.45, 0.45)))
+            snr = float(np.exp(rng.uniform(np.log(6.0), np.log(40.0))))
+            seg, _, _ = synth.simulate_at_snr(1, seed * 1000 + i, params, snr, T, 20.0, 1024.0)
+        else:
+            seg, _, _ = synth.simulate(1, seed * 1000 + i, lambda r: (30.0, 0.8, 0.1), lambda r: 0.0, T, 20.0, 1024.0)
+        out.append((float(t[i]), seg, rows))
+    return out
+
+
+def main_serve(args):
+    """the request loop under Poisson arrivals: one JSON line with throughput and latency"""
+    from exmarg_amd import _hip
+    from exmarg_amd.search import Server, store as stm
+    _hip.init(0)
+    info = _hip.device_info()
+    st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
+    reqs = serve_requests(args.requests, args.seed, args.rate)
+    srv = Server(st, n_loud=args.n_loud, n_random=args.n_random, max_request=(8000, 8 * 65536, 64257))
+    # warm-up: one small request per segment length (FFT plans, kernel loads)
+    for T in (4.0, 8.0, 16.0, 32.0, 64.0):
+        for t_arr, seg, rows in reqs:
+            if seg["T"] == T:
+                srv.submit(seg, rows[:40 * 20][::10], seed=1)
+                break
+    srv.run()
+    srv.stats.update(batches=0, admitted=0, admitted_during_gather=0, finished=0, refused=0)
+    _hip.profile(True)
+    t0 = time.perf_counter()
+    pending = list(reqs)
+    handles = []
+    while pending or not srv.idle:
+        now = time.perf_counter() - t0
+        while pending and pending[0][0] <= now:
+            t_arr, seg, rows = pending.pop(0)
+            handles.append(srv.submit(seg, rows, seed=len(handles), t_submit=t0 + t_arr))
+        if not srv.step():
+            time.sleep(2e-4)
+    wall = time.perf_counter() - t0
+    g_ms, g_n = _hip.profile_get("gather")
+    f_ms, f_n = _hip.profile_get("filter")
+    _hip.profile(False)
+    srv.close()
+    ok = [h for h in handles if h.error is None]
+    lat = np.array([h.latency for h in ok])
+    span = max(h.t_done for h in ok) - t0 - reqs[0][0]
+    n_tmpl = np.array([h.rows.shape[0] for h in ok])
+    out = {
+        "metric": "serving: requests/s and p50/p99 latency under Poisson arrivals",
+        "value": len(ok) / span, "unit": "requests/s", "n_gpus": 1, "higher_is_better": True,
+        "offered_rate": args.rate, "requests": len(handles), "finished": len(ok),
+        "refused": len(handles) - len(ok),
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "max": float(lat.max())},
+        "wall_s": wall, "batches": srv.stats["batches"],
+        "admitted_during_gather": srv.stats["admitted_during_gather"],
+        "templates_mean": float(n_tmpl.mean()), "gather_ms_total": g_ms, "filter_ms_total": f_ms,
+        "gpu_gather_busy_frac": g_ms / 1e3 / wall,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "Poisson arrivals; T in {4,8,16,32,64} s at 20-1024 Hz; banks of 500-8000 templates "
+                               "(log-uniform); 30%% injections SNR 6-40; 53 HBM-resident table nodes; "
+                               "pe_intrinsic 'volume', n_loud=n_random=%d" % args.n_loud,
+                   "device": info["name"], "num_cu": info["num_cu"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path runs one process per GPU and this build targets a single "
               "MI355X (--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.mode == "serve":
+        return main_serve(args)
     from exmarg_amd import _hip
     from exmarg_amd.search import mass_bank, pe_intrinsic, store as stm
     from exmarg_amd.search.pe import _node_geometry

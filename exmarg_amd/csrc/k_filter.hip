@@ -72,9 +72,10 @@ int launch_filter(Bank* bank, const double* dH, const double* dL, const double* 
     const int64_t n_time = bank->n_time;
     const int nf = bank->n_freq;
     // chunk of templates whose complex128 padded rows fit in 1 GiB of workspace
-    int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(bank->n_tmpl, (1ll << 30) / (16 * n_time)));
+    const int chunk_max = (int)std::max<int64_t>(1, (1ll << 30) / (16 * n_time));
+    const int chunk = std::min(bank->n_tmpl, chunk_max);
     double2* pad = nullptr;
-    EXM_TRY(scratch(0, sizeof(double2) * (size_t)chunk * n_time, (void**)&pad));
+    EXM_TRY(scratch(0, sizeof(double2) * (size_t)chunk_max * n_time, (void**)&pad));
     double* seg = nullptr;
     EXM_TRY(scratch(2, sizeof(double) * 6 * (size_t)nf + 256, (void**)&seg));
     double2* ddH = (double2*)seg;
@@ -97,8 +98,13 @@ int launch_filter(Bank* bank, const double* dH, const double* dL, const double* 
             hipLaunchKernelGGL(k_filter_pack, grid, dim3(256), 0, c.stream, bank->d_W, d, s2, nf, bank->lo, n_time,
                                m0, pad);
             EXM_HIP(hipGetLastError());
+            // plans exist for power-of-two batches (and the full chunk) only, so that banks of any size
+            // share a few plans; rows past nb in the work buffer are transformed and ignored
+            int nplan = 1;
+            while (nplan < nb) nplan <<= 1;
+            nplan = std::min(nplan, chunk_max);
             hipfftHandle plan;
-            EXM_TRY(fft_plan(n_time, nb, &plan));
+            EXM_TRY(fft_plan(n_time, nplan, &plan));
             EXM_FFT(hipfftExecZ2Z(plan, (hipfftDoubleComplex*)pad, (hipfftDoubleComplex*)pad, HIPFFT_FORWARD));
             hipLaunchKernelGGL(k_filter_post, grid, dim3(256), 0, c.stream, pad, n_time, m0, lq, ph);
             EXM_HIP(hipGetLastError());

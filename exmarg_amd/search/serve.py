@@ -56,12 +56,14 @@ class Server:
 
     store           RedshiftStore (uploaded on construction)
     n_loud/n_random arrival times walked per template (select_times)
-    slot_templates  templates of walk records per slot (default: half of the HBM left after
-                    reserve_bytes, split over the two slots)
+    slot_templates  templates of walk records per slot (default: the HBM left after reserve_bytes,
+                    split over the two slots, at most SLOT_TEMPLATES)
     max_request     (n_tmpl, n_time, n_freq) of the largest request expected: the work memory for
                     admitting it is reserved up front so that no admission allocates while a gather
                     runs
     prior, ymin     distance prior of the node weights (pe_intrinsic)"""
+
+    SLOT_TEMPLATES = 32768          # default cap of a slot: larger batches only lengthen the wait of others
 
     def __init__(self, store, n_loud=1024, n_random=1024, slot_templates=None, reserve_bytes=16 << 30,
                  max_request=None, prior="volume", ymin=0.1267, clock=time.perf_counter):
@@ -74,7 +76,7 @@ class Server:
         per = template_bytes(np.asarray(store.tau_idx).size, self.K + self.S)
         if slot_templates is None:
             free, _ = _hip.mem_info()
-            slot_templates = int((free - reserve_bytes) // (2 * per))
+            slot_templates = int(min((free - reserve_bytes) // (2 * per), self.SLOT_TEMPLATES))
         if slot_templates <= 0:
             raise MemoryError("no device memory for a batch slot (%d bytes per template)" % per)
         self.slots = [SegmentBatch(store, self.K, self.S, max_templates=slot_templates) for _ in range(2)]
