@@ -974,6 +974,49 @@ int exm_profile_get(const char* name, double* total_ms, int64_t* count) {
     return EXM_OK;
 }
 
+int exm_source_frame(const double* lamN, const double* yN, int nrow, int nm, const double* Mdet, const double* yq,
+                     const double* m1, int nm1, const double* z, int nz, const double* w, double* out, double* ln_b) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    EXM_CHECK(lamN && yN && Mdet && yq && m1 && z, EXM_ERR_ARG, "NULL argument");
+    EXM_CHECK(nm1 > 0 && nz > 0, EXM_ERR_ARG, "empty (m1, z) grid");
+    EXM_CHECK(w == nullptr || ln_b != nullptr, EXM_ERR_ARG, "ln_b is NULL");
+    EXM_TRY(source_frame_limits(nrow, nm));
+    for (int m = 1; m < nm; ++m)
+        EXM_CHECK(Mdet[m] > Mdet[m - 1], EXM_ERR_ARG, "detector-frame masses must ascend (template %d)", m);
+    // device copies: lamN, yN, M, yq, m1, z, w | surf | out | ln_b (slot 8)
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t n_in = 2 * (size_t)nrow * nm + nm + nz + nm1 + nz;
+    const size_t n_grid = (size_t)nm1 * nz;
+    const size_t o_w = al(8 * n_in), o_surf = o_w + (w ? al(8 * n_grid) : 0), o_out = o_surf + al(8 * (size_t)nz * nm),
+                 o_lnb = o_out + al(8 * n_grid), total = o_lnb + 256;
+    void* buf = nullptr;
+    EXM_TRY(scratch(8, total, &buf));
+    char* b = (char*)buf;
+    double* d = (double*)b;
+    double *dL = d, *dY = dL + (size_t)nrow * nm, *dM = dY + (size_t)nrow * nm, *dyq = dM + nm, *dm1 = dyq + nz,
+           *dz = dm1 + nm1;
+    const hipStream_t s = g_ctx.stream;
+    EXM_HIP(hipMemcpyAsync(dL, lamN, 8 * (size_t)nrow * nm, hipMemcpyHostToDevice, s));
+    EXM_HIP(hipMemcpyAsync(dY, yN, 8 * (size_t)nrow * nm, hipMemcpyHostToDevice, s));
+    EXM_HIP(hipMemcpyAsync(dM, Mdet, 8 * (size_t)nm, hipMemcpyHostToDevice, s));
+    EXM_HIP(hipMemcpyAsync(dyq, yq, 8 * (size_t)nz, hipMemcpyHostToDevice, s));
+    EXM_HIP(hipMemcpyAsync(dm1, m1, 8 * (size_t)nm1, hipMemcpyHostToDevice, s));
+    EXM_HIP(hipMemcpyAsync(dz, z, 8 * (size_t)nz, hipMemcpyHostToDevice, s));
+    double* dw = w ? (double*)(b + o_w) : nullptr;
+    if (w) EXM_HIP(hipMemcpyAsync(dw, w, 8 * n_grid, hipMemcpyHostToDevice, s));
+    double* dout = (double*)(b + o_out);
+    double* dlnb = (double*)(b + o_lnb);
+    {
+        ProfileScope prof("source_frame");
+        EXM_TRY(launch_source_frame(dL, dY, nrow, nm, dM, dyq, dm1, nm1, dz, nz, dw, (double*)(b + o_surf), dout, dlnb));
+    }
+    if (out) EXM_HIP(hipMemcpyAsync(out, dout, 8 * n_grid, hipMemcpyDeviceToHost, s));
+    if (w) EXM_HIP(hipMemcpyAsync(ln_b, dlnb, 8, hipMemcpyDeviceToHost, s));
+    EXM_HIP(hipStreamSynchronize(s));
+    return EXM_OK;
+}
+
 int exm_copy_calibration(int64_t bytes, double* ms) {
     LOCK;
     EXM_TRY(ensure_init());

@@ -169,7 +169,7 @@ struct Context {
     hipStream_t launch_stream = nullptr;  // when set, gather-side launches go here (a batch's own stream)
     std::recursive_mutex mu;
     // grow-only scratch buffers
-    static constexpr int NSCRATCH = 8;
+    static constexpr int NSCRATCH = 9;
     void* scratch[NSCRATCH] = {};
     size_t scratch_size[NSCRATCH] = {};
     std::unordered_map<uint64_t, hipfftHandle> fft_plans;  // key: (n_time << 20) | batch
@@ -230,6 +230,10 @@ int launch_lse(const JobSet& js, int ntau, const double* om, const double* os, c
                double* d_out);
 int launch_fill(float* dst, int64_t n, float v);
 int launch_copy_calibration(float4* dst, const float4* src, int64_t n);
+int launch_source_frame(const double* lamN, const double* yN, int nrow, int nm, const double* M, const double* yq,
+                        const double* m1, int nm1, const double* z, int nz, const double* w, double* surf,
+                        double* out, double* ln_b);
+int source_frame_limits(int nrow, int nm);
 int launch_taylorf2(const double* d_M, const double* d_q, const double* d_chi, int n_tmpl, int n_freq, int lo,
                     double T, double d0_mpc, const double* d_sig2H, float2* d_W, double* d_u);
 int launch_fill(double* dst, int64_t n, double v);

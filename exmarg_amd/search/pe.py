@@ -12,7 +12,7 @@ distance-prior weights (float64) and averages over the bank.
 
 import numpy as np
 
-from .. import cosmology
+from .. import _hip, cosmology
 from ..priors import distance_density
 from . import gather as _gather, store as _store
 from .bank import check_device
@@ -235,29 +235,6 @@ def pe_intrinsic(segments, bank, store, rows=None, prior="volume", ymin=0.1267, 
     return dict(M=bank.M, q=bank.q, chi=bank.chi, u=bank.u, ln_lambda=out, ln_b=ln_b, rows=seg["rows"])
 
 
-def _to_source_frame(lamN, yN, Mdet, yq, m1, z):
-    """node values (n_node + 1, n_template) -> ln Lambda on (m1, z) (src/exmarg/search/pe.py:87-132):
-    linear in y per template between its finite points, then linear in M_det = 2 m1 (1 + z) across the
-    templates finite at that z; -inf outside the points"""
-    nm = lamN.shape[1]
-    surf = np.full((yq.size, nm), -np.inf)
-    ok = np.isfinite(lamN) & (yN > 0)
-    for m in np.nonzero(ok.sum(axis=0) >= 2)[0]:
-        ys, ls = yN[ok[:, m], m], lamN[ok[:, m], m]
-        o = np.argsort(ys, kind="stable")
-        ys, ls = ys[o], ls[o]
-        inside = (yq >= ys[0]) & (yq <= ys[-1])
-        surf[inside, m] = np.interp(yq[inside], ys, ls)
-    out = np.full((m1.size, z.size), -np.inf)
-    good = np.isfinite(surf)
-    for iz in np.nonzero(good.sum(axis=1) >= 2)[0]:
-        Mg, Lg = Mdet[good[iz]], surf[iz, good[iz]]
-        Md = 2.0 * m1 * (1.0 + z[iz])
-        inb = (Md >= Mg[0]) & (Md <= Mg[-1])
-        out[inb, iz] = np.interp(Md[inb], Mg, Lg)
-    return out
-
-
 def pe_redshift(segments, bank, store, rows=None, chunk=8, zmax=8.0, nz=1600, m1_range=(3.0, 150.0), nm1=1600,
                 ymin=0.1267, nodes_per_launch=6, block=256, mdet_range=(10.0, 500.0), resident=False, n_loud=1024,
                 n_random=1024, seed=0):
@@ -275,18 +252,49 @@ def pe_redshift(segments, bank, store, rows=None, chunk=8, zmax=8.0, nz=1600, m1
     m1 = np.geomspace(m1_range[0], m1_range[1], int(nm1))
     dl = _dl_of_z(z)
     yq = D0 / dl
+    w = redshift_prior_weights(m1, z, dl, mdet_range)
     an = _Analysis(bank, store, ymin, resident).start()
     out = np.full((seg["rows"].size, m1.size, z.size), -np.inf)
+    ln_b = np.empty(seg["rows"].size)
     try:
         for j in range(seg["rows"].size):
             lamN = an.node_values(seg, j, n_loud, n_random, seed)
-            out[j] = _to_source_frame(lamN, an.geometry[2], bank.M, yq, m1, z)
+            ln_b[j] = source_frame(lamN, an.geometry[2], bank.M, yq, m1, z, w, out=out[j])
     finally:
         an.close()
+    return dict(m1=m1, z=z, ln_lambda=out, ln_b=ln_b, rows=seg["rows"])
+
+
+def redshift_prior_weights(m1, z, dl, mdet_range):
+    """normalized weights of the pe_redshift prior on the (m1, z) cells: uniform in M_det = 2 m1 (1 + z)
+    on mdet_range and p(z) ~ (1 + z) D_L^2 dD_L/dz (src/exmarg/search/pe.py:250-255)"""
     lnp = np.log(np.maximum((1.0 + z) * dl ** 2 * np.gradient(dl, z), 1e-300))
     Md = 2.0 * m1[:, None] * (1.0 + z[None, :])
     inside = (Md >= mdet_range[0]) & (Md <= mdet_range[1])
     w = np.where(inside, np.exp(lnp)[None, :], 0.0) * np.gradient(m1)[:, None] * np.gradient(z)[None, :]
-    w /= w.sum()
-    ln_b = np.array([_log_mean(L, w) for L in out])
-    return dict(m1=m1, z=z, ln_lambda=out, ln_b=ln_b, rows=seg["rows"])
+    return w / w.sum()
+
+
+def source_frame(lamN, yN, Mdet, yq, m1, z, w=None, out=None):
+    """ln Lambda on the (m1, z) grid from the node values of one segment, on the device
+    (exm_source_frame: src/exmarg/search/pe.py:87-142).  Writes the grid into out (nm1, nz) when given
+    and returns ln B = ln sum w e^Lambda when w is given (else None)."""
+    lamN = np.ascontiguousarray(lamN, np.float64)
+    yN = np.ascontiguousarray(yN, np.float64)
+    nrow, nm = lamN.shape
+    if yN.shape != (nrow, nm):
+        raise ValueError("yN has shape %s, expected %s" % (yN.shape, (nrow, nm)))
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (Mdet, yq, m1, z)]
+    Md, yq_, m1_, z_ = arrs
+    if Md.shape != (nm,) or yq_.shape != z_.shape:
+        raise ValueError("Mdet must have one mass per template and yq one value per z")
+    wv = None if w is None else np.ascontiguousarray(w, np.float64)
+    if wv is not None and wv.shape != (m1_.size, z_.size):
+        raise ValueError("w has shape %s, expected %s" % (wv.shape, (m1_.size, z_.size)))
+    if out is not None and (out.shape != (m1_.size, z_.size) or out.dtype != np.float64 or not out.flags.c_contiguous):
+        raise ValueError("out must be a C-contiguous float64 (nm1, nz) array")
+    ln_b = np.zeros(1)
+    _hip.call("exm_source_frame", _hip.ptr(lamN), _hip.ptr(yN), nrow, nm, _hip.ptr(Md), _hip.ptr(yq_), _hip.ptr(m1_),
+              m1_.size, _hip.ptr(z_), z_.size, _hip.ptr(wv) if wv is not None else None,
+              _hip.ptr(out) if out is not None else None, _hip.ptr(ln_b))
+    return float(ln_b[0]) if wv is not None else None

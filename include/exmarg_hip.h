@@ -192,6 +192,16 @@ int exm_set_gather_piece(int n);
  * src/exmarg/search/gather.py:143-306).  Applies to the band kernel only. */
 int exm_set_gather_precision(int p);
 
+/* ln Lambda of one segment on the source-frame grid (m1, z) from its node values, and its prior
+ * average: lamN and yN (nrow, nm) float64 (rows: the nodes and the boundary row, nrow <= 64), Mdet
+ * (nm) ascending detector-frame masses, yq (nz) = D0 / D_L(z), m1 (nm1), z (nz); out (nm1, nz)
+ * (may be NULL: not copied back); w (nm1, nz) normalized prior weights or NULL; ln_b = ln sum
+ * w e^out over the finite cells (written when w is given).  float64 throughout.  Replaces
+ * src/exmarg/search/pe.py:87-132 (_to_source_frame) and :135-142 (_log_mean) inside pe_redshift
+ * (pe.py:236-257). */
+int exm_source_frame(const double* lamN, const double* yN, int nrow, int nm, const double* Mdet, const double* yq,
+                     const double* m1, int nm1, const double* z, int nz, const double* w, double* out, double* ln_b);
+
 /* ---------------------------------------------------------------- measurement
  * No reference counterpart: kernel timing with HIP events recorded on the library's stream around
  * every launch of a named stage ("filter", "select", "gather"), for the benchmark. */

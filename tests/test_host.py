@@ -75,21 +75,6 @@ def test_luminosity_distance_golden_pairs(oracle):
     np.testing.assert_allclose(cosmology.luminosity_distance(zz), oracle.dl_of_z(zz), rtol=1e-12)
 
 
-def test_to_source_frame_matches_oracle(oracle):
-    rng = np.random.default_rng(3)
-    nn, nm = 9, 14
-    yN = np.where(rng.uniform(size=(nn, nm)) < 0.8, rng.uniform(0.1, 3.0, (nn, nm)), 0.0)
-    lamN = np.where(rng.uniform(size=(nn, nm)) < 0.9, rng.standard_normal((nn, nm)) * 3, -np.inf)
-    M = np.sort(rng.uniform(20, 120, nm))
-    z = np.linspace(0.02, 2.0, 40)
-    yq = pe.D0 / cosmology.luminosity_distance(z)
-    m1 = np.geomspace(3.0, 60.0, 33)
-    got = pe._to_source_frame(lamN, yN, M, yq, m1, z)
-    ref = oracle.to_source_frame(lamN, yN, M, yq, m1, z)
-    np.testing.assert_array_equal(np.isfinite(got), np.isfinite(ref))
-    np.testing.assert_allclose(got[np.isfinite(got)], ref[np.isfinite(ref)], rtol=1e-13)
-
-
 def test_log_mean_and_cells_match_oracle(oracle):
     rng = np.random.default_rng(5)
     lam = np.where(rng.uniform(size=50) < 0.9, rng.standard_normal(50) * 4, -np.inf)
