@@ -16,3 +16,4 @@ from .store import load_redshift_store, synthetic_store
 from .pe import pe_redshift, pe_intrinsic, distance_weights
 from .batch import pe_intrinsic_batch, SegmentBatch
 from .serve import Server
+from .stream import pe_redshift_stream, psd_estimate, noise_periodograms
