@@ -44,9 +44,10 @@ def parse():
     p.add_argument("--piece", type=int, default=0, help="walked times per piece of the band kernel (0 default)")
     p.add_argument("--precision", choices=["f64", "f32"], default="f64",
                    help="table interpolation of the band kernel (index arithmetic and log-sum-exp stay float64)")
-    p.add_argument("--mode", choices=["plain", "serve", "stream"], default="plain",
+    p.add_argument("--mode", choices=["plain", "serve", "stream", "batch"], default="plain",
                    help="plain: pe_intrinsic on one segment per step (the headline); serve: the request loop "
-                        "under Poisson arrivals; stream: pe_redshift over 8 s segments with an estimated spectrum")
+                        "under Poisson arrivals; stream: pe_redshift over 8 s segments with an estimated spectrum; "
+                        "batch: 256 segments of uneven length in batched launches (float32 form, autotuned)")
     p.add_argument("--segments", type=int, default=64, help="stream: segments in the timed run")
     p.add_argument("--group", type=int, default=32, help="stream: segments sharing one spectrum estimate")
     p.add_argument("--rate", type=float, default=50.0, help="serve: mean arrival rate (requests/s)")
@@ -308,6 +309,100 @@ def main_stream(args):
     print(json.dumps(out), flush=True)
 
 
+def batch_items(n, seed):
+    """the batched workload: n segments with T uniform over {4, 8, 16, 32, 64} s at 20-1024 Hz, half noise
+    only, half with one TaylorF2 injection (M_det, q, chi inside the bank, network SNR log-uniform in
+    [8, 30]); one 2000-template 20 x 10 x 10 bank per segment length"""
+    from exmarg_amd import synth
+    rng = np.random.default_rng(seed)
+    Ts = rng.choice([4.0, 8.0, 16.0, 32.0, 64.0], n)
+    items = []
+    for T in (4.0, 8.0, 16.0, 32.0, 64.0):
+        k = int((Ts == T).sum())
+        if k == 0:
+            continue
+        params = [(float(rng.uniform(12.0, 75.0)), float(rng.uniform(0.25, 1.0)), float(rng.uniform(-0.45, 0.45)))
+                  for _ in range(k)]
+        snr = [float(np.exp(rng.uniform(np.log(8.0), np.log(30.0)))) if rng.uniform() < 0.5 else 0.0
+               for _ in range(k)]
+        it_p = iter(params)
+        seg1, _, snr1 = synth.simulate(k, seed * 1000 + int(T), lambda r: next(it_p), lambda r: 1.0, T, 20.0, 1024.0)
+        it_p, it_y = iter(params), iter(np.array(snr) / snr1)
+        seg, _, _ = synth.simulate(k, seed * 1000 + int(T), lambda r: next(it_p), lambda r: float(next(it_y)), T,
+                                   20.0, 1024.0)
+        items.append((seg, T))
+    return items
+
+
+def main_batch(args):
+    """pe_intrinsic_batch over uneven segments: one JSON line with segments/s, the plain drop-in's rate on
+    the same segments and the float32 form's ln B error against the float64 form"""
+    from exmarg_amd import _hip, tune
+    from exmarg_amd.search import SegmentBatch, mass_bank, pe_intrinsic, pe_intrinsic_batch, store as stm
+    from exmarg_amd.search import pe as _pe
+    from exmarg_amd.search.segments import read_segments
+    from exmarg_amd import synth
+    _hip.init(0)
+    info = _hip.device_info()
+    st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
+    raw = batch_items(args.segments, args.seed)
+    rows = synth.grid_bank(shape=(20, 10, 10))
+    items = [(seg, mass_bank(rows, seg, "hip")) for seg, T in raw]
+    kw = dict(n_loud=args.n_loud, n_random=args.n_random, seed=0)
+    prec = 1 if args.precision == "f32" else 0
+    _hip.call("exm_set_gather_precision", prec)
+    # autotune the band kernel on a sample batch (8 segments of the 16 s bank, or the first bank)
+    sample_seg, sample_bank = next(((s, b) for s, b in items if s["T"] == 16.0), items[0])
+    rseg = read_segments(sample_seg)
+    an = _pe._Analysis(sample_bank, st, 0.1267, True)
+    an.start()
+    sb = SegmentBatch(st, args.n_loud, args.n_random, max_templates=8 * sample_bank.u.size)
+    for j in range(min(8, rseg["rows"].size)):
+        sb.add(an, {I: rseg["d_" + I][j] for I in "HL"}, {I: rseg["sig2_" + I] for I in "HL"}, j)
+    tuned = tune.tune_batch(sb)
+    sb.free()
+    # warm-up, then the timed batched run
+    pe_intrinsic_batch(items[:1], st, **kw)
+    _hip.profile(True)
+    t0 = time.perf_counter()
+    res = pe_intrinsic_batch(items, st, **kw)
+    wall = time.perf_counter() - t0
+    g_ms, g_n = _hip.profile_get("gather")
+    f_ms, _ = _hip.profile_get("filter")
+    _hip.profile(False)
+    nseg = sum(r["ln_b"].size for r in res)
+    # the float64 form of the same batch (the reference for the float32 error)
+    _hip.call("exm_set_gather_precision", 0)
+    res64 = pe_intrinsic_batch(items, st, **kw)
+    dlnb = float(max(np.max(np.abs(a["ln_b"] - b["ln_b"])) for a, b in zip(res, res64)))
+    dlnl = float(max(np.nanmax(np.abs(a["ln_lambda"] - b["ln_lambda"])) for a, b in zip(res, res64)))
+    # the plain drop-in (float64, one segment per call) on a sample of the same segments
+    tune.set_knobs(0, 0, 0)
+    plain_n, t_plain = 0, 0.0
+    for seg, bank in items:
+        for j in range(min(2, read_segments(seg)["rows"].size)):
+            t1 = time.perf_counter()
+            pe_intrinsic(seg, bank, st, rows=[j], resident=True, **kw)
+            t_plain += time.perf_counter() - t1
+            plain_n += 1
+    plain = plain_n / t_plain
+    out = {
+        "metric": "batched: segments/s over uneven segments (float32 gather, float64 log-sum-exp)",
+        "value": nseg / wall, "unit": "segments/s", "n_gpus": 1, "higher_is_better": True,
+        "segments": nseg, "wall_s": wall, "gather_launches": g_n, "gather_ms_total": g_ms, "filter_ms_total": f_ms,
+        "plain_dropin_segments_per_s": plain, "plain_sample_segments": plain_n, "speedup_vs_plain": nseg / wall / plain,
+        "float32_vs_float64": {"max_abs_dln_b": dlnb, "max_abs_dln_lambda": dlnl},
+        "autotune": tuned,
+        "dtype": "f32 interpolation, f64 index arithmetic and log-sum-exp" if prec else "f64", "data": "synthetic",
+        "config": {"workload": "%d segments, T uniform over {4,8,16,32,64} s at 20-1024 Hz, half noise, half one "
+                               "TaylorF2 injection (SNR 8-30); 2000-template 20x10x10 bank per length; 53 "
+                               "HBM-resident table nodes; pe_intrinsic 'volume', n_loud=n_random=%d"
+                               % (nseg, args.n_loud),
+                   "device": info["name"], "num_cu": info["num_cu"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -318,6 +413,12 @@ def main():
         return main_serve(args)
     if args.mode == "stream":
         return main_stream(args)
+    if args.mode == "batch":
+        if args.precision == "f64" and "--precision" not in " ".join(sys.argv):
+            args.precision = "f32"
+        if "--segments" not in " ".join(sys.argv):
+            args.segments = 256
+        return main_batch(args)
     from exmarg_amd import _hip
     from exmarg_amd.search import mass_bank, pe_intrinsic, store as stm
     from exmarg_amd.search.pe import _node_geometry

@@ -108,7 +108,8 @@ int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const doub
         int64_t b = a;
         while (b < njobs && st[b] == st[a]) ++b;
         const int64_t n = b - a;
-        const int64_t pieces = (n + TILE_JOBS - 1) / TILE_JOBS;
+        const int64_t tile = g_ctx.tile ? g_ctx.tile : TILE_JOBS;
+        const int64_t pieces = (n + tile - 1) / tile;
         for (int64_t q = 0; q < pieces; ++q) {
             const int64_t lo = a + n * q / pieces, hi = a + n * (q + 1) / pieces;
             tiles.push_back((int32_t)lo);
@@ -939,6 +940,22 @@ int exm_set_gather_piece(int n) {
     LOCK;
     EXM_CHECK(n == 0 || n == 4 || n == 8 || n == 16, EXM_ERR_ARG, "piece length must be 0 (default), 4, 8 or 16");
     g_ctx.piece = n;
+    return EXM_OK;
+}
+
+int exm_set_gather_tile(int n) {
+    LOCK;
+    EXM_CHECK(n == 0 || n == 256 || n == 512 || n == TILE_JOBS, EXM_ERR_ARG,
+              "reads per workgroup must be 0 (default), 256, 512 or %d", TILE_JOBS);
+    g_ctx.tile = n;
+    return EXM_OK;
+}
+
+int exm_get_gather_tuning(int* piece, int* tile, int* precision) {
+    LOCK;
+    if (piece) *piece = g_ctx.piece;
+    if (tile) *tile = g_ctx.tile;
+    if (precision) *precision = g_ctx.f32_interp;
     return EXM_OK;
 }
 

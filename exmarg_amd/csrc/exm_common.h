@@ -175,6 +175,7 @@ struct Context {
     std::unordered_map<uint64_t, hipfftHandle> fft_plans;  // key: (n_time << 20) | batch
     int piece = 0;                                          // walked times per piece (0: default)
     int f32_interp = 0;                                     // band kernel interpolates in float32
+    int tile = 0;                                           // reads per band workgroup (0: TILE_JOBS)
     int num_cu = 0;
     int64_t total_mem = 0;
     char name[256] = {0};

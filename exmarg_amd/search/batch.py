@@ -21,9 +21,11 @@ from .segments import read_segments
 RECORD_BYTES = 8          # bytes of walk record per template, lag and walked time
 
 
-def template_bytes(ntau, n_walk):
-    """device bytes one template of one segment holds in a batch"""
-    return ntau * n_walk * RECORD_BYTES + n_walk * 4 + 12
+def template_bytes(ntau, n_walk, reads=54, nb=200):
+    """device bytes one template of one segment needs in a batch: its walk records, and the launch's
+    work for its reads (at most one per node plus a boundary point): (max, sum) per lag, row bounds
+    and the read list"""
+    return ntau * n_walk * RECORD_BYTES + n_walk * 4 + 12 + reads * (16 * ntau + 2 * (nb + 1) + 32)
 
 
 class SegmentBatch:
@@ -41,7 +43,7 @@ class SegmentBatch:
         self.nw = self.K + self.S
         self.ntau = int(np.asarray(store.tau_idx).size)
         self.lag_lo, self.lag_hi = lag_range(store.tau_idx)
-        per = template_bytes(self.ntau, self.nw)
+        per = template_bytes(self.ntau, self.nw, np.asarray(store.s).size + 1, store.table_shape()[2])
         if max_templates is None:
             free, _ = _hip.mem_info()
             budget = free - reserve_bytes
@@ -133,6 +135,9 @@ def pe_intrinsic_batch(items, store, prior="volume", ymin=0.1267, n_loud=1024, n
         lnW = _pe._bank_weights(an.geometry[2], prior, ymin, bank)
         an.start()
         prepared.append((seg, bank, an, lnW))
+    # work memory for admitting the largest segment first, so that the batch is sized on what is left
+    big = max(prepared, key=lambda p: p[1].u.size * p[1].n_time)[1]
+    _hip.call("exm_reserve", big.u.size, big.n_time, big.nu.size, int(n_loud) + int(n_random))
     batch = SegmentBatch(store, n_loud, n_random, max_templates=max_templates, max_bytes=max_bytes)
     outs = [np.full((seg["rows"].size, bank.u.size), -np.inf) for seg, bank, _, _ in prepared]
     pending = []

@@ -73,7 +73,8 @@ class Server:
         if max_request is not None:
             nm, nt, nf = (int(v) for v in max_request)
             _hip.call("exm_reserve", nm, nt, nf, self.K + self.S)
-        per = template_bytes(np.asarray(store.tau_idx).size, self.K + self.S)
+        per = template_bytes(np.asarray(store.tau_idx).size, self.K + self.S, np.asarray(store.s).size + 1,
+                             store.table_shape()[2])
         if slot_templates is None:
             free, _ = _hip.mem_info()
             slot_templates = int(min((free - reserve_bytes) // (2 * per), self.SLOT_TEMPLATES))

@@ -1,0 +1,107 @@
+"""
+Autotuner of the band gather kernel.
+
+The band kernel has two shape knobs: the piece length (walked times one lane reads in a row, 4, 8 or
+16) and the tile (reads per workgroup, 256, 512 or 1024).  The best pair depends on the walk, the
+table geometry and how the reads spread over the beta rows, so it is measured, not modelled:
+tune_batch times every candidate on the caller's own filled batch (the gather is rerun, nothing is
+consumed) with the kernel's own HIP events, keeps the fastest and records it in a JSON cache keyed by
+device, table geometry, walk length and precision; apply() restores a recorded choice.
+
+    batch = SegmentBatch(store, ...); batch.add(...)     # a sample of the workload
+    best = tune.tune_batch(batch)                        # sets the knobs
+    tune.apply(store, n_walk)                            # later runs, from the cache
+
+No reference counterpart (the reference's CUDA kernel has one fixed block size,
+src/exmarg/search/gather.py:313-337).
+"""
+
+import ctypes
+import json
+import os
+
+from . import _hip
+
+PIECES = (4, 8, 16)
+TILES = (256, 512, 1024)
+
+
+def cache_path():
+    return os.environ.get("EXMARG_TUNE_CACHE") or os.path.join(os.path.expanduser("~"), ".cache", "exmarg_amd",
+                                                               "tuning.json")
+
+
+def current():
+    """(piece, tile, precision) in force (0 for the defaults)"""
+    v = [ctypes.c_int() for _ in range(3)]
+    _hip.call("exm_get_gather_tuning", *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
+
+
+def set_knobs(piece=0, tile=0, precision=None):
+    _hip.call("exm_set_gather_piece", int(piece))
+    _hip.call("exm_set_gather_tile", int(tile))
+    if precision is not None:
+        _hip.call("exm_set_gather_precision", int(precision))
+
+
+def key(store, n_walk, precision):
+    name = _hip.device_info()["name"]
+    shp = store.table_shape()
+    return "%s|%s|walk%d|%s" % (name, "x".join(str(int(v)) for v in shp), int(n_walk), "f32" if precision else "f64")
+
+
+def _load():
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(table):
+    path = cache_path()
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def tune_batch(batch, repeats=2, pieces=PIECES, tiles=TILES):
+    """time the gather of the filled batch for every (piece, tile), keep the fastest (knobs set) and
+    record it; returns dict(piece, tile, ms, table={"piece,tile": ms}).  Uses (and resets) the
+    library's stage timers."""
+    if not batch.entries:
+        raise ValueError("tune a batch that holds segments")
+    _, _, precision = current()
+    _hip.profile(True)
+    times = {}
+    try:
+        for p in pieces:
+            for t in tiles:
+                set_knobs(p, t)
+                batch.run()                      # warm (code objects, work memory)
+                ms0, n0 = _hip.profile_get("gather")
+                for _ in range(repeats):
+                    batch.run()
+                ms1, n1 = _hip.profile_get("gather")
+                times[(p, t)] = (ms1 - ms0) / max(n1 - n0, 1)
+    finally:
+        _hip.profile(False)
+    (p, t), ms = min(times.items(), key=lambda kv: kv[1])
+    set_knobs(p, t)
+    table = _load()
+    table[key(batch.store, batch.nw, precision)] = dict(piece=p, tile=t, ms=ms)
+    _save(table)
+    return dict(piece=p, tile=t, ms=ms, table={"%d,%d" % k: v for k, v in times.items()})
+
+
+def apply(store, n_walk):
+    """set the recorded knobs for this store geometry and walk (precision in force); returns them or None"""
+    _, _, precision = current()
+    rec = _load().get(key(store, n_walk, precision))
+    if rec is None:
+        return None
+    set_knobs(rec["piece"], rec["tile"])
+    return rec

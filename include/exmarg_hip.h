@@ -192,6 +192,14 @@ int exm_set_gather_piece(int n);
  * src/exmarg/search/gather.py:143-306).  Applies to the band kernel only. */
 int exm_set_gather_precision(int p);
 
+/* Reads per workgroup of the band kernel: 256, 512 or 1024 (0 restores the default, 1024).  Smaller
+ * tiles give more workgroups, each sweeping fewer table rows of its reads' band.  A tuning knob (the
+ * autotuner exmarg_amd.tune chooses it); no reference counterpart. */
+int exm_set_gather_tile(int n);
+
+/* The current tuning: piece length, tile and precision (0 where the default applies). */
+int exm_get_gather_tuning(int* piece, int* tile, int* precision);
+
 /* ln Lambda of one segment on the source-frame grid (m1, z) from its node values, and its prior
  * average: lamN and yN (nrow, nm) float64 (rows: the nodes and the boundary row, nrow <= 64), Mdet
  * (nm) ascending detector-frame masses, yq (nz) = D0 / D_L(z), m1 (nm1), z (nz); out (nm1, nz)
