@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cmath>
+#include <map>
 
 #include "exm_common.h"
 
@@ -32,9 +33,42 @@ int ensure_init() {
     return EXM_OK;
 }
 
-static int dev_alloc(void** p, size_t bytes) {
+// Device memory: blocks are rounded up (2 MiB above 16 MiB) and freed blocks are kept in a pool by
+// size, up to a quarter of the device, so that the per-call objects of repeated analyses (trajectories,
+// times, read lists) reuse memory instead of mapping and unmapping it every call.  A failed hipMalloc
+// releases the pool and tries once more.
+static std::multimap<size_t, void*> g_pool;
+static size_t g_pool_bytes = 0;
+
+static size_t block_size(size_t bytes) {
     if (bytes == 0) bytes = 256;
+    if (bytes > ((size_t)16 << 20)) return (bytes + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
+    return (bytes + 255) & ~(size_t)255;
+}
+
+static void pool_release() {
+    for (auto& kv : g_pool) (void)hipFree(kv.second);
+    g_pool.clear();
+    g_pool_bytes = 0;
+}
+
+static int dev_alloc(void** p, size_t bytes) {
+    bytes = block_size(bytes);
+    auto it = g_pool.find(bytes);
+    if (it != g_pool.end()) {
+        *p = it->second;
+        g_pool.erase(it);
+        g_pool_bytes -= bytes;
+        g_mem_in_use += (int64_t)bytes;
+        return EXM_OK;
+    }
     hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess && !g_pool.empty()) {
+        (void)hipGetLastError();
+        (void)hipDeviceSynchronize();
+        pool_release();
+        e = hipMalloc(p, bytes);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
@@ -46,9 +80,16 @@ static int dev_alloc(void** p, size_t bytes) {
 }
 
 static void dev_free(void* p, size_t bytes) {
-    if (p) {
+    if (!p) return;
+    bytes = block_size(bytes);
+    g_mem_in_use -= (int64_t)bytes;
+    // a pooled block may be handed out at once: the work that used it must have finished
+    (void)hipDeviceSynchronize();
+    if (g_ctx.total_mem > 0 && g_pool_bytes + bytes <= (size_t)g_ctx.total_mem / 4) {
+        g_pool.emplace(bytes, p);
+        g_pool_bytes += bytes;
+    } else {
         (void)hipFree(p);
-        g_mem_in_use -= (int64_t)(bytes ? bytes : 256);
     }
 }
 
@@ -278,7 +319,7 @@ int exm_mem_info(int64_t* free_bytes, int64_t* total_bytes) {
     EXM_TRY(ensure_init());
     size_t f = 0, t = 0;
     EXM_HIP(hipMemGetInfo(&f, &t));
-    if (free_bytes) *free_bytes = (int64_t)f;
+    if (free_bytes) *free_bytes = (int64_t)(f + g_pool_bytes);   // pooled blocks are free for this library
     if (total_bytes) *total_bytes = (int64_t)t;
     return EXM_OK;
 }
