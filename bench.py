@@ -64,6 +64,23 @@ def workload(args):
     return seg, rows, float(snr[0])
 
 
+def measured_traffic(kernel_ms):
+    """HBM bytes per launch of the gather kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    record (profiles/gather_traffic.json), used only when it was taken on this workload with the
+    kernel source now in the tree"""
+    import hashlib
+    try:
+        with open(os.path.join(ROOT, "profiles", "gather_traffic.json")) as f:
+            rec = json.load(f)
+        with open(os.path.join(ROOT, "exmarg_amd", "csrc", "k_band.hip"), "rb") as f:
+            sha = hashlib.sha256(f.read()).hexdigest()[:16]
+    except (OSError, ValueError):
+        return None
+    if rec.get("source_sha256_16") != sha or rec.get("workload") != "16s_2000tmpl_53nodes_walk2048":
+        return None
+    return rec
+
+
 def segment_bytes(F, n_tmpl, n_time, n_nodes):
     """algorithmic bytes of one segment: data, template factors, filter output written and read once,
     every table read once"""
@@ -489,6 +506,11 @@ def main():
         "path_hbm_frac": seg_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS,
         "ln_b": float(res["ln_b"][0]),
     }
+    trf = measured_traffic(1e3 * g_s)
+    if trf is not None and args.grid == "20,10,10" and args.nodes == 53 and walk == 2048 and args.precision == "f64":
+        out["roofline"]["traffic"] = trf["traffic_bytes"]
+        out["roofline"]["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this kernel on this "
+                                             "workload, scaled by a 1 GiB copy (profiles/gather_traffic.json)")
     if f64_check is not None:
         out["float32_form_vs_float64"] = f64_check
     if not args.no_cpu:
