@@ -376,7 +376,7 @@ def main_batch(args):
     sb = SegmentBatch(st, args.n_loud, args.n_random, max_templates=8 * sample_bank.u.size)
     for j in range(min(8, rseg["rows"].size)):
         sb.add(an, {I: rseg["d_" + I][j] for I in "HL"}, {I: rseg["sig2_" + I] for I in "HL"}, j)
-    tuned = tune.tune_batch(sb)
+    tuned = tune.tune_batch(sb, repeats=1)
     sb.free()
     # warm-up, then the timed batched run
     pe_intrinsic_batch(items[:1], st, **kw)
@@ -394,7 +394,7 @@ def main_batch(args):
     dlnb = float(max(np.max(np.abs(a["ln_b"] - b["ln_b"])) for a, b in zip(res, res64)))
     dlnl = float(max(np.nanmax(np.abs(a["ln_lambda"] - b["ln_lambda"])) for a, b in zip(res, res64)))
     # the plain drop-in (float64, one segment per call) on a sample of the same segments
-    tune.set_knobs(0, 0, 0)
+    tune.set_knobs(0, 0, 0, form=0)
     plain_n, t_plain = 0, 0.0
     for seg, bank in items:
         for j in range(min(2, read_segments(seg)["rows"].size)):

@@ -57,7 +57,7 @@ SIGNATURES = {
     "exm_set_gather_piece": [I32],
     "exm_set_gather_precision": [I32],
     "exm_set_gather_tile": [I32],
-    "exm_get_gather_tuning": [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+    "exm_get_gather_tuning": [ctypes.POINTER(ctypes.c_int)] * 4,
     "exm_profile_enable": [I32],
     "exm_profile_reset": [],
     "exm_profile_get": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)],

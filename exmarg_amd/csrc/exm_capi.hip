@@ -972,7 +972,8 @@ int exm_batch_info(exm_handle hbatch, int64_t* templates_used, int64_t* reads, i
 
 int exm_set_gather_kernel(int which) {
     LOCK;
-    EXM_CHECK(which >= 0 && which <= 2, EXM_ERR_ARG, "gather kernel must be 0 (auto), 1 (reference) or 2 (band)");
+    EXM_CHECK(which >= 0 && which <= 3, EXM_ERR_ARG,
+              "gather kernel must be 0 (auto), 1 (reference), 2 (band, piece form) or 3 (band, wave form)");
     g_gather_mode = which;
     return EXM_OK;
 }
@@ -992,8 +993,9 @@ int exm_set_gather_tile(int n) {
     return EXM_OK;
 }
 
-int exm_get_gather_tuning(int* piece, int* tile, int* precision) {
+int exm_get_gather_tuning(int* kernel, int* piece, int* tile, int* precision) {
     LOCK;
+    if (kernel) *kernel = g_gather_mode;
     if (piece) *piece = g_ctx.piece;
     if (tile) *tile = g_ctx.tile;
     if (precision) *precision = g_ctx.f32_interp;

@@ -176,6 +176,40 @@ __device__ __forceinline__ void row_to_lds(const float* __restrict__ src, float*
 __device__ __forceinline__ double vfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float vfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// Geometry of one workgroup's table slice, shared by the band kernels.
+struct BandGeom {
+    double rdb, offd, rdd, qbd, xbmax, xdmax;
+    int nd;
+};
+
+// One lookup of App. B (the reference's gather.py:45-98) with the walk record (aL, e) of a walked time
+// whose beta row is b (R0 row b, R1 row b + 1 in LDS): float64 index and fraction arithmetic, the
+// bilinear (beta, Delta) interpolation at both phase bins and the phase interpolation in V.
+template <typename V>
+__device__ __forceinline__ double band_lookup(const float* R0, const float* R1, const BandGeom& g, double bd,
+                                              double offb, double cj, double lw_rand, float aLf, uint2 e) {
+    const double aL = (double)aLf;
+    const uint32_t code = e.x;
+    // beta coordinate (clamped) and its fraction in row b
+    const double xb = fmin(fmax(fma(aL, g.rdb, offb), 0.0), g.xbmax);
+    const double fb = xb - bd;
+    // Delta coordinate: ln q_L moves with the beta clamp, ln q_H stays
+    const double xd = fmin(fmax(fma(-g.qbd, xb, fma((double)__uint_as_float(e.y), g.rdd, cj)), 0.0), g.xdmax);
+    int d0 = (int)xd;
+    d0 = min(max(d0, 0), g.nd - 2);
+    const double fd = xd - (double)d0;
+    const V fc = (V)(code & FC_MASK) * (V)(1.0 / FC_SCALE);
+    const int o0 = (int)__umul24((code >> BIN_SHIFT) & 0x3F, (uint32_t)g.nd) + d0;
+    const int o1 = (code & DOWN) ? o0 - g.nd : o0 + g.nd;
+    const V fdv = (V)fd, fbv = (V)fb;
+    const V t00 = R0[o0], t01 = R0[o0 + 1], t10 = R1[o0], t11 = R1[o0 + 1];
+    const V u00 = R0[o1], u01 = R0[o1 + 1], u10 = R1[o1], u11 = R1[o1 + 1];
+    const V v0a = vfma(fdv, t01 - t00, t00), v0b = vfma(fdv, t11 - t10, t10);
+    const V v1a = vfma(fdv, u01 - u00, u00), v1b = vfma(fdv, u11 - u10, u10);
+    const V v0 = vfma(fbv, v0b - v0a, v0a), v1 = vfma(fbv, v1b - v1a, v1a);
+    return (double)vfma(fc, v1 - v0, v0) + ((code & DRAWN) ? lw_rand : 0.0);
+}
+
 // one workgroup per (tile of reads of one table, lag); PN walked times per piece; V the type of the
 // table interpolation (double, or float for the float32 form)
 template <int PN, typename V>
@@ -199,8 +233,8 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     const int j0 = tiles[2 * blockIdx.x], nj = tiles[2 * blockIdx.x + 1];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const float* Tr = T + (int64_t)jtab[j0] * tstride + (int64_t)r * nb * rowlen;
-    const double xbmax = nb - 1 - 1e-6, xdmax = nd - 1 - 1e-6;
     const double qbd = rdd / rdb;     // Delta coordinate per beta coordinate
+    const BandGeom geo{rdb, offd, rdd, qbd, nb - 1 - 1e-6, nd - 1 - 1e-6, nd};
 
     int my_first = INT_MAX, my_last = INT_MIN;
     if (tid < nj) {
@@ -302,29 +336,8 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
                     }
                     // value of lookup k of the chunk (-1e300 past the piece's end)
                     auto lookup = [&](int k) -> double {
-                        const double aL = (double)aLv[k];
-                        const uint2 e = ev[k];
-                        const uint32_t code = e.x;
-                        // beta coordinate (clamped) and its fraction in row b
-                        const double xb = fmin(fmax(fma(aL, rdb, offb), 0.0), xbmax);
-                        const double fb = xb - bd;
-                        // Delta coordinate: ln q_L moves with the beta clamp, ln q_H stays
-                        const double xd = fmin(fmax(fma(-qbd, xb, fma((double)__uint_as_float(e.y), rdd, cj)), 0.0),
-                                               xdmax);
-                        int d0 = (int)xd;
-                        d0 = min(max(d0, 0), nd - 2);
-                        const double fd = xd - (double)d0;
-                        const V fc = (V)(code & FC_MASK) * (V)(1.0 / FC_SCALE);
-                        const int o0 = (int)__umul24((code >> BIN_SHIFT) & 0x3F, (uint32_t)nd) + d0;
-                        const int o1 = (code & DOWN) ? o0 - nd : o0 + nd;
-                        const V fdv = (V)fd, fbv = (V)fb;
-                        const V t00 = R0[o0], t01 = R0[o0 + 1], t10 = R1[o0], t11 = R1[o0 + 1];
-                        const V u00 = R0[o1], u01 = R0[o1 + 1], u10 = R1[o1], u11 = R1[o1 + 1];
-                        const V v0a = vfma(fdv, t01 - t00, t00), v0b = vfma(fdv, t11 - t10, t10);
-                        const V v1a = vfma(fdv, u01 - u00, u00), v1b = vfma(fdv, u11 - u10, u10);
-                        const V v0 = vfma(fbv, v0b - v0a, v0a), v1 = vfma(fbv, v1b - v1a, v1a);
-                        return pa + c0 + k < pe ? (double)vfma(fc, v1 - v0, v0) + ((code & DRAWN) ? lw_rand : 0.0)
-                                   : -1e300;
+                        return pa + c0 + k < pe ? band_lookup<V>(R0, R1, geo, bd, offb, cj, lw_rand, aLv[k], ev[k])
+                                                : -1e300;
                     };
                     if constexpr (sizeof(V) == 4) {
                         // independent lookups (their LDS reads and interpolations overlap), then the
@@ -385,6 +398,190 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     }
 }
 
+// lane-shuffle helpers (wave64)
+__device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ int shfl_i(int v, int src) { return __shfl(v, src, 64); }
+
+// Combine (m2, s2) into the running (m, s) of a log-sum-exp: one exp of the non-positive difference.
+__device__ __forceinline__ void lse_merge(double& m, double& s, double m2, double s2) {
+    const double d = m2 - m;
+    const double e = (double)__expf(-(float)fabs(d));
+    s = d > 0.0 ? fma(s, e, s2) : fma(s2, e, s);
+    m = fmax(m, m2);
+}
+
+// The wave form of the band kernel: one workgroup per (tile of reads of one table, lag) and the same
+// three-row LDS ring as k_gather_piece, but each wave deals the pieces of its own 64 reads (lane l of
+// wave w owns read w + 16 l) in rounds of 64 without workgroup barriers: a lane finds its piece's read
+// by a binary search over the wave's piece offsets, takes the read's parameters from the owning lane
+// by shuffles, computes the piece, and a segmented reduction over lanes (shuffles) leaves each read's
+// (max, sum) of the round at its first lane, which the owning lane merges.  Only the ring rotation at
+// the end of a row synchronizes the workgroup, so the waves of a CU overlap each other's memory
+// latency.  Merges are float64; their order follows lane positions, so a read's result depends on its
+// neighbours in the workgroup only through float64 rounding.
+template <int PN, typename V>
+__global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
+    const uint2* __restrict__ stream, const float* __restrict__ wa, int nw, const float* __restrict__ lw_g,
+    const float* __restrict__ T, int64_t tstride, int nphif, int nb, int nd, int ntau,
+    const int32_t* __restrict__ jm, const int32_t* __restrict__ jtab, const double* __restrict__ joffb,
+    const int32_t* __restrict__ tiles, const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds,
+    double rdb, double offd, double rdd, double* __restrict__ om, double* __restrict__ os) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    constexpr int NWAVE = BAND_THREADS / 64;
+    const int rowlen = nphif * nd, slot = row_slot(rowlen);
+    float* rows = (float*)lds;                                              // 3 ring slots
+    int* band = (int*)(lds + piece_res_offset(rowlen));                     // band range of the tile
+    const int r = blockIdx.y;
+    const int j0 = tiles[2 * blockIdx.x], nj = tiles[2 * blockIdx.x + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int jl = wid + NWAVE * lane;                                      // this lane's read in the tile
+    const bool mine = jl < nj;
+    const float* Tr = T + (int64_t)jtab[j0] * tstride + (int64_t)r * nb * rowlen;
+    const double qbd = rdd / rdb;
+    const BandGeom geo{rdb, offd, rdd, qbd, nb - 1 - 1e-6, nd - 1 - 1e-6, nd};
+
+    // this lane's read: rows, template, offsets
+    int my_first = INT_MAX, my_last = INT_MIN, my_m = 0;
+    double my_offb = 0.0;
+    float my_lw = 0.0f;
+    if (mine) {
+        const int64_t job = (int64_t)j0 + jl;
+        my_first = first_last[2 * job];
+        my_last = first_last[2 * job + 1];
+        my_m = jm[job];
+        my_offb = joffb[job];
+        my_lw = lw_g[my_m];
+    }
+    double amx = -1e300, asum = 0.0;
+    if (tid == 0) {
+        band[0] = INT_MAX;
+        band[1] = INT_MIN;
+    }
+    __syncthreads();
+    if (mine) {
+        atomicMin(&band[0], my_first);
+        atomicMax(&band[1], my_last);
+    }
+    __syncthreads();
+    const int b_start = band[0], b_end = band[1];
+    const int b_top = min(b_end + 1, nb - 1);
+    for (int q = 0; q < 3 && b_start + q <= b_top; ++q)
+        row_to_lds(Tr + (int64_t)(b_start + q) * rowlen, rows + ((b_start + q) % 3) * slot, slot, wid, lane);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    for (int b = b_start; b <= b_end; ++b) {
+        if (b > b_start && b + 2 <= b_top)
+            row_to_lds(Tr + (int64_t)(b + 2) * rowlen, rows + ((b + 2) % 3) * slot, slot, wid, lane);
+        // this lane's read in row b: walk range and pieces; the wave's piece offsets
+        int lo = 0, hi = 0;
+        if (mine && b >= my_first && b <= my_last) {
+            const uint16_t* bj = bounds + ((int64_t)j0 + jl) * (nb + 1);
+            lo = bj[b - my_first];
+            hi = bj[b + 1 - my_first];
+        }
+        const int q = (hi - lo + PN - 1) / PN;
+        int x = q;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        const int qex = x - q;                    // first piece of this lane's read
+        const int Qw = __shfl(x, 63, 64);         // pieces of the wave in this row
+        const int sb = b % 3;
+        const float* R0 = rows + sb * slot;
+        const float* R1 = rows + (sb + 1 == 3 ? 0 : sb + 1) * slot;
+        const double bd = (double)b;
+        for (int base = 0; base < Qw; base += 64) {
+            const int p = base + lane;
+            const bool act = p < Qw;
+            // owner: the largest lane whose first piece is <= p (piece offsets ascend with the lane)
+            int o = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int c = o + step;
+                const int qc = shfl_i(qex, c & 63);
+                if (c < 64 && qc <= p) o = c;
+            }
+            const int lo_o = shfl_i(lo, o), hi_o = shfl_i(hi, o), qex_o = shfl_i(qex, o), m_o = shfl_i(my_m, o);
+            const double offb_o = shfl_d(my_offb, o);
+            const double lw_o = (double)__shfl(my_lw, o, 64);
+            double mx = -1e300, sm = 0.0;
+            if (act) {
+                const int pa = lo_o + (p - qex_o) * PN;
+                const int pe = min(pa + PN, hi_o);
+                const double cj = offd + offb_o * qbd;
+                const float* wam = wa + (int64_t)m_o * nw;
+                const uint2* smp = stream + ((int64_t)m_o * ntau + r) * nw;
+                constexpr int CH = PN < 8 ? PN : 8;
+#pragma unroll
+                for (int c0 = 0; c0 < PN; c0 += CH) {
+                    if (c0 > 0 && pa + c0 >= pe) break;
+                    float aLv[CH];
+                    uint2 ev[CH];
+#pragma unroll
+                    for (int k = 0; k < CH; ++k) {
+                        const int i = min(pa + c0 + k, pe - 1);
+                        aLv[k] = wam[i];
+                        ev[k] = smp[i];
+                    }
+                    if constexpr (sizeof(V) == 4) {
+                        double av[CH];
+#pragma unroll
+                        for (int k = 0; k < CH; ++k)
+                            av[k] = pa + c0 + k < pe ? band_lookup<V>(R0, R1, geo, bd, offb_o, cj, lw_o, aLv[k], ev[k])
+                                                     : -1e300;
+                        double cm = av[0];
+#pragma unroll
+                        for (int k = 1; k < CH; ++k) cm = fmax(cm, av[k]);
+                        double cs = 0.0;
+#pragma unroll
+                        for (int k = 0; k < CH; ++k) cs += (double)__expf((float)(av[k] - cm));
+                        if (c0 == 0) {
+                            mx = cm;
+                            sm = cs;
+                        } else {
+                            lse_merge(mx, sm, cm, cs);
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < CH; ++k) {
+                            const double a = pa + c0 + k < pe
+                                                 ? band_lookup<V>(R0, R1, geo, bd, offb_o, cj, lw_o, aLv[k], ev[k])
+                                                 : -1e300;
+                            const double dlt = a - mx;
+                            const double ex = (double)__expf(-(float)fabs(dlt));
+                            sm = dlt > 0.0 ? fma(sm, ex, 1.0) : sm + ex;
+                            mx = fmax(mx, a);
+                        }
+                    }
+                }
+            }
+            // segmented reduction towards the first lane of each read's run of pieces in this round
+            const int seg = act ? o : -1 - lane;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int src = (lane + d) & 63;
+                const double m2 = shfl_d(mx, src), s2 = shfl_d(sm, src);
+                const int g2 = shfl_i(seg, src);
+                if (lane + d < 64 && g2 == seg) lse_merge(mx, sm, m2, s2);
+            }
+            // the owning lane takes its read's result of this round from the run's first lane
+            const bool in_round = q > 0 && qex < base + 64 && qex + q > base;
+            const int head = in_round ? max(qex, base) - base : 0;
+            const double hm = shfl_d(mx, head), hs = shfl_d(sm, head);
+            if (in_round) lse_merge(amx, asum, hm, hs);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
+    if (mine) {
+        om[((int64_t)j0 + jl) * ntau + r] = amx;
+        os[((int64_t)j0 + jl) * ntau + r] = asum;
+    }
+}
+
 // walk records of one segment into wa_dst (n_tmpl, nw) and stream_dst (n_tmpl, ntau, nw)
 int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* stream_dst) {
     Context& c = ctx();
@@ -435,13 +632,23 @@ int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const 
     EXM_CHECK(piece_res_offset(st->nphif * st->nd) % 16 == 0, EXM_ERR_ARG, "piece results misaligned");
     EXM_CHECK(row_slot(st->nphif * st->nd) - st->nphif * st->nd <= TABLE_PAD_FLOATS, EXM_ERR_ARG,
               "row copy overruns the table padding");
-    const size_t lds = piece_lds_bytes(st->nphif * st->nd);
+    const size_t lds = gather_mode() != 2 ? piece_res_offset(st->nphif * st->nd) + 16 : piece_lds_bytes(st->nphif * st->nd);
     EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "three table rows and the piece results (%zu bytes) exceed the LDS", lds);
-    const int pn = c.piece ? c.piece : PIECE;
-    auto kern = c.f32_interp ? (pn == 4 ? k_gather_piece<4, float> : pn == 16 ? k_gather_piece<16, float>
-                                                                               : k_gather_piece<8, float>)
-                             : (pn == 4 ? k_gather_piece<4, double> : pn == 16 ? k_gather_piece<16, double>
-                                                                                : k_gather_piece<8, double>);
+    // the wave form unless the piece form is asked for; 4 walked times per piece in the wave form
+    const bool wave = gather_mode() != 2;
+    const int pn = c.piece ? c.piece : (wave ? 4 : PIECE);
+    using Kern = decltype(&k_gather_piece<8, double>);
+    Kern kern;
+    if (wave)
+        kern = c.f32_interp ? (pn == 4 ? k_gather_wave<4, float> : pn == 16 ? k_gather_wave<16, float>
+                                                                            : k_gather_wave<8, float>)
+                            : (pn == 4 ? k_gather_wave<4, double> : pn == 16 ? k_gather_wave<16, double>
+                                                                             : k_gather_wave<8, double>);
+    else
+        kern = c.f32_interp ? (pn == 4 ? k_gather_piece<4, float> : pn == 16 ? k_gather_piece<16, float>
+                                                                             : k_gather_piece<8, float>)
+                            : (pn == 4 ? k_gather_piece<4, double> : pn == 16 ? k_gather_piece<16, double>
+                                                                              : k_gather_piece<8, double>);
     EXM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     ProfileScope prof("gather");
     hipLaunchKernelGGL(kern, dim3(js.ntiles, st->ntau), dim3(BAND_THREADS), lds, cs, stream, wa, nw, lw_g,

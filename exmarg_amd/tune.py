@@ -1,9 +1,10 @@
 """
 Autotuner of the band gather kernel.
 
-The band kernel has two shape knobs: the piece length (walked times one lane reads in a row, 4, 8 or
-16) and the tile (reads per workgroup, 256, 512 or 1024).  The best pair depends on the walk, the
-table geometry and how the reads spread over the beta rows, so it is measured, not modelled:
+The band kernel has a form and two shape knobs: the form (3: each wave deals its own reads' pieces;
+2: pieces dealt over the whole workgroup), the piece length (walked times one lane reads in a row, 4,
+8 or 16) and the tile (reads per workgroup, 256, 512 or 1024).  The best choice depends on the walk,
+the table geometry and how the reads spread over the beta rows, so it is measured, not modelled:
 tune_batch times every candidate on the caller's own filled batch (the gather is rerun, nothing is
 consumed) with the kernel's own HIP events, keeps the fastest and records it in a JSON cache keyed by
 device, table geometry, walk length and precision; apply() restores a recorded choice.
@@ -22,6 +23,7 @@ import os
 
 from . import _hip
 
+FORMS = (3, 2)
 PIECES = (4, 8, 16)
 TILES = (256, 512, 1024)
 
@@ -32,17 +34,19 @@ def cache_path():
 
 
 def current():
-    """(piece, tile, precision) in force (0 for the defaults)"""
-    v = [ctypes.c_int() for _ in range(3)]
+    """(form, piece, tile, precision) in force (0 for the defaults)"""
+    v = [ctypes.c_int() for _ in range(4)]
     _hip.call("exm_get_gather_tuning", *[ctypes.byref(x) for x in v])
     return tuple(x.value for x in v)
 
 
-def set_knobs(piece=0, tile=0, precision=None):
+def set_knobs(piece=0, tile=0, precision=None, form=None):
     _hip.call("exm_set_gather_piece", int(piece))
     _hip.call("exm_set_gather_tile", int(tile))
     if precision is not None:
         _hip.call("exm_set_gather_precision", int(precision))
+    if form is not None:
+        _hip.call("exm_set_gather_kernel", int(form))
 
 
 def key(store, n_walk, precision):
@@ -68,40 +72,41 @@ def _save(table):
     os.replace(tmp, path)
 
 
-def tune_batch(batch, repeats=2, pieces=PIECES, tiles=TILES):
-    """time the gather of the filled batch for every (piece, tile), keep the fastest (knobs set) and
-    record it; returns dict(piece, tile, ms, table={"piece,tile": ms}).  Uses (and resets) the
-    library's stage timers."""
+def tune_batch(batch, repeats=2, forms=FORMS, pieces=PIECES, tiles=TILES):
+    """time the gather of the filled batch for every (form, piece, tile), keep the fastest (knobs set)
+    and record it; returns dict(form, piece, tile, ms, table={"form,piece,tile": ms}).  Uses (and
+    resets) the library's stage timers."""
     if not batch.entries:
         raise ValueError("tune a batch that holds segments")
-    _, _, precision = current()
+    precision = current()[3]
     _hip.profile(True)
     times = {}
     try:
-        for p in pieces:
-            for t in tiles:
-                set_knobs(p, t)
-                batch.run()                      # warm (code objects, work memory)
-                ms0, n0 = _hip.profile_get("gather")
-                for _ in range(repeats):
-                    batch.run()
-                ms1, n1 = _hip.profile_get("gather")
-                times[(p, t)] = (ms1 - ms0) / max(n1 - n0, 1)
+        for f in forms:
+            for p in pieces:
+                for t in tiles:
+                    set_knobs(p, t, form=f)
+                    batch.run()                      # warm (code objects, work memory)
+                    ms0, n0 = _hip.profile_get("gather")
+                    for _ in range(repeats):
+                        batch.run()
+                    ms1, n1 = _hip.profile_get("gather")
+                    times[(f, p, t)] = (ms1 - ms0) / max(n1 - n0, 1)
     finally:
         _hip.profile(False)
-    (p, t), ms = min(times.items(), key=lambda kv: kv[1])
-    set_knobs(p, t)
+    (f, p, t), ms = min(times.items(), key=lambda kv: kv[1])
+    set_knobs(p, t, form=f)
     table = _load()
-    table[key(batch.store, batch.nw, precision)] = dict(piece=p, tile=t, ms=ms)
+    table[key(batch.store, batch.nw, precision)] = dict(form=f, piece=p, tile=t, ms=ms)
     _save(table)
-    return dict(piece=p, tile=t, ms=ms, table={"%d,%d" % k: v for k, v in times.items()})
+    return dict(form=f, piece=p, tile=t, ms=ms, table={"%d,%d,%d" % k: v for k, v in times.items()})
 
 
 def apply(store, n_walk):
     """set the recorded knobs for this store geometry and walk (precision in force); returns them or None"""
-    _, _, precision = current()
+    precision = current()[3]
     rec = _load().get(key(store, n_walk, precision))
     if rec is None:
         return None
-    set_knobs(rec["piece"], rec["tile"])
+    set_knobs(rec["piece"], rec["tile"], form=rec.get("form", 0))
     return rec

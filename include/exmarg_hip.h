@@ -177,9 +177,11 @@ int exm_batch_reset(exm_handle batch);
 
 int exm_batch_info(exm_handle batch, int64_t* templates_used, int64_t* reads, int* segments);
 
-/* Choose the gather kernel: 0 automatic (the band kernel for a selection of at most 8192 walked
- * times per template, the reference-order kernel otherwise), 1 the reference-order kernel, 2 the
- * band kernel where it applies.  No reference counterpart (the reference has one kernel per device). */
+/* Choose the gather kernel: 0 automatic (the band kernel in its wave form for a selection of at most
+ * 8192 walked times per template, the reference-order kernel otherwise), 1 the reference-order
+ * kernel, 2 the band kernel's piece form (pieces dealt over the whole workgroup), 3 its wave form
+ * (each wave deals its own reads' pieces).  No reference counterpart (the reference has one kernel
+ * per device). */
 int exm_set_gather_kernel(int which);
 
 /* Walked times per piece of the band kernel: 4, 8 or 16 (0 restores the default, 8).  Any length
@@ -197,8 +199,9 @@ int exm_set_gather_precision(int p);
  * autotuner exmarg_amd.tune chooses it); no reference counterpart. */
 int exm_set_gather_tile(int n);
 
-/* The current tuning: piece length, tile and precision (0 where the default applies). */
-int exm_get_gather_tuning(int* piece, int* tile, int* precision);
+/* The current tuning: gather kernel choice, piece length, tile and precision (0 where the default
+ * applies). */
+int exm_get_gather_tuning(int* kernel, int* piece, int* tile, int* precision);
 
 /* ln Lambda of one segment on the source-frame grid (m1, z) from its node values, and its prior
  * average: lamN and yN (nrow, nm) float64 (rows: the nodes and the boundary row, nrow <= 64), Mdet

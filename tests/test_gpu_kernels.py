@@ -135,7 +135,7 @@ def _tiny_store(kind, shape, n_nodes=2, seed=1):
     return st.synthetic_store(n_nodes=n_nodes, seed=seed, kind=kind, shape=shape, host=True)
 
 
-KERNELS = {1: 1e-12, 2: 1e-6}     # reference-order kernel: same operation order as the oracle; band kernel
+KERNELS = {1: 1e-12, 2: 1e-6, 3: 1e-6}     # reference-order kernel: same operation order as the oracle; band kernel
 
 
 @pytest.fixture
@@ -145,7 +145,7 @@ def kernel(hip, request):
     hip.call("exm_set_gather_kernel", 0)
 
 
-@pytest.mark.parametrize("kernel", [1, 2], indirect=True)
+@pytest.mark.parametrize("kernel", [1, 2, 3], indirect=True)
 @pytest.mark.parametrize("kind", ["smooth", "rough"])
 @pytest.mark.parametrize("subsampled", [False, True])
 def test_gather_matches_oracle(hip, oracle, kernel, kind, subsampled):
@@ -177,12 +177,12 @@ def piece(hip, request):
     hip.call("exm_set_gather_piece", 0)
 
 
-@pytest.mark.parametrize("kernel", [2], indirect=True)
+@pytest.mark.parametrize("kernel", [2, 3], indirect=True)
 @pytest.mark.parametrize("piece", [4, 8, 16], indirect=True)
 def test_band_gather_clamps_and_many_reads(hip, oracle, kernel, piece):
     # beta clamped at both table edges, Delta clamped, more reads of one table than one tile holds
     from exmarg_amd.search import gather as g
-    say("gather: band kernel, piece %d, clamped rows, 1500 reads of one table" % piece)
+    say("gather: band kernel %d, piece %d, clamped rows, 1500 reads of one table" % (kernel, piece))
     s = _tiny_store("rough", (5, 49, 24, 17), n_nodes=1, seed=4)
     nm, nt = 3, 4096
     traj = _rand_traj(nm, nt, seed=8)
@@ -200,13 +200,14 @@ def test_band_gather_clamps_and_many_reads(hip, oracle, kernel, piece):
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
 
 
+@pytest.mark.parametrize("form", [2, 3])
 @pytest.mark.parametrize("kind", ["smooth", "rough"])
-def test_band_gather_float32_form(hip, oracle, kind):
+def test_band_gather_float32_form(hip, oracle, kind, form):
     """float32 table interpolation (index arithmetic and log-sum-exp in float64) stays within 2e-6
     of the float64 oracle; the float64 form of the same reads (float32 exp inside the log-sum-exp)
     stays within 1e-7 of it"""
     from exmarg_amd.search import gather as g
-    say("gather: band kernel, float32 interpolation, %s table" % kind)
+    say("gather: band kernel %d, float32 interpolation, %s table" % (form, kind))
     s = _tiny_store(kind, (7, 49, 30, 23), n_nodes=2, seed=6)
     traj = _rand_traj(4, 4096, seed=12)
     traj["lqH"] *= 3.0
@@ -219,7 +220,7 @@ def test_band_gather_float32_form(hip, oracle, kind):
     tab = rng.integers(0, 2, (300, 2))
     ref = oracle.gather(traj, oracle.Tables(tables=s.tables, tau_idx=s.tau_idx), s.beta[0], s.delta[0], ids, lna,
                         tab, times)
-    hip.call("exm_set_gather_kernel", 2)
+    hip.call("exm_set_gather_kernel", form)
     try:
         hip.call("exm_set_gather_precision", 1)
         got32 = g.gather(traj, s, ids, lna, tab, times=times_dev)

@@ -9,6 +9,10 @@ pytestmark = pytest.mark.gpu
 
 LNB_TOL = 1e-5          # |ln B - ln B_oracle| bar of the analysis
 LNL_TOL = 1e-5          # per-template ln Lambda
+# the same segment alone or beside others in a launch: the band kernel groups a read's lookups with
+# float32 exps of non-positive differences, so the grouping (which depends on the neighbours in a
+# workgroup) moves ln Lambda by at most this much
+MIX_TOL = 1e-7
 
 
 def say(*a):
@@ -104,7 +108,7 @@ def test_pe_intrinsic_rejects_templates_outside_the_store(hip):
 
 def test_batch_of_uneven_segments_matches_single_calls(hip, oracle):
     """segments of 2 s and 4 s with their own banks in one batch (capacity forcing two gather launches)
-    give the same ln Lambda as pe_intrinsic one set at a time, and match the oracle"""
+    give the same ln Lambda as pe_intrinsic one set at a time (to MIX_TOL), and match the oracle"""
     from exmarg_amd.search import mass_bank, pe_intrinsic, pe_intrinsic_batch, read_segments, store as stm
     say("batch: 2 s and 4 s segments, two banks, capacity of 3 segments")
     st = stm.synthetic_store(seed=0, s_nodes=synth.S_NODES[::4])
@@ -118,7 +122,7 @@ def test_batch_of_uneven_segments_matches_single_calls(hip, oracle):
     res = pe_intrinsic_batch(items, st, n_loud=128, n_random=128, seed=4, max_templates=3 * 8)
     for (seg, bank), r in zip(items, res):
         single = pe_intrinsic(seg, bank, st, n_loud=128, n_random=128, seed=4, resident=True)
-        np.testing.assert_allclose(r["ln_lambda"], single["ln_lambda"], rtol=0, atol=1e-9)
+        np.testing.assert_allclose(r["ln_lambda"], single["ln_lambda"], rtol=0, atol=MIX_TOL)
         b, store = oracle_inputs(bank, st)
         tabs = oracle.Tables(synth=dict(seed=0, kind="smooth", nodes=np.arange(st.s.size)), shape=st.table_shape(),
                              tau_idx=st.tau_idx)

@@ -8,7 +8,9 @@ from exmarg_amd import synth
 pytestmark = pytest.mark.gpu
 
 LNB_TOL = 1e-5
-MIX_TOL = 1e-9          # alone vs in the mix: float64 sums in another order at most
+# alone vs in the mix: a read's lookups are grouped with float32 exps of non-positive differences, and
+# the grouping depends on the neighbouring reads of its workgroup
+MIX_TOL = 1e-7
 
 
 def say(*a):

@@ -7,6 +7,8 @@ from exmarg_amd import synth
 
 pytestmark = pytest.mark.gpu
 
+MIX_TOL = 1e-7          # a segment batched beside others vs alone (float32 exps grouped differently)
+
 
 def say(*a):
     print(*a, flush=True)
@@ -36,11 +38,11 @@ def test_stream_equals_pe_redshift(hip):
     for seg, res in zip(gs, got):
         bank = mass_bank(masses, seg, "hip")
         ref = pe_redshift(seg, bank, st, resident=True, **kw)
-        np.testing.assert_allclose(res["ln_b"], ref["ln_b"], rtol=0, atol=1e-9)
+        np.testing.assert_allclose(res["ln_b"], ref["ln_b"], rtol=0, atol=MIX_TOL)
         for j, row in enumerate(ref["rows"]):
             L = res["kept"][int(row)]
             np.testing.assert_array_equal(np.isfinite(L), np.isfinite(ref["ln_lambda"][j]))
             fin = np.isfinite(L)
-            np.testing.assert_allclose(L[fin], ref["ln_lambda"][j][fin], rtol=0, atol=1e-9)
+            np.testing.assert_allclose(L[fin], ref["ln_lambda"][j][fin], rtol=0, atol=MIX_TOL)
     assert np.all(np.isfinite(got[0]["ln_b"]))
     stm.free(st)

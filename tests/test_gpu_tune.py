@@ -18,7 +18,7 @@ def test_tuner_candidates_agree_and_choice_is_recorded(hip, tmp_path, monkeypatc
     from exmarg_amd.search import SegmentBatch, mass_bank, read_segments, store as stm
     from exmarg_amd.search import pe as _pe
     monkeypatch.setenv("EXMARG_TUNE_CACHE", str(tmp_path / "tuning.json"))
-    say("tune: 3 segments, pieces (4, 8, 16) x tiles (256, 1024)")
+    say("tune: 3 segments, forms (2, 3) x pieces (4, 8, 16) x tiles (256, 1024)")
     st = stm.synthetic_store(seed=0, s_nodes=synth.S_NODES[::4])
     seg, _, _ = synth.simulate_at_snr(3, 71, (30.0, 0.8, 0.1), 14.0, 4.0, 20.0, 256.0)
     bank = mass_bank(synth.chirp_mass_grid_bank(30.0, 0.8, 0.1, 40), seg, "hip")
@@ -30,23 +30,24 @@ def test_tuner_candidates_agree_and_choice_is_recorded(hip, tmp_path, monkeypatc
     try:
         ref = sb.run()
         outs = {}
-        for p in (4, 8, 16):
-            for t in (256, 1024):
-                tune.set_knobs(p, t)
-                outs[(p, t)] = sb.run()
+        for f in (2, 3):
+            for p in (4, 8, 16):
+                for t in (256, 1024):
+                    tune.set_knobs(p, t, form=f)
+                    outs[(f, p, t)] = sb.run()
         for k, v in outs.items():
             for a, b in zip(v, ref):
                 fin = np.isfinite(b)
                 np.testing.assert_array_equal(np.isfinite(a), fin)
                 np.testing.assert_allclose(a[fin], b[fin], rtol=0, atol=1e-7, err_msg=str(k))
-        best = tune.tune_batch(sb, repeats=1, pieces=(4, 8), tiles=(256, 1024))
+        best = tune.tune_batch(sb, repeats=1, forms=(2, 3), pieces=(4, 8), tiles=(256, 1024))
         say("best", best)
-        assert (best["piece"], best["tile"]) in {(4, 256), (4, 1024), (8, 256), (8, 1024)}
-        assert tune.current()[:2] == (best["piece"], best["tile"])
-        tune.set_knobs(0, 0)
+        assert best["form"] in (2, 3) and best["piece"] in (4, 8) and best["tile"] in (256, 1024)
+        assert tune.current()[:3] == (best["form"], best["piece"], best["tile"])
+        tune.set_knobs(0, 0, form=0)
         rec = tune.apply(st, 256)
-        assert rec is not None and tune.current()[:2] == (best["piece"], best["tile"])
+        assert rec is not None and tune.current()[:3] == (best["form"], best["piece"], best["tile"])
     finally:
-        tune.set_knobs(0, 0)
+        tune.set_knobs(0, 0, form=0)
         sb.free()
         stm.free(st)
