@@ -558,19 +558,34 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
                     }
                 }
             }
-            // segmented reduction towards the first lane of each read's run of pieces in this round
+            // segmented reduction over each read's run of lanes in this round, in two passes: the run's
+            // maximum (float, suffix scan towards the run's first lane), then the sum of every lane's
+            // sum rescaled to that maximum (float64), so that one exp per lane suffices
             const int seg = act ? o : -1 - lane;
+            int same = 0;                                   // bit k: lane + 2^k is in this lane's run
+            float M = (float)mx;
 #pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int src = (lane + d) & 63;
-                const double m2 = shfl_d(mx, src), s2 = shfl_d(sm, src);
+            for (int k = 0; k < 6; ++k) {
+                const int src = (lane + (1 << k)) & 63;
                 const int g2 = shfl_i(seg, src);
-                if (lane + d < 64 && g2 == seg) lse_merge(mx, sm, m2, s2);
+                const float m2 = __shfl(M, src, 64);
+                if (lane + (1 << k) < 64 && g2 == seg) {
+                    same |= 1 << k;
+                    M = fmaxf(M, m2);
+                }
             }
-            // the owning lane takes its read's result of this round from the run's first lane
+            const int first = act ? max(qex_o, base) - base : lane;     // the run's first lane
+            const double Mr = (double)__shfl(M, first, 64);
+            double t = act ? sm * (double)__expf((float)(mx - Mr)) : 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const double t2 = shfl_d(t, (lane + (1 << k)) & 63);
+                if ((same >> k) & 1) t += t2;
+            }
+            // the owning lane takes its read's (max, sum) of this round from the run's first lane
             const bool in_round = q > 0 && qex < base + 64 && qex + q > base;
             const int head = in_round ? max(qex, base) - base : 0;
-            const double hm = shfl_d(mx, head), hs = shfl_d(sm, head);
+            const double hm = (double)__shfl(M, head, 64), hs = shfl_d(t, head);
             if (in_round) lse_merge(amx, asum, hm, hs);
         }
         __builtin_amdgcn_s_waitcnt(0);
