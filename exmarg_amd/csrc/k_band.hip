@@ -493,9 +493,19 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
         const float* R0 = rows + sb * slot;
         const float* R1 = rows + (sb + 1 == 3 ? 0 : sb + 1) * slot;
         const double bd = (double)b;
-        for (int base = 0; base < Qw; base += 64) {
+        // a round's piece: its read (owning lane), walk range, parameters and records.  The next round's
+        // piece is found and its records requested before the current round's lookups, so that the
+        // shuffles and global loads of one round overlap the arithmetic of the previous one.
+        struct Piece {
+            int o, pa, pe, m, first;
+            bool act;
+            double offb, lw;
+            float aL[PN];
+            uint2 ev[PN];
+        };
+        auto fetch = [&](int base, Piece& P) {
             const int p = base + lane;
-            const bool act = p < Qw;
+            P.act = p < Qw;
             // owner: the largest lane whose first piece is <= p (piece offsets ascend with the lane)
             int o = 0;
 #pragma unroll
@@ -504,57 +514,58 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
                 const int qc = shfl_i(qex, c & 63);
                 if (c < 64 && qc <= p) o = c;
             }
-            const int lo_o = shfl_i(lo, o), hi_o = shfl_i(hi, o), qex_o = shfl_i(qex, o), m_o = shfl_i(my_m, o);
-            const double offb_o = shfl_d(my_offb, o);
-            const double lw_o = (double)__shfl(my_lw, o, 64);
+            const int lo_o = shfl_i(lo, o), hi_o = shfl_i(hi, o), qex_o = shfl_i(qex, o);
+            P.o = o;
+            P.m = shfl_i(my_m, o);
+            P.offb = shfl_d(my_offb, o);
+            P.lw = (double)__shfl(my_lw, o, 64);
+            P.pa = lo_o + (p - qex_o) * PN;
+            P.pe = min(P.pa + PN, hi_o);
+            P.first = P.act ? max(qex_o, base) - base : lane;          // the first lane of the read's run
+            if (P.act) {
+                const float* wam = wa + (int64_t)P.m * nw;
+                const uint2* smp = stream + ((int64_t)P.m * ntau + r) * nw;
+#pragma unroll
+                for (int k = 0; k < PN; ++k) {
+                    const int i = min(P.pa + k, P.pe - 1);
+                    P.aL[k] = wam[i];
+                    P.ev[k] = smp[i];
+                }
+            }
+        };
+        Piece cur;
+        if (Qw > 0) fetch(0, cur);
+        for (int base = 0; base < Qw; base += 64) {
+            const bool more = base + 64 < Qw;
+            Piece nxt;
+            if (more) fetch(base + 64, nxt);
+            const bool act = cur.act;
+            const int o = cur.o;
             double mx = -1e300, sm = 0.0;
             if (act) {
-                const int pa = lo_o + (p - qex_o) * PN;
-                const int pe = min(pa + PN, hi_o);
-                const double cj = offd + offb_o * qbd;
-                const float* wam = wa + (int64_t)m_o * nw;
-                const uint2* smp = stream + ((int64_t)m_o * ntau + r) * nw;
-                constexpr int CH = PN < 8 ? PN : 8;
+                const int pa = cur.pa, pe = cur.pe;
+                const double cj = offd + cur.offb * qbd;
+                if constexpr (sizeof(V) == 4) {
+                    double av[PN];
 #pragma unroll
-                for (int c0 = 0; c0 < PN; c0 += CH) {
-                    if (c0 > 0 && pa + c0 >= pe) break;
-                    float aLv[CH];
-                    uint2 ev[CH];
+                    for (int k = 0; k < PN; ++k)
+                        av[k] = pa + k < pe ? band_lookup<V>(R0, R1, geo, bd, cur.offb, cj, cur.lw, cur.aL[k], cur.ev[k])
+                                            : -1e300;
+                    mx = av[0];
 #pragma unroll
-                    for (int k = 0; k < CH; ++k) {
-                        const int i = min(pa + c0 + k, pe - 1);
-                        aLv[k] = wam[i];
-                        ev[k] = smp[i];
-                    }
-                    if constexpr (sizeof(V) == 4) {
-                        double av[CH];
+                    for (int k = 1; k < PN; ++k) mx = fmax(mx, av[k]);
 #pragma unroll
-                        for (int k = 0; k < CH; ++k)
-                            av[k] = pa + c0 + k < pe ? band_lookup<V>(R0, R1, geo, bd, offb_o, cj, lw_o, aLv[k], ev[k])
-                                                     : -1e300;
-                        double cm = av[0];
+                    for (int k = 0; k < PN; ++k) sm += (double)__expf((float)(av[k] - mx));
+                } else {
 #pragma unroll
-                        for (int k = 1; k < CH; ++k) cm = fmax(cm, av[k]);
-                        double cs = 0.0;
-#pragma unroll
-                        for (int k = 0; k < CH; ++k) cs += (double)__expf((float)(av[k] - cm));
-                        if (c0 == 0) {
-                            mx = cm;
-                            sm = cs;
-                        } else {
-                            lse_merge(mx, sm, cm, cs);
-                        }
-                    } else {
-#pragma unroll
-                        for (int k = 0; k < CH; ++k) {
-                            const double a = pa + c0 + k < pe
-                                                 ? band_lookup<V>(R0, R1, geo, bd, offb_o, cj, lw_o, aLv[k], ev[k])
-                                                 : -1e300;
-                            const double dlt = a - mx;
-                            const double ex = (double)__expf(-(float)fabs(dlt));
-                            sm = dlt > 0.0 ? fma(sm, ex, 1.0) : sm + ex;
-                            mx = fmax(mx, a);
-                        }
+                    for (int k = 0; k < PN; ++k) {
+                        const double a = pa + k < pe
+                                             ? band_lookup<V>(R0, R1, geo, bd, cur.offb, cj, cur.lw, cur.aL[k], cur.ev[k])
+                                             : -1e300;
+                        const double dlt = a - mx;
+                        const double ex = (double)__expf(-(float)fabs(dlt));
+                        sm = dlt > 0.0 ? fma(sm, ex, 1.0) : sm + ex;
+                        mx = fmax(mx, a);
                     }
                 }
             }
@@ -574,8 +585,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
                     M = fmaxf(M, m2);
                 }
             }
-            const int first = act ? max(qex_o, base) - base : lane;     // the run's first lane
-            const double Mr = (double)__shfl(M, first, 64);
+            const double Mr = (double)__shfl(M, cur.first, 64);
             double t = act ? sm * (double)__expf((float)(mx - Mr)) : 0.0;
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
@@ -587,6 +597,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
             const int head = in_round ? max(qex, base) - base : 0;
             const double hm = (double)__shfl(M, head, 64), hs = shfl_d(t, head);
             if (in_round) lse_merge(amx, asum, hm, hs);
+            if (more) cur = nxt;
         }
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
@@ -652,13 +663,12 @@ int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const 
     // the wave form unless the piece form is asked for; 4 walked times per piece in the wave form
     const bool wave = gather_mode() != 2;
     const int pn = c.piece ? c.piece : (wave ? 4 : PIECE);
+    EXM_CHECK(!wave || pn != 16, EXM_ERR_ARG, "the wave form of the band kernel takes pieces of 4 or 8 walked times");
     using Kern = decltype(&k_gather_piece<8, double>);
     Kern kern;
     if (wave)
-        kern = c.f32_interp ? (pn == 4 ? k_gather_wave<4, float> : pn == 16 ? k_gather_wave<16, float>
-                                                                            : k_gather_wave<8, float>)
-                            : (pn == 4 ? k_gather_wave<4, double> : pn == 16 ? k_gather_wave<16, double>
-                                                                             : k_gather_wave<8, double>);
+        kern = c.f32_interp ? (pn == 4 ? k_gather_wave<4, float> : k_gather_wave<8, float>)
+                            : (pn == 4 ? k_gather_wave<4, double> : k_gather_wave<8, double>);
     else
         kern = c.f32_interp ? (pn == 4 ? k_gather_piece<4, float> : pn == 16 ? k_gather_piece<16, float>
                                                                              : k_gather_piece<8, float>)

@@ -177,8 +177,7 @@ def piece(hip, request):
     hip.call("exm_set_gather_piece", 0)
 
 
-@pytest.mark.parametrize("kernel", [2, 3], indirect=True)
-@pytest.mark.parametrize("piece", [4, 8, 16], indirect=True)
+@pytest.mark.parametrize("kernel,piece", [(2, 4), (2, 8), (2, 16), (3, 4), (3, 8)], indirect=True)
 def test_band_gather_clamps_and_many_reads(hip, oracle, kernel, piece):
     # beta clamped at both table edges, Delta clamped, more reads of one table than one tile holds
     from exmarg_amd.search import gather as g
@@ -248,6 +247,21 @@ def test_gather_full_geometry_one_node(hip, oracle):
     ref = oracle.gather(traj, tabs, s.beta[0], s.delta[0], ids, lna, tab)
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
     st.free(s)
+
+
+def test_wave_form_rejects_pieces_of_16(hip):
+    from exmarg_amd.search import gather as g
+    s = _tiny_store("smooth", (5, 49, 8, 9), n_nodes=1)
+    traj = _rand_traj(2, 256, 0)
+    times = g.select_times(traj, s.tau_idx, 16, 16, seed=1)
+    hip.call("exm_set_gather_kernel", 3)
+    hip.call("exm_set_gather_piece", 16)
+    try:
+        with pytest.raises(ValueError, match="4 or 8"):
+            g.gather(traj, s, np.array([0]), np.array([[0.0]]), np.array([[0]]), times=times)
+    finally:
+        hip.call("exm_set_gather_piece", 0)
+        hip.call("exm_set_gather_kernel", 0)
 
 
 def test_gather_rejects_bad_ids(hip):
