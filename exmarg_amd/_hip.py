@@ -40,12 +40,15 @@ SIGNATURES = {
     "exm_traj_create": [I32, I64, P, P, P, P, HP],
     "exm_traj_read": [H, P, P, P, P],
     "exm_select_times": [H, I32, I32, I32, I32, P, F32, HP],
+    "exm_select_times_seeded": [H, I32, I32, I32, I32, P, F32, HP],
+    "exm_draw_integers": [P, I64, I64, P],
     "exm_times_read": [H, P, P],
     "exm_gather": [H, H, P, I64, P, P, I32, H, P],
     "exm_plan_create": [H, I32, I64, P, P, P, HP],
     "exm_plan_gather": [H, H, H, H, P],
     "exm_batch_create": [H, I64, I32, HP],
     "exm_batch_add": [H, H, H, P, P, P, P, I32, I32, I32, I32, P, F32, ctypes.POINTER(ctypes.c_int)],
+    "exm_batch_add_seeded": [H, H, H, P, P, P, P, I32, I32, I32, I32, P, F32, ctypes.POINTER(ctypes.c_int)],
     "exm_batch_run": [H, P],
     "exm_batch_launch": [H],
     "exm_reserve": [I32, I64, I32, I32],

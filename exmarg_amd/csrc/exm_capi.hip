@@ -578,10 +578,17 @@ int exm_traj_read(exm_handle h, float* lqH, float* phH, float* lqL, float* phL) 
 }
 
 // ---------------------------------------------------------------- times
-int exm_select_times(exm_handle ht, int lag_lo, int lag_hi, int K, int S, const int64_t* pick, float lw_rand,
-                     exm_handle* times_out) {
-    LOCK;
-    EXM_TRY(ensure_init());
+// the draw of the random times on the device into scratch slot 2
+static int device_draw(const uint64_t* rng_state, int64_t n, int64_t high, int64_t** out) {
+    EXM_CHECK(rng_state != nullptr, EXM_ERR_ARG, "rng_state is NULL");
+    void* buf = nullptr;
+    EXM_TRY(scratch(2, sizeof(int64_t) * (size_t)std::max<int64_t>(n, 1), &buf));
+    *out = (int64_t*)buf;
+    return launch_draw(rng_state, n, high, *out);
+}
+
+static int select_times(exm_handle ht, int lag_lo, int lag_hi, int K, int S, const int64_t* pick,
+                        const uint64_t* rng_state, float lw_rand, exm_handle* times_out) {
     NEED(Traj, traj, ht, K_TRAJ);
     EXM_CHECK(lag_lo <= lag_hi, EXM_ERR_ARG, "lag range [%d, %d] is empty", lag_lo, lag_hi);
     EXM_CHECK((int64_t)(lag_hi - lag_lo) < traj->n_time, EXM_ERR_ARG, "lag range wider than the segment");
@@ -589,11 +596,15 @@ int exm_select_times(exm_handle ht, int lag_lo, int lag_hi, int K, int S, const 
               (long long)traj->n_time);
     EXM_CHECK(S > 0, EXM_ERR_ARG, "n_random must be positive, got %d", S);
     EXM_CHECK(K + S <= 16384, EXM_ERR_ARG, "n_loud + n_random = %d exceeds 16384", K + S);
-    EXM_CHECK(pick != nullptr, EXM_ERR_ARG, "pick is NULL");
     const int64_t nrest = traj->n_time - K;
-    for (int64_t i = 0; i < (int64_t)traj->n_tmpl * S; ++i)
-        EXM_CHECK(pick[i] >= 0 && pick[i] < nrest, EXM_ERR_ARG, "pick[%lld] = %lld outside [0, %lld)",
-                  (long long)i, (long long)pick[i], (long long)nrest);
+    int64_t* d_pick = nullptr;
+    if (pick) {
+        for (int64_t i = 0; i < (int64_t)traj->n_tmpl * S; ++i)
+            EXM_CHECK(pick[i] >= 0 && pick[i] < nrest, EXM_ERR_ARG, "pick[%lld] = %lld outside [0, %lld)",
+                      (long long)i, (long long)pick[i], (long long)nrest);
+    } else {
+        EXM_TRY(device_draw(rng_state, (int64_t)traj->n_tmpl * S, nrest, &d_pick));
+    }
     Times* tm = new Times();
     tm->kind = K_TIMES;
     tm->n_tmpl = traj->n_tmpl;
@@ -601,7 +612,7 @@ int exm_select_times(exm_handle ht, int lag_lo, int lag_hi, int K, int S, const 
     tm->lw_rand = lw_rand;
     int rc = dev_alloc((void**)&tm->d_sel, sizeof(int32_t) * (size_t)tm->n_tmpl * tm->n_walk);
     if (rc == EXM_OK) rc = dev_alloc((void**)&tm->d_lw, sizeof(float) * (size_t)tm->n_tmpl * tm->n_walk);
-    if (rc == EXM_OK) rc = launch_select(traj, lag_lo, lag_hi, K, S, pick, lw_rand, tm);
+    if (rc == EXM_OK) rc = launch_select(traj, lag_lo, lag_hi, K, S, pick, d_pick, lw_rand, tm);
     if (rc == EXM_OK) {
         hipError_t e = hipStreamSynchronize(ctx().stream);
         if (e != hipSuccess) {
@@ -614,6 +625,32 @@ int exm_select_times(exm_handle ht, int lag_lo, int lag_hi, int K, int S, const 
         return rc;
     }
     *times_out = register_object(tm);
+    return EXM_OK;
+}
+
+int exm_select_times(exm_handle ht, int lag_lo, int lag_hi, int K, int S, const int64_t* pick, float lw_rand,
+                     exm_handle* times_out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    EXM_CHECK(pick != nullptr, EXM_ERR_ARG, "pick is NULL");
+    return select_times(ht, lag_lo, lag_hi, K, S, pick, nullptr, lw_rand, times_out);
+}
+
+int exm_select_times_seeded(exm_handle ht, int lag_lo, int lag_hi, int K, int S, const uint64_t* rng_state,
+                            float lw_rand, exm_handle* times_out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    return select_times(ht, lag_lo, lag_hi, K, S, nullptr, rng_state, lw_rand, times_out);
+}
+
+int exm_draw_integers(const uint64_t* rng_state, int64_t n, int64_t high, int64_t* out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    EXM_CHECK(out != nullptr || n == 0, EXM_ERR_ARG, "out is NULL");
+    int64_t* d = nullptr;
+    EXM_TRY(device_draw(rng_state, n, high, &d));
+    if (n > 0) EXM_HIP(hipMemcpyAsync(out, d, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToHost, g_ctx.stream));
+    EXM_HIP(hipStreamSynchronize(g_ctx.stream));
     return EXM_OK;
 }
 
@@ -771,16 +808,14 @@ int exm_batch_create(exm_handle hstore, int64_t max_templates, int n_walk, exm_h
     return EXM_OK;
 }
 
-int exm_batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, const double* dH, const double* dL,
-                  const double* s2H, const double* s2L, int lag_lo, int lag_hi, int K, int S, const int64_t* pick,
-                  float lw_rand, int* slot) {
-    LOCK;
-    EXM_TRY(ensure_init());
+static int batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, const double* dH, const double* dL,
+                     const double* s2H, const double* s2L, int lag_lo, int lag_hi, int K, int S, const int64_t* pick,
+                     const uint64_t* rng_state, float lw_rand, int* slot) {
     NEED(Batch, b, hbatch, K_BATCH);
     NEED(Bank, bank, hbank, K_BANK);
     NEED(Plan, plan, hplan, K_PLAN);
     NEED(Store, st, b->store, K_STORE);
-    EXM_CHECK(dH && dL && s2H && s2L && pick, EXM_ERR_ARG, "NULL argument");
+    EXM_CHECK(dH && dL && s2H && s2L && (pick || rng_state), EXM_ERR_ARG, "NULL argument");
     EXM_CHECK(!b->in_flight, EXM_ERR_STATE, "batch in flight; collect it before adding segments");
     EXM_CHECK(K + S == b->nw, EXM_ERR_ARG, "n_loud + n_random = %d, the batch walks %d times", K + S, b->nw);
     EXM_CHECK(plan->n_tmpl == bank->n_tmpl, EXM_ERR_ARG, "plan is for %d templates, bank has %d", plan->n_tmpl,
@@ -791,9 +826,10 @@ int exm_batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, const d
     EXM_CHECK(K > 0 && K < bank->n_time && S > 0, EXM_ERR_ARG, "n_loud %d / n_random %d invalid for n_time %lld", K,
               S, (long long)bank->n_time);
     const int64_t nrest = bank->n_time - K;
-    for (int64_t i = 0; i < (int64_t)bank->n_tmpl * S; ++i)
-        EXM_CHECK(pick[i] >= 0 && pick[i] < nrest, EXM_ERR_ARG, "pick[%lld] outside [0, %lld)", (long long)i,
-                  (long long)nrest);
+    if (pick)
+        for (int64_t i = 0; i < (int64_t)bank->n_tmpl * S; ++i)
+            EXM_CHECK(pick[i] >= 0 && pick[i] < nrest, EXM_ERR_ARG, "pick[%lld] outside [0, %lld)", (long long)i,
+                      (long long)nrest);
     // transient trajectories and times in scratch slot 7
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t tr_bytes = al(sizeof(float) * 4 * (size_t)bank->n_tmpl * bank->n_time);
@@ -811,7 +847,9 @@ int exm_batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, const d
     tm.d_sel = (int32_t*)((char*)tbuf + tr_bytes);
     tm.d_lw = (float*)((char*)tbuf + tr_bytes + sel_bytes);
     int rc = launch_filter(bank, dH, dL, s2H, s2L, &tr);
-    if (rc == EXM_OK) rc = launch_select(&tr, lag_lo, lag_hi, K, S, pick, lw_rand, &tm);
+    int64_t* d_pick = nullptr;
+    if (rc == EXM_OK && !pick) rc = device_draw(rng_state, (int64_t)bank->n_tmpl * S, nrest, &d_pick);
+    if (rc == EXM_OK) rc = launch_select(&tr, lag_lo, lag_hi, K, S, pick, d_pick, lw_rand, &tm);
     if (rc == EXM_OK)
         rc = launch_walk(&tr, &tm, st, b->d_wa + b->used * b->nw, b->d_stream + b->used * b->ntau * b->nw);
     if (rc == EXM_OK) rc = launch_fill(b->d_lw + b->used, bank->n_tmpl, lw_rand);
@@ -831,6 +869,24 @@ int exm_batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, const d
     b->used += bank->n_tmpl;
     if (slot) *slot = (int)b->seg_reads.size() - 1;
     return EXM_OK;
+}
+
+
+int exm_batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, const double* dH, const double* dL,
+                  const double* s2H, const double* s2L, int lag_lo, int lag_hi, int K, int S, const int64_t* pick,
+                  float lw_rand, int* slot) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    EXM_CHECK(pick != nullptr, EXM_ERR_ARG, "pick is NULL");
+    return batch_add(hbatch, hbank, hplan, dH, dL, s2H, s2L, lag_lo, lag_hi, K, S, pick, nullptr, lw_rand, slot);
+}
+
+int exm_batch_add_seeded(exm_handle hbatch, exm_handle hbank, exm_handle hplan, const double* dH, const double* dL,
+                         const double* s2H, const double* s2L, int lag_lo, int lag_hi, int K, int S,
+                         const uint64_t* rng_state, float lw_rand, int* slot) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    return batch_add(hbatch, hbank, hplan, dH, dL, s2H, s2L, lag_lo, lag_hi, K, S, nullptr, rng_state, lw_rand, slot);
 }
 
 static void batch_release_jobs(Batch* b) {

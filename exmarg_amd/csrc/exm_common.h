@@ -215,8 +215,11 @@ T* get(exm_handle h, uint32_t kind) {
 
 // ---------------------------------------------------------------- kernels (host launchers)
 int launch_filter(Bank* bank, const double* dH, const double* dL, const double* s2H, const double* s2L, Traj* traj);
-int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_t* pick, float lw_rand,
-                  Times* times);
+// arrival-time selection; the draw comes from the host (pick) or the device (d_pick, pick NULL)
+int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_t* pick, const int64_t* d_pick,
+                  float lw_rand, Times* times);
+// numpy's default_rng(...).integers(0, high, n_out) from its PCG64 state (state hi, lo, inc hi, lo)
+int launch_draw(const uint64_t rng_state[4], int64_t n_out, int64_t high, int64_t* d_out);
 // ln Lambda of every job of js into d_out (table-sorted order)
 int launch_gather(Traj* traj, Store* store, const JobSet& js, Times* times, double* d_out);
 int launch_gather_ref(Traj* traj, Store* store, const JobSet& js, Times* times, double* om, double* os);

@@ -199,8 +199,8 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict_
     }
 }
 
-int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_t* pick, float lw_rand,
-                  Times* times) {
+int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_t* pick, const int64_t* d_pick,
+                  float lw_rand, Times* times) {
     Context& c = ctx();
     const int64_t nt = traj->n_time;
     int Sp = 1;
@@ -212,10 +212,15 @@ int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_
     const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(traj->n_tmpl, (1ll << 30) / (8 * nt)));
     uint64_t* keys = nullptr;
     EXM_TRY(scratch(0, sizeof(uint64_t) * (size_t)chunk * nt, (void**)&keys));
-    int64_t* dpick = nullptr;
-    EXM_TRY(scratch(2, sizeof(int64_t) * (size_t)traj->n_tmpl * S, (void**)&dpick));
-    EXM_HIP(hipMemcpyAsync(dpick, pick, sizeof(int64_t) * (size_t)traj->n_tmpl * S, hipMemcpyHostToDevice,
-                           c.stream));
+    // the draw: uploaded from the host (pick) or already on the device (d_pick)
+    const int64_t* dpick = d_pick;
+    if (!dpick) {
+        int64_t* up = nullptr;
+        EXM_TRY(scratch(2, sizeof(int64_t) * (size_t)traj->n_tmpl * S, (void**)&up));
+        EXM_HIP(hipMemcpyAsync(up, pick, sizeof(int64_t) * (size_t)traj->n_tmpl * S, hipMemcpyHostToDevice,
+                               c.stream));
+        dpick = up;
+    }
     EXM_HIP(hipFuncSetAttribute((const void*)k_select, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     ProfileScope prof("select");
     for (int m0 = 0; m0 < traj->n_tmpl; m0 += chunk) {

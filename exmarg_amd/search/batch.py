@@ -15,7 +15,7 @@ import numpy as np
 
 from .. import _hip
 from . import pe as _pe
-from .gather import lag_range, random_draw
+from .gather import lag_range, rng_state
 from .segments import read_segments
 
 RECORD_BYTES = 8          # bytes of walk record per template, lag and walked time
@@ -66,13 +66,13 @@ class SegmentBatch:
         nm, nt = bank.u.size, bank.n_time
         if not 0 < self.K < nt:
             raise ValueError("n_loud must lie in (0, n_time), got %d" % self.K)
-        pick = np.ascontiguousarray(random_draw(nm, nt, self.K, self.S, seed), np.int64)
+        state = rng_state(seed)
         lw = np.float32(np.log((nt - self.K) / self.S))
         arrs = [_hip.contiguous(d["H"], np.complex128), _hip.contiguous(d["L"], np.complex128),
                 _hip.contiguous(sig2["H"], np.float64), _hip.contiguous(sig2["L"], np.float64)]
         slot = _hip.ctypes.c_int()
-        _hip.call("exm_batch_add", int(self.handle), int(bank.handle), int(analysis.plan.handle),
-                  *[_hip.ptr(a) for a in arrs], self.lag_lo, self.lag_hi, self.K, self.S, _hip.ptr(pick), float(lw),
+        _hip.call("exm_batch_add_seeded", int(self.handle), int(bank.handle), int(analysis.plan.handle),
+                  *[_hip.ptr(a) for a in arrs], self.lag_lo, self.lag_hi, self.K, self.S, _hip.ptr(state), float(lw),
                   _hip.ctypes.byref(slot))
         self.used += nm
         self.entries.append(analysis)

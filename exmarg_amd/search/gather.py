@@ -51,13 +51,23 @@ def random_draw(n_template, n_time, n_loud, n_random, seed):
     return np.random.default_rng(seed).integers(0, n_time - n_loud, size=(n_template, n_random))
 
 
+def rng_state(seed):
+    """the PCG64 state of numpy.random.default_rng(seed) as four uint64 (state hi, lo, inc hi, lo): the
+    device makes the draw of random_draw from it (exm_select_times_seeded), bit for bit"""
+    st = np.random.default_rng(seed).bit_generator.state["state"]
+    s, inc = int(st["state"]), int(st["inc"])
+    m = (1 << 64) - 1
+    return np.array([s >> 64, s & m, inc >> 64, inc & m], dtype=np.uint64)
+
+
 def select_times(traj, tau_idx, n_loud, n_random, seed, xp=None, block=256):
     """the arrival times visited for every template (src/exmarg/search/gather.py:374-420).
 
     The n_loud times with the largest |z_L(t)|^2 + max_tau |z_H(t + tau)|^2 are kept exactly (float64
     score, ties to the earlier time); n_random more are drawn uniformly with replacement from the rest
-    and carry the log weight ln((n_time - n_loud) / n_random).  xp and block are accepted for the
-    reference signature.  returns Times on the device."""
+    (numpy's default_rng(seed) stream, drawn on the device) and carry the log weight
+    ln((n_time - n_loud) / n_random).  xp and block are accepted for the reference signature.
+    returns Times on the device."""
     if not isinstance(traj, Trajectories):
         traj = Trajectories.from_host(traj)
     K, S = int(n_loud), int(n_random)
@@ -67,9 +77,10 @@ def select_times(traj, tau_idx, n_loud, n_random, seed, xp=None, block=256):
         raise ValueError("n_loud must lie in (0, n_time), got %d" % K)
     if S <= 0:
         raise ValueError("n_random must be positive, got %d" % S)
-    pick = np.ascontiguousarray(random_draw(nm, nt, K, S, seed), np.int64)
     lw = np.float32(np.log((nt - K) / S))
-    h = _hip.new_handle("exm_select_times", int(traj.handle), lo, hi, K, S, _hip.ptr(pick), float(lw), kind="times")
+    state = rng_state(seed)
+    h = _hip.new_handle("exm_select_times_seeded", int(traj.handle), lo, hi, K, S, _hip.ptr(state), float(lw),
+                        kind="times")
     return Times(h, nm, K + S)
 
 

@@ -115,6 +115,17 @@ int exm_traj_read(exm_handle traj, float* lqH, float* phH, float* lqL, float* ph
 int exm_select_times(exm_handle traj, int lag_lo, int lag_hi, int K, int S, const int64_t* pick, float lw_rand,
                      exm_handle* times_out);
 
+/* As exm_select_times, with the random draw made on the device from the generator state of
+ * numpy.random.default_rng(seed + row): rng_state = (state >> 64, state & (2^64 - 1), inc >> 64,
+ * inc & (2^64 - 1)) of its PCG64 bit generator.  The draw is numpy's integers(0, n_time - K,
+ * (n_tmpl, S)) exactly (PCG64 XSL-RR words, low half first, Lemire's bounded method), the one
+ * src/exmarg/search/gather.py:397 makes on the host. */
+int exm_select_times_seeded(exm_handle traj, int lag_lo, int lag_hi, int K, int S, const uint64_t* rng_state,
+                            float lw_rand, exm_handle* times_out);
+
+/* The same draw copied to out (n values in [0, high)): numpy default_rng integers from rng_state. */
+int exm_draw_integers(const uint64_t* rng_state, int64_t n, int64_t high, int64_t* out);
+
 /* sel (n_tmpl, K+S) int32 and lw (n_tmpl, K+S) float32 of a times object. */
 int exm_times_read(exm_handle times, int32_t* sel, float* lw);
 
@@ -152,6 +163,11 @@ int exm_batch_create(exm_handle store, int64_t max_templates, int n_walk, exm_ha
 int exm_batch_add(exm_handle batch, exm_handle bank, exm_handle plan, const double* dH, const double* dL,
                   const double* s2H, const double* s2L, int lag_lo, int lag_hi, int K, int S, const int64_t* pick,
                   float lw_rand, int* slot);
+/* As exm_batch_add, with the draw made on the device from the generator state (see
+ * exm_select_times_seeded). */
+int exm_batch_add_seeded(exm_handle batch, exm_handle bank, exm_handle plan, const double* dH, const double* dL,
+                         const double* s2H, const double* s2L, int lag_lo, int lag_hi, int K, int S,
+                         const uint64_t* rng_state, float lw_rand, int* slot);
 
 /* ln Lambda of every read of the batch: out holds, segment after segment, the reads of each segment's
  * plan in plan order (float64). */

@@ -272,3 +272,15 @@ def test_gather_rejects_bad_ids(hip):
         g.gather(traj, s, np.array([5]), np.zeros((1, 1)), np.zeros((1, 1), np.int64))
     with pytest.raises(ValueError):
         g.gather(traj, s, np.array([0]), np.zeros((1, 1)), np.ones((1, 1), np.int64))
+
+
+@pytest.mark.parametrize("seed,n,high", [(5, 2_048_000, 130048), (0, 100_000, 3 << 30), (123, 9999, 7),
+                                         (9, 30_000, (1 << 32) - 5), (77, 1, 2), (4, 50_000, 1)])
+def test_device_draw_equals_numpy(hip, seed, n, high):
+    """the device draw is numpy's default_rng(seed).integers(0, high, n) bit for bit, at rejection rates
+    from none to a quarter of the words"""
+    from exmarg_amd.search.gather import rng_state
+    say("draw: seed %d, %d values below %d" % (seed, n, high))
+    out = np.empty(n, np.int64)
+    hip.call("exm_draw_integers", hip.ptr(rng_state(seed)), n, high, hip.ptr(out))
+    np.testing.assert_array_equal(out, np.random.default_rng(seed).integers(0, high, size=n))
