@@ -132,7 +132,7 @@ def pe_intrinsic_batch(items, store, prior="volume", ymin=0.1267, n_loud=1024, n
     for segments, bank in items:
         seg = read_segments(segments)
         an = _pe._Analysis(bank, store, ymin, True)
-        lnW = _pe._bank_weights(an.geometry[2], prior, ymin, bank)
+        lnW = _pe.bank_weights(an, prior)
         an.start()
         prepared.append((seg, bank, an, lnW))
     # work memory for admitting the largest segment first, so that the batch is sized on what is left

@@ -10,6 +10,8 @@ every (template, node) pair plus the boundary points; the host assembles the nod
 distance-prior weights (float64) and averages over the bank.
 """
 
+import itertools
+
 import numpy as np
 
 from .. import _hip, cosmology
@@ -17,6 +19,14 @@ from ..priors import distance_density
 from . import gather as _gather, store as _store
 from .bank import check_device
 from .segments import D0, read_segments
+
+
+_store_tokens = itertools.count()
+
+
+def _store_key(store):
+    """a key that identifies the store object for the bank's caches (never reused, unlike id())"""
+    return store.__dict__.setdefault("_cache_token", next(_store_tokens))
 
 
 def _dl_of_z(z):
@@ -121,6 +131,16 @@ def _bank_weights(yN, prior, ymin, bank):
     return lnW
 
 
+def bank_weights(an, prior):
+    """lnW of the analysis' bank over its points for a prior, computed once per (store, ymin, prior) and
+    kept with the bank (the distance weights depend on the bank and the store, not on the segment)"""
+    cache = an.bank.__dict__.setdefault("_lnW", {})
+    key = (_store_key(an.store), an.ymin, prior)
+    if key not in cache:
+        cache[key] = _bank_weights(an.geometry[2], prior, an.ymin, an.bank)
+    return cache[key]
+
+
 def _log_mean(lam, w):
     """ln sum_i w_i exp(lam_i) for normalized weights w, lam = -inf allowed (src/exmarg/search/pe.py:135-142)"""
     lam, w = np.asarray(lam), np.asarray(w)
@@ -161,7 +181,12 @@ class _Analysis:
     def __init__(self, bank, store, ymin, resident):
         check_device(bank.device)
         self.bank, self.store, self.resident, self.ymin = bank, store, resident, float(ymin)
-        self.geometry = _node_geometry(bank, store, ymin)
+        # per (store, ymin): the geometry of the bank's templates over the nodes, kept with the bank
+        cache = bank.__dict__.setdefault("_geometry", {})
+        key = (_store_key(store), self.ymin)
+        if key not in cache:
+            cache[key] = _node_geometry(bank, store, ymin)
+        self.geometry = cache[key]
         self.was_resident = store.device is not None
         self.dev = None
 
@@ -169,7 +194,7 @@ class _Analysis:
         """upload the tables and build the read plan (after the host-side checks have passed)"""
         bank, store = self.bank, self.store
         self.dev = _store.upload(store)
-        key = (id(store), self.ymin)
+        key = (_store_key(store), self.ymin)
         cache = bank.__dict__.setdefault("_plans", {})
         if key not in cache:
             job_m, job_tab, job_lna, row = _jobs(self.geometry)
@@ -220,7 +245,7 @@ def pe_intrinsic(segments, bank, store, rows=None, prior="volume", ymin=0.1267, 
     returns dict: M, q, chi, u (n_template,), ln_lambda (n_segment, n_template), ln_b (n_segment,), rows."""
     seg = read_segments(segments, rows)
     an = _Analysis(bank, store, ymin, resident)
-    lnW = _bank_weights(an.geometry[2], prior, ymin, bank)
+    lnW = bank_weights(an, prior)
     an.start()
     try:
         out = np.full((seg["rows"].size, bank.u.size), -np.inf)

@@ -149,7 +149,7 @@ class Server:
         bank = mass_bank(req.rows, seg, "hip")
         an = _pe._Analysis(bank, self.store, self.ymin, True)
         try:
-            lnW = _pe._bank_weights(an.geometry[2], self.prior, self.ymin, bank)
+            lnW = _pe.bank_weights(an, self.prior)
             an.start()
         except ValueError as e:
             self.release.append(bank.handle)
