@@ -2,11 +2,16 @@
 //
 // One workgroup of 1024 threads per template:
 //   A  score(t) = e^{2 lqL(t)} + e^{2 max_{lag_lo <= tau <= lag_hi} lqH(t + tau)} (cyclic in t), in
-//      float64 by pexp, stored as its IEEE bits (order-preserving for positive values);
-//      the window maximum reads an LDS tile of lqH with its lag halo.
-//   B  radix select (11-bit digits, LDS histogram) of the K-th largest key;
-//   C  ordered compaction: the K winners (keys above the K-th, then keys equal to it by
-//      increasing t) in increasing t;
+//      float64 by pexp, stored as its IEEE bits (order-preserving for positive values).  The window
+//      maximum comes from a sparse table over an LDS tile of lqH with its lag halo: maxima of 2^k
+//      consecutive values by k doubling steps, then two overlapping blocks cover the window.  Every
+//      stride-th key is kept as a sample.
+//   B  the K largest keys: the samples are sorted in LDS and a threshold taken below the expected
+//      K-th key; one pass collects every key at or above it (at most CAND of them) and an LDS bitonic
+//      sort by (key descending, time ascending) gives the K winners exactly (ties to the earlier
+//      time).  When the threshold catches fewer than K or more than CAND keys, an exact radix select
+//      (11-bit digits, LDS histogram) and an ordered compaction pass give the same winners;
+//   C  the winners in increasing t;
 //   D  the drawn times rest[pick] of the remaining times in time order,
 //      rest[p] = p + #{i : top[i] - i <= p}, sorted by a bitonic sort and merged with the winners.
 #include "exm_common.h"
@@ -17,6 +22,8 @@ constexpr int SEL_THREADS = 1024;
 constexpr int SEL_WAVES = SEL_THREADS / 64;
 constexpr int SEL_TILE = 4096;
 constexpr int RADIX = 2048;
+constexpr int NSAMP = 2048;          // samples of the keys
+constexpr int CAND = 4096;           // candidates sorted in LDS
 
 // exclusive scan over the block; *total receives the block sum
 __device__ int block_excl_scan(int v, int* wsum, int* total) {
@@ -66,17 +73,63 @@ __device__ __forceinline__ int lower_bound_i(const int32_t* a, int n, int32_t v)
     return lo;
 }
 
+__device__ __forceinline__ void bitonic_desc_pairs(uint64_t* k, int32_t* t, int n) {
+    // sort (key, time) pairs: key descending, then time ascending; n a power of two
+    for (int size = 2; size <= n; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < n; i += SEL_THREADS) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool first = (i & size) == 0;           // this half in the requested order
+                    const uint64_t ka = k[i], kb = k[j];
+                    const int32_t ta = t[i], tb = t[j];
+                    const bool a_before_b = ka > kb || (ka == kb && ta < tb);
+                    if (a_before_b != first) {
+                        k[i] = kb;
+                        k[j] = ka;
+                        t[i] = tb;
+                        t[j] = ta;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ void bitonic_asc_int(int32_t* a, int n) {
+    for (int size = 2; size <= n; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < n; i += SEL_THREADS) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const int32_t x = a[i], y = a[j];
+                    if ((x > y) == up) {
+                        a[i] = y;
+                        a[j] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict__ lqH, const float* __restrict__ lqL,
-                                                        int64_t nt, int lag_lo, int lag_hi, int K, int S, int Sp,
-                                                        const int64_t* __restrict__ pick, float lw_rand, int m0,
-                                                        uint64_t* __restrict__ keys, int32_t* __restrict__ sel,
-                                                        float* __restrict__ lw) {
-    extern __shared__ unsigned char smem[];
+                                                        int64_t nt, int lag_lo, int lag_hi, int K, int Kp, int S,
+                                                        int Sp, const int64_t* __restrict__ pick, float lw_rand,
+                                                        int m0, uint64_t* __restrict__ keys,
+                                                        int32_t* __restrict__ sel, float* __restrict__ lw) {
+    extern __shared__ __align__(16) unsigned char smem[];
     const int span = lag_hi - lag_lo;
-    float* tile = (float*)smem;                                   // SEL_TILE + span
-    uint32_t* hist = (uint32_t*)(tile + SEL_TILE + span + 1);     // RADIX
-    int32_t* top = (int32_t*)(hist + RADIX);                      // K
-    int32_t* picked = top + K;                                    // Sp
+    const int tlen = SEL_TILE + span + 1;
+    uint64_t* ckey = (uint64_t*)smem;                             // CAND (samples first use NSAMP of it)
+    int32_t* ctime = (int32_t*)(ckey + CAND);                     // CAND
+    float* tile = (float*)(ctime + CAND);                         // 2 tlen (sparse-table ping-pong)
+    uint32_t* hist = (uint32_t*)(tile + 2 * tlen);                // RADIX
+    int32_t* top = (int32_t*)(hist + RADIX);                      // Kp
+    int32_t* picked = top + Kp;                                   // Sp
     int* wsum = picked + Sp;                                      // SEL_WAVES
     int* misc = wsum + SEL_WAVES;                                 // 4
 
@@ -85,78 +138,137 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict_
     const float* qH = lqH + (int64_t)m * nt;
     const float* qL = lqL + (int64_t)m * nt;
     uint64_t* key = keys + (int64_t)blockIdx.x * nt;
+    // window of W = span + 1 lags: blocks of 2^kw and the offset of the second block
+    const int W = span + 1;
+    int kw = 0;
+    while ((2 << kw) <= W) ++kw;
+    const int off2 = W - (1 << kw);
+    const int64_t stride = nt / NSAMP > 1 ? nt / NSAMP : 1;      // one sample every stride times
+    const int nsamp = (int)((nt + stride - 1) / stride);
 
-    // ---------------- A: keys
+    // ---------------- A: keys and samples
     for (int64_t t0 = 0; t0 < nt; t0 += SEL_TILE) {
-        const int64_t nthis = (nt - t0) < SEL_TILE ? (nt - t0) : SEL_TILE;
+        const int nthis = (int)((nt - t0) < SEL_TILE ? (nt - t0) : SEL_TILE);
+        float* src_buf = tile;
         for (int i = tid; i < nthis + span; i += SEL_THREADS) {
             int64_t src = (t0 + lag_lo + i) % nt;
             if (src < 0) src += nt;
-            tile[i] = qH[src];
+            src_buf[i] = qH[src];
         }
         __syncthreads();
+        // doubling: after step j, src_buf[i] = max of 2^j values starting at i
+        int len = nthis + span;
+        for (int j = 0; j < kw; ++j) {
+            float* dst = src_buf == tile ? tile + tlen : tile;
+            const int h = 1 << j;
+            const int nlen = len - h;
+            for (int i = tid; i < nlen; i += SEL_THREADS) dst[i] = fmaxf(src_buf[i], src_buf[i + h]);
+            __syncthreads();
+            src_buf = dst;
+            len = nlen;
+        }
         for (int i = tid; i < nthis; i += SEL_THREADS) {
-            float mx = tile[i];
-            for (int j = 1; j <= span; ++j) mx = fmaxf(mx, tile[i + j]);
-            const double sc = __dadd_rn(pexp(2.0 * (double)qL[t0 + i]), pexp(2.0 * (double)mx));
-            key[t0 + i] = (uint64_t)__double_as_longlong(sc);
+            const float mx = fmaxf(src_buf[i], src_buf[i + off2]);
+            const int64_t t = t0 + i;
+            const double sc = __dadd_rn(pexp(2.0 * (double)qL[t]), pexp(2.0 * (double)mx));
+            const uint64_t k = (uint64_t)__double_as_longlong(sc);
+            key[t] = k;
+            if (t % stride == 0) ckey[t / stride] = k;
         }
         __syncthreads();
     }
 
-    // ---------------- B: radix select of the K-th largest key
-    uint64_t prefix = 0, pmask = 0;
-    int krem = K;
-    const int shifts[6] = {53, 42, 31, 20, 9, 0};
-    const int widths[6] = {11, 11, 11, 11, 11, 9};
-    for (int pass = 0; pass < 6; ++pass) {
-        const int shift = shifts[pass];
-        const uint64_t dmask = (1ull << widths[pass]) - 1;
-        for (int i = tid; i < RADIX; i += SEL_THREADS) hist[i] = 0;
-        __syncthreads();
-        for (int64_t t = tid; t < nt; t += SEL_THREADS) {
-            const uint64_t k = key[t];
-            if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & dmask], 1u);
+    // ---------------- B: the threshold from the samples, the candidates, the exact K largest
+    int np2 = 1;
+    while (np2 < nsamp) np2 <<= 1;
+    for (int i = nsamp + tid; i < np2; i += SEL_THREADS) ckey[i] = 0;
+    for (int i = tid; i < np2; i += SEL_THREADS) ctime[i] = i;
+    __syncthreads();
+    bitonic_desc_pairs(ckey, ctime, np2);
+    // expected count at or above sample r is (r + 1) stride; aim for 1.5 K plus a margin
+    const int64_t rr = (int64_t)(1.5 * K / (double)stride) + 8;
+    const uint64_t thr0 = rr < nsamp ? ckey[rr] : 0;
+    if (tid == 0) misc[2] = 0;
+    __syncthreads();
+    for (int64_t t = tid; t < nt; t += SEL_THREADS) {
+        const uint64_t k = key[t];
+        if (k >= thr0) {
+            const int pos = atomicAdd(&misc[2], 1);
+            if (pos < CAND) {
+                ckey[pos] = k;
+                ctime[pos] = (int32_t)t;
+            }
         }
-        __syncthreads();
-        // bins in descending order: thread i holds bins RADIX-1-2i and RADIX-2-2i
-        const int b0 = RADIX - 1 - 2 * tid, b1 = RADIX - 2 - 2 * tid;
-        const int h0 = (int)hist[b0], h1 = (int)hist[b1];
-        int total;
-        const int before = block_excl_scan(h0 + h1, wsum, &total);
-        if (before < krem && krem <= before + h0) {
-            misc[0] = b0;
-            misc[1] = krem - before;
-        } else if (before + h0 < krem && krem <= before + h0 + h1) {
-            misc[0] = b1;
-            misc[1] = krem - before - h0;
-        }
-        __syncthreads();
-        prefix |= (uint64_t)misc[0] << shift;
-        pmask |= dmask << shift;
-        krem = misc[1];
-        __syncthreads();
-    }
-    const uint64_t thr = prefix;
-    const int need = krem;
-
-    // ---------------- C: ordered compaction of the winners
-    int eq_base = 0, take_base = 0;
-    for (int64_t t0 = 0; t0 < nt; t0 += SEL_THREADS) {
-        const int64_t t = t0 + tid;
-        const bool valid = t < nt;
-        const uint64_t k = valid ? key[t] : 0;
-        const int gt = valid && k > thr;
-        const int eq = valid && k == thr;
-        int tot_eq, tot_take;
-        const int ex_eq = block_excl_scan(eq, wsum, &tot_eq);
-        const int take = gt || (eq && eq_base + ex_eq < need);
-        const int ex_take = block_excl_scan(take, wsum, &tot_take);
-        if (take) top[take_base + ex_take] = (int32_t)t;
-        eq_base += tot_eq;
-        take_base += tot_take;
     }
     __syncthreads();
+    const int ncand = misc[2];
+    __syncthreads();
+    if (ncand >= K && ncand <= CAND) {
+        int nc2 = 1;
+        while (nc2 < ncand) nc2 <<= 1;
+        for (int i = ncand + tid; i < nc2; i += SEL_THREADS) {
+            ckey[i] = 0;
+            ctime[i] = INT32_MAX;
+        }
+        __syncthreads();
+        bitonic_desc_pairs(ckey, ctime, nc2);
+        for (int i = tid; i < Kp; i += SEL_THREADS) top[i] = i < K ? ctime[i] : INT32_MAX;
+        __syncthreads();
+        bitonic_asc_int(top, Kp);
+    } else {
+        // exact radix select of the K-th largest key
+        uint64_t prefix = 0, pmask = 0;
+        int krem = K;
+        const int shifts[6] = {53, 42, 31, 20, 9, 0};
+        const int widths[6] = {11, 11, 11, 11, 11, 9};
+        for (int pass = 0; pass < 6; ++pass) {
+            const int shift = shifts[pass];
+            const uint64_t dmask = (1ull << widths[pass]) - 1;
+            for (int i = tid; i < RADIX; i += SEL_THREADS) hist[i] = 0;
+            __syncthreads();
+            for (int64_t t = tid; t < nt; t += SEL_THREADS) {
+                const uint64_t k = key[t];
+                if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & dmask], 1u);
+            }
+            __syncthreads();
+            // bins in descending order: thread i holds bins RADIX-1-2i and RADIX-2-2i
+            const int b0 = RADIX - 1 - 2 * tid, b1 = RADIX - 2 - 2 * tid;
+            const int h0 = (int)hist[b0], h1 = (int)hist[b1];
+            int total;
+            const int before = block_excl_scan(h0 + h1, wsum, &total);
+            if (before < krem && krem <= before + h0) {
+                misc[0] = b0;
+                misc[1] = krem - before;
+            } else if (before + h0 < krem && krem <= before + h0 + h1) {
+                misc[0] = b1;
+                misc[1] = krem - before - h0;
+            }
+            __syncthreads();
+            prefix |= (uint64_t)misc[0] << shift;
+            pmask |= dmask << shift;
+            krem = misc[1];
+            __syncthreads();
+        }
+        const uint64_t thr = prefix;
+        const int need = krem;
+        // ordered compaction of the winners (keys above the K-th, then keys equal to it by time)
+        int eq_base = 0, take_base = 0;
+        for (int64_t t0 = 0; t0 < nt; t0 += SEL_THREADS) {
+            const int64_t t = t0 + tid;
+            const bool valid = t < nt;
+            const uint64_t k = valid ? key[t] : 0;
+            const int gt = valid && k > thr;
+            const int eq = valid && k == thr;
+            int tot_eq, tot_take;
+            const int ex_eq = block_excl_scan(eq, wsum, &tot_eq);
+            const int take = gt || (eq && eq_base + ex_eq < need);
+            const int ex_take = block_excl_scan(take, wsum, &tot_take);
+            if (take) top[take_base + ex_take] = (int32_t)t;
+            eq_base += tot_eq;
+            take_base += tot_take;
+        }
+        __syncthreads();
+    }
 
     // ---------------- D: drawn times, sorted, merged with the winners
     for (int s = tid; s < Sp; s += SEL_THREADS) {
@@ -205,9 +317,11 @@ int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_
     const int64_t nt = traj->n_time;
     int Sp = 1;
     while (Sp < S) Sp <<= 1;
+    int Kp = 1;
+    while (Kp < K) Kp <<= 1;
     const int span = lag_hi - lag_lo;
-    const size_t lds = sizeof(float) * (SEL_TILE + span + 1) + sizeof(uint32_t) * RADIX +
-                       sizeof(int32_t) * (K + Sp) + sizeof(int) * (SEL_WAVES + 4);
+    const size_t lds = (sizeof(uint64_t) + sizeof(int32_t)) * CAND + sizeof(float) * 2 * (SEL_TILE + span + 1) +
+                       sizeof(uint32_t) * RADIX + sizeof(int32_t) * (Kp + Sp) + sizeof(int) * (SEL_WAVES + 4);
     EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "selection needs %zu bytes of LDS", lds);
     const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(traj->n_tmpl, (1ll << 30) / (8 * nt)));
     uint64_t* keys = nullptr;
@@ -226,7 +340,7 @@ int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_
     for (int m0 = 0; m0 < traj->n_tmpl; m0 += chunk) {
         const int nb = std::min(chunk, traj->n_tmpl - m0);
         hipLaunchKernelGGL(k_select, dim3(nb), dim3(SEL_THREADS), lds, c.stream, traj->lqH(), traj->lqL(), nt,
-                           lag_lo, lag_hi, K, S, Sp, dpick, lw_rand, m0, keys, times->d_sel, times->d_lw);
+                           lag_lo, lag_hi, K, Kp, S, Sp, dpick, lw_rand, m0, keys, times->d_sel, times->d_lw);
         EXM_HIP(hipGetLastError());
     }
     return EXM_OK;

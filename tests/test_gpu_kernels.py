@@ -98,7 +98,8 @@ def _rand_traj(nm, nt, seed):
             "phL": rng.uniform(-np.pi, np.pi, (nm, nt)).astype(np.float32)}
 
 
-@pytest.mark.parametrize("nm,nt,K,S,lag", [(5, 4096, 64, 100, 43), (3, 20000, 1024, 1024, 43), (4, 300, 299, 5, 3)])
+@pytest.mark.parametrize("nm,nt,K,S,lag", [(5, 4096, 64, 100, 43), (3, 20000, 1024, 1024, 43), (4, 300, 299, 5, 3),
+                                           (2, 131072, 1024, 1024, 43), (2, 9000, 4000, 512, 1)])
 def test_select_matches_oracle(hip, oracle, nm, nt, K, S, lag):
     from exmarg_amd.search import gather as g
     say("select: nm=%d nt=%d K=%d S=%d" % (nm, nt, K, S))
@@ -110,12 +111,38 @@ def test_select_matches_oracle(hip, oracle, nm, nt, K, S, lag):
     np.testing.assert_array_equal(lw, rlw)
 
 
+@pytest.mark.parametrize("layout", ["off_sample", "one_template_flat"])
+def test_select_threshold_fallbacks(hip, oracle, layout):
+    """scores arranged so that the sample threshold catches too many or too few keys: the exact
+    selection must give the oracle's times all the same"""
+    from exmarg_amd.search import gather as g
+    say("select: %s" % layout)
+    nm, nt = 2, 65536
+    traj = _rand_traj(nm, nt, seed=4)
+    if layout == "off_sample":
+        # loud times only where no sample is taken (every time but multiples of 32)
+        traj["lqL"][:, np.arange(nt) % 32 != 0] += 6.0
+    else:
+        traj["lqL"][0] = 0.25                     # template 0: many equal scores around the threshold
+        traj["lqH"][0] = -3.0
+    tau = synth.lag_indices(43)
+    sel, lw = g.select_times(traj, tau, 1024, 512, seed=3).to_host()
+    rsel, rlw = oracle.select_times(traj, tau, 1024, 512, seed=3)
+    np.testing.assert_array_equal(sel, rsel)
+    np.testing.assert_array_equal(lw, rlw)
+
+
 def test_select_ties_take_earliest(hip, oracle):
     from exmarg_amd.search import gather as g
     say("select: all-equal scores")
     traj = {k: np.zeros((2, 1000), np.float32) for k in ("lqH", "phH", "lqL", "phL")}
     sel, lw = g.select_times(traj, synth.lag_indices(5), 100, 50, seed=1).to_host()
     np.testing.assert_array_equal(sel[:, :100][lw[:, :100] == 0].reshape(2, -1)[:, :10], np.tile(np.arange(10), (2, 1)))
+    rsel, rlw = oracle.select_times(traj, synth.lag_indices(5), 100, 50, seed=1)
+    np.testing.assert_array_equal(sel, rsel)
+    # all equal over more times than the candidate sort holds
+    traj = {k: np.zeros((1, 20000), np.float32) for k in ("lqH", "phH", "lqL", "phL")}
+    sel, lw = g.select_times(traj, synth.lag_indices(5), 100, 50, seed=1).to_host()
     rsel, rlw = oracle.select_times(traj, synth.lag_indices(5), 100, 50, seed=1)
     np.testing.assert_array_equal(sel, rsel)
 
