@@ -44,6 +44,9 @@ def parse():
     p.add_argument("--piece", type=int, default=0, help="walked times per piece of the band kernel (0 default)")
     p.add_argument("--precision", choices=["f64", "f32"], default="f64",
                    help="table interpolation of the band kernel (index arithmetic and log-sum-exp stay float64)")
+    p.add_argument("--filter-mib", type=int, default=0, help="matched-filter workspace per transform (MiB, 0 default)")
+    p.add_argument("--filter-precision", choices=["f64", "f32"], default="f64",
+                   help="matched-filter transform (weighting, modulus and phase stay float64)")
     p.add_argument("--mode", choices=["plain", "serve", "stream", "batch"], default="plain",
                    help="plain: pe_intrinsic on one segment per step (the headline); serve: the request loop "
                         "under Poisson arrivals; stream: pe_redshift over 8 s segments with an estimated spectrum; "
@@ -368,6 +371,7 @@ def main_batch(args):
     kw = dict(n_loud=args.n_loud, n_random=args.n_random, seed=0)
     prec = 1 if args.precision == "f32" else 0
     _hip.call("exm_set_gather_precision", prec)
+    _hip.call("exm_set_filter_precision", prec)
     # autotune the band kernel on a sample batch (8 segments of the 16 s bank, or the first bank)
     sample_seg, sample_bank = next(((s, b) for s, b in items if s["T"] == 16.0), items[0])
     rseg = read_segments(sample_seg)
@@ -390,6 +394,7 @@ def main_batch(args):
     nseg = sum(r["ln_b"].size for r in res)
     # the float64 form of the same batch (the reference for the float32 error)
     _hip.call("exm_set_gather_precision", 0)
+    _hip.call("exm_set_filter_precision", 0)
     res64 = pe_intrinsic_batch(items, st, **kw)
     dlnb = float(max(np.max(np.abs(a["ln_b"] - b["ln_b"])) for a, b in zip(res, res64)))
     dlnl = float(max(np.nanmax(np.abs(a["ln_lambda"] - b["ln_lambda"])) for a, b in zip(res, res64)))
@@ -410,7 +415,8 @@ def main_batch(args):
         "plain_dropin_segments_per_s": plain, "plain_sample_segments": plain_n, "speedup_vs_plain": nseg / wall / plain,
         "float32_vs_float64": {"max_abs_dln_b": dlnb, "max_abs_dln_lambda": dlnl},
         "autotune": tuned,
-        "dtype": "f32 interpolation, f64 index arithmetic and log-sum-exp" if prec else "f64", "data": "synthetic",
+        "dtype": "f32 transform and interpolation, f64 weighting, index arithmetic and log-sum-exp" if prec else "f64",
+        "data": "synthetic",
         "config": {"workload": "%d segments, T uniform over {4,8,16,32,64} s at 20-1024 Hz, half noise, half one "
                                "TaylorF2 injection (SNR 8-30); 2000-template 20x10x10 bank per length; 53 "
                                "HBM-resident table nodes; pe_intrinsic 'volume', n_loud=n_random=%d"
@@ -445,6 +451,8 @@ def main():
     _hip.call("exm_set_gather_kernel", args.gather_kernel)
     _hip.call("exm_set_gather_piece", args.piece)
     _hip.call("exm_set_gather_precision", 1 if args.precision == "f32" else 0)
+    _hip.call("exm_set_filter_workspace", args.filter_mib << 20)
+    _hip.call("exm_set_filter_precision", 1 if args.filter_precision == "f32" else 0)
     info = _hip.device_info()
     st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
     bank = mass_bank(rows, seg, "hip")
@@ -466,11 +474,13 @@ def main():
     b_ms, b_n = _hip.profile_get("bounds")
     _hip.profile(False)
     f64_check = None
-    if args.precision == "f32":
+    if args.precision == "f32" or args.filter_precision == "f32":
         # the same segment through the float64 form: the size-independent check of the float32 form
         _hip.call("exm_set_gather_precision", 0)
+        _hip.call("exm_set_filter_precision", 0)
         ref64 = pe_intrinsic(seg, bank, st, **kw)
-        _hip.call("exm_set_gather_precision", 1)
+        _hip.call("exm_set_gather_precision", 1 if args.precision == "f32" else 0)
+        _hip.call("exm_set_filter_precision", 1 if args.filter_precision == "f32" else 0)
         f64_check = {"max_abs_dln_lambda": float(np.nanmax(np.abs(res["ln_lambda"] - ref64["ln_lambda"]))),
                      "dln_b": float(abs(res["ln_b"][0] - ref64["ln_b"][0]))}
 
@@ -490,7 +500,10 @@ def main():
         "metric": "segments/sec at ln B err <=1e-5 vs FP64 CPU; p50/p99 latency; HBM GB/s",
         "value": value, "unit": "segments/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "f64" else "f32 interpolation, f64 index arithmetic and log-sum-exp",
+        "dtype": ("f64" if args.precision == "f64" and args.filter_precision == "f64"
+                  else "f32 %s, f64 elsewhere" % " and ".join(
+                      n for n, v in (("transform", args.filter_precision), ("interpolation", args.precision))
+                      if v == "f32")),
         "data": "synthetic",
         "config": {"workload": "one 16 s segment, 20-1024 Hz, 2000-template (M_det, q, chi) bank, 53 HBM-resident "
                                "table nodes, pe_intrinsic 'volume' prior, n_loud=n_random=1024",

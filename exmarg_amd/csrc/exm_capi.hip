@@ -1049,6 +1049,21 @@ int exm_set_gather_tile(int n) {
     return EXM_OK;
 }
 
+int exm_set_filter_workspace(int64_t bytes) {
+    LOCK;
+    EXM_CHECK(bytes == 0 || (bytes >= (1ll << 20) && bytes <= (1ll << 32)), EXM_ERR_ARG,
+              "filter workspace must be 0 (default, 128 MiB) or 1 MiB .. 4 GiB, got %lld", (long long)bytes);
+    g_ctx.filter_bytes = bytes;
+    return EXM_OK;
+}
+
+int exm_set_filter_precision(int p) {
+    LOCK;
+    EXM_CHECK(p == 0 || p == 1, EXM_ERR_ARG, "filter precision must be 0 (float64) or 1 (float32 transform)");
+    g_ctx.filter_f32 = p;
+    return EXM_OK;
+}
+
 int exm_get_gather_tuning(int* kernel, int* piece, int* tile, int* precision) {
     LOCK;
     if (kernel) *kernel = g_gather_mode;

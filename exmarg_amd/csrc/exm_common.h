@@ -176,6 +176,8 @@ struct Context {
     int piece = 0;                                          // walked times per piece (0: default)
     int f32_interp = 0;                                     // band kernel interpolates in float32
     int tile = 0;                                           // reads per band workgroup (0: TILE_JOBS)
+    int64_t filter_bytes = 0;                               // matched-filter workspace per transform (0: 1 GiB)
+    int filter_f32 = 0;                                     // matched-filter transform in float32
     int num_cu = 0;
     int64_t total_mem = 0;
     char name[256] = {0};

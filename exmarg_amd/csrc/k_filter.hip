@@ -5,7 +5,10 @@
 //   lq  = 1/2 ln max(|z|^2, 1e-38),  ph = atan2(Im z, Re z)
 //
 // Weighting, transform and modulus/phase run in float64 (hipFFT Z2Z, batched over a chunk of
-// templates); the trajectories are rounded once to float32 on output.
+// templates); the trajectories are rounded once to float32 on output.  The float32 form
+// (exm_set_filter_precision(1), part of the precision form of the batched analysis) keeps the
+// weighting and the modulus/phase in float64 but transforms in float32 (C2C), as the reference's GPU
+// path does (complex64, src/exmarg/search/bank.py:66-75).
 #include <cmath>
 
 #include "exm_common.h"
@@ -36,6 +39,39 @@ __global__ void k_filter_pack(const float2* __restrict__ W, const double2* __res
     pad[(int64_t)j * n_time + nu] = out;
 }
 
+// the same weighting rounded to complex64 for the float32 transform
+__global__ void k_filter_pack32(const float2* __restrict__ W, const double2* __restrict__ d,
+                                const double* __restrict__ s2, int n_freq, int lo, int64_t n_time, int m0,
+                                float2* __restrict__ pad) {
+    const int j = blockIdx.y;
+    const int64_t nu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nu >= n_time) return;
+    float2 out = make_float2(0.0f, 0.0f);
+    const int64_t f = nu - lo;
+    if (f >= 0 && f < n_freq) {
+        const float2 wt = W[(int64_t)(m0 + j) * n_freq + f];
+        const double2 dd = d[f];
+        const double inv = 1.0 / s2[f];
+        const double wr = dd.x * inv, wi = (-dd.y) * inv;
+        const double ar = (double)wt.x, ai = (double)wt.y;
+        out = make_float2((float)(ar * wr - ai * wi), (float)(ar * wi + ai * wr));
+    }
+    pad[(int64_t)j * n_time + nu] = out;
+}
+
+__global__ void k_filter_post32(const float2* __restrict__ z, int64_t n_time, int m0, float* __restrict__ lq,
+                                float* __restrict__ ph) {
+    const int j = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_time) return;
+    const float2 v = z[(int64_t)j * n_time + t];
+    const double x = (double)v.x, y = (double)v.y;
+    const double p = x * x + y * y;
+    const int64_t o = (int64_t)(m0 + j) * n_time + t;
+    lq[o] = (float)(0.5 * log(p > 1e-38 ? p : 1e-38));
+    ph[o] = (float)atan2(y, x);
+}
+
 __global__ void k_filter_post(const double2* __restrict__ z, int64_t n_time, int m0, float* __restrict__ lq,
                               float* __restrict__ ph) {
     const int j = blockIdx.y;
@@ -48,9 +84,9 @@ __global__ void k_filter_post(const double2* __restrict__ z, int64_t n_time, int
     ph[o] = (float)atan2(v.y, v.x);
 }
 
-static int fft_plan(int64_t n_time, int batch, hipfftHandle* out) {
+static int fft_plan(int64_t n_time, int batch, bool f32, hipfftHandle* out) {
     Context& c = ctx();
-    const uint64_t key = ((uint64_t)n_time << 20) | (uint64_t)batch;
+    const uint64_t key = ((uint64_t)n_time << 21) | ((uint64_t)f32 << 20) | (uint64_t)batch;
     auto it = c.fft_plans.find(key);
     if (it != c.fft_plans.end()) {
         *out = it->second;
@@ -60,7 +96,8 @@ static int fft_plan(int64_t n_time, int batch, hipfftHandle* out) {
     EXM_FFT(hipfftCreate(&p));
     int n[1] = {(int)n_time};
     size_t ws = 0;
-    EXM_FFT(hipfftMakePlanMany(p, 1, n, nullptr, 1, (int)n_time, nullptr, 1, (int)n_time, HIPFFT_Z2Z, batch, &ws));
+    EXM_FFT(hipfftMakePlanMany(p, 1, n, nullptr, 1, (int)n_time, nullptr, 1, (int)n_time, f32 ? HIPFFT_C2C : HIPFFT_Z2Z,
+                               batch, &ws));
     EXM_FFT(hipfftSetStream(p, c.stream));
     c.fft_plans[key] = p;
     *out = p;
@@ -71,11 +108,16 @@ int launch_filter(Bank* bank, const double* dH, const double* dL, const double* 
     Context& c = ctx();
     const int64_t n_time = bank->n_time;
     const int nf = bank->n_freq;
-    // chunk of templates whose complex128 padded rows fit in 1 GiB of workspace
-    const int chunk_max = (int)std::max<int64_t>(1, (1ll << 30) / (16 * n_time));
+    // chunk of templates whose padded rows fit in the workspace (128 MiB by default: each transform's
+    // passes then stay in the 256 MB last-level cache; measured 11.1 ms against 12.6 ms with 1 GiB for
+    // 2000 templates x 131072 samples)
+    const bool f32 = c.filter_f32 != 0;
+    const int64_t elem = f32 ? (int64_t)sizeof(float2) : (int64_t)sizeof(double2);
+    const int64_t ws = c.filter_bytes > 0 ? c.filter_bytes : (128ll << 20);
+    const int chunk_max = (int)std::max<int64_t>(1, ws / (elem * n_time));
     const int chunk = std::min(bank->n_tmpl, chunk_max);
-    double2* pad = nullptr;
-    EXM_TRY(scratch(0, sizeof(double2) * (size_t)chunk_max * n_time, (void**)&pad));
+    void* pad = nullptr;
+    EXM_TRY(scratch(0, (size_t)elem * chunk_max * n_time, &pad));
     double* seg = nullptr;
     EXM_TRY(scratch(2, sizeof(double) * 6 * (size_t)nf + 256, (void**)&seg));
     double2* ddH = (double2*)seg;
@@ -95,8 +137,12 @@ int launch_filter(Bank* bank, const double* dH, const double* dL, const double* 
         for (int m0 = 0; m0 < bank->n_tmpl; m0 += chunk) {
             const int nb = std::min(chunk, bank->n_tmpl - m0);
             dim3 grid((unsigned)((n_time + 255) / 256), (unsigned)nb);
-            hipLaunchKernelGGL(k_filter_pack, grid, dim3(256), 0, c.stream, bank->d_W, d, s2, nf, bank->lo, n_time,
-                               m0, pad);
+            if (f32)
+                hipLaunchKernelGGL(k_filter_pack32, grid, dim3(256), 0, c.stream, bank->d_W, d, s2, nf, bank->lo,
+                                   n_time, m0, (float2*)pad);
+            else
+                hipLaunchKernelGGL(k_filter_pack, grid, dim3(256), 0, c.stream, bank->d_W, d, s2, nf, bank->lo,
+                                   n_time, m0, (double2*)pad);
             EXM_HIP(hipGetLastError());
             // plans exist for power-of-two batches (and the full chunk) only, so that banks of any size
             // share a few plans; rows past nb in the work buffer are transformed and ignored
@@ -104,9 +150,16 @@ int launch_filter(Bank* bank, const double* dH, const double* dL, const double* 
             while (nplan < nb) nplan <<= 1;
             nplan = std::min(nplan, chunk_max);
             hipfftHandle plan;
-            EXM_TRY(fft_plan(n_time, nplan, &plan));
-            EXM_FFT(hipfftExecZ2Z(plan, (hipfftDoubleComplex*)pad, (hipfftDoubleComplex*)pad, HIPFFT_FORWARD));
-            hipLaunchKernelGGL(k_filter_post, grid, dim3(256), 0, c.stream, pad, n_time, m0, lq, ph);
+            EXM_TRY(fft_plan(n_time, nplan, f32, &plan));
+            if (f32) {
+                EXM_FFT(hipfftExecC2C(plan, (hipfftComplex*)pad, (hipfftComplex*)pad, HIPFFT_FORWARD));
+                hipLaunchKernelGGL(k_filter_post32, grid, dim3(256), 0, c.stream, (const float2*)pad, n_time, m0, lq,
+                                   ph);
+            } else {
+                EXM_FFT(hipfftExecZ2Z(plan, (hipfftDoubleComplex*)pad, (hipfftDoubleComplex*)pad, HIPFFT_FORWARD));
+                hipLaunchKernelGGL(k_filter_post, grid, dim3(256), 0, c.stream, (const double2*)pad, n_time, m0, lq,
+                                   ph);
+            }
             EXM_HIP(hipGetLastError());
         }
     }

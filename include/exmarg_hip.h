@@ -215,6 +215,16 @@ int exm_set_gather_precision(int p);
  * autotuner exmarg_amd.tune chooses it); no reference counterpart. */
 int exm_set_gather_tile(int n);
 
+/* Workspace of one batched transform of the matched filter (templates per transform = bytes / (16
+ * n_time), / (8 n_time) for the float32 transform): 0 restores the default 128 MiB, which keeps a
+ * transform's passes in the last-level cache.  A tuning knob; no reference counterpart. */
+int exm_set_filter_workspace(int64_t bytes);
+
+/* Arithmetic of the matched filter's transform: 0 float64 (Z2Z, default), 1 float32 (C2C; weighting,
+ * modulus and phase stay float64).  The float32 transform is the reference GPU path's
+ * (src/exmarg/search/bank.py:66-75 keeps complex64); part of the precision form. */
+int exm_set_filter_precision(int p);
+
 /* The current tuning: gather kernel choice, piece length, tile and precision (0 where the default
  * applies). */
 int exm_get_gather_tuning(int* kernel, int* piece, int* tile, int* precision);

@@ -72,6 +72,32 @@ def test_filter_matches_fp64_oracle(hip, oracle):
         assert np.abs(dph).max() < 1e-6, k
 
 
+def test_filter_float32_transform(hip, oracle):
+    """the float32 transform (precision form) against the FP64 oracle: ln|z| and the phase within 2e-5
+    wherever |z| is above a hundredth of its maximum (the transform's float32 error is relative to the
+    largest values, so it grows as 1 / |z| below)"""
+    from exmarg_amd.search.bank import MassBank
+    say("filter: float32 transform, 24 templates, T=4 s, 20-256 Hz")
+    seg = _segment()
+    bank_rows = synth.chirp_mass_grid_bank(30.0, 0.8, 0.1, 24)
+    bank = MassBank(bank_rows[:, 0], bank_rows[:, 1], bank_rows[:, 2], seg)
+    d = {I: seg["d_" + I][0] for I in "HL"}
+    s2 = {I: seg["sig2"] for I in "HL"}
+    hip.call("exm_set_filter_precision", 1)
+    try:
+        got = bank.trajectories(d, s2).to_host()
+    finally:
+        hip.call("exm_set_filter_precision", 0)
+    ref = oracle.filter_fp64(bank.W, bank.lo, bank.n_time, d, s2)
+    for I in "HL":
+        lq, ph = ref["lq" + I], ref["ph" + I]
+        big = lq > lq.max(axis=1, keepdims=True) - np.log(1e2)
+        assert np.abs(got["lq" + I].astype(np.float64) - lq)[big].max() < 2e-5
+        dph = np.angle(np.exp(1j * (got["ph" + I].astype(np.float64) - ph)))
+        assert np.abs(dph[big]).max() < 2e-5
+        assert np.abs(got["lq" + I].astype(np.float64) - lq).max() > 0.0      # a different computation
+
+
 def test_filter_single_bin_closed_form(hip):
     from exmarg_amd.search.bank import MassBank
     say("filter: single-bin closed form")
