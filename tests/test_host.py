@@ -189,3 +189,29 @@ def test_library_fails_loudly_without_a_device():
     rc = _hip.load().exm_store_create(1, 1, 49, 2, 2, None, 0.0, 1.0, 0.0, 1.0, None)
     assert rc == _hip.EXM_ERR_STATE
     assert "exm_init" in _hip.last_error()
+
+
+def test_rng_state_is_numpys_pcg64_state():
+    """the four words handed to the device draw are the PCG64 state and increment of default_rng(seed)"""
+    from exmarg_amd.search.gather import rng_state
+    for seed in (0, 5, 2**40 + 3):
+        st = np.random.default_rng(seed).bit_generator.state
+        assert st["bit_generator"] == "PCG64" and st["has_uint32"] == 0
+        w = [int(x) for x in rng_state(seed)]
+        assert (w[0] << 64) | w[1] == st["state"]["state"]
+        assert (w[2] << 64) | w[3] == st["state"]["inc"]
+
+
+def test_template_bytes_covers_records_and_launch_work():
+    from exmarg_amd.search.batch import template_bytes
+    per = template_bytes(87, 2048)
+    assert per >= 87 * 2048 * 8 + 2048 * 4          # walk records and walk
+    assert per - template_bytes(87, 2048, reads=0) == 54 * (16 * 87 + 2 * 201 + 32)
+
+
+def test_tune_cache_roundtrip(tmp_path, monkeypatch):
+    from exmarg_amd import tune
+    monkeypatch.setenv("EXMARG_TUNE_CACHE", str(tmp_path / "t.json"))
+    assert tune._load() == {}
+    tune._save({"k": {"form": 3, "piece": 4, "tile": 1024, "ms": 1.0}})
+    assert tune._load()["k"]["piece"] == 4
