@@ -31,6 +31,7 @@ SIGNATURES = {
     "exm_mem_info": [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
     "exm_store_create": [I32, I32, I32, I32, I32, P, F64, F64, F64, F64, HP],
     "exm_store_set_table": [H, I32, P],
+    "exm_store_load_files": [H, I32, P, P, P, I32, ctypes.POINTER(ctypes.c_double)],
     "exm_store_fill_synthetic": [H, I32, U64, I64, I32],
     "exm_store_read_table": [H, I32, P],
     "exm_bank_create": [P, I32, I32, I32, I64, HP],

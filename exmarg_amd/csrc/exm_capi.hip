@@ -391,6 +391,21 @@ int exm_store_set_table(exm_handle h, int k, const float* table) {
     return EXM_OK;
 }
 
+int exm_store_load_files(exm_handle h, int n, const char* const* paths, const int64_t* offsets, const int* tables,
+                         int readers, double* seconds) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Store, st, h, K_STORE);
+    EXM_CHECK(n >= 0 && (n == 0 || (paths && offsets && tables)), EXM_ERR_ARG, "NULL argument");
+    for (int k = 0; k < n; ++k) {
+        EXM_CHECK(tables[k] >= 0 && tables[k] < st->ntab, EXM_ERR_ARG, "table index %d out of range [0, %d)",
+                  tables[k], st->ntab);
+        EXM_CHECK(paths[k] != nullptr && offsets[k] >= 0, EXM_ERR_ARG, "bad file or offset for table %d", k);
+    }
+    if (n == 0) return EXM_OK;
+    return store_load_files(st, n, paths, offsets, tables, readers, seconds);
+}
+
 int exm_store_fill_synthetic(exm_handle h, int k, uint64_t seed, int64_t node, int kind) {
     LOCK;
     EXM_TRY(ensure_init());

@@ -236,6 +236,9 @@ int launch_lse(const JobSet& js, int ntau, const double* om, const double* os, c
                double* d_out);
 int launch_fill(float* dst, int64_t n, float v);
 int launch_copy_calibration(float4* dst, const float4* src, int64_t n);
+// tables from store files (file, byte offset of the float32 member, device index) through pinned buffers
+int store_load_files(Store* st, int n, const char* const* paths, const int64_t* offsets, const int* tables,
+                     int readers, double* seconds);
 int launch_source_frame(const double* lamN, const double* yN, int nrow, int nm, const double* M, const double* yq,
                         const double* m1, int nm1, const double* z, int nz, const double* w, double* surf,
                         double* out, double* ln_b);

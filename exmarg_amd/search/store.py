@@ -70,6 +70,7 @@ class RedshiftStore:
     device: object = field(default=None, repr=False)
     shape: tuple = None
     synthetic: dict = None
+    files: list = None          # (path, byte offset of the float32 T member) per node, for direct ingest
 
     def table_shape(self):
         if self.tables is not None:
@@ -77,10 +78,14 @@ class RedshiftStore:
         return tuple(self.shape)
 
 
-def load_redshift_store(store_dir, pinned=False, readers=6, mmap=False):
-    """every node of the redshift store in host memory (src/exmarg/search/store.py:90-107).
-    pinned is accepted for the reference signature: the upload stages through the library's own
-    buffers.  mmap=True maps the tables from disk instead of reading them."""
+def load_redshift_store(store_dir, pinned=False, readers=6, mmap=False, host=True):
+    """every node of the redshift store (src/exmarg/search/store.py:90-107).
+
+    host=True (the reference's behaviour): the tables are read into host memory (mmap=True maps them
+    from disk instead).  host=False: only the node metadata and the files' member offsets are read;
+    upload() then streams the tables from the files into HBM with `readers` threads through pinned
+    buffers (exm_store_load_files), never holding them in host memory.  pinned is accepted for the
+    reference signature: uploads stage through the library's own pinned buffers."""
     files = glob.glob(os.path.join(store_dir, "bd_*.npz"))
     if not files:
         raise FileNotFoundError("no bd_*.npz in %s" % store_dir)
@@ -91,14 +96,27 @@ def load_redshift_store(store_dir, pinned=False, readers=6, mmap=False):
                          h["beta"], h["delta"], h["tau_idx"], f))
     head.sort(key=lambda r: r[0])
     s, alo, ahi, beta, delta, tau, files = zip(*head)
-    if mmap:
+    members = []
+    for f in files:
+        off, shape, fortran, dtype = _member_offset(f, "T.npy")
+        if fortran or np.dtype(dtype) != np.float32 or len(shape) != 4:
+            raise ValueError("%s: T must be a C-order float32 (ntau, nphif, nb, nd) array" % f)
+        members.append((f, off, tuple(shape)))
+    if len({m[2] for m in members}) != 1:
+        raise ValueError("the tables of a store must share their shape")
+    if not host:
+        tables = None
+    elif mmap:
         tables = [map_npz_member(f, "T.npy") for f in files]
     else:
         with ThreadPoolExecutor(max_workers=max(1, int(readers))) as ex:
             tables = list(ex.map(lambda f: read_npz_member(f, "T.npy"), files))
-    return RedshiftStore(s=np.array(s), alpha_lo=np.array(alo), alpha_hi=np.array(ahi), beta=list(beta),
-                         delta=list(delta), tau_idx=tau[0], tables=tables,
-                         n_max=int(os.path.basename(files[0]).split("_n")[1].split("_")[0]))
+    st = RedshiftStore(s=np.array(s), alpha_lo=np.array(alo), alpha_hi=np.array(ahi), beta=list(beta),
+                       delta=list(delta), tau_idx=tau[0], tables=tables,
+                       n_max=int(os.path.basename(files[0]).split("_n")[1].split("_")[0]),
+                       shape=members[0][2], files=[(m[0], m[1]) for m in members])
+    st.readers = int(readers)
+    return st
 
 
 class DeviceStore:
@@ -151,6 +169,23 @@ def upload(store):
     if store.device is not None:
         return store.device
     _check_grids(store)
+    in_ram = store.tables is not None and not any(isinstance(T, np.memmap) or
+                                                  (getattr(T, "base", None) is not None and
+                                                   isinstance(T.base, np.memmap)) for T in store.tables)
+    if store.files is not None and not in_ram:
+        # straight from the files (mapped or not read): reader threads, pinned buffers, async copies
+        dev = _create(tuple(store.shape), len(store.files), store.beta[0], store.delta[0], store.tau_idx)
+        paths = [os.fsencode(p) for p, _ in store.files]
+        cpaths = (ctypes.c_char_p * len(paths))(*paths)
+        offs = np.array([o for _, o in store.files], np.int64)
+        idx = np.arange(len(paths), dtype=np.int32)
+        secs = ctypes.c_double()
+        _hip.call("exm_store_load_files", int(dev.handle), len(paths), ctypes.cast(cpaths, ctypes.c_void_p),
+                  _hip.ptr(offs), _hip.ptr(idx),
+                  int(getattr(store, "readers", 6)), ctypes.byref(secs))
+        store.ingest_seconds = secs.value
+        store.device = dev
+        return dev
     if store.tables is None:
         raise ValueError("store has no host tables to upload")
     shape = tuple(store.tables[0].shape)

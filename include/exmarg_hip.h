@@ -68,6 +68,15 @@ int exm_store_create(int ntab, int ntau, int nphif, int nb, int nd, const int64_
  * Replaces one copy of src/exmarg/search/store.py:117-118. */
 int exm_store_set_table(exm_handle store, int k, const float* table);
 
+/* Tables read straight from the store files: for table k, the file paths[k] holds the float32 C-order
+ * (ntau, nphif, nb, nd) array at byte offset offsets[k] (the uncompressed T.npy member of a bd_*.npz)
+ * and goes to device table tables[k].  `readers` threads read chunks into pinned buffers, copies run on
+ * their own stream and the layout transform of one table overlaps the reading of the next; *seconds
+ * receives the wall time.  Replaces the read and upload of src/exmarg/search/store.py:55-87 and
+ * pe.py:146-154 (readers=6 there); the tables never pass through host arrays. */
+int exm_store_load_files(exm_handle store, int n, const char* const* paths, const int64_t* offsets,
+                         const int* tables, int readers, double* seconds);
+
 /* Fill table k on the device with synthetic store values (seed, node, kind 0 smooth / 1 rough),
  * bit-identical to the host writer of the synthetic store files.  No reference counterpart:
  * it stands in for reading 0.75 GB files per node (src/exmarg/search/store.py:55-67). */
