@@ -47,7 +47,8 @@ def parse():
     p.add_argument("--filter-mib", type=int, default=0, help="matched-filter workspace per transform (MiB, 0 default)")
     p.add_argument("--filter-precision", choices=["f64", "f32"], default="f64",
                    help="matched-filter transform (weighting, modulus and phase stay float64)")
-    p.add_argument("--mode", choices=["plain", "serve", "stream", "batch"], default="plain",
+    p.add_argument("--ingest-nodes", type=int, default=4, help="ingest: full-size tables written and read back")
+    p.add_argument("--mode", choices=["plain", "serve", "stream", "batch", "ingest"], default="plain",
                    help="plain: pe_intrinsic on one segment per step (the headline); serve: the request loop "
                         "under Poisson arrivals; stream: pe_redshift over 8 s segments with an estimated spectrum; "
                         "batch: 256 segments of uneven length in batched launches (float32 form, autotuned)")
@@ -426,6 +427,51 @@ def main_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def main_ingest(args):
+    """store ingest: full-size tables written as bd_*.npz files, then loaded into HBM through host arrays
+    (the reference's way) and straight from the files; one JSON line with both rates"""
+    import shutil
+    import tempfile
+    from exmarg_amd import _hip, synth
+    from exmarg_amd.search import store as stm
+    _hip.init(0)
+    info = _hip.device_info()
+    n = args.ingest_nodes
+    tmp = tempfile.mkdtemp(prefix="exm_store_")
+    try:
+        src = stm.synthetic_store(n_nodes=n, seed=0)
+        beta, delta, _ = synth.store_grids()
+        arrays = dict(s=src.s, beta=beta, delta=delta, tau_idx=src.tau_idx, alpha=synth.ALPHA.copy(),
+                      tables=[src.device.read_table(k) for k in range(n)])
+        stm.free(src)
+        synth.write_store(tmp, arrays)
+        nbytes = sum(T.nbytes for T in arrays["tables"])
+        del arrays
+        t0 = time.perf_counter()
+        a = stm.load_redshift_store(tmp, readers=6)
+        t1 = time.perf_counter()
+        stm.upload(a)
+        t2 = time.perf_counter()
+        stm.free(a)
+        del a
+        t3 = time.perf_counter()
+        b = stm.load_redshift_store(tmp, readers=6, host=False)
+        stm.upload(b)
+        t4 = time.perf_counter()
+        stm.free(b)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    out = {"metric": "store ingest: GB/s from bd_*.npz files to HBM-resident device layout",
+           "value": nbytes / (t4 - t3) / 1e9, "unit": "GB/s", "n_gpus": 1, "higher_is_better": True,
+           "tables": n, "bytes": nbytes,
+           "direct_s": t4 - t3, "host_path_s": {"read": t1 - t0, "upload": t2 - t1, "total": t2 - t0},
+           "host_path_GB_per_s": nbytes / (t2 - t0) / 1e9,
+           "note": "files freshly written, so reads come from the page cache; 6 reader threads both ways",
+           "data": "synthetic", "config": {"workload": "%d full (87, 49, 200, 219) float32 tables" % n,
+                                           "device": info["name"]}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -436,6 +482,8 @@ def main():
         return main_serve(args)
     if args.mode == "stream":
         return main_stream(args)
+    if args.mode == "ingest":
+        return main_ingest(args)
     if args.mode == "batch":
         if args.precision == "f64" and "--precision" not in " ".join(sys.argv):
             args.precision = "f32"
