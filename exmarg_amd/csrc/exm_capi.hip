@@ -805,8 +805,8 @@ int exm_batch_create(exm_handle hstore, int64_t max_templates, int n_walk, exm_h
     b->ntau = st->ntau;
     b->cap = max_templates;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t s_bytes = al(sizeof(uint2) * (size_t)max_templates * st->ntau * n_walk);
-    const size_t wa_bytes = al(sizeof(float) * (size_t)max_templates * n_walk);
+    const size_t s_bytes = al(sizeof(uint2) * (size_t)max_templates * st->ntau * n_walk + WALK_PAD_BYTES);
+    const size_t wa_bytes = al(sizeof(float) * (size_t)max_templates * n_walk + WALK_PAD_BYTES);
     const size_t lw_bytes = al(sizeof(float) * (size_t)max_templates);
     const size_t nt_bytes = al(sizeof(double) * (size_t)max_templates);
     b->bytes = s_bytes + wa_bytes + lw_bytes + nt_bytes;

@@ -70,6 +70,9 @@ struct Object {
 // (float32): one beta row of all phases is contiguous, nphif * nd floats.  The allocation ends with
 // TABLE_PAD_FLOATS spare floats so that row copies in whole 64-float blocks stay inside it.
 constexpr int TABLE_PAD_FLOATS = 256;
+// Spare bytes after the walk buffers (walked ln|z_L| and the lag stream): the band kernel's wave form
+// reads every piece as PN consecutive records from one address, up to 7 records past a read's range.
+constexpr size_t WALK_PAD_BYTES = 64;
 struct Store : Object {
     int ntab = 0, ntau = 0, nphif = 0, nb = 0, nd = 0;
     int64_t tstride = 0;   // floats per table

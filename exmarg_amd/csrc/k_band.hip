@@ -194,13 +194,15 @@ __device__ __forceinline__ double band_lookup(const float* R0, const float* R1, 
     const double xb = fmin(fmax(fma(aL, g.rdb, offb), 0.0), g.xbmax);
     const double fb = xb - bd;
     // Delta coordinate: ln q_L moves with the beta clamp, ln q_H stays
+    // xd lies in [0, nd - 1 - 1e-6] (NaN included: fmax returns the number), so d0 = floor(xd) is in
+    // [0, nd - 2] and its fraction xd - d0 is exact
     const double xd = fmin(fmax(fma(-g.qbd, xb, fma((double)__uint_as_float(e.y), g.rdd, cj)), 0.0), g.xdmax);
-    int d0 = (int)xd;
-    d0 = min(max(d0, 0), g.nd - 2);
-    const double fd = xd - (double)d0;
+    const int d0 = (int)xd;
+    const double fd = __builtin_amdgcn_fract(xd);
     const V fc = (V)(code & FC_MASK) * (V)(1.0 / FC_SCALE);
     const int o0 = (int)__umul24((code >> BIN_SHIFT) & 0x3F, (uint32_t)g.nd) + d0;
-    const int o1 = (code & DOWN) ? o0 - g.nd : o0 + g.nd;
+    // the second phase bin: the stored bin above (+nd) or, from the mirror on, below (-nd)
+    const int o1 = o0 + __mul24(1 - 2 * (int)((code >> 24) & 1u), g.nd);
     const V fdv = (V)fd, fbv = (V)fb;
     const V t00 = R0[o0], t01 = R0[o0 + 1], t10 = R1[o0], t11 = R1[o0 + 1];
     const V u00 = R0[o1], u01 = R0[o1 + 1], u10 = R1[o1], u11 = R1[o1 + 1];
@@ -398,6 +400,18 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     }
 }
 
+// records of one piece: PN consecutive walked times (positions past the read's range end lie inside
+// the buffers' WALK_PAD_BYTES and are not used)
+template <int PN>
+__device__ __forceinline__ void load_piece(const float* __restrict__ wp, const uint2* __restrict__ sp,
+                                           float (&aL)[PN], uint2 (&ev)[PN]) {
+#pragma unroll
+    for (int k = 0; k < PN; ++k) {
+        aL[k] = wp[k];
+        ev[k] = sp[k];
+    }
+}
+
 // lane-shuffle helpers (wave64)
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 __device__ __forceinline__ int shfl_i(int v, int src) { return __shfl(v, src, 64); }
@@ -410,12 +424,54 @@ __device__ __forceinline__ void lse_merge(double& m, double& s, double m2, doubl
     m = fmax(m, m2);
 }
 
+// DPP lane moves within a wave64 (VALU, no LDS round trip): lane l receives the value of lane l - n of
+// its row of 16 lanes (row_shr:n), or of lane 15 / 31 of the rows below it (row_bcast:15 / :31); lanes
+// without a source lane, or outside the row and bank masks, receive `old`.
+constexpr int DPP_ROW_SHR = 0x110, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+
+template <int CTRL, int ROWM, int BANKM>
+__device__ __forceinline__ int dpp_i(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWM, BANKM, false);
+}
+
+template <int CTRL, int ROWM, int BANKM>
+__device__ __forceinline__ float dpp_f(float old, float v) {
+    return __int_as_float(dpp_i<CTRL, ROWM, BANKM>(__float_as_int(old), __float_as_int(v)));
+}
+
+template <int CTRL, int ROWM, int BANKM>
+__device__ __forceinline__ double dpp_d(double old, double v) {
+    const int lo = dpp_i<CTRL, ROWM, BANKM>(__double2loint(old), __double2loint(v));
+    const int hi = dpp_i<CTRL, ROWM, BANKM>(__double2hiint(old), __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// One step of a segmented inclusive scan towards the last lane of each run of equal `seg`: the value of
+// the DPP source lane (`src`, the scan's input for the first three steps, its running value after) is
+// combined when that lane is in the same run.  Runs are contiguous, so the sequence shr 1, 2, 3 (from the
+// input), shr 4 (banks 1-3), shr 8 (banks 2-3), bcast 15 (rows 1, 3), bcast 31 (rows 2, 3) leaves at every
+// lane the combination over its run up to itself.
+template <int CTRL, int ROWM, int BANKM>
+__device__ __forceinline__ bool seg_max_step(int seg, float src, float& M) {
+    const int s2 = dpp_i<CTRL, ROWM, BANKM>(INT_MIN, seg);
+    const float m2 = dpp_f<CTRL, ROWM, BANKM>(-INFINITY, src);
+    const bool same = s2 == seg;
+    M = same && m2 > M ? m2 : M;
+    return same;
+}
+
+template <int CTRL, int ROWM, int BANKM>
+__device__ __forceinline__ void seg_sum_step(bool same, double src, double& t) {
+    const double t2 = dpp_d<CTRL, ROWM, BANKM>(0.0, src);
+    t = same ? t + t2 : t;
+}
+
 // The wave form of the band kernel: one workgroup per (tile of reads of one table, lag) and the same
 // three-row LDS ring as k_gather_piece, but each wave deals the pieces of its own 64 reads (lane l of
 // wave w owns read w + 16 l) in rounds of 64 without workgroup barriers: a lane finds its piece's read
 // by a binary search over the wave's piece offsets, takes the read's parameters from the owning lane
-// by shuffles, computes the piece, and a segmented reduction over lanes (shuffles) leaves each read's
-// (max, sum) of the round at its first lane, which the owning lane merges.  Only the ring rotation at
+// by shuffles, computes the piece, and a segmented scan over lanes (DPP moves) leaves each read's
+// (max, sum) of the round at its last lane, which the owning lane merges.  Only the ring rotation at
 // the end of a row synchronizes the workgroup, so the waves of a CU overlap each other's memory
 // latency.  Merges are float64; their order follows lane positions, so a read's result depends on its
 // neighbours in the workgroup only through float64 rounding.
@@ -497,9 +553,10 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
         // piece is found and its records requested before the current round's lookups, so that the
         // shuffles and global loads of one round overlap the arithmetic of the previous one.
         struct Piece {
-            int o, pa, pe, m, first;
+            int o, pa, pe, m, last;
             bool act;
-            double offb, lw;
+            double offb;
+            float lw;
             float aL[PN];
             uint2 ev[PN];
         };
@@ -518,86 +575,83 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
             P.o = o;
             P.m = shfl_i(my_m, o);
             P.offb = shfl_d(my_offb, o);
-            P.lw = (double)__shfl(my_lw, o, 64);
+            P.lw = __shfl(my_lw, o, 64);
             P.pa = lo_o + (p - qex_o) * PN;
             P.pe = min(P.pa + PN, hi_o);
-            P.first = P.act ? max(qex_o, base) - base : lane;          // the first lane of the read's run
-            if (P.act) {
-                const float* wam = wa + (int64_t)P.m * nw;
-                const uint2* smp = stream + ((int64_t)P.m * ntau + r) * nw;
-#pragma unroll
-                for (int k = 0; k < PN; ++k) {
-                    const int i = min(P.pa + k, P.pe - 1);
-                    P.aL[k] = wam[i];
-                    P.ev[k] = smp[i];
-                }
-            }
+            // the last lane of the read's run in this round
+            P.last = P.act ? min(qex_o + (hi_o - lo_o + PN - 1) / PN, base + 64) - 1 - base : lane;
+            if (P.act)
+                load_piece<PN>(wa + (int64_t)P.m * nw + P.pa, stream + ((int64_t)P.m * ntau + r) * nw + P.pa, P.aL,
+                               P.ev);
         };
-        Piece cur;
-        if (Qw > 0) fetch(0, cur);
-        for (int base = 0; base < Qw; base += 64) {
-            const bool more = base + 64 < Qw;
-            Piece nxt;
-            if (more) fetch(base + 64, nxt);
+        // one round: the next round's piece is fetched into nxt, then cur is computed and merged
+        auto round = [&](int base, Piece& cur, Piece& nxt) __attribute__((always_inline)) {
+            if (base + 64 < Qw) fetch(base + 64, nxt);
             const bool act = cur.act;
-            const int o = cur.o;
             double mx = -1e300, sm = 0.0;
             if (act) {
                 const int pa = cur.pa, pe = cur.pe;
-                const double cj = offd + cur.offb * qbd;
-                if constexpr (sizeof(V) == 4) {
+                const double cj = offd + cur.offb * qbd, lw = (double)cur.lw;
+                if constexpr (PN <= 4) {
+                    // the piece's values first, then its (max, sum); -inf values leave the sum unchanged
                     double av[PN];
 #pragma unroll
                     for (int k = 0; k < PN; ++k)
-                        av[k] = pa + k < pe ? band_lookup<V>(R0, R1, geo, bd, cur.offb, cj, cur.lw, cur.aL[k], cur.ev[k])
+                        av[k] = pa + k < pe ? band_lookup<V>(R0, R1, geo, bd, cur.offb, cj, lw, cur.aL[k], cur.ev[k])
                                             : -1e300;
-                    mx = av[0];
 #pragma unroll
-                    for (int k = 1; k < PN; ++k) mx = fmax(mx, av[k]);
+                    for (int k = 0; k < PN; ++k) mx = av[k] > mx ? av[k] : mx;
 #pragma unroll
                     for (int k = 0; k < PN; ++k) sm += (double)__expf((float)(av[k] - mx));
                 } else {
+                    // a running (max, sum) over the lookups (fewer live registers for long pieces)
 #pragma unroll
                     for (int k = 0; k < PN; ++k) {
                         const double a = pa + k < pe
-                                             ? band_lookup<V>(R0, R1, geo, bd, cur.offb, cj, cur.lw, cur.aL[k], cur.ev[k])
+                                             ? band_lookup<V>(R0, R1, geo, bd, cur.offb, cj, lw, cur.aL[k], cur.ev[k])
                                              : -1e300;
                         const double dlt = a - mx;
                         const double ex = (double)__expf(-(float)fabs(dlt));
                         sm = dlt > 0.0 ? fma(sm, ex, 1.0) : sm + ex;
-                        mx = fmax(mx, a);
+                        mx = dlt > 0.0 ? a : mx;
                     }
                 }
             }
-            // segmented reduction over each read's run of lanes in this round, in two passes: the run's
-            // maximum (float, suffix scan towards the run's first lane), then the sum of every lane's
-            // sum rescaled to that maximum (float64), so that one exp per lane suffices
-            const int seg = act ? o : -1 - lane;
-            int same = 0;                                   // bit k: lane + 2^k is in this lane's run
-            float M = (float)mx;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                const int src = (lane + (1 << k)) & 63;
-                const int g2 = shfl_i(seg, src);
-                const float m2 = __shfl(M, src, 64);
-                if (lane + (1 << k) < 64 && g2 == seg) {
-                    same |= 1 << k;
-                    M = fmaxf(M, m2);
-                }
-            }
-            const double Mr = (double)__shfl(M, cur.first, 64);
-            double t = act ? sm * (double)__expf((float)(mx - Mr)) : 0.0;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                const double t2 = shfl_d(t, (lane + (1 << k)) & 63);
-                if ((same >> k) & 1) t += t2;
-            }
-            // the owning lane takes its read's (max, sum) of this round from the run's first lane
+            // segmented scan over each read's run of lanes in this round, in two passes: the run's maximum
+            // (float), then the sum of every lane's sum rescaled to that maximum (float64), so that one
+            // exp per lane suffices; both end at the run's last lane
+            const int seg = act ? cur.o : -1 - lane;
+            const float M0 = (float)mx;
+            float M = M0;
+            const bool s1 = seg_max_step<DPP_ROW_SHR | 1, 0xf, 0xf>(seg, M0, M);
+            const bool s2 = seg_max_step<DPP_ROW_SHR | 2, 0xf, 0xf>(seg, M0, M);
+            const bool s3 = seg_max_step<DPP_ROW_SHR | 3, 0xf, 0xf>(seg, M0, M);
+            const bool s4 = seg_max_step<DPP_ROW_SHR | 4, 0xf, 0xe>(seg, M, M);
+            const bool s5 = seg_max_step<DPP_ROW_SHR | 8, 0xf, 0xc>(seg, M, M);
+            const bool s6 = seg_max_step<DPP_ROW_BCAST15, 0xa, 0xf>(seg, M, M);
+            const bool s7 = seg_max_step<DPP_ROW_BCAST31, 0xc, 0xf>(seg, M, M);
+            const double Mr = (double)__shfl(M, cur.last, 64);
+            const double t0 = act ? sm * (double)__expf((float)(mx - Mr)) : 0.0;
+            double t = t0;
+            seg_sum_step<DPP_ROW_SHR | 1, 0xf, 0xf>(s1, t0, t);
+            seg_sum_step<DPP_ROW_SHR | 2, 0xf, 0xf>(s2, t0, t);
+            seg_sum_step<DPP_ROW_SHR | 3, 0xf, 0xf>(s3, t0, t);
+            seg_sum_step<DPP_ROW_SHR | 4, 0xf, 0xe>(s4, t, t);
+            seg_sum_step<DPP_ROW_SHR | 8, 0xf, 0xc>(s5, t, t);
+            seg_sum_step<DPP_ROW_BCAST15, 0xa, 0xf>(s6, t, t);
+            seg_sum_step<DPP_ROW_BCAST31, 0xc, 0xf>(s7, t, t);
+            // the owning lane takes its read's (max, sum) of this round from the run's last lane
             const bool in_round = q > 0 && qex < base + 64 && qex + q > base;
-            const int head = in_round ? max(qex, base) - base : 0;
-            const double hm = (double)__shfl(M, head, 64), hs = shfl_d(t, head);
+            const int tail = in_round ? min(qex + q, base + 64) - 1 - base : 0;
+            const double hm = (double)__shfl(M, tail, 64), hs = shfl_d(t, tail);
             if (in_round) lse_merge(amx, asum, hm, hs);
-            if (more) cur = nxt;
+        };
+        // two rounds per iteration, the pieces alternating between pa_ and pb_
+        Piece pa_, pb_;
+        if (Qw > 0) fetch(0, pa_);
+        for (int base = 0; base < Qw; base += 128) {
+            round(base, pa_, pb_);
+            if (base + 64 < Qw) round(base + 64, pb_, pa_);
         }
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
@@ -658,10 +712,11 @@ int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const 
     EXM_CHECK(piece_res_offset(st->nphif * st->nd) % 16 == 0, EXM_ERR_ARG, "piece results misaligned");
     EXM_CHECK(row_slot(st->nphif * st->nd) - st->nphif * st->nd <= TABLE_PAD_FLOATS, EXM_ERR_ARG,
               "row copy overruns the table padding");
-    const size_t lds = gather_mode() != 2 ? piece_res_offset(st->nphif * st->nd) + 16 : piece_lds_bytes(st->nphif * st->nd);
-    EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "three table rows and the piece results (%zu bytes) exceed the LDS", lds);
     // the wave form unless the piece form is asked for; 4 walked times per piece in the wave form
     const bool wave = gather_mode() != 2;
+    const int rowlen = st->nphif * st->nd;
+    const size_t lds = wave ? piece_res_offset(rowlen) + 16 : piece_lds_bytes(rowlen);
+    EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "three table rows and the piece results (%zu bytes) exceed the LDS", lds);
     const int pn = c.piece ? c.piece : (wave ? 4 : PIECE);
     EXM_CHECK(!wave || pn != 16, EXM_ERR_ARG, "the wave form of the band kernel takes pieces of 4 or 8 walked times");
     using Kern = decltype(&k_gather_piece<8, double>);
@@ -710,8 +765,8 @@ int launch_fill(double* dst, int64_t n, double v) {
 int launch_gather_band(Traj* traj, Store* st, const JobSet& js, Times* times, double* om, double* os) {
     const int nm = traj->n_tmpl, nw = times->n_walk;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t wa_bytes = al((size_t)nm * nw * 4), lw_bytes = al((size_t)nm * 4);
-    const size_t st_bytes = sizeof(uint2) * (size_t)nm * st->ntau * nw;
+    const size_t wa_bytes = al((size_t)nm * nw * 4 + WALK_PAD_BYTES), lw_bytes = al((size_t)nm * 4);
+    const size_t st_bytes = sizeof(uint2) * (size_t)nm * st->ntau * nw + WALK_PAD_BYTES;
     void* buf = nullptr;
     EXM_TRY(scratch(4, wa_bytes + lw_bytes + st_bytes, &buf));
     float* wa = (float*)buf;

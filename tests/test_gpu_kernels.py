@@ -302,6 +302,34 @@ def test_gather_full_geometry_one_node(hip, oracle):
     st.free(s)
 
 
+@pytest.mark.parametrize("kernel,piece", [(2, 8), (3, 4), (3, 8)], indirect=True)
+@pytest.mark.parametrize("n_walk", [8192, 24])
+def test_band_gather_skewed_rows(hip, oracle, kernel, piece, n_walk):
+    """reads whose whole walk falls in one clamped row (one read's pieces fill whole rounds of lanes)
+    beside reads spread over many rows and reads with a single piece per row, at the longest walk and a
+    short one"""
+    from exmarg_amd.search import gather as g
+    say("gather: band kernel %d, piece %d, skewed rows, walk %d" % (kernel, piece, n_walk))
+    s = _tiny_store("rough", (5, 49, 24, 17), n_nodes=1, seed=11)
+    nm = 4
+    traj = _rand_traj(nm, 16384, seed=13)
+    traj["lqH"] *= 3.0
+    traj["lqL"] *= 3.0
+    times_dev = g.select_times(traj, s.tau_idx, n_walk // 2, n_walk // 2, seed=5)
+    times = times_dev.to_host()
+    rng = np.random.default_rng(7)
+    nr = 150
+    ids = rng.integers(0, nm, nr)
+    lna = rng.uniform(-5.0, 5.0, (nr, 1))
+    lna[::3] = -40.0                                  # every walked time in the lowest row
+    lna[1::7] = 40.0                                  # every walked time in the highest row
+    tab = np.zeros((nr, 1), np.int64)
+    got = g.gather(traj, s, ids, lna, tab, times=times_dev)
+    ref = oracle.gather(traj, oracle.Tables(tables=s.tables, tau_idx=s.tau_idx), s.beta[0], s.delta[0], ids, lna,
+                        tab, times)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
+
+
 def test_wave_form_rejects_pieces_of_16(hip):
     from exmarg_amd.search import gather as g
     s = _tiny_store("smooth", (5, 49, 8, 9), n_nodes=1)
