@@ -576,29 +576,35 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
             P.m = shfl_i(my_m, o);
             P.offb = shfl_d(my_offb, o);
             P.lw = __shfl(my_lw, o, 64);
-            P.pa = lo_o + (p - qex_o) * PN;
-            P.pe = min(P.pa + PN, hi_o);
-            // the last lane of the read's run in this round
-            P.last = P.act ? min(qex_o + (hi_o - lo_o + PN - 1) / PN, base + 64) - 1 - base : lane;
-            if (P.act)
-                load_piece<PN>(wa + (int64_t)P.m * nw + P.pa, stream + ((int64_t)P.m * ntau + r) * nw + P.pa, P.aL,
-                               P.ev);
+            // lanes past the wave's last piece keep an empty range at the start of a walk
+            P.pa = P.act ? lo_o + (p - qex_o) * PN : 0;
+            P.pe = P.act ? min(P.pa + PN, hi_o) : 0;
+            // the last lane of the read's run in this round (any lane for lanes past the last piece)
+            P.last = (min(qex_o + (int)((unsigned)(hi_o - lo_o + PN - 1) / PN), base + 64) - 1 - base) & 63;
+            load_piece<PN>(wa + (int64_t)P.m * nw + P.pa, stream + ((int64_t)P.m * ntau + r) * nw + P.pa, P.aL, P.ev);
         };
-        // one round: the next round's piece is fetched into nxt, then cur is computed and merged
+        // One round: the next round's piece is fetched into nxt (after the wave's last round, a fetch of
+        // empty pieces), then cur is computed and merged.  The round has no branches, so the compiler can
+        // overlap the fetch's lane shuffles with the lookups: every slot of a piece is looked up, unused
+        // slots with the record of phase bin 0 (addresses inside the row) and their values discarded.
         auto round = [&](int base, Piece& cur, Piece& nxt) __attribute__((always_inline)) {
-            if (base + 64 < Qw) fetch(base + 64, nxt);
+            fetch(base + 64, nxt);
             const bool act = cur.act;
             double mx = -1e300, sm = 0.0;
-            if (act) {
+            {
                 const int pa = cur.pa, pe = cur.pe;
                 const double cj = offd + cur.offb * qbd, lw = (double)cur.lw;
+                auto lookup = [&](int k) -> double {
+                    const bool use = pa + k < pe;
+                    const uint2 e = make_uint2(use ? cur.ev[k].x : 0u, cur.ev[k].y);
+                    const double a = band_lookup<V>(R0, R1, geo, bd, cur.offb, cj, lw, cur.aL[k], e);
+                    return use ? a : -1e300;
+                };
                 if constexpr (PN <= 4) {
                     // the piece's values first, then its (max, sum); -inf values leave the sum unchanged
                     double av[PN];
 #pragma unroll
-                    for (int k = 0; k < PN; ++k)
-                        av[k] = pa + k < pe ? band_lookup<V>(R0, R1, geo, bd, cur.offb, cj, lw, cur.aL[k], cur.ev[k])
-                                            : -1e300;
+                    for (int k = 0; k < PN; ++k) av[k] = lookup(k);
 #pragma unroll
                     for (int k = 0; k < PN; ++k) mx = av[k] > mx ? av[k] : mx;
 #pragma unroll
@@ -607,9 +613,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
                     // a running (max, sum) over the lookups (fewer live registers for long pieces)
 #pragma unroll
                     for (int k = 0; k < PN; ++k) {
-                        const double a = pa + k < pe
-                                             ? band_lookup<V>(R0, R1, geo, bd, cur.offb, cj, lw, cur.aL[k], cur.ev[k])
-                                             : -1e300;
+                        const double a = lookup(k);
                         const double dlt = a - mx;
                         const double ex = (double)__expf(-(float)fabs(dlt));
                         sm = dlt > 0.0 ? fma(sm, ex, 1.0) : sm + ex;
