@@ -1051,7 +1051,8 @@ int exm_set_gather_kernel(int which) {
 
 int exm_set_gather_piece(int n) {
     LOCK;
-    EXM_CHECK(n == 0 || n == 4 || n == 8 || n == 16, EXM_ERR_ARG, "piece length must be 0 (default), 4, 8 or 16");
+    EXM_CHECK(n == 0 || n == 4 || n == 6 || n == 8 || n == 16, EXM_ERR_ARG,
+              "piece length must be 0 (default), 4, 6, 8 or 16");
     g_ctx.piece = n;
     return EXM_OK;
 }

@@ -209,8 +209,9 @@ int exm_batch_info(exm_handle batch, int64_t* templates_used, int64_t* reads, in
  * per device). */
 int exm_set_gather_kernel(int which);
 
-/* Walked times per piece of the band kernel: 4, 8 or 16 (0 restores the default, 8).  Any length
- * gives the same results to rounding; the choice is a tuning knob.  No reference counterpart. */
+/* Walked times per piece of the band kernel: 4, 6 or 8 in its wave form, 4, 8 or 16 in its piece form
+ * (0 restores the default: 6 in the wave form, 8 in the piece form).  Any length gives the same results
+ * to rounding; the choice is a tuning knob.  No reference counterpart. */
 int exm_set_gather_piece(int n);
 
 /* Arithmetic of the band kernel's table interpolation: 0 float64 (default), 1 float32.  Index and

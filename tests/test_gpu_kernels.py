@@ -230,7 +230,7 @@ def piece(hip, request):
     hip.call("exm_set_gather_piece", 0)
 
 
-@pytest.mark.parametrize("kernel,piece", [(2, 4), (2, 8), (2, 16), (3, 4), (3, 8)], indirect=True)
+@pytest.mark.parametrize("kernel,piece", [(2, 4), (2, 8), (2, 16), (3, 4), (3, 6), (3, 8)], indirect=True)
 def test_band_gather_clamps_and_many_reads(hip, oracle, kernel, piece):
     # beta clamped at both table edges, Delta clamped, more reads of one table than one tile holds
     from exmarg_amd.search import gather as g
@@ -302,7 +302,7 @@ def test_gather_full_geometry_one_node(hip, oracle):
     st.free(s)
 
 
-@pytest.mark.parametrize("kernel,piece", [(2, 8), (3, 4), (3, 8)], indirect=True)
+@pytest.mark.parametrize("kernel,piece", [(2, 8), (3, 4), (3, 6), (3, 8)], indirect=True)
 @pytest.mark.parametrize("n_walk", [8192, 24])
 def test_band_gather_skewed_rows(hip, oracle, kernel, piece, n_walk):
     """reads whose whole walk falls in one clamped row (one read's pieces fill whole rounds of lanes)
@@ -338,7 +338,7 @@ def test_wave_form_rejects_pieces_of_16(hip):
     hip.call("exm_set_gather_kernel", 3)
     hip.call("exm_set_gather_piece", 16)
     try:
-        with pytest.raises(ValueError, match="4 or 8"):
+        with pytest.raises(ValueError, match="4, 6 or 8"):
             g.gather(traj, s, np.array([0]), np.array([[0.0]]), np.array([[0]]), times=times)
     finally:
         hip.call("exm_set_gather_piece", 0)
