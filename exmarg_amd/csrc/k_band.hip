@@ -446,11 +446,10 @@ __device__ __forceinline__ double dpp_d(double old, double v) {
     return __hiloint2double(hi, lo);
 }
 
-// One step of a segmented inclusive scan towards the last lane of each run of equal `seg`: the value of
-// the DPP source lane (`src`, the scan's input for the first three steps, its running value after) is
-// combined when that lane is in the same run.  Runs are contiguous, so the sequence shr 1, 2, 3 (from the
-// input), shr 4 (banks 1-3), shr 8 (banks 2-3), bcast 15 (rows 1, 3), bcast 31 (rows 2, 3) leaves at every
-// lane the combination over its run up to itself.
+// One step of a segmented inclusive scan towards the last lane of each run of equal `seg`: the running
+// value of the DPP source lane is combined when that lane is in the same run.  Runs are contiguous, so
+// the sequence shr 1, shr 2, shr 4 (banks 1-3), shr 8 (banks 2-3), bcast 15 (rows 1, 3), bcast 31 (rows
+// 2, 3) leaves at every lane the combination over its run up to itself.
 template <int CTRL, int ROWM, int BANKM>
 __device__ __forceinline__ bool seg_max_step(int seg, float src, float& M) {
     const int s2 = dpp_i<CTRL, ROWM, BANKM>(INT_MIN, seg);
@@ -601,12 +600,15 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
                     return use ? a : -1e300;
                 };
                 if constexpr (PN <= 6) {
-                    // the piece's values first, then its (max, sum); -inf values leave the sum unchanged
+                    // the piece's values first, then its (max, sum); the max, only the reference of the
+                    // sum, is taken in float; -inf values leave the sum unchanged
                     double av[PN];
 #pragma unroll
                     for (int k = 0; k < PN; ++k) av[k] = lookup(k);
+                    float mf = (float)av[0];
 #pragma unroll
-                    for (int k = 0; k < PN; ++k) mx = av[k] > mx ? av[k] : mx;
+                    for (int k = 1; k < PN; ++k) mf = fmaxf(mf, (float)av[k]);
+                    mx = (double)mf;
 #pragma unroll
                     for (int k = 0; k < PN; ++k) sm += (double)__expf((float)(av[k] - mx));
                 } else {
@@ -625,25 +627,21 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
             // (float), then the sum of every lane's sum rescaled to that maximum (float64), so that one
             // exp per lane suffices; both end at the run's last lane
             const int seg = act ? cur.o : -1 - lane;
-            const float M0 = (float)mx;
-            float M = M0;
-            const bool s1 = seg_max_step<DPP_ROW_SHR | 1, 0xf, 0xf>(seg, M0, M);
-            const bool s2 = seg_max_step<DPP_ROW_SHR | 2, 0xf, 0xf>(seg, M0, M);
-            const bool s3 = seg_max_step<DPP_ROW_SHR | 3, 0xf, 0xf>(seg, M0, M);
+            float M = (float)mx;
+            const bool s1 = seg_max_step<DPP_ROW_SHR | 1, 0xf, 0xf>(seg, M, M);
+            const bool s2 = seg_max_step<DPP_ROW_SHR | 2, 0xf, 0xf>(seg, M, M);
             const bool s4 = seg_max_step<DPP_ROW_SHR | 4, 0xf, 0xe>(seg, M, M);
-            const bool s5 = seg_max_step<DPP_ROW_SHR | 8, 0xf, 0xc>(seg, M, M);
-            const bool s6 = seg_max_step<DPP_ROW_BCAST15, 0xa, 0xf>(seg, M, M);
-            const bool s7 = seg_max_step<DPP_ROW_BCAST31, 0xc, 0xf>(seg, M, M);
+            const bool s8 = seg_max_step<DPP_ROW_SHR | 8, 0xf, 0xc>(seg, M, M);
+            const bool s15 = seg_max_step<DPP_ROW_BCAST15, 0xa, 0xf>(seg, M, M);
+            const bool s31 = seg_max_step<DPP_ROW_BCAST31, 0xc, 0xf>(seg, M, M);
             const double Mr = (double)__shfl(M, cur.last, 64);
-            const double t0 = act ? sm * (double)__expf((float)(mx - Mr)) : 0.0;
-            double t = t0;
-            seg_sum_step<DPP_ROW_SHR | 1, 0xf, 0xf>(s1, t0, t);
-            seg_sum_step<DPP_ROW_SHR | 2, 0xf, 0xf>(s2, t0, t);
-            seg_sum_step<DPP_ROW_SHR | 3, 0xf, 0xf>(s3, t0, t);
+            double t = act ? sm * (double)__expf((float)(mx - Mr)) : 0.0;
+            seg_sum_step<DPP_ROW_SHR | 1, 0xf, 0xf>(s1, t, t);
+            seg_sum_step<DPP_ROW_SHR | 2, 0xf, 0xf>(s2, t, t);
             seg_sum_step<DPP_ROW_SHR | 4, 0xf, 0xe>(s4, t, t);
-            seg_sum_step<DPP_ROW_SHR | 8, 0xf, 0xc>(s5, t, t);
-            seg_sum_step<DPP_ROW_BCAST15, 0xa, 0xf>(s6, t, t);
-            seg_sum_step<DPP_ROW_BCAST31, 0xc, 0xf>(s7, t, t);
+            seg_sum_step<DPP_ROW_SHR | 8, 0xf, 0xc>(s8, t, t);
+            seg_sum_step<DPP_ROW_BCAST15, 0xa, 0xf>(s15, t, t);
+            seg_sum_step<DPP_ROW_BCAST31, 0xc, 0xf>(s31, t, t);
             // the owning lane takes its read's (max, sum) of this round from the run's last lane
             const bool in_round = q > 0 && qex < base + 64 && qex + q > base;
             const int tail = in_round ? min(qex + q, base + 64) - 1 - base : 0;
