@@ -579,69 +579,98 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
+    // walk range of this lane's read in row b (empty outside its rows)
+    const uint16_t* bj = bounds + ((int64_t)j0 + (mine ? jl : 0)) * (nb + 1);
+    auto bounds_of = [&](int b, int& lo_, int& hi_) {
+        lo_ = hi_ = 0;
+        if (mine && b >= my_first && b <= my_last) {
+            lo_ = bj[b - my_first];
+            hi_ = bj[b + 1 - my_first];
+        }
+    };
+    // the wave's pieces in one row: this lane's read's walk range, its pieces, its first piece (an
+    // exclusive sum over the lanes, by DPP) and the wave's total
+    struct Layout {
+        int lo, hi, q, qex, Qw;
+    };
+    auto layout = [&](int lo_, int hi_) -> Layout {
+        Layout L;
+        L.lo = lo_;
+        L.hi = hi_;
+        L.q = (int)((unsigned)(hi_ - lo_ + PN - 1) / PN);
+        int x = L.q;
+        x += dpp_i<DPP_ROW_SHR | 1, 0xf, 0xf>(0, x);
+        x += dpp_i<DPP_ROW_SHR | 2, 0xf, 0xf>(0, x);
+        x += dpp_i<DPP_ROW_SHR | 4, 0xf, 0xe>(0, x);
+        x += dpp_i<DPP_ROW_SHR | 8, 0xf, 0xc>(0, x);
+        x += dpp_i<DPP_ROW_BCAST15, 0xa, 0xf>(0, x);
+        x += dpp_i<DPP_ROW_BCAST31, 0xc, 0xf>(0, x);
+        L.qex = x - L.q;
+        L.Qw = __builtin_amdgcn_readlane(x, 63);
+        return L;
+    };
+    // a round's piece: its read (owning lane), walk range, parameters and records
+    struct Piece {
+        int o, pa, pe, m, last;
+        bool act;
+        double offb;
+        float lw;
+        float aL[PN];
+        uint2 ev[PN];
+    };
+    // the piece of this lane in the round starting at piece `base` of a row with layout L; lanes past the
+    // row's last piece keep an empty range at the start of a walk
+    auto fetch = [&](const Layout& L, int base, Piece& P) {
+        const int p = base + lane;
+        P.act = p < L.Qw;
+        // owner: the largest lane whose first piece is <= p (piece offsets ascend with the lane)
+        int o = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+            const int c = o + step;
+            const int qc = shfl_i(L.qex, c & 63);
+            if (c < 64 && qc <= p) o = c;
+        }
+        const int lo_o = shfl_i(L.lo, o), hi_o = shfl_i(L.hi, o), qex_o = shfl_i(L.qex, o);
+        P.o = o;
+        P.m = shfl_i(my_m, o);
+        P.offb = shfl_d(my_offb, o);
+        P.lw = __shfl(my_lw, o, 64);
+        P.pa = P.act ? lo_o + (p - qex_o) * PN : 0;
+        P.pe = P.act ? min(P.pa + PN, hi_o) : 0;
+        // the last lane of the read's run in this round (any lane for lanes past the last piece)
+        P.last = (min(qex_o + (int)((unsigned)(hi_o - lo_o + PN - 1) / PN), base + 64) - 1 - base) & 63;
+        load_piece<PN>(wa + (int64_t)P.m * nw + P.pa, stream + ((int64_t)P.m * ntau + r) * nw + P.pa, P.aL, P.ev);
+    };
+
+    // Rows flow into each other: row b + 1's layout is formed while row b runs (its bounds were requested
+    // a row earlier), and the last round of row b fetches row b + 1's first piece.
+    int n_lo, n_hi, c_lo, c_hi;
+    bounds_of(b_start, c_lo, c_hi);
+    bounds_of(b_start + 1, n_lo, n_hi);
+    Layout S = layout(c_lo, c_hi);
+    Piece pa_, pb_;
+    bool pre = false;                           // pa_ holds row b's first piece
     for (int b = b_start; b <= b_end; ++b) {
         if (b > b_start && b + 2 <= b_top)
             row_to_lds(Tr + (int64_t)(b + 2) * rowlen, rows + ((b + 2) % 3) * slot, slot, wid, lane);
-        // this lane's read in row b: walk range and pieces; the wave's piece offsets
-        int lo = 0, hi = 0;
-        if (mine && b >= my_first && b <= my_last) {
-            const uint16_t* bj = bounds + ((int64_t)j0 + jl) * (nb + 1);
-            lo = bj[b - my_first];
-            hi = bj[b + 1 - my_first];
-        }
-        const int q = (hi - lo + PN - 1) / PN;
-        int x = q;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int y = __shfl_up(x, d, 64);
-            if (lane >= d) x += y;
-        }
-        const int qex = x - q;                    // first piece of this lane's read
-        const int Qw = __shfl(x, 63, 64);         // pieces of the wave in this row
+        int f_lo, f_hi;
+        bounds_of(b + 2, f_lo, f_hi);           // row b + 2's walk range, used at the end of this row
+        const Layout N = layout(n_lo, n_hi);    // row b + 1
         const int sb = b % 3;
         const float* R0 = rows + sb * slot;
         const float* R1 = rows + (sb + 1 == 3 ? 0 : sb + 1) * slot;
         const double bd = (double)b;
-        // a round's piece: its read (owning lane), walk range, parameters and records.  The next round's
-        // piece is found and its records requested before the current round's lookups, so that the
-        // shuffles and global loads of one round overlap the arithmetic of the previous one.
-        struct Piece {
-            int o, pa, pe, m, last;
-            bool act;
-            double offb;
-            float lw;
-            float aL[PN];
-            uint2 ev[PN];
-        };
-        auto fetch = [&](int base, Piece& P) {
-            const int p = base + lane;
-            P.act = p < Qw;
-            // owner: the largest lane whose first piece is <= p (piece offsets ascend with the lane)
-            int o = 0;
-#pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                const int c = o + step;
-                const int qc = shfl_i(qex, c & 63);
-                if (c < 64 && qc <= p) o = c;
-            }
-            const int lo_o = shfl_i(lo, o), hi_o = shfl_i(hi, o), qex_o = shfl_i(qex, o);
-            P.o = o;
-            P.m = shfl_i(my_m, o);
-            P.offb = shfl_d(my_offb, o);
-            P.lw = __shfl(my_lw, o, 64);
-            // lanes past the wave's last piece keep an empty range at the start of a walk
-            P.pa = P.act ? lo_o + (p - qex_o) * PN : 0;
-            P.pe = P.act ? min(P.pa + PN, hi_o) : 0;
-            // the last lane of the read's run in this round (any lane for lanes past the last piece)
-            P.last = (min(qex_o + (int)((unsigned)(hi_o - lo_o + PN - 1) / PN), base + 64) - 1 - base) & 63;
-            load_piece<PN>(wa + (int64_t)P.m * nw + P.pa, stream + ((int64_t)P.m * ntau + r) * nw + P.pa, P.aL, P.ev);
-        };
-        // One round: the next round's piece is fetched into nxt (after the wave's last round, a fetch of
-        // empty pieces), then cur is computed and merged.  The round has no branches, so the compiler can
-        // overlap the fetch's lane shuffles with the lookups: every slot of a piece is looked up, unused
-        // slots with the record of phase bin 0 (addresses inside the row) and their values discarded.
+        const int q = S.q, qex = S.qex, Qw = S.Qw;
+        // One round: the next piece (this row's next round, or row b + 1's first) is fetched into nxt,
+        // then cur is computed and merged.  The round has no branches, so the compiler can overlap the
+        // fetch's lane shuffles with the lookups: every slot of a piece is looked up, unused slots with
+        // the record of phase bin 0 (addresses inside the row) and their values discarded.
         auto round = [&](int base, Piece& cur, Piece& nxt) __attribute__((always_inline)) {
-            fetch(base + 64, nxt);
+            const bool same_row = base + 64 < Qw;
+            const Layout F{same_row ? S.lo : N.lo, same_row ? S.hi : N.hi, same_row ? S.q : N.q,
+                           same_row ? S.qex : N.qex, same_row ? Qw : N.Qw};
+            fetch(F, same_row ? base + 64 : 0, nxt);
             const bool act = cur.act;
             double mx = -1e300, sm = 0.0;
             {
@@ -702,15 +731,24 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
             const double hm = (double)__shfl(M, tail, 64), hs = shfl_d(t, tail);
             if (in_round) lse_merge(amx, asum, hm, hs);
         };
-        // two rounds per iteration, the pieces alternating between pa_ and pb_
-        Piece pa_, pb_;
-        if (Qw > 0) fetch(0, pa_);
-        for (int base = 0; base < Qw; base += 128) {
-            round(base, pa_, pb_);
-            if (base + 64 < Qw) round(base + 64, pb_, pa_);
+        if (Qw > 0) {
+            if (!pre) fetch(S, 0, pa_);
+            // two rounds per iteration, the pieces alternating between pa_ and pb_
+            for (int base = 0; base < Qw; base += 128) {
+                round(base, pa_, pb_);
+                if (base + 64 < Qw) round(base + 64, pb_, pa_);
+            }
+            // row b + 1's first piece came with the last round: into pb_ after an odd number of rounds
+            if (((Qw + 63) >> 6) & 1) pa_ = pb_;
+            pre = N.Qw > 0;
+        } else {
+            pre = false;
         }
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
+        S = N;
+        n_lo = f_lo;
+        n_hi = f_hi;
     }
     if (mine) {
         om[((int64_t)j0 + jl) * ntau + r] = amx;
