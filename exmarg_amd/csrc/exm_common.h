@@ -172,15 +172,19 @@ struct Context {
     hipStream_t launch_stream = nullptr;  // when set, gather-side launches go here (a batch's own stream)
     std::recursive_mutex mu;
     // grow-only scratch buffers
-    static constexpr int NSCRATCH = 9;
+    static constexpr int NSCRATCH = 10;
     void* scratch[NSCRATCH] = {};
     size_t scratch_size[NSCRATCH] = {};
     std::unordered_map<uint64_t, hipfftHandle> fft_plans;  // key: (n_time << 20) | batch
     int piece = 0;                                          // walked times per piece (0: default)
     int f32_interp = 0;                                     // band kernel interpolates in float32
     int tile = 0;                                           // reads per band workgroup (0: TILE_JOBS)
-    int64_t filter_bytes = 0;                               // matched-filter workspace per transform (0: 1 GiB)
+    int64_t filter_bytes = 0;                               // matched-filter workspace per transform (0: 64 MiB)
     int filter_f32 = 0;                                     // matched-filter transform in float32
+    // geometry of the zero-padded transform input (scratch slot 9) whose padding is already zero
+    void* filter_in = nullptr;
+    int64_t filter_in_nt = 0, filter_in_rows = 0;
+    int filter_in_lo = -1, filter_in_nf = -1, filter_in_f32 = -1;
     int num_cu = 0;
     int64_t total_mem = 0;
     char name[256] = {0};

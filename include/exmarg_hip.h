@@ -226,8 +226,9 @@ int exm_set_gather_precision(int p);
 int exm_set_gather_tile(int n);
 
 /* Workspace of one batched transform of the matched filter (templates per transform = bytes / (16
- * n_time), / (8 n_time) for the float32 transform): 0 restores the default 128 MiB, which keeps a
- * transform's passes in the last-level cache.  A tuning knob; no reference counterpart. */
+ * n_time), / (8 n_time) for the float32 transform; the transform's input and output take this much
+ * each): 0 restores the default 64 MiB, which keeps both in the last-level cache.  A tuning knob; no
+ * reference counterpart. */
 int exm_set_filter_workspace(int64_t bytes);
 
 /* Arithmetic of the matched filter's transform: 0 float64 (Z2Z, default), 1 float32 (C2C; weighting,
