@@ -1,6 +1,7 @@
 // Arrival-time selection of every template (src/exmarg/search/gather.py:374-420).
 //
-// One workgroup of 1024 threads per template:
+// One workgroup of 512 threads per template (about 52 KB of LDS at K = 1024, so that three workgroups
+// share a CU and one's barriers overlap the others' work):
 //   A  score(t) = e^{2 lqL(t)} + e^{2 max_{lag_lo <= tau <= lag_hi} lqH(t + tau)} (cyclic in t), in
 //      float64 by pexp, stored as its IEEE bits (order-preserving for positive values).  The window
 //      maximum comes from a sparse table over an LDS tile of lqH with its lag halo: maxima of 2^k
@@ -18,12 +19,15 @@
 
 namespace exm {
 
-constexpr int SEL_THREADS = 1024;
+constexpr int SEL_THREADS = 512;
 constexpr int SEL_WAVES = SEL_THREADS / 64;
-constexpr int SEL_TILE = 4096;
 constexpr int RADIX = 2048;
-constexpr int NSAMP = 2048;          // samples of the keys
+constexpr int RADIX_PER_THREAD = RADIX / SEL_THREADS;
+constexpr int NSAMP = 2048;          // at most this many samples of the keys
 constexpr int CAND = 4096;           // candidates sorted in LDS
+// The candidate arrays (CAND keys, then CAND times) also hold, while they are not in use, the sparse-
+// table tile of phase A (after the samples), the radix histogram and the drawn times of phase D.
+constexpr int TILE_FLOATS = ((CAND - NSAMP) * 8 + CAND * 4) / 4;   // room for the two tile halves
 
 // exclusive scan over the block; *total receives the block sum
 __device__ int block_excl_scan(int v, int* wsum, int* total) {
@@ -123,14 +127,15 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict_
                                                         int32_t* __restrict__ sel, float* __restrict__ lw) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int span = lag_hi - lag_lo;
-    const int tlen = SEL_TILE + span + 1;
+    const int tlen = TILE_FLOATS / 2;                             // one tile half: times plus the lag halo
+    const int sel_tile = tlen - span - 1;                         // times per tile
     uint64_t* ckey = (uint64_t*)smem;                             // CAND (samples first use NSAMP of it)
     int32_t* ctime = (int32_t*)(ckey + CAND);                     // CAND
-    float* tile = (float*)(ctime + CAND);                         // 2 tlen (sparse-table ping-pong)
-    uint32_t* hist = (uint32_t*)(tile + 2 * tlen);                // RADIX
-    int32_t* top = (int32_t*)(hist + RADIX);                      // Kp
-    int32_t* picked = top + Kp;                                   // Sp
-    int* wsum = picked + Sp;                                      // SEL_WAVES
+    float* tile = (float*)(ckey + NSAMP);                         // 2 tlen, phase A only
+    uint32_t* hist = (uint32_t*)ckey;                             // RADIX, exact radix select only
+    int32_t* picked = (int32_t*)ckey;                             // Sp, phase D only
+    int32_t* top = ctime + CAND;                                  // Kp
+    int* wsum = top + Kp;                                         // SEL_WAVES
     int* misc = wsum + SEL_WAVES;                                 // 4
 
     const int tid = threadIdx.x;
@@ -143,12 +148,12 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict_
     int kw = 0;
     while ((2 << kw) <= W) ++kw;
     const int off2 = W - (1 << kw);
-    const int64_t stride = nt / NSAMP > 1 ? nt / NSAMP : 1;      // one sample every stride times
-    const int nsamp = (int)((nt + stride - 1) / stride);
+    const int64_t stride = (nt + NSAMP - 1) / NSAMP;             // one sample every stride times
+    const int nsamp = (int)((nt + stride - 1) / stride);          // <= NSAMP
 
     // ---------------- A: keys and samples
-    for (int64_t t0 = 0; t0 < nt; t0 += SEL_TILE) {
-        const int nthis = (int)((nt - t0) < SEL_TILE ? (nt - t0) : SEL_TILE);
+    for (int64_t t0 = 0; t0 < nt; t0 += sel_tile) {
+        const int nthis = (int)((nt - t0) < sel_tile ? (nt - t0) : sel_tile);
         float* src_buf = tile;
         for (int i = tid; i < nthis + span; i += SEL_THREADS) {
             int64_t src = (t0 + lag_lo + i) % nt;
@@ -231,17 +236,22 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict_
                 if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & dmask], 1u);
             }
             __syncthreads();
-            // bins in descending order: thread i holds bins RADIX-1-2i and RADIX-2-2i
-            const int b0 = RADIX - 1 - 2 * tid, b1 = RADIX - 2 - 2 * tid;
-            const int h0 = (int)hist[b0], h1 = (int)hist[b1];
+            // bins in descending order: thread i holds bins RADIX-1-RPT i down to RADIX-RPT-RPT i
+            int h[RADIX_PER_THREAD], hsum = 0;
+#pragma unroll
+            for (int k = 0; k < RADIX_PER_THREAD; ++k) {
+                h[k] = (int)hist[RADIX - 1 - RADIX_PER_THREAD * tid - k];
+                hsum += h[k];
+            }
             int total;
-            const int before = block_excl_scan(h0 + h1, wsum, &total);
-            if (before < krem && krem <= before + h0) {
-                misc[0] = b0;
-                misc[1] = krem - before;
-            } else if (before + h0 < krem && krem <= before + h0 + h1) {
-                misc[0] = b1;
-                misc[1] = krem - before - h0;
+            int run = block_excl_scan(hsum, wsum, &total);
+#pragma unroll
+            for (int k = 0; k < RADIX_PER_THREAD; ++k) {
+                if (run < krem && krem <= run + h[k]) {
+                    misc[0] = RADIX - 1 - RADIX_PER_THREAD * tid - k;
+                    misc[1] = krem - run;
+                }
+                run += h[k];
             }
             __syncthreads();
             prefix |= (uint64_t)misc[0] << shift;
@@ -320,8 +330,9 @@ int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_
     int Kp = 1;
     while (Kp < K) Kp <<= 1;
     const int span = lag_hi - lag_lo;
-    const size_t lds = (sizeof(uint64_t) + sizeof(int32_t)) * CAND + sizeof(float) * 2 * (SEL_TILE + span + 1) +
-                       sizeof(uint32_t) * RADIX + sizeof(int32_t) * (Kp + Sp) + sizeof(int) * (SEL_WAVES + 4);
+    EXM_CHECK(span + 1 <= TILE_FLOATS / 4, EXM_ERR_ARG, "selection window of %d lags too wide", span + 1);
+    EXM_CHECK(Sp <= 2 * CAND && RADIX <= 2 * CAND, EXM_ERR_ARG, "selection of %d drawn times too large", S);
+    const size_t lds = (sizeof(uint64_t) + sizeof(int32_t)) * CAND + sizeof(int32_t) * Kp + sizeof(int) * (SEL_WAVES + 4);
     EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "selection needs %zu bytes of LDS", lds);
     const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(traj->n_tmpl, (1ll << 30) / (8 * nt)));
     uint64_t* keys = nullptr;
