@@ -256,7 +256,7 @@ __device__ __forceinline__ double band_lookup(const float* R0, const float* R1, 
     const V fc = (V)(code & FC_MASK) * (V)(1.0 / FC_SCALE);
     const int o0 = (int)__umul24((code >> BIN_SHIFT) & 0x3F, (uint32_t)g.nd) + d0;
     // the second phase bin: the stored bin above (+nd) or, from the mirror on, below (-nd)
-    const int o1 = o0 + __mul24(1 - 2 * (int)((code >> 24) & 1u), g.nd);
+    const int o1 = o0 + g.nd - (int)__umul24((code >> 24) & 1u, 2u * (uint32_t)g.nd);
     const V fdv = (V)fd, fbv = (V)fb;
     const V t00 = R0[o0], t01 = R0[o0 + 1], t10 = R1[o0], t11 = R1[o0 + 1];
     const V u00 = R0[o1], u01 = R0[o1 + 1], u10 = R1[o1], u11 = R1[o1 + 1];
@@ -674,10 +674,10 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
             const bool act = cur.act;
             double mx = -1e300, sm = 0.0;
             {
-                const int pa = cur.pa, pe = cur.pe;
+                const int nuse = cur.pe - cur.pa;   // slots of the piece that hold its read's walked times
                 const double cj = offd + cur.offb * qbd, lw = (double)cur.lw;
                 auto lookup = [&](int k) -> double {
-                    const bool use = pa + k < pe;
+                    const bool use = k < nuse;
                     const uint2 e = make_uint2(use ? cur.ev[k].x : 0u, cur.ev[k].y);
                     const double a = band_lookup<V>(R0, R1, geo, bd, cur.offb, cj, lw, cur.aL[k], e);
                     return use ? a : -1e300;
