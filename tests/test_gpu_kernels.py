@@ -98,6 +98,31 @@ def test_filter_float32_transform(hip, oracle):
         assert np.abs(got["lq" + I].astype(np.float64) - lq).max() > 0.0      # a different computation
 
 
+def test_filter_padding_follows_the_band(hip, oracle):
+    """the transform input keeps its zero padding between calls; banks whose frequency bands differ
+    (same transform length), and the float32 transform in between, must each still match the oracle"""
+    from exmarg_amd.search.bank import MassBank
+    say("filter: alternating frequency bands on one transform length")
+    bank_rows = synth.chirp_mass_grid_bank(30.0, 0.8, 0.1, 8)
+    cases = [(20.0, 256.0), (60.0, 200.0), (20.0, 256.0), (30.0, 128.0)]
+    for i, (fmin, fmax) in enumerate(cases):
+        seg = _segment(fmin=fmin, fmax=fmax, seed=11 + i)
+        bank = MassBank(bank_rows[:, 0], bank_rows[:, 1], bank_rows[:, 2], seg)
+        d = {I: seg["d_" + I][0] for I in "HL"}
+        s2 = {I: seg["sig2"] for I in "HL"}
+        if i == 2:                                        # a float32 transform between two float64 ones
+            hip.call("exm_set_filter_precision", 1)
+            try:
+                bank.trajectories(d, s2).free()
+            finally:
+                hip.call("exm_set_filter_precision", 0)
+        got = bank.trajectories(d, s2).to_host()
+        ref = oracle.filter_fp64(bank.W, bank.lo, bank.n_time, d, s2)
+        for k in ("lqH", "lqL"):
+            diff = np.abs(got[k].astype(np.float64) - ref[k])
+            assert (diff <= np.spacing(np.abs(ref[k]))).all(), (fmin, fmax, k)
+
+
 def test_filter_single_bin_closed_form(hip):
     from exmarg_amd.search.bank import MassBank
     say("filter: single-bin closed form")
