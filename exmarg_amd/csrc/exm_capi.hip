@@ -916,8 +916,11 @@ int exm_reserve(int n_tmpl, int64_t n_time, int n_freq, int n_walk) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t nm = (size_t)n_tmpl, nt = (size_t)n_time, nw = (size_t)n_walk, nf = (size_t)n_freq;
     void* p = nullptr;
-    // filter and selection work (<= 1 GiB plus one row), uploads, walk temporaries, batch admission
+    // filter and selection work (<= 1 GiB plus one row), the filter's zero-padded input (its workspace,
+    // at least one row), uploads, walk temporaries, batch admission
     EXM_TRY(scratch(0, std::max<size_t>((size_t)1 << 30, 16 * nt) + 16 * nt, &p));
+    const size_t fws = g_ctx.filter_bytes > 0 ? (size_t)g_ctx.filter_bytes : ((size_t)64 << 20);
+    EXM_TRY(scratch(9, std::max(fws, 16 * nt), &p));
     EXM_TRY(scratch(2, 8 * std::max(std::max(6 * nf + 32, nm * nw), 4 * nm + nf), &p));
     EXM_TRY(scratch(5, 2 * al(4 * nm * nw), &p));
     EXM_TRY(scratch(7, al(16 * nm * nt) + 2 * al(4 * nm * nw), &p));
@@ -1068,7 +1071,7 @@ int exm_set_gather_tile(int n) {
 int exm_set_filter_workspace(int64_t bytes) {
     LOCK;
     EXM_CHECK(bytes == 0 || (bytes >= (1ll << 20) && bytes <= (1ll << 32)), EXM_ERR_ARG,
-              "filter workspace must be 0 (default, 128 MiB) or 1 MiB .. 4 GiB, got %lld", (long long)bytes);
+              "filter workspace must be 0 (default, 64 MiB) or 1 MiB .. 4 GiB, got %lld", (long long)bytes);
     g_ctx.filter_bytes = bytes;
     return EXM_OK;
 }

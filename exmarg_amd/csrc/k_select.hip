@@ -151,16 +151,39 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict_
     const int64_t stride = (nt + NSAMP - 1) / NSAMP;             // one sample every stride times
     const int nsamp = (int)((nt + stride - 1) / stride);          // <= NSAMP
 
-    // ---------------- A: keys and samples
+    // ---------------- A: keys and samples.  The lqH tile (with its lag halo) of the next tile is
+    // requested into registers while this one is processed, and this tile's lqL before its doubling
+    // steps, so that the loads' latency overlaps work.
+    constexpr int QH_PT = (TILE_FLOATS / 2 + SEL_THREADS - 1) / SEL_THREADS;   // halo tile values per thread
+    constexpr int QL_PT = QH_PT;                                               // tile times per thread
+    float qh_next[QH_PT];
+    auto load_qh = [&](int64_t t0, int nload) {
+#pragma unroll
+        for (int k = 0; k < QH_PT; ++k) {
+            const int i = tid + k * SEL_THREADS;
+            int64_t src = (t0 + lag_lo + i) % nt;
+            if (src < 0) src += nt;
+            qh_next[k] = i < nload ? qH[src] : 0.0f;
+        }
+    };
+    load_qh(0, (int)((nt < sel_tile ? nt : sel_tile) + span));
     for (int64_t t0 = 0; t0 < nt; t0 += sel_tile) {
         const int nthis = (int)((nt - t0) < sel_tile ? (nt - t0) : sel_tile);
         float* src_buf = tile;
-        for (int i = tid; i < nthis + span; i += SEL_THREADS) {
-            int64_t src = (t0 + lag_lo + i) % nt;
-            if (src < 0) src += nt;
-            src_buf[i] = qH[src];
+#pragma unroll
+        for (int k = 0; k < QH_PT; ++k) {
+            const int i = tid + k * SEL_THREADS;
+            if (i < nthis + span) src_buf[i] = qh_next[k];
+        }
+        float ql[QL_PT];
+#pragma unroll
+        for (int k = 0; k < QL_PT; ++k) {
+            const int i = tid + k * SEL_THREADS;
+            ql[k] = i < nthis ? qL[t0 + i] : 0.0f;
         }
         __syncthreads();
+        const int64_t t1 = t0 + sel_tile;
+        if (t1 < nt) load_qh(t1, (int)((nt - t1) < sel_tile ? (nt - t1) : sel_tile) + span);
         // doubling: after step j, src_buf[i] = max of 2^j values starting at i
         int len = nthis + span;
         for (int j = 0; j < kw; ++j) {
@@ -172,13 +195,17 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict_
             src_buf = dst;
             len = nlen;
         }
-        for (int i = tid; i < nthis; i += SEL_THREADS) {
-            const float mx = fmaxf(src_buf[i], src_buf[i + off2]);
-            const int64_t t = t0 + i;
-            const double sc = __dadd_rn(pexp(2.0 * (double)qL[t]), pexp(2.0 * (double)mx));
-            const uint64_t k = (uint64_t)__double_as_longlong(sc);
-            key[t] = k;
-            if (t % stride == 0) ckey[t / stride] = k;
+#pragma unroll
+        for (int k = 0; k < QL_PT; ++k) {
+            const int i = tid + k * SEL_THREADS;
+            if (i < nthis) {
+                const float mx = fmaxf(src_buf[i], src_buf[i + off2]);
+                const int64_t t = t0 + i;
+                const double sc = __dadd_rn(pexp(2.0 * (double)ql[k]), pexp(2.0 * (double)mx));
+                const uint64_t kk = (uint64_t)__double_as_longlong(sc);
+                key[t] = kk;
+                if (t % stride == 0) ckey[t / stride] = kk;
+            }
         }
         __syncthreads();
     }
@@ -195,13 +222,24 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict_
     const uint64_t thr0 = rr < nsamp ? ckey[rr] : 0;
     if (tid == 0) misc[2] = 0;
     __syncthreads();
-    for (int64_t t = tid; t < nt; t += SEL_THREADS) {
-        const uint64_t k = key[t];
-        if (k >= thr0) {
-            const int pos = atomicAdd(&misc[2], 1);
-            if (pos < CAND) {
-                ckey[pos] = k;
-                ctime[pos] = (int32_t)t;
+    // eight keys per thread are requested before any is tested
+    constexpr int KB = 8;
+    for (int64_t t0 = 0; t0 < nt; t0 += KB * SEL_THREADS) {
+        uint64_t kb[KB];
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+            const int64_t t = t0 + j * SEL_THREADS + tid;
+            kb[j] = t < nt ? key[t] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+            const int64_t t = t0 + j * SEL_THREADS + tid;
+            if (t < nt && kb[j] >= thr0) {
+                const int pos = atomicAdd(&misc[2], 1);
+                if (pos < CAND) {
+                    ckey[pos] = kb[j];
+                    ctime[pos] = (int32_t)t;
+                }
             }
         }
     }
