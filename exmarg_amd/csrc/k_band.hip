@@ -120,15 +120,30 @@ __global__ __launch_bounds__(256) void k_walk_stream(const float* __restrict__ l
         drs[threadIdx.x] = wl[(int64_t)m * nw + p] != 0.0f ? DRAWN : 0u;
     }
     __syncthreads();
-#pragma unroll 4
-    for (int pl = wid; pl < np; pl += 4) {
-        const int64_t t = wts[pl];
-        const float pL = pLs[pl];
-        const uint32_t drawn = drs[pl];
-        for (int r = lane; r < ntau; r += 64) {
-            int64_t i1 = t + tau0[r];
-            if (i1 >= nt) i1 -= nt;
-            const float dph = __fsub_rn(phH[base + i1], pL);
+    // per chunk of 64 lags: the wave's walked times' phi_H and ln|z_H| are all requested, then the
+    // records computed (the loads' latency overlaps)
+    constexpr int PPW = WS_TILE / 4;                    // walked times per wave
+    for (int r0 = 0; r0 < ntau; r0 += 64) {
+        const int r = r0 + lane;
+        const bool rv = r < ntau;
+        const int64_t tr = rv ? tau0[r] : 0;
+        float phv[PPW], aHv[PPW];
+#pragma unroll
+        for (int k = 0; k < PPW; ++k) {
+            const int pl = wid + 4 * k;
+            phv[k] = aHv[k] = 0.0f;
+            if (rv && pl < np) {
+                int64_t i1 = wts[pl] + tr;
+                if (i1 >= nt) i1 -= nt;
+                phv[k] = phH[base + i1];
+                aHv[k] = lqH[base + i1];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PPW; ++k) {
+            const int pl = wid + 4 * k;
+            if (!(rv && pl < np)) continue;
+            const float dph = __fsub_rn(phv[k], pLs[pl]);
             double x3 = (double)dph * rph;
             if (x3 < 0.0) x3 += NPHI;
             if (x3 >= NPHI) x3 -= NPHI;
@@ -140,8 +155,8 @@ __global__ __launch_bounds__(256) void k_walk_stream(const float* __restrict__ l
             // stored bins of c0 and c0 + 1 (mod 96) are adjacent: ci1 = ci0 + 1 below the mirror, ci0 - 1 from it on
             const uint32_t ci0 = (uint32_t)(c0 > NPHI / 2 ? NPHI - c0 : c0);
             const uint32_t down = c0 >= NPHI / 2 ? DOWN : 0u;
-            wcode[r * WS_STRIDE + pl] = drawn | (ci0 << BIN_SHIFT) | down | fq;
-            waH[r * WS_STRIDE + pl] = __float_as_uint(lqH[base + i1]);
+            wcode[r * WS_STRIDE + pl] = drs[pl] | (ci0 << BIN_SHIFT) | down | fq;
+            waH[r * WS_STRIDE + pl] = __float_as_uint(aHv[k]);
         }
     }
     __syncthreads();
