@@ -48,7 +48,7 @@ def parse():
     p.add_argument("--filter-precision", choices=["f64", "f32"], default="f64",
                    help="matched-filter transform (weighting, modulus and phase stay float64)")
     p.add_argument("--ingest-nodes", type=int, default=4, help="ingest: full-size tables written and read back")
-    p.add_argument("--mode", choices=["plain", "serve", "stream", "batch", "ingest"], default="plain",
+    p.add_argument("--mode", choices=["plain", "serve", "stream", "batch", "ingest", "tables"], default="plain",
                    help="plain: pe_intrinsic on one segment per step (the headline); serve: the request loop "
                         "under Poisson arrivals; stream: pe_redshift over 8 s segments with an estimated spectrum; "
                         "batch: 256 segments of uneven length in batched launches (float32 form, autotuned)")
@@ -472,6 +472,62 @@ def main_ingest(args):
     print(json.dumps(out), flush=True)
 
 
+def main_tables(args):
+    """table build, stage 3 (the sum over the orders): one node at the reference geometry (121 orders, 87 lags,
+    110 q points, 49 phases -> 87 x 49 x 200 x 219 float32) from synthetic stage-2 slices; nodes/s on the GPU
+    with the host copies, the kernel time, and the oracle on a bounded sample of lags"""
+    import math
+    from exmarg_amd import _hip, build_bd
+    _hip.init(0)
+    info = _hip.device_info()
+    nord, ntau, nq, nphif = 121, 87, 110, 49
+    lnq = np.log(np.geomspace(100.0 / 2000.0, 100.0, nq))
+    n = np.arange(nord, dtype=np.float64)[:, None, None, None]
+    lgam = np.array([math.lgamma(k + 1.0) for k in range(nord)])[:, None, None, None]
+    cph = np.cos(np.arange(nphif) * np.pi / 48.0)[None, None, None, :]
+    lag = (np.arange(ntau) / ntau)[None, :, None, None]
+    col = (2.0 * n * lnq[-1] - lgam + 0.3 * n * cph - lag - 0.02 * n * np.arange(nq)[None, None, :, None]).astype(np.float32)
+    row = (2.0 * n * lnq[None, None, :, None] - lgam + 0.3 * n * cph - lag).astype(np.float32)
+    ladder = [nord - 1]
+    build_bd.sum_over_orders(col, row, lnq, ladder)                     # warm-up
+    _hip.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = build_bd.sum_over_orders(col, row, lnq, ladder)
+    wall = (time.perf_counter() - t0) / args.steps
+    k_ms, k_n = _hip.profile_get("tables")
+    _hip.profile(False)
+    kernel_ms = k_ms / max(k_n, 1)
+    beta, delta = build_bd.grids(lnq)
+    outputs = ntau * nphif * beta.size * delta.size
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle"))
+        import exm_oracle
+        lags = 2
+        c0 = time.perf_counter()
+        ref = exm_oracle.bd_sum_orders(col[:, :lags], row[:, :lags], lnq, ladder)
+        cs = (time.perf_counter() - c0) * ntau / lags
+        same = float(np.mean(ref[ladder[0]] == out[ladder[0]][:lags]))
+        cpu = {"value": 1.0 / cs, "unit": "nodes/s", "cores": 1, "kind": "port",
+               "sample": "the oracle (numpy, float64) on %d of the %d lags, scaled" % (lags, ntau),
+               "seconds_per_node": cs, "identical_fraction": same}
+    in_bytes = 2 * col.nbytes
+    out_bytes = 4 * outputs * len(ladder)
+    line = {"metric": "table build stage 3: nodes/s (sum over 121 orders -> one 87 x 49 x 200 x 219 float32 table)",
+            "value": 1.0 / wall, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
+            "seconds_per_node": wall, "kernel_ms": kernel_ms,
+            "exp_per_s": outputs * nord / (kernel_ms * 1e-3),
+            "roofline": {"bound": "hbm", "achieved": (in_bytes + out_bytes) / (kernel_ms * 1e-3) / 1e9,
+                         "peak": 8000.0, "unit": "GB/s",
+                         "frac": (in_bytes + out_bytes) / (kernel_ms * 1e-3) / 8e12, "traffic": None,
+                         "kernel": "k_bd_sum_orders", "bytes_per_launch": in_bytes + out_bytes},
+            "cpu_baseline": cpu, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "one node: 121 orders, 87 lags, 110 q points, 49 phases -> 200 beta x 219 Delta",
+                       "device": info["name"]}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -484,6 +540,8 @@ def main():
         return main_stream(args)
     if args.mode == "ingest":
         return main_ingest(args)
+    if args.mode == "tables":
+        return main_tables(args)
     if args.mode == "batch":
         if args.precision == "f64" and "--precision" not in " ".join(sys.argv):
             args.precision = "f32"

@@ -68,6 +68,7 @@ SIGNATURES = {
     "exm_profile_reset": [],
     "exm_profile_get": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)],
     "exm_copy_calibration": [I64, ctypes.POINTER(ctypes.c_double)],
+    "exm_tables_sum_orders": [P, P, I32, I32, I32, I32, P, P, I32, P, I32, P],
     "exm_source_frame": [P, P, I32, I32, P, P, P, I32, P, I32, P, P, P],
 }
 

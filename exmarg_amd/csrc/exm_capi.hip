@@ -52,7 +52,7 @@ static void pool_release() {
     g_pool_bytes = 0;
 }
 
-static int dev_alloc(void** p, size_t bytes) {
+int dev_alloc(void** p, size_t bytes) {
     bytes = block_size(bytes);
     auto it = g_pool.find(bytes);
     if (it != g_pool.end()) {
@@ -79,7 +79,7 @@ static int dev_alloc(void** p, size_t bytes) {
     return EXM_OK;
 }
 
-static void dev_free(void* p, size_t bytes) {
+void dev_free(void* p, size_t bytes) {
     if (!p) return;
     bytes = block_size(bytes);
     g_mem_in_use -= (int64_t)bytes;
@@ -1193,6 +1193,13 @@ int exm_copy_calibration(int64_t bytes, double* ms) {
     dev_free(a, (size_t)bytes);
     if (b) dev_free(b, (size_t)bytes);
     return rc;
+}
+
+int exm_tables_sum_orders(const float* col, const float* row, int nord, int ntau, int nq, int nphif,
+                          const double* lnq, const double* beta, int nb, const int* ladder, int nlad, float* const* out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    return tables_sum_orders(col, row, nord, ntau, nq, nphif, lnq, beta, nb, ladder, nlad, out);
 }
 
 }  // extern "C"

@@ -244,6 +244,11 @@ int launch_lse(const JobSet& js, int ntau, const double* om, const double* os, c
 int launch_fill(float* dst, int64_t n, float v);
 int launch_copy_calibration(float4* dst, const float4* src, int64_t n);
 // tables from store files (file, byte offset of the float32 member, device index) through pinned buffers
+// pooled device memory of the library (blocks are kept for reuse until memory runs short)
+int dev_alloc(void** p, size_t bytes);
+void dev_free(void* p, size_t bytes);
+int tables_sum_orders(const float* col, const float* row, int nord, int ntau, int nq, int nphif, const double* lnq,
+                      const double* beta, int nb, const int* ladder, int nlad, float* const* out);
 int store_load_files(Store* st, int n, const char* const* paths, const int64_t* offsets, const int* tables,
                      int readers, double* seconds);
 int launch_source_frame(const double* lamN, const double* yN, int nrow, int nm, const double* M, const double* yq,

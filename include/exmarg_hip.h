@@ -250,6 +250,18 @@ int exm_get_gather_tuning(int* kernel, int* piece, int* tile, int* precision);
 int exm_source_frame(const double* lamN, const double* yN, int nrow, int nm, const double* Mdet, const double* yq,
                      const double* m1, int nm1, const double* z, int nz, const double* w, double* out, double* ln_b);
 
+/* ---------------------------------------------------------------- table build, stage 3
+ * The sum over the orders of one node (scripts/build_tables/build_bd.py:52-79, build_node): col and
+ * row are the last column (q2 = q_max) and last row (q1 = q_max) of every (q1, q2) block of the
+ * stage-2 array lnK_n(tau, q1, q2, phi), each float32 (nord, ntau, nq, nphif) C order (build_bd.py:61-62);
+ * lnq (nq) and beta (nb) float64 the node's ln q grid and the beta grid (build_bd.py:52-59).  For every
+ * truncation order ladder[l] (ascending, < nord; BD_NMAX_LADDER of build_bd.py:27,38) out[l] receives
+ * T (ntau, nphif, nb, 2 nq - 1) float32 = max_n E_n + ln sum_n exp(E_n - max) over n <= ladder[l],
+ * E_n = n beta + g_n(Delta) (build_bd.py:63-76), computed in float64.  nord <= 128, at most 32 orders
+ * in the ladder. */
+int exm_tables_sum_orders(const float* col, const float* row, int nord, int ntau, int nq, int nphif,
+                          const double* lnq, const double* beta, int nb, const int* ladder, int nlad, float* const* out);
+
 /* ---------------------------------------------------------------- measurement
  * No reference counterpart: kernel timing with HIP events recorded on the library's stream around
  * every launch of a named stage ("filter", "select", "gather"), for the benchmark. */

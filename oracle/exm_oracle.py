@@ -29,6 +29,8 @@ Functions and the reference lines they follow:
   dl_of_z              src/exmarg/search/pe.py:26-28 (astropy Planck18 restated: flat LCDM with
                        photons and one massive neutrino species, the Komatsu et al. 2011
                        Eq. 26 fitting function for the neutrino density)
+  bd_grids             scripts/build_tables/build_bd.py:52-60 (beta and Delta grids of stage 3)
+  bd_sum_orders        scripts/build_tables/build_bd.py:61-76 (stage 3: the sum over the orders)
 """
 
 import ctypes
@@ -476,3 +478,58 @@ def pe_redshift_weights(m1, z, mdet_range):
     inside = (Md >= mdet_range[0]) & (Md <= mdet_range[1])
     w = np.where(inside, np.exp(lnp)[None, :], 0.0) * np.gradient(m1)[:, None] * np.gradient(z)[None, :]
     return w / w.sum()
+
+
+# ---------------------------------------------------------------- table build, stage 3
+BD_ALPHA = np.geomspace(0.16, 4.0e4, 30)        # scripts/build_tables/build_bd.py:26
+
+
+def bd_grids(lnq):
+    """(beta, delta) grids of one node from its ln q grid (scripts/build_tables/build_bd.py:52-60):
+    Delta = ln q1 - ln q2 on the 2 nq - 1 differences of the grid, sorted; beta from ln alpha_min + 2 ln q_min
+    to ln alpha_max + 2 ln q_max in steps of 2 d(ln q)"""
+    lnq = np.asarray(lnq, np.float64)
+    last = lnq.size - 1
+    dlq = float(lnq[1] - lnq[0])
+    delta = np.concatenate([lnq - lnq[last], (lnq[last] - lnq)[::-1][1:]])
+    dbeta = 2.0 * dlq
+    b_lo = np.log(BD_ALPHA[0]) + 2.0 * lnq[0]
+    b_hi = np.log(BD_ALPHA[-1]) + 2.0 * lnq[-1]
+    nb = int(np.ceil((b_hi - b_lo) / dbeta)) + 1
+    return b_lo + dbeta * np.arange(nb), delta
+
+
+def bd_sum_orders(col, row, lnq, ladder):
+    """stage 3 of the tables for one node (scripts/build_tables/build_bd.py:61-76), restated in numpy.
+
+    col (nord, ntau, nq, nphif): lnK_n at q2 = q_max (the last column of each (q1, q2) block),
+    row (nord, ntau, nq, nphif): lnK_n at q1 = q_max (the last row); float32 as stage 2 stores them.
+    g_n(Delta) = col - 2 n ln q_max for Delta <= 0, row - 2 n ln q2 for Delta >= 0 (sorted in Delta);
+    E_n(beta, Delta) = n beta + g_n(Delta); for every n_max of the ladder
+    T = max_{n <= n_max} E_n + ln sum_{n <= n_max} exp(E_n - max), in float64, rounded to float32.
+    Returns {n_max: T (ntau, nphif, nb, nd) float32}."""
+    col = np.asarray(col)
+    row = np.asarray(row)
+    lnq = np.asarray(lnq, np.float64)
+    nord, ntau, nq, nphif = col.shape
+    last = nq - 1
+    beta, delta = bd_grids(lnq)
+    n = np.arange(nord, dtype=np.float64)
+    E_nb = n[:, None] * beta[None, :]
+    out = {nm: np.empty((ntau, nphif, beta.size, delta.size), np.float32) for nm in ladder}
+    for ti in range(ntau):
+        c = col[:, ti].astype(np.float64)
+        r = row[:, ti].astype(np.float64)
+        g_neg = c - 2.0 * n[:, None, None] * lnq[last]
+        g_pos = r - 2.0 * n[:, None, None] * lnq[None, :, None]
+        g = np.concatenate([g_neg, g_pos[:, ::-1, :][:, 1:, :]], axis=1)
+        for pi in range(nphif):
+            E = E_nb[:, :, None] + g[:, None, :, pi]
+            mx_run = np.maximum.accumulate(E, axis=0)
+            for nm in ladder:
+                mxn = mx_run[nm]
+                tot = np.zeros_like(mxn)
+                for k in range(nm + 1):                  # in order of n
+                    tot += np.exp(E[k] - mxn)
+                out[nm][ti, pi] = (mxn + np.log(tot)).astype(np.float32)
+    return out
