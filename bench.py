@@ -167,8 +167,33 @@ def cpu_leg(args, seg, bank, store, res, an_geometry, seed_row):
     sample = ("filter+selection of %d of %d templates (scaled x%.1f) and the gather of those templates on the "
               "4 most-used nodes (%d reads x 87 lags x %d times, tables in RAM), scaled to all %d reads"
               % (samp.size, nm, nm / samp.size, ids.size, K + S, n_jobs_all))
+    # the same on one core: selection of a quarter of the sample, gather of it on one node
+    O.set_threads(1)
+    try:
+        q = samp[: max(1, samp.size // 4)]
+        trq = O.filter_fp64(bank.W[q], bank.lo, nt, d, s2)
+        t0 = time.perf_counter()
+        tq = O.select_times(trq, store.tau_idx, K, S, 0, pick=full_pick[q])
+        t_sel1 = time.perf_counter() - t0
+        ids1, tabs1, lnas1, _ = jobs_for(q, nodes={nodes[0]})
+        t0 = time.perf_counter()
+        O.gather(trq, tabs_mem, beta, delta, ids1, lnas1, tabs1, tq)
+        rate1 = ids1.size * store.tau_idx.size * (K + S) / (time.perf_counter() - t0)
+    finally:
+        O.set_threads(threads)
+    t_seg1 = t_filter * nm / samp.size + t_sel1 * nm / q.size + lookups_all / rate1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     cpu = dict(value=1.0 / t_seg, unit="segments/s", cores=threads, kind="port", sample=sample,
-               seconds_per_segment=t_seg, gather_lookups_per_s=cpu_lookup_rate)
+               seconds_per_segment=t_seg, gather_lookups_per_s=cpu_lookup_rate, cpu_model=model,
+               host_cpus=os.cpu_count(),
+               one_core=dict(value=1.0 / t_seg1, seconds_per_segment=t_seg1, gather_lookups_per_s=rate1,
+                             sample="selection of %d templates and their gather on 1 node (%d reads), 1 thread; "
+                                    "the filter (numpy, single-threaded) as above" % (q.size, ids1.size)))
     return cpu, err, chk.size
 
 

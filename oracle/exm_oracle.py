@@ -68,8 +68,14 @@ def lib():
                                           P, f64, f64, f64, P, P, i64, i32, P, P]
         L.oracle_select.argtypes = [P, P, i64, i64, i32, i32, i32, i32, P, f32, P, P]
         L.oracle_scores.argtypes = [P, P, i64, i32, i32, P]
+        L.oracle_set_threads.argtypes = [i32]
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """threads of the oracle's OpenMP loops"""
+    lib().oracle_set_threads(int(n))
 
 
 def _p(a):

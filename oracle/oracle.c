@@ -24,6 +24,9 @@
  *                      polynomial; the HIP library evaluates the identical
  *                      operation sequence, so both rank times identically.
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -36,6 +39,15 @@ static const double PEXP_LOG2E = 1.4426950408889634;
 static const double PEXP_LN2_HI = 6.93147180369123816490e-01;
 static const double PEXP_LN2_LO = 1.90821492927058770002e-10;
 static const double PEXP_C[14] = {1.0, 1.0, 0.5, 0.16666666666666666, 0.041666666666666664, 0.008333333333333333, 0.001388888888888889, 0.0001984126984126984, 2.48015873015873e-05, 2.7557319223985893e-06, 2.755731922398589e-07, 2.505210838544172e-08, 2.08767569878681e-09, 1.6059043836821613e-10};
+
+// threads of the OpenMP loops below (the CPU baseline is timed on all cores and on one)
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
 
 double oracle_pexp(double x) {
     double k = nearbyint(x * PEXP_LOG2E);
