@@ -66,7 +66,7 @@ struct Object {
     virtual ~Object() {}
 };
 
-// Tables of every node in one device allocation, layout [table][lag][beta][phase][Delta]
+// Tables of every node in one device allocation, layout [table][lag][beta][Delta][phase]
 // (float32): one beta row of all phases is contiguous, nphif * nd floats.  The allocation ends with
 // TABLE_PAD_FLOATS spare floats so that row copies in whole 64-float blocks stay inside it.
 constexpr int TABLE_PAD_FLOATS = 256;

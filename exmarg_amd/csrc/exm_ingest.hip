@@ -7,7 +7,7 @@
 // members are stored uncompressed).  Each table is cut into chunks; reader threads pread the chunks in
 // order into a ring of pinned buffers; the calling thread copies each chunk to a device staging slab
 // on a copy stream as soon as it is read, and when a table's slab is complete the layout transform to
-// [lag][beta][phase][Delta] runs on the library's stream while the next table streams into the other
+// [lag][beta][Delta][phase] runs on the library's stream while the next table streams into the other
 // slab.  Host memory never holds more than the ring.
 #include <fcntl.h>
 #include <unistd.h>

@@ -4,10 +4,10 @@
 // For every template, beta = ln a + 2 ln|z_L(t)| of a walked time t is increasing in ln|z_L(t)| at
 // every node and lag.  So, once per segment:
 //   k_walk_sort    sorts each template's walked times by ln|z_L|;
-//   k_walk_stream  writes, for every (template, lag) and walked time in that order, the stored phase bin
-//                  of phi_H(t + tau) - phi_L(t) (the bin mirrored to [0, 48]), the side of its
-//                  neighbour, the fraction, a drawn-time flag and ln|z_H(t + tau)|: 8 bytes per lookup,
-//                  read contiguously by the gather;
+//   k_walk_stream  writes, for every (template, lag) and walked time in that order, the pair of adjacent
+//                  stored phase bins that phi_H(t + tau) - phi_L(t) falls between (bins mirrored to
+//                  [0, 48]; the lower one is stored), the weight of the upper bin, a drawn-time flag and
+//                  ln|z_H(t + tau)|: 8 bytes per lookup, read contiguously by the gather;
 //   k_read_bounds  finds, for every read (template, table, ln a), where each beta row starts in the
 //                  sorted walk.
 // k_gather_piece then runs one workgroup per (table, lag, tile of at most 1024 reads of that table).
@@ -30,16 +30,20 @@ namespace exm {
 
 constexpr int WALK_THREADS = 1024;
 constexpr int BAND_THREADS = 1024;
-constexpr double FC_SCALE = 16777216.0;   // phase fraction in 24-bit fixed point
-constexpr uint32_t FC_MASK = 0xFFFFFFu;
+constexpr double FC_SCALE = 16777216.0;   // phase weights in fixed point, units of 2^-24
+constexpr uint32_t FC_MASK = 0x1FFFFFFu;  // 25 bits: the weight of the upper bin, 0 .. 2^24
 constexpr uint32_t DRAWN = 0x80000000u;   // walked time from the random draw (log weight lw_rand)
-constexpr uint32_t DOWN = 0x01000000u;    // the second phase bin is the stored bin below the first
-constexpr int BIN_SHIFT = 25;             // 6 bits of stored bin (0..48)
+constexpr int BIN_SHIFT = 25;             // 6 bits: the lower stored bin of the pair (0..47)
 constexpr int PIECE = 8;                  // default walked times per piece
 
 __device__ __forceinline__ uint32_t float_order_key(float f) {
     const uint32_t u = __float_as_uint(f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// the float of an order key (the inverse of float_order_key)
+__device__ __forceinline__ float order_key_float(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
 __global__ __launch_bounds__(WALK_THREADS) void k_walk_sort(const float* __restrict__ lqL, int64_t nt,
@@ -151,11 +155,15 @@ __global__ __launch_bounds__(256) void k_walk_stream(const float* __restrict__ l
             c0 = c0 < 0 ? 0 : (c0 > NPHI - 1 ? NPHI - 1 : c0);
             const double fc = x3 - c0;
             uint32_t fq = (uint32_t)__double2uint_rn(fc * FC_SCALE);
-            fq = fq > FC_MASK ? FC_MASK : fq;
-            // stored bins of c0 and c0 + 1 (mod 96) are adjacent: ci1 = ci0 + 1 below the mirror, ci0 - 1 from it on
+            fq = fq > (1u << 24) - 1 ? (1u << 24) - 1 : fq;
+            // stored bins of c0 and c0 + 1 (mod 96) are adjacent: ci1 = ci0 + 1 below the mirror, ci0 - 1
+            // from it on.  The record holds the lower bin of the pair and the weight of the upper one (fc,
+            // or 1 - fc when c0's stored bin is the upper).
             const uint32_t ci0 = (uint32_t)(c0 > NPHI / 2 ? NPHI - c0 : c0);
-            const uint32_t down = c0 >= NPHI / 2 ? DOWN : 0u;
-            wcode[r * WS_STRIDE + pl] = drs[pl] | (ci0 << BIN_SHIFT) | down | fq;
+            const bool down = c0 >= NPHI / 2;
+            const uint32_t lo = down ? ci0 - 1 : ci0;
+            const uint32_t wq = down ? (1u << 24) - fq : fq;
+            wcode[r * WS_STRIDE + pl] = drs[pl] | (lo << BIN_SHIFT) | wq;
             waH[r * WS_STRIDE + pl] = __float_as_uint(aHv[k]);
         }
     }
@@ -253,7 +261,10 @@ struct BandGeom {
 
 // One lookup of App. B (the reference's gather.py:45-98) with the walk record (aL, e) of a walked time
 // whose beta row is b (R0 row b, R1 row b + 1 in LDS): float64 index and fraction arithmetic, the
-// bilinear (beta, Delta) interpolation at both phase bins and the phase interpolation in V.
+// bilinear (beta, Delta) interpolation at both phase bins and the phase interpolation in V.  A row is
+// stored Delta-major, [Delta][phase bin], so the four values of a row that one lookup reads (Delta
+// columns d0 and d0 + 1, stored bins lo and lo + 1) sit at o, o + 1, o + NPHIF and o + NPHIF + 1 from
+// one address: two ds_read2 per row.
 template <typename V>
 __device__ __forceinline__ double band_lookup(const float* R0, const float* R1, const BandGeom& g, double bd,
                                               double offb, double cj, double lw_rand, float aLf, uint2 e) {
@@ -268,17 +279,17 @@ __device__ __forceinline__ double band_lookup(const float* R0, const float* R1, 
     const double xd = fmin(fmax(fma(-g.qbd, xb, fma((double)__uint_as_float(e.y), g.rdd, cj)), 0.0), g.xdmax);
     const int d0 = (int)xd;
     const double fd = __builtin_amdgcn_fract(xd);
-    const V fc = (V)(code & FC_MASK) * (V)(1.0 / FC_SCALE);
-    const int o0 = (int)__umul24((code >> BIN_SHIFT) & 0x3F, (uint32_t)g.nd) + d0;
-    // the second phase bin: the stored bin above (+nd) or, from the mirror on, below (-nd)
-    const int o1 = o0 + g.nd - (int)__umul24((code >> 24) & 1u, 2u * (uint32_t)g.nd);
+    // weight of the upper bin of the pair (exact: at most 25 significant bits)
+    const V wc = (V)(code & FC_MASK) * (V)(1.0 / FC_SCALE);
+    const int o = (int)__umul24((uint32_t)d0, (uint32_t)NPHIF) + (int)((code >> BIN_SHIFT) & 0x3F);
     const V fdv = (V)fd, fbv = (V)fb;
-    const V t00 = R0[o0], t01 = R0[o0 + 1], t10 = R1[o0], t11 = R1[o0 + 1];
-    const V u00 = R0[o1], u01 = R0[o1 + 1], u10 = R1[o1], u11 = R1[o1 + 1];
-    const V v0a = vfma(fdv, t01 - t00, t00), v0b = vfma(fdv, t11 - t10, t10);
-    const V v1a = vfma(fdv, u01 - u00, u00), v1b = vfma(fdv, u11 - u10, u10);
-    const V v0 = vfma(fbv, v0b - v0a, v0a), v1 = vfma(fbv, v1b - v1a, v1a);
-    return (double)vfma(fc, v1 - v0, v0) + ((code & DRAWN) ? lw_rand : 0.0);
+    const V a00 = R0[o], a01 = R0[o + 1], a10 = R0[o + NPHIF], a11 = R0[o + NPHIF + 1];
+    const V b00 = R1[o], b01 = R1[o + 1], b10 = R1[o + NPHIF], b11 = R1[o + NPHIF + 1];
+    // Delta, then beta, at the lower (x0) and upper (x1) bin, then the phase
+    const V r0l = vfma(fdv, a10 - a00, a00), r0u = vfma(fdv, a11 - a01, a01);
+    const V r1l = vfma(fdv, b10 - b00, b00), r1u = vfma(fdv, b11 - b01, b01);
+    const V vl = vfma(fbv, r1l - r0l, r0l), vu = vfma(fbv, r1u - r0u, r0u);
+    return (double)vfma(wc, vu - vl, vl) + ((code & DRAWN) ? lw_rand : 0.0);
 }
 
 // one workgroup per (tile of reads of one table, lag); PN walked times per piece; V the type of the
@@ -495,7 +506,10 @@ __device__ __forceinline__ void lse_merge(double& m, double& s, double m2, doubl
 
 // DPP lane moves within a wave64 (VALU, no LDS round trip): lane l receives the value of lane l - n of
 // its row of 16 lanes (row_shr:n), or of lane 15 / 31 of the rows below it (row_bcast:15 / :31); lanes
-// without a source lane, or outside the row and bank masks, receive `old`.
+// without a source lane, or outside the row and bank masks, receive `old`.  Combining each lane with
+// the moved value in the sequence shr 1, shr 2, shr 4 (banks 1-3), shr 8 (banks 2-3), bcast 15 (rows 1,
+// 3), bcast 31 (rows 2, 3), with `old` the identity of the combination, leaves at every lane the
+// combination over the lanes up to itself (an inclusive scan).
 constexpr int DPP_ROW_SHR = 0x110, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
 
 template <int CTRL, int ROWM, int BANKM>
@@ -504,8 +518,17 @@ __device__ __forceinline__ int dpp_i(int old, int v) {
 }
 
 template <int CTRL, int ROWM, int BANKM>
-__device__ __forceinline__ float dpp_f(float old, float v) {
-    return __int_as_float(dpp_i<CTRL, ROWM, BANKM>(__float_as_int(old), __float_as_int(v)));
+__device__ __forceinline__ uint32_t dpp_u(uint32_t old, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWM, BANKM, false);
+}
+
+// lane l receives lane l - n of its row of 16, or 0 where l - n lies outside the row (every lane
+// written, so no previous value is needed)
+template <int N>
+__device__ __forceinline__ double shr_zero_d(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), DPP_ROW_SHR | N, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), DPP_ROW_SHR | N, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
 }
 
 template <int CTRL, int ROWM, int BANKM>
@@ -513,25 +536,6 @@ __device__ __forceinline__ double dpp_d(double old, double v) {
     const int lo = dpp_i<CTRL, ROWM, BANKM>(__double2loint(old), __double2loint(v));
     const int hi = dpp_i<CTRL, ROWM, BANKM>(__double2hiint(old), __double2hiint(v));
     return __hiloint2double(hi, lo);
-}
-
-// One step of a segmented inclusive scan towards the last lane of each run of equal `seg`: the running
-// value of the DPP source lane is combined when that lane is in the same run.  Runs are contiguous, so
-// the sequence shr 1, shr 2, shr 4 (banks 1-3), shr 8 (banks 2-3), bcast 15 (rows 1, 3), bcast 31 (rows
-// 2, 3) leaves at every lane the combination over its run up to itself.
-template <int CTRL, int ROWM, int BANKM>
-__device__ __forceinline__ bool seg_max_step(int seg, float src, float& M) {
-    const int s2 = dpp_i<CTRL, ROWM, BANKM>(INT_MIN, seg);
-    const float m2 = dpp_f<CTRL, ROWM, BANKM>(-INFINITY, src);
-    const bool same = s2 == seg;
-    M = same && m2 > M ? m2 : M;
-    return same;
-}
-
-template <int CTRL, int ROWM, int BANKM>
-__device__ __forceinline__ void seg_sum_step(bool same, double src, double& t) {
-    const double t2 = dpp_d<CTRL, ROWM, BANKM>(0.0, src);
-    t = same ? t + t2 : t;
 }
 
 // The wave form of the band kernel: one workgroup per (tile of reads of one table, lag) and the same
@@ -721,29 +725,37 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
                     }
                 }
             }
-            // segmented scan over each read's run of lanes in this round, in two passes: the run's maximum
-            // (float), then the sum of every lane's sum rescaled to that maximum (float64), so that one
-            // exp per lane suffices; both end at the run's last lane
-            const int seg = act ? cur.o : -1 - lane;
-            float M = (float)mx;
-            const bool s1 = seg_max_step<DPP_ROW_SHR | 1, 0xf, 0xf>(seg, M, M);
-            const bool s2 = seg_max_step<DPP_ROW_SHR | 2, 0xf, 0xf>(seg, M, M);
-            const bool s4 = seg_max_step<DPP_ROW_SHR | 4, 0xf, 0xe>(seg, M, M);
-            const bool s8 = seg_max_step<DPP_ROW_SHR | 8, 0xf, 0xc>(seg, M, M);
-            const bool s15 = seg_max_step<DPP_ROW_BCAST15, 0xa, 0xf>(seg, M, M);
-            const bool s31 = seg_max_step<DPP_ROW_BCAST31, 0xc, 0xf>(seg, M, M);
-            const double Mr = (double)__shfl(M, cur.last, 64);
+            // Each read's (max, sum) over its run of lanes in this round, from two unsegmented scans.
+            // The max: the owning lane ascends with the lane, so the prefix maximum of the key (owner,
+            // piece max) at a lane is the maximum over its own run up to itself; the piece max enters
+            // the key truncated to 26 bits of its order (the run max is only the reference of the sums,
+            // so any value within 2^-17 relative of it serves).  Lanes past the last piece hold key 0 and
+            // come after every active lane.  The sum: every lane's sum rescaled to its run's reference,
+            // prefix-summed over the wave (float64); a run's sum is the difference of the prefix sums at
+            // its last lane and before its first, and is at least ~1 (the lane holding the run's max
+            // contributes its own largest term, ~1), so the difference loses ~1e-14 relative at most.
+            uint32_t key = act ? ((uint32_t)cur.o << 26) | (float_order_key((float)mx) >> 6) : 0u;
+            key = max(key, dpp_u<DPP_ROW_SHR | 1, 0xf, 0xf>(0u, key));
+            key = max(key, dpp_u<DPP_ROW_SHR | 2, 0xf, 0xf>(0u, key));
+            key = max(key, dpp_u<DPP_ROW_SHR | 4, 0xf, 0xe>(0u, key));
+            key = max(key, dpp_u<DPP_ROW_SHR | 8, 0xf, 0xc>(0u, key));
+            key = max(key, dpp_u<DPP_ROW_BCAST15, 0xa, 0xf>(0u, key));
+            key = max(key, dpp_u<DPP_ROW_BCAST31, 0xc, 0xf>(0u, key));
+            const double Mr = (double)order_key_float(__shfl(key, cur.last, 64) << 6);
             double t = act ? sm * (double)__expf((float)(mx - Mr)) : 0.0;
-            seg_sum_step<DPP_ROW_SHR | 1, 0xf, 0xf>(s1, t, t);
-            seg_sum_step<DPP_ROW_SHR | 2, 0xf, 0xf>(s2, t, t);
-            seg_sum_step<DPP_ROW_SHR | 4, 0xf, 0xe>(s4, t, t);
-            seg_sum_step<DPP_ROW_SHR | 8, 0xf, 0xc>(s8, t, t);
-            seg_sum_step<DPP_ROW_BCAST15, 0xa, 0xf>(s15, t, t);
-            seg_sum_step<DPP_ROW_BCAST31, 0xc, 0xf>(s31, t, t);
-            // the owning lane takes its read's (max, sum) of this round from the run's last lane
+            t += shr_zero_d<1>(t);
+            t += shr_zero_d<2>(t);
+            t += shr_zero_d<4>(t);
+            t += shr_zero_d<8>(t);
+            t += dpp_d<DPP_ROW_BCAST15, 0xa, 0xf>(0.0, t);
+            t += dpp_d<DPP_ROW_BCAST31, 0xc, 0xf>(0.0, t);
+            // the owning lane takes its read's (max, sum) of this round from the run's lanes
             const bool in_round = q > 0 && qex < base + 64 && qex + q > base;
             const int tail = in_round ? min(qex + q, base + 64) - 1 - base : 0;
-            const double hm = (double)__shfl(M, tail, 64), hs = shfl_d(t, tail);
+            const int head = in_round ? max(qex - base, 0) : 0;
+            const double hm = (double)order_key_float(__shfl(key, tail, 64) << 6);
+            const double before = shfl_d(t, (head + 63) & 63);
+            const double hs = shfl_d(t, tail) - (head > 0 ? before : 0.0);
             if (in_round) lse_merge(amx, asum, hm, hs);
         };
         if (Qw > 0) {

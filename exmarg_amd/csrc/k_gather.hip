@@ -69,10 +69,11 @@ __global__ __launch_bounds__(256) void k_gather_ref(
             int ci = cc ? c1 : c0;
             if (ci > NPHI / 2) ci = NPHI - ci;
             const double fci = cc ? fc : 1.0 - fc;
-            const float* r0 = Tn + b0 * row + (int64_t)ci * nd;
-            const float* r1 = Tn + b1 * row + (int64_t)ci * nd;
-            const double v = fb0 * (fd0 * (double)r0[d0] + fdf * (double)r0[d1]) +
-                             fbf * (fd0 * (double)r1[d0] + fdf * (double)r1[d1]);
+            // device rows are Delta-major: [beta][Delta][phase bin]
+            const float* r0 = Tn + b0 * row + ci;
+            const float* r1 = Tn + b1 * row + ci;
+            const double v = fb0 * (fd0 * (double)r0[d0 * nphif] + fdf * (double)r0[d1 * nphif]) +
+                             fbf * (fd0 * (double)r1[d0 * nphif] + fdf * (double)r1[d1 * nphif]);
             acc += fci * v;
         }
         if (sel) acc += (double)lw[(int64_t)m * n_walk_sel + k];

@@ -3,33 +3,57 @@
 //
 // Host (file) layout of one table: [lag][phase][beta][Delta]
 //   (scripts/build_tables/build_bd.py:64,79 writes it; src/exmarg/search/store.py:110-120 stacks it).
-// Device layout: [table][lag][beta][phase][Delta], so that one beta row of every phase bin is one
-// contiguous span of nphif * nd floats: the unit the gather stages through LDS.
+// Device layout: [table][lag][beta][Delta][phase], so that one beta row (every Delta column and phase
+// bin) is one contiguous span of nd * nphif floats, the unit the gather stages through LDS, and the two
+// phase bins a lookup interpolates between are adjacent floats.
 #include "exm_common.h"
 
 namespace exm {
 
-// one thread block per (lag, beta, phase) row of nd floats
-__global__ void k_transpose_in(const float* __restrict__ src, float* __restrict__ dst, int nphif, int nb, int nd) {
-    const int64_t row = blockIdx.x;                 // device order: (r * nb + b) * nphif + c
-    const int c = (int)(row % nphif);
-    const int64_t rb = row / nphif;
+constexpr int TR_DT = 64;        // Delta columns per transpose block
+
+// one block per (lag, beta, 64 Delta columns): the nphif x 64 tile through LDS, read along Delta from
+// the host layout and written along the phase bins of the device layout
+__global__ __launch_bounds__(256) void k_transpose_in(const float* __restrict__ src, float* __restrict__ dst,
+                                                      int nphif, int nb, int nd) {
+    __shared__ float tile[NPHIF][TR_DT + 1];
+    const int nc = (nd + TR_DT - 1) / TR_DT;
+    const int64_t rb = blockIdx.x / nc;             // r * nb + b
+    const int d0 = (int)(blockIdx.x - rb * nc) * TR_DT;
     const int b = (int)(rb % nb);
     const int64_t r = rb / nb;
-    const float* s = src + ((r * nphif + c) * nb + b) * (int64_t)nd;
-    float* d = dst + row * (int64_t)nd;
-    for (int i = threadIdx.x; i < nd; i += blockDim.x) d[i] = s[i];
+    const int w = min(TR_DT, nd - d0);
+    for (int i = threadIdx.x; i < nphif * TR_DT; i += blockDim.x) {
+        const int c = i / TR_DT, dl = i - c * TR_DT;
+        if (dl < w) tile[c][dl] = src[((r * nphif + c) * nb + b) * (int64_t)nd + d0 + dl];
+    }
+    __syncthreads();
+    float* d = dst + (rb * nd + d0) * (int64_t)nphif;
+    for (int i = threadIdx.x; i < w * nphif; i += blockDim.x) {
+        const int dl = i / nphif, c = i - dl * nphif;
+        d[i] = tile[c][dl];
+    }
 }
 
-__global__ void k_transpose_out(const float* __restrict__ src, float* __restrict__ dst, int nphif, int nb, int nd) {
-    const int64_t row = blockIdx.x;
-    const int c = (int)(row % nphif);
-    const int64_t rb = row / nphif;
+__global__ __launch_bounds__(256) void k_transpose_out(const float* __restrict__ src, float* __restrict__ dst,
+                                                       int nphif, int nb, int nd) {
+    __shared__ float tile[NPHIF][TR_DT + 1];
+    const int nc = (nd + TR_DT - 1) / TR_DT;
+    const int64_t rb = blockIdx.x / nc;
+    const int d0 = (int)(blockIdx.x - rb * nc) * TR_DT;
     const int b = (int)(rb % nb);
     const int64_t r = rb / nb;
-    const float* s = src + row * (int64_t)nd;
-    float* d = dst + ((r * nphif + c) * nb + b) * (int64_t)nd;
-    for (int i = threadIdx.x; i < nd; i += blockDim.x) d[i] = s[i];
+    const int w = min(TR_DT, nd - d0);
+    const float* s = src + (rb * nd + d0) * (int64_t)nphif;
+    for (int i = threadIdx.x; i < w * nphif; i += blockDim.x) {
+        const int dl = i / nphif, c = i - dl * nphif;
+        tile[c][dl] = s[i];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nphif * TR_DT; i += blockDim.x) {
+        const int c = i / TR_DT, dl = i - c * TR_DT;
+        if (dl < w) dst[((r * nphif + c) * nb + b) * (int64_t)nd + d0 + dl] = tile[c][dl];
+    }
 }
 
 __device__ __forceinline__ uint64_t splitmix(uint64_t z) {
@@ -39,21 +63,21 @@ __device__ __forceinline__ uint64_t splitmix(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-// synthetic value at reference-layout flat index idx (integer recipe, exact in float32)
+// synthetic value at reference-layout flat index idx (integer recipe, exact in float32); one block per
+// (lag, beta) row, written in the device order
 __global__ void k_fill_synthetic(float* __restrict__ dst, const int32_t* __restrict__ tau, uint64_t seed,
                                  int64_t node, int kind, int nphif, int nb, int nd) {
-    const int64_t row = blockIdx.x;
-    const int c = (int)(row % nphif);
-    const int64_t rb = row / nphif;
+    const int64_t rb = blockIdx.x;                  // r * nb + b
     const int b = (int)(rb % nb);
     const int64_t r = rb / nb;
-    const int64_t ref_row = (r * nphif + c) * nb + b;
     const uint64_t key0 = (seed * 0x100000001B3ull) ^ ((uint64_t)node << 40);
     int64_t ta = tau[r];
     ta = ta < 0 ? -ta : ta;
     const int64_t bb = b - 56 > 0 ? b - 56 : 0;
-    for (int d = threadIdx.x; d < nd; d += blockDim.x) {
-        const int64_t idx = ref_row * nd + d;
+    float* row = dst + rb * (int64_t)nd * nphif;
+    for (int i = threadIdx.x; i < nd * nphif; i += blockDim.x) {
+        const int d = i / nphif, c = i - d * nphif;
+        const int64_t idx = ((r * nphif + c) * nb + b) * (int64_t)nd + d;
         const uint64_t h = splitmix(key0 ^ (uint64_t)idx);
         float v;
         if (kind == 1) {
@@ -66,28 +90,30 @@ __global__ void k_fill_synthetic(float* __restrict__ dst, const int32_t* __restr
             const int64_t num = 8 * bb * (64 - c) - 64 * dd - 32 * ta + hv;
             v = (float)((double)num / 4096.0);
         }
-        dst[row * (int64_t)nd + d] = v;
+        row[i] = v;
     }
 }
 
 int launch_store_transpose_in(Store* st, int k, const float* d_src) {
-    const int64_t rows = (int64_t)st->ntau * st->nb * st->nphif;
-    hipLaunchKernelGGL(k_transpose_in, dim3((unsigned)rows), dim3(256), 0, ctx().stream, d_src,
+    EXM_CHECK(st->nphif <= NPHIF, EXM_ERR_ARG, "tables of %d phase bins exceed %d", st->nphif, NPHIF);
+    const int64_t blocks = (int64_t)st->ntau * st->nb * ((st->nd + TR_DT - 1) / TR_DT);
+    hipLaunchKernelGGL(k_transpose_in, dim3((unsigned)blocks), dim3(256), 0, ctx().stream, d_src,
                        st->d_T + (int64_t)k * st->tstride, st->nphif, st->nb, st->nd);
     EXM_HIP(hipGetLastError());
     return EXM_OK;
 }
 
 int launch_store_transpose_out(Store* st, int k, float* d_dst) {
-    const int64_t rows = (int64_t)st->ntau * st->nb * st->nphif;
-    hipLaunchKernelGGL(k_transpose_out, dim3((unsigned)rows), dim3(256), 0, ctx().stream,
+    EXM_CHECK(st->nphif <= NPHIF, EXM_ERR_ARG, "tables of %d phase bins exceed %d", st->nphif, NPHIF);
+    const int64_t blocks = (int64_t)st->ntau * st->nb * ((st->nd + TR_DT - 1) / TR_DT);
+    hipLaunchKernelGGL(k_transpose_out, dim3((unsigned)blocks), dim3(256), 0, ctx().stream,
                        st->d_T + (int64_t)k * st->tstride, d_dst, st->nphif, st->nb, st->nd);
     EXM_HIP(hipGetLastError());
     return EXM_OK;
 }
 
 int launch_fill_synthetic(Store* st, int k, uint64_t seed, int64_t node, int kind) {
-    const int64_t rows = (int64_t)st->ntau * st->nb * st->nphif;
+    const int64_t rows = (int64_t)st->ntau * st->nb;
     hipLaunchKernelGGL(k_fill_synthetic, dim3((unsigned)rows), dim3(256), 0, ctx().stream,
                        st->d_T + (int64_t)k * st->tstride, st->d_tau, seed, node, kind, st->nphif, st->nb, st->nd);
     EXM_HIP(hipGetLastError());
