@@ -355,6 +355,31 @@ def test_band_gather_skewed_rows(hip, oracle, kernel, piece, n_walk):
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
 
 
+@pytest.mark.parametrize("kernel,piece", [(2, 8), (3, 4), (3, 6), (3, 8)], indirect=True)
+def test_band_gather_negative_log_weight(hip, oracle, kernel, piece):
+    """more random draws than times outside the top K: the drawn times' log weight ln((nt - K) / S)
+    (gather.py:433-435) is negative, so the drawn terms weigh less than the loud ones"""
+    from exmarg_amd.search import gather as g
+    say("gather: band kernel %d, piece %d, negative log weight" % (kernel, piece))
+    s = _tiny_store("rough", (5, 49, 24, 17), n_nodes=1, seed=12)
+    nm, nt, K, S = 3, 2048, 1024, 4096
+    traj = _rand_traj(nm, nt, seed=14)
+    traj["lqH"] *= 3.0
+    traj["lqL"] *= 3.0
+    times_dev = g.select_times(traj, s.tau_idx, K, S, seed=6)
+    times = times_dev.to_host()
+    assert np.all(times[1][:, :] <= 0.0) and np.any(times[1] < -1.0)
+    rng = np.random.default_rng(9)
+    nr = 200
+    ids = rng.integers(0, nm, nr)
+    lna = rng.uniform(-5.0, 5.0, (nr, 1))
+    tab = np.zeros((nr, 1), np.int64)
+    got = g.gather(traj, s, ids, lna, tab, times=times_dev)
+    ref = oracle.gather(traj, oracle.Tables(tables=s.tables, tau_idx=s.tau_idx), s.beta[0], s.delta[0], ids, lna,
+                        tab, times)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
+
+
 def test_wave_form_rejects_pieces_of_16(hip):
     from exmarg_amd.search import gather as g
     s = _tiny_store("smooth", (5, 49, 8, 9), n_nodes=1)
