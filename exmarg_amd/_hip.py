@@ -63,6 +63,7 @@ SIGNATURES = {
     "exm_set_gather_tile": [I32],
     "exm_set_filter_workspace": [I64],
     "exm_set_filter_precision": [I32],
+    "exm_set_filter_transform": [I32],
     "exm_get_gather_tuning": [ctypes.POINTER(ctypes.c_int)] * 4,
     "exm_profile_enable": [I32],
     "exm_profile_reset": [],

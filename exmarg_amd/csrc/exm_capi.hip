@@ -1083,6 +1083,13 @@ int exm_set_filter_precision(int p) {
     return EXM_OK;
 }
 
+int exm_set_filter_transform(int t) {
+    LOCK;
+    EXM_CHECK(t == 0 || t == 1, EXM_ERR_ARG, "filter transform must be 0 (band transform where it fits) or 1 (hipFFT)");
+    g_ctx.filter_transform = t;
+    return EXM_OK;
+}
+
 int exm_get_gather_tuning(int* kernel, int* piece, int* tile, int* precision) {
     LOCK;
     if (kernel) *kernel = g_gather_mode;

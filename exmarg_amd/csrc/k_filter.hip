@@ -4,14 +4,17 @@
 //   z   = FFT_n_time(pad)      (forward, unnormalized)
 //   lq  = 1/2 ln max(|z|^2, 1e-38),  ph = atan2(Im z, Re z)
 //
-// Weighting, transform and modulus/phase run in float64 (hipFFT Z2Z, batched over a chunk of
-// templates); the trajectories are rounded once to float32 on output.  The transform runs out of
-// place from an input buffer whose zero padding outside [lo, lo + n_freq) is written once (and again
-// only when the geometry changes), so each chunk writes only its n_freq weighted bins.  The float32 form
+// Weighting, transform and modulus/phase run in float64; the trajectories are rounded once to float32
+// on output.  The default transform is the band transform below (two passes that read only the band
+// and write the trajectories directly); exm_set_filter_transform(1), and lengths it does not take, use
+// hipFFT Z2Z batched over a chunk of templates, out of place from an input buffer whose zero padding
+// outside [lo, lo + n_freq) is written once (and again only when the geometry changes), so each chunk
+// writes only its n_freq weighted bins.  The float32 form
 // (exm_set_filter_precision(1), part of the precision form of the batched analysis) keeps the
 // weighting and the modulus/phase in float64 but transforms in float32 (C2C), as the reference's GPU
 // path does (complex64, src/exmarg/search/bank.py:66-75).
 #include <cmath>
+#include <vector>
 
 #include "exm_common.h"
 
@@ -81,6 +84,213 @@ __global__ void k_filter_post(const double2* __restrict__ z, int64_t n_time, int
     ph[o] = (float)atan2(v.y, v.x);
 }
 
+// ---------------------------------------------------------------- the band transform (float64)
+// The transform's input is zero outside the band [lo, lo + n_freq) (n_freq / n_time = 12 % at the
+// headline geometry), so the default float64 path does not write the zero-padded input nor read it:
+// with n_time = N1 * N2, N2 = 512, n = N2 n1 + n2 and k = k1 + N1 k2,
+//   A[n2][k1]     = w_N^{n2 k1} sum_{n1} x[N2 n1 + n2] w_N1^{n1 k1}     (pass A, one column per n2)
+//   z[k1 + N1 k2] = sum_{n2} A[n2][k1] w_N2^{n2 k2}                     (pass B, one column per k1)
+// with w_L = e^{-2 pi i / L}.  Pass A reads only the band bins (the weighting of k_filter_pack fused
+// into its load) and writes A (16 n_time bytes per template); pass B reads A and writes ln|z| and
+// arg z (the arithmetic of k_filter_post).  Each pass transforms its columns in LDS by Stockham stages
+// (radix 4, a last radix-2 stage when the length is 2 * 4^s), several columns interleaved so that
+// consecutive lanes touch consecutive addresses; every stage reads its values into registers, waits
+// for the workgroup, then writes its outputs in place.  Twiddles come from one table of w_N^t, t < N,
+// built on the host to ~1 ulp.
+constexpr int BT_N2 = 512;
+constexpr int BT_TILE_A = 2048;                       // complex values per pass-A workgroup
+constexpr int BT_COLS_B = 16;                         // k1 columns per pass-B workgroup
+constexpr int BT_THREADS_A = BT_TILE_A / 8;           // every thread holds 8 values per stage
+constexpr int BT_THREADS_B = BT_COLS_B * BT_N2 / 8;
+constexpr int BT_MIN_N1 = BT_COLS_B, BT_MAX_N1 = BT_TILE_A;
+constexpr double BT_TWO_PI = 6.283185307179586476925286766559;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// One Stockham stage of radix R over ncol interleaved transforms of length L, buf[idx * ncol + c]:
+// thread t takes butterflies t + q nthr (q < 8 / R), butterfly b of column b % ncol and index
+// i = b / ncol < L / R; k = i mod p, inputs x_r = buf[i + r L / R] times w_L^{k r L / (R p)}, outputs
+// at j + r p with j = (i - k) R + k (p: the length of the transforms already formed).  ncol and p are
+// powers of two and L ncol = 8 nthr.
+template <int R>
+__device__ __forceinline__ void stockham_stage(double2* buf, const double2* tw, int L, int ncol, int p, int tid,
+                                               int nthr) {
+    constexpr int NB = 8 / R;
+    const int T = L / R;
+    const int cmask = ncol - 1, cshift = __builtin_ctz((unsigned)ncol);
+    double2 v[8];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int b = tid + q * nthr;
+        const int c = b & cmask, i = b >> cshift;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[q * R + r] = buf[(i + r * T) * ncol + c];
+    }
+    __syncthreads();
+    const int tstride = L / (R * p);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int b = tid + q * nthr;
+        const int c = b & cmask, i = b >> cshift;
+        const int k = i & (p - 1);
+        double2* x = v + q * R;
+        if (p > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) x[r] = cmul(x[r], tw[k * r * tstride]);
+        }
+        const int j = (i - k) * R + k;
+        if constexpr (R == 4) {
+            const double2 s02 = cadd(x[0], x[2]), d02 = csub(x[0], x[2]);
+            const double2 s13 = cadd(x[1], x[3]), d13 = csub(x[1], x[3]);
+            buf[j * ncol + c] = cadd(s02, s13);
+            buf[(j + p) * ncol + c] = make_double2(d02.x + d13.y, d02.y - d13.x);      // d02 - i d13
+            buf[(j + 2 * p) * ncol + c] = csub(s02, s13);
+            buf[(j + 3 * p) * ncol + c] = make_double2(d02.x - d13.y, d02.y + d13.x);  // d02 + i d13
+        } else {
+            buf[j * ncol + c] = cadd(x[0], x[1]);
+            buf[(j + p) * ncol + c] = csub(x[0], x[1]);
+        }
+    }
+    __syncthreads();
+}
+
+// the forward transforms of length L of the ncol interleaved columns in buf
+__device__ __forceinline__ void stockham(double2* buf, const double2* tw, int L, int ncol, int tid, int nthr) {
+    int p = 1;
+    for (; p * 4 <= L; p *= 4) stockham_stage<4>(buf, tw, L, ncol, p, tid, nthr);
+    if (p < L) stockham_stage<2>(buf, tw, L, ncol, p, tid, nthr);
+}
+
+// pass A: columns n2_0 .. n2_0 + TA - 1 (TA = BT_TILE_A / N1) of template m0 + blockIdx.y
+__global__ __launch_bounds__(BT_THREADS_A) void k_bt_pass_a(const float2* __restrict__ W, const double2* __restrict__ d,
+                                                            const double* __restrict__ s2, int n_freq, int lo,
+                                                            int n1len, int64_t n_time, int m0,
+                                                            const double2* __restrict__ twN,
+                                                            double2* __restrict__ A) {
+    extern __shared__ double2 bsh[];
+    double2* buf = bsh;                                // [n1][TA]
+    double2* tw = bsh + BT_TILE_A;                     // w_N1^t
+    const int TA = BT_TILE_A / n1len;
+    const int j = blockIdx.y, tid = threadIdx.x;
+    const int n2_0 = blockIdx.x * TA;
+    const int64_t tstep = n_time / n1len;
+    for (int t = tid; t < n1len; t += BT_THREADS_A) tw[t] = twN[t * tstep];
+    const float2* Wm = W + (int64_t)(m0 + j) * n_freq;
+    for (int i = tid; i < BT_TILE_A; i += BT_THREADS_A) {
+        const int c = i & (TA - 1), n1 = i / TA;
+        const int f = n1 * BT_N2 + n2_0 + c - lo;
+        double2 v = make_double2(0.0, 0.0);
+        if (f >= 0 && f < n_freq) {
+            // w = conj(d) / sig2 (times 1/sig2, as numpy divides a complex by a real), times W
+            const float2 wt = Wm[f];
+            const double2 dd = d[f];
+            const double inv = 1.0 / s2[f];
+            const double wr = dd.x * inv, wi = (-dd.y) * inv;
+            const double ar = (double)wt.x, ai = (double)wt.y;
+            v = make_double2(ar * wr - ai * wi, ar * wi + ai * wr);
+        }
+        buf[i] = v;
+    }
+    __syncthreads();
+    stockham(buf, tw, n1len, TA, tid, BT_THREADS_A);
+    double2* Aj = A + (int64_t)j * n_time;
+    for (int i = tid; i < BT_TILE_A; i += BT_THREADS_A) {
+        const int c = i / n1len, k1 = i & (n1len - 1);
+        const int n2 = n2_0 + c;
+        Aj[(int64_t)n2 * n1len + k1] = cmul(buf[k1 * TA + c], twN[(int64_t)n2 * k1]);
+    }
+}
+
+// pass B: columns k1_0 .. k1_0 + 15 of template m0 + blockIdx.y, then ln|z| and arg z
+__global__ __launch_bounds__(BT_THREADS_B) void k_bt_pass_b(const double2* __restrict__ A, int n1len, int64_t n_time,
+                                                            int m0, const double2* __restrict__ twN,
+                                                            float* __restrict__ lq, float* __restrict__ ph) {
+    extern __shared__ double2 bsh[];
+    double2* buf = bsh;                                // [n2][16]
+    double2* tw = bsh + BT_COLS_B * BT_N2;             // w_512^t
+    const int j = blockIdx.y, tid = threadIdx.x;
+    const int k1_0 = blockIdx.x * BT_COLS_B;
+    const int64_t tstep = n_time / BT_N2;
+    for (int t = tid; t < BT_N2; t += BT_THREADS_B) tw[t] = twN[t * tstep];
+    const double2* Aj = A + (int64_t)j * n_time;
+    for (int i = tid; i < BT_COLS_B * BT_N2; i += BT_THREADS_B) {
+        const int c = i & (BT_COLS_B - 1), n2 = i / BT_COLS_B;
+        buf[i] = Aj[(int64_t)n2 * n1len + k1_0 + c];
+    }
+    __syncthreads();
+    stockham(buf, tw, BT_N2, BT_COLS_B, tid, BT_THREADS_B);
+    const int64_t row = (int64_t)(m0 + j) * n_time;
+    for (int i = tid; i < BT_COLS_B * BT_N2; i += BT_THREADS_B) {
+        const int c = i & (BT_COLS_B - 1), k2 = i / BT_COLS_B;
+        const double2 v = buf[i];
+        const double p = v.x * v.x + v.y * v.y;
+        const int64_t o = row + k1_0 + c + (int64_t)n1len * k2;
+        lq[o] = (float)(0.5 * log(p > 1e-38 ? p : 1e-38));
+        ph[o] = (float)atan2(v.y, v.x);
+    }
+}
+
+constexpr size_t bt_lds_a(int n1len) { return sizeof(double2) * (size_t)(BT_TILE_A + n1len); }
+constexpr size_t bt_lds_b() { return sizeof(double2) * (size_t)(BT_COLS_B * BT_N2 + BT_N2); }
+
+// whether the band transform takes transforms of length n_time (N1 = n_time / 512 a power of two in
+// [16, 2048])
+static bool band_transform_fits(int64_t n_time) {
+    if (n_time % BT_N2) return false;
+    const int64_t n1 = n_time / BT_N2;
+    return n1 >= BT_MIN_N1 && n1 <= BT_MAX_N1 && (n1 & (n1 - 1)) == 0;
+}
+
+// e^{-2 pi i t / n} for t < n (n a multiple of 8): cos and sin of angles <= pi/4 mapped by the
+// symmetries of the octants, so every entry is within ~1 ulp
+static void twiddle_table(int64_t n, std::vector<double2>& w) {
+    w.resize((size_t)n);
+    const int64_t q4 = n / 4, q8 = n / 8;
+    for (int64_t t = 0; t < n; ++t) {
+        const int64_t quad = t / q4, r = t - quad * q4;
+        double c, s;                                   // cos and sin of 2 pi r / n, r < n / 4
+        if (r <= q8) {
+            const double a = BT_TWO_PI * ((double)r / (double)n);
+            c = std::cos(a);
+            s = std::sin(a);
+        } else {
+            const double a = BT_TWO_PI * ((double)(q4 - r) / (double)n);
+            c = std::sin(a);
+            s = std::cos(a);
+        }
+        double cc, ss;                                 // of 2 pi t / n = quad pi / 2 + 2 pi r / n
+        switch (quad) {
+            case 0: cc = c; ss = s; break;
+            case 1: cc = -s; ss = c; break;
+            case 2: cc = -c; ss = -s; break;
+            default: cc = s; ss = -c; break;
+        }
+        w[(size_t)t] = make_double2(cc, -ss);
+    }
+}
+
+// the device twiddle table of length n, built once per length and kept
+static int twiddles(int64_t n, const double2** out) {
+    Context& c = ctx();
+    auto it = c.bt_twiddles.find(n);
+    if (it != c.bt_twiddles.end()) {
+        *out = (const double2*)it->second;
+        return EXM_OK;
+    }
+    std::vector<double2> w;
+    twiddle_table(n, w);
+    void* d = nullptr;
+    EXM_HIP(hipMalloc(&d, sizeof(double2) * (size_t)n));
+    EXM_HIP(hipMemcpy(d, w.data(), sizeof(double2) * (size_t)n, hipMemcpyHostToDevice));
+    c.bt_twiddles[n] = d;
+    *out = (const double2*)d;
+    return EXM_OK;
+}
+
 static int fft_plan(int64_t n_time, int batch, bool f32, hipfftHandle* out) {
     Context& c = ctx();
     const uint64_t key = ((uint64_t)n_time << 21) | ((uint64_t)f32 << 20) | (uint64_t)batch;
@@ -101,14 +311,66 @@ static int fft_plan(int64_t n_time, int batch, bool f32, hipfftHandle* out) {
     return EXM_OK;
 }
 
+// the float64 band transform of every template for one segment's weighted data (d, s2 on the device)
+static int filter_band(Bank* bank, const double2* const d[2], const double* const s2[2], Traj* traj) {
+    Context& c = ctx();
+    const int64_t n_time = bank->n_time;
+    const int nf = bank->n_freq;
+    const int n1len = (int)(n_time / BT_N2);
+    // templates per pass pair: A (16 n_time bytes per template) in the workspace, 64 MiB by default so
+    // that pass B reads it from the last-level cache
+    const int64_t ws = c.filter_bytes > 0 ? c.filter_bytes : (64ll << 20);
+    const int chunk = (int)std::min<int64_t>(bank->n_tmpl, std::max<int64_t>(1, ws / ((int64_t)sizeof(double2) * n_time)));
+    void* A = nullptr;
+    EXM_TRY(scratch(0, sizeof(double2) * (size_t)chunk * n_time, &A));
+    const double2* twN = nullptr;
+    EXM_TRY(twiddles(n_time, &twN));
+    const size_t lds_a = bt_lds_a(n1len), lds_b = bt_lds_b();
+    EXM_HIP(hipFuncSetAttribute((const void*)k_bt_pass_a, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_a));
+    EXM_HIP(hipFuncSetAttribute((const void*)k_bt_pass_b, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+    for (int det = 0; det < 2; ++det) {
+        float* lq = det == 0 ? traj->lqH() : traj->lqL();
+        float* ph = det == 0 ? traj->phH() : traj->phL();
+        for (int m0 = 0; m0 < bank->n_tmpl; m0 += chunk) {
+            const int nb = std::min(chunk, bank->n_tmpl - m0);
+            hipLaunchKernelGGL(k_bt_pass_a, dim3((unsigned)(BT_N2 * n1len / BT_TILE_A), (unsigned)nb),
+                               dim3(BT_THREADS_A), lds_a, c.stream, bank->d_W, d[det], s2[det], nf, bank->lo, n1len,
+                               n_time, m0, twN, (double2*)A);
+            EXM_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_bt_pass_b, dim3((unsigned)(n1len / BT_COLS_B), (unsigned)nb), dim3(BT_THREADS_B),
+                               lds_b, c.stream, (const double2*)A, n1len, n_time, m0, twN, lq, ph);
+            EXM_HIP(hipGetLastError());
+        }
+    }
+    return EXM_OK;
+}
+
 int launch_filter(Bank* bank, const double* dH, const double* dL, const double* s2H, const double* s2L, Traj* traj) {
     Context& c = ctx();
     const int64_t n_time = bank->n_time;
     const int nf = bank->n_freq;
-    // chunk of templates whose padded rows fit in the workspace (64 MiB by default: the transform's input
-    // and output, 128 MiB together, stay in the 256 MB last-level cache; measured for 2000 templates x
-    // 131072 samples: 10.8 ms, against 12.1 ms with 128 MiB and 12.5 ms with 32 MiB)
+    EXM_CHECK(bank->lo >= 0 && (int64_t)bank->lo + nf <= n_time, EXM_ERR_ARG, "band [%d, %d) outside %lld samples",
+              bank->lo, bank->lo + nf, (long long)n_time);
     const bool f32 = c.filter_f32 != 0;
+    double* seg = nullptr;
+    EXM_TRY(scratch(2, sizeof(double) * 6 * (size_t)nf + 256, (void**)&seg));
+    double2* ddH = (double2*)seg;
+    double2* ddL = ddH + nf;
+    double* ds2H = (double*)(ddL + nf);
+    double* ds2L = ds2H + nf;
+    EXM_HIP(hipMemcpyAsync(ddH, dH, sizeof(double2) * nf, hipMemcpyHostToDevice, c.stream));
+    EXM_HIP(hipMemcpyAsync(ddL, dL, sizeof(double2) * nf, hipMemcpyHostToDevice, c.stream));
+    EXM_HIP(hipMemcpyAsync(ds2H, s2H, sizeof(double) * nf, hipMemcpyHostToDevice, c.stream));
+    EXM_HIP(hipMemcpyAsync(ds2L, s2L, sizeof(double) * nf, hipMemcpyHostToDevice, c.stream));
+    ProfileScope prof("filter");
+    if (!f32 && c.filter_transform == 0 && band_transform_fits(n_time)) {
+        const double2* d[2] = {ddH, ddL};
+        const double* s2[2] = {ds2H, ds2L};
+        return filter_band(bank, d, s2, traj);
+    }
+    // hipFFT: chunk of templates whose padded rows fit in the workspace (64 MiB by default: the
+    // transform's input and output, 128 MiB together, stay in the 256 MB last-level cache; measured for
+    // 2000 templates x 131072 samples: 10.8 ms, against 12.1 ms with 128 MiB and 12.5 ms with 32 MiB)
     const int64_t elem = f32 ? (int64_t)sizeof(float2) : (int64_t)sizeof(double2);
     const int64_t ws = c.filter_bytes > 0 ? c.filter_bytes : (64ll << 20);
     const int chunk_max = (int)std::max<int64_t>(1, ws / (elem * n_time));
@@ -130,17 +392,6 @@ int launch_filter(Bank* bank, const double* dH, const double* dL, const double* 
         c.filter_in_nf = nf;
         c.filter_in_f32 = (int)f32;
     }
-    double* seg = nullptr;
-    EXM_TRY(scratch(2, sizeof(double) * 6 * (size_t)nf + 256, (void**)&seg));
-    double2* ddH = (double2*)seg;
-    double2* ddL = ddH + nf;
-    double* ds2H = (double*)(ddL + nf);
-    double* ds2L = ds2H + nf;
-    EXM_HIP(hipMemcpyAsync(ddH, dH, sizeof(double2) * nf, hipMemcpyHostToDevice, c.stream));
-    EXM_HIP(hipMemcpyAsync(ddL, dL, sizeof(double2) * nf, hipMemcpyHostToDevice, c.stream));
-    EXM_HIP(hipMemcpyAsync(ds2H, s2H, sizeof(double) * nf, hipMemcpyHostToDevice, c.stream));
-    EXM_HIP(hipMemcpyAsync(ds2L, s2L, sizeof(double) * nf, hipMemcpyHostToDevice, c.stream));
-    ProfileScope prof("filter");
     for (int det = 0; det < 2; ++det) {
         const double2* d = det == 0 ? ddH : ddL;
         const double* s2 = det == 0 ? ds2H : ds2L;

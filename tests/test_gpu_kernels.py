@@ -72,6 +72,45 @@ def test_filter_matches_fp64_oracle(hip, oracle):
         assert np.abs(dph).max() < 1e-6, k
 
 
+@pytest.mark.parametrize("T,fmin,fmax", [(1.0, 20.0, 1024.0), (2.0, 20.0, 1024.0), (16.0, 20.0, 1024.0),
+                                          (64.0, 20.0, 1024.0), (2.0, 100.0, 1024.0)])
+def test_band_transform_matches_fp64_oracle(hip, oracle, T, fmin, fmax):
+    """the default float64 filter (the band transform: two passes reading only the band) against the
+    FP64 oracle (numpy's transform of the zero-padded band) and against the hipFFT path, from the
+    shortest length it takes (n_time / 512 = 16) through one with a radix-2 stage to the longest segment"""
+    from exmarg_amd.search.bank import MassBank
+    seg = _segment(T=T, fmin=fmin, fmax=fmax, seed=5)
+    bank_rows = synth.chirp_mass_grid_bank(30.0, 0.8, 0.1, 6)
+    bank = MassBank(bank_rows[:, 0], bank_rows[:, 1], bank_rows[:, 2], seg)
+    n1 = bank.n_time // 512
+    say("filter: band transform, T=%g s, %g-%g Hz, n_time %d, band %d bins" % (T, fmin, fmax, bank.n_time,
+                                                                             bank.W.shape[1]))
+    assert bank.n_time % 512 == 0 and 16 <= n1 <= 2048 and n1 & (n1 - 1) == 0
+    d = {I: seg["d_" + I][0] for I in "HL"}
+    s2 = {I: seg["sig2"] for I in "HL"}
+    got = bank.trajectories(d, s2).to_host()
+    hip.call("exm_set_filter_transform", 1)
+    try:
+        fft = bank.trajectories(d, s2).to_host()
+    finally:
+        hip.call("exm_set_filter_transform", 0)
+    ref = oracle.filter_fp64(bank.W, bank.lo, bank.n_time, d, s2)
+    for k in ("lqH", "lqL"):
+        ulp = np.spacing(np.abs(ref[k])).astype(np.float64)
+        for g in (got, fft):
+            assert (np.abs(g[k].astype(np.float64) - ref[k]) <= ulp).all(), k
+            assert np.mean(g[k] == ref[k]) > 0.999, k
+        assert (np.abs(got[k].astype(np.float64) - fft[k]) <= 2 * ulp).all(), k
+    for k in ("phH", "phL"):
+        dph = np.angle(np.exp(1j * (got[k].astype(np.float64) - ref[k])))
+        assert np.abs(dph).max() < 1e-6, k
+
+
+def test_filter_transform_choice_is_checked(hip):
+    with pytest.raises(ValueError, match="filter transform"):
+        hip.call("exm_set_filter_transform", 2)
+
+
 def test_filter_float32_transform(hip, oracle):
     """the float32 transform (precision form) against the FP64 oracle: ln|z| and the phase within 2e-5
     wherever |z| is above a hundredth of its maximum (the transform's float32 error is relative to the
