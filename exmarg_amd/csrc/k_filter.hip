@@ -116,26 +116,24 @@ __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_doub
 // i = b / ncol < L / R; k = i mod p, inputs x_r = buf[i + r L / R] times w_L^{k r L / (R p)}, outputs
 // at j + r p with j = (i - k) R + k (p: the length of the transforms already formed).  ncol and p are
 // powers of two and L ncol = 8 nthr.
-template <int R>
-__device__ __forceinline__ void stockham_stage(double2* buf, const double2* tw, int L, int ncol, int p, int tid,
-                                               int nthr) {
-    constexpr int NB = 8 / R;
-    const int T = L / R;
-    const int cmask = ncol - 1, cshift = __builtin_ctz((unsigned)ncol);
+template <int R, int L, int NCOL, int NTHR>
+__device__ __forceinline__ void stockham_stage(double2* buf, const double2* tw, int p, int tid) {
+    static_assert(L * NCOL == 8 * NTHR, "every thread holds 8 values per stage");
+    constexpr int NB = 8 / R, T = L / R;
     double2 v[8];
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
-        const int b = tid + q * nthr;
-        const int c = b & cmask, i = b >> cshift;
+        const int b = tid + q * NTHR;
+        const int c = b % NCOL, i = b / NCOL;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[q * R + r] = buf[(i + r * T) * ncol + c];
+        for (int r = 0; r < R; ++r) v[q * R + r] = buf[(i + r * T) * NCOL + c];
     }
     __syncthreads();
     const int tstride = L / (R * p);
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
-        const int b = tid + q * nthr;
-        const int c = b & cmask, i = b >> cshift;
+        const int b = tid + q * NTHR;
+        const int c = b % NCOL, i = b / NCOL;
         const int k = i & (p - 1);
         double2* x = v + q * R;
         if (p > 1) {
@@ -146,42 +144,107 @@ __device__ __forceinline__ void stockham_stage(double2* buf, const double2* tw, 
         if constexpr (R == 4) {
             const double2 s02 = cadd(x[0], x[2]), d02 = csub(x[0], x[2]);
             const double2 s13 = cadd(x[1], x[3]), d13 = csub(x[1], x[3]);
-            buf[j * ncol + c] = cadd(s02, s13);
-            buf[(j + p) * ncol + c] = make_double2(d02.x + d13.y, d02.y - d13.x);      // d02 - i d13
-            buf[(j + 2 * p) * ncol + c] = csub(s02, s13);
-            buf[(j + 3 * p) * ncol + c] = make_double2(d02.x - d13.y, d02.y + d13.x);  // d02 + i d13
+            buf[j * NCOL + c] = cadd(s02, s13);
+            buf[(j + p) * NCOL + c] = make_double2(d02.x + d13.y, d02.y - d13.x);      // d02 - i d13
+            buf[(j + 2 * p) * NCOL + c] = csub(s02, s13);
+            buf[(j + 3 * p) * NCOL + c] = make_double2(d02.x - d13.y, d02.y + d13.x);  // d02 + i d13
         } else {
-            buf[j * ncol + c] = cadd(x[0], x[1]);
-            buf[(j + p) * ncol + c] = csub(x[0], x[1]);
+            buf[j * NCOL + c] = cadd(x[0], x[1]);
+            buf[(j + p) * NCOL + c] = csub(x[0], x[1]);
         }
     }
     __syncthreads();
 }
 
-// the forward transforms of length L of the ncol interleaved columns in buf
-__device__ __forceinline__ void stockham(double2* buf, const double2* tw, int L, int ncol, int tid, int nthr) {
+// the forward transforms of length L of the NCOL interleaved columns in buf
+template <int L, int NCOL, int NTHR>
+__device__ __forceinline__ void stockham(double2* buf, const double2* tw, int tid) {
     int p = 1;
-    for (; p * 4 <= L; p *= 4) stockham_stage<4>(buf, tw, L, ncol, p, tid, nthr);
-    if (p < L) stockham_stage<2>(buf, tw, L, ncol, p, tid, nthr);
+#pragma unroll
+    for (; p * 4 <= L; p *= 4) stockham_stage<4, L, NCOL, NTHR>(buf, tw, p, tid);
+    if (p < L) stockham_stage<2, L, NCOL, NTHR>(buf, tw, p, tid);
+}
+
+// ln p for finite p > 0, for outputs rounded to float32 (relative error ~3e-16): p = m 2^e with m in
+// [1/sqrt 2, sqrt 2), ln m = 2 atanh(s) = 2 s (1 + z/3 + z^2/5 + ...), s = (m - 1) / (m + 1), z = s^2
+// <= 0.0295, nine terms (the first omitted z^9 / 19 < 1e-15)
+__device__ __forceinline__ double ln_pos(double p) {
+    int e;
+    double m = frexp(p, &e);
+    const bool low = m < 0.70710678118654752440;
+    m = low ? 2.0 * m : m;
+    e = low ? e - 1 : e;
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double q = 1.0 / 17.0;
+    q = fma(q, z, 1.0 / 15.0);
+    q = fma(q, z, 1.0 / 13.0);
+    q = fma(q, z, 1.0 / 11.0);
+    q = fma(q, z, 1.0 / 9.0);
+    q = fma(q, z, 1.0 / 7.0);
+    q = fma(q, z, 1.0 / 5.0);
+    q = fma(q, z, 1.0 / 3.0);
+    const double s2 = s + s;
+    return fma((double)e, 0.69314718055994530942, fma(s2 * z, q, s2));
+}
+
+// atan2(y, x) for outputs rounded to float32 (error ~2e-16 of the result): the angle from the nearer
+// axis, a = atan(mn / mx) in [0, pi/4], is atan(t) with t = mn / mx up to tan(pi/8), else pi/4 +
+// atan(t) with t = (mn - mx) / (mn + mx) (one division either way), |t| <= tan(pi/8) = 0.414, by 19 terms
+// of t (1 - z/3 + z^2/5 - ...) (the first omitted t z^19 / 39 < 1e-16 t); then the octant.
+// atan2(+-0, x) is +-0 for x >= 0 and +-pi for x < 0.
+__device__ __forceinline__ double atan2_f(double y, double x) {
+    const double ax = fabs(x), ay = fabs(y);
+    const double mx = fmax(ax, ay), mn = fmin(ax, ay);
+    const bool mid = mn > 0.41421356237309504880 * mx;
+    const double num = mid ? mn - mx : mn;
+    const double den = mid ? mn + mx : (mx > 0.0 ? mx : 1.0);
+    const double t = num / den;
+    const double z = t * t;
+    double q = 1.0 / 39.0;
+    q = fma(q, -z, 1.0 / 37.0);
+    q = fma(q, -z, 1.0 / 35.0);
+    q = fma(q, -z, 1.0 / 33.0);
+    q = fma(q, -z, 1.0 / 31.0);
+    q = fma(q, -z, 1.0 / 29.0);
+    q = fma(q, -z, 1.0 / 27.0);
+    q = fma(q, -z, 1.0 / 25.0);
+    q = fma(q, -z, 1.0 / 23.0);
+    q = fma(q, -z, 1.0 / 21.0);
+    q = fma(q, -z, 1.0 / 19.0);
+    q = fma(q, -z, 1.0 / 17.0);
+    q = fma(q, -z, 1.0 / 15.0);
+    q = fma(q, -z, 1.0 / 13.0);
+    q = fma(q, -z, 1.0 / 11.0);
+    q = fma(q, -z, 1.0 / 9.0);
+    q = fma(q, -z, 1.0 / 7.0);
+    q = fma(q, -z, 1.0 / 5.0);
+    q = fma(q, -z, 1.0 / 3.0);
+    double a = fma(-t * z, q, t);
+    a = mid ? a + 0.78539816339744830962 : a;
+    a = ay > ax ? 1.57079632679489661923 - a : a;
+    a = x < 0.0 ? 3.14159265358979323846 - a : a;
+    return copysign(a, y);
 }
 
 // pass A: columns n2_0 .. n2_0 + TA - 1 (TA = BT_TILE_A / N1) of template m0 + blockIdx.y
+template <int N1>
 __global__ __launch_bounds__(BT_THREADS_A) void k_bt_pass_a(const float2* __restrict__ W, const double2* __restrict__ d,
                                                             const double* __restrict__ s2, int n_freq, int lo,
-                                                            int n1len, int64_t n_time, int m0,
-                                                            const double2* __restrict__ twN,
+                                                            int64_t n_time, int m0, const double2* __restrict__ twN,
                                                             double2* __restrict__ A) {
-    extern __shared__ double2 bsh[];
-    double2* buf = bsh;                                // [n1][TA]
-    double2* tw = bsh + BT_TILE_A;                     // w_N1^t
-    const int TA = BT_TILE_A / n1len;
+    constexpr int TA = BT_TILE_A / N1;
+    __shared__ double2 buf[BT_TILE_A];                 // [n1][TA]
+    __shared__ double2 tw[N1];                         // w_N1^t = w_N^{512 t}
+    __shared__ double2 twl[BT_N2];                     // w_N^t, t < 512
     const int j = blockIdx.y, tid = threadIdx.x;
     const int n2_0 = blockIdx.x * TA;
-    const int64_t tstep = n_time / n1len;
-    for (int t = tid; t < n1len; t += BT_THREADS_A) tw[t] = twN[t * tstep];
+    for (int t = tid; t < N1; t += BT_THREADS_A) tw[t] = twN[(int64_t)t * BT_N2];
+    for (int t = tid; t < BT_N2; t += BT_THREADS_A) twl[t] = twN[t];
     const float2* Wm = W + (int64_t)(m0 + j) * n_freq;
     for (int i = tid; i < BT_TILE_A; i += BT_THREADS_A) {
-        const int c = i & (TA - 1), n1 = i / TA;
+        const int c = i % TA, n1 = i / TA;
         const int f = n1 * BT_N2 + n2_0 + c - lo;
         double2 v = make_double2(0.0, 0.0);
         if (f >= 0 && f < n_freq) {
@@ -196,12 +259,14 @@ __global__ __launch_bounds__(BT_THREADS_A) void k_bt_pass_a(const float2* __rest
         buf[i] = v;
     }
     __syncthreads();
-    stockham(buf, tw, n1len, TA, tid, BT_THREADS_A);
+    stockham<N1, TA, BT_THREADS_A>(buf, tw, tid);
+    // times w_N^{n2 k1} = w_N^{512 u} w_N^{v} with n2 k1 = 512 u + v (n2 k1 < 512 N1)
     double2* Aj = A + (int64_t)j * n_time;
     for (int i = tid; i < BT_TILE_A; i += BT_THREADS_A) {
-        const int c = i / n1len, k1 = i & (n1len - 1);
+        const int c = i / N1, k1 = i % N1;
         const int n2 = n2_0 + c;
-        Aj[(int64_t)n2 * n1len + k1] = cmul(buf[k1 * TA + c], twN[(int64_t)n2 * k1]);
+        const int e = n2 * k1;
+        Aj[(int64_t)n2 * N1 + k1] = cmul(buf[k1 * TA + c], cmul(tw[e / BT_N2], twl[e % BT_N2]));
     }
 }
 
@@ -222,19 +287,18 @@ __global__ __launch_bounds__(BT_THREADS_B) void k_bt_pass_b(const double2* __res
         buf[i] = Aj[(int64_t)n2 * n1len + k1_0 + c];
     }
     __syncthreads();
-    stockham(buf, tw, BT_N2, BT_COLS_B, tid, BT_THREADS_B);
+    stockham<BT_N2, BT_COLS_B, BT_THREADS_B>(buf, tw, tid);
     const int64_t row = (int64_t)(m0 + j) * n_time;
     for (int i = tid; i < BT_COLS_B * BT_N2; i += BT_THREADS_B) {
         const int c = i & (BT_COLS_B - 1), k2 = i / BT_COLS_B;
         const double2 v = buf[i];
         const double p = v.x * v.x + v.y * v.y;
         const int64_t o = row + k1_0 + c + (int64_t)n1len * k2;
-        lq[o] = (float)(0.5 * log(p > 1e-38 ? p : 1e-38));
-        ph[o] = (float)atan2(v.y, v.x);
+        lq[o] = (float)(0.5 * ln_pos(p > 1e-38 ? p : 1e-38));
+        ph[o] = (float)atan2_f(v.y, v.x);
     }
 }
 
-constexpr size_t bt_lds_a(int n1len) { return sizeof(double2) * (size_t)(BT_TILE_A + n1len); }
 constexpr size_t bt_lds_b() { return sizeof(double2) * (size_t)(BT_COLS_B * BT_N2 + BT_N2); }
 
 // whether the band transform takes transforms of length n_time (N1 = n_time / 512 a power of two in
@@ -325,17 +389,27 @@ static int filter_band(Bank* bank, const double2* const d[2], const double* cons
     EXM_TRY(scratch(0, sizeof(double2) * (size_t)chunk * n_time, &A));
     const double2* twN = nullptr;
     EXM_TRY(twiddles(n_time, &twN));
-    const size_t lds_a = bt_lds_a(n1len), lds_b = bt_lds_b();
-    EXM_HIP(hipFuncSetAttribute((const void*)k_bt_pass_a, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_a));
+    const size_t lds_b = bt_lds_b();
     EXM_HIP(hipFuncSetAttribute((const void*)k_bt_pass_b, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+    void (*pass_a)(const float2*, const double2*, const double*, int, int, int64_t, int, const double2*, double2*) = nullptr;
+    switch (n1len) {
+        case 16: pass_a = k_bt_pass_a<16>; break;
+        case 32: pass_a = k_bt_pass_a<32>; break;
+        case 64: pass_a = k_bt_pass_a<64>; break;
+        case 128: pass_a = k_bt_pass_a<128>; break;
+        case 256: pass_a = k_bt_pass_a<256>; break;
+        case 512: pass_a = k_bt_pass_a<512>; break;
+        case 1024: pass_a = k_bt_pass_a<1024>; break;
+        case 2048: pass_a = k_bt_pass_a<2048>; break;
+        default: EXM_CHECK(false, EXM_ERR_ARG, "band transform of %d x 512 points", n1len);
+    }
     for (int det = 0; det < 2; ++det) {
         float* lq = det == 0 ? traj->lqH() : traj->lqL();
         float* ph = det == 0 ? traj->phH() : traj->phL();
         for (int m0 = 0; m0 < bank->n_tmpl; m0 += chunk) {
             const int nb = std::min(chunk, bank->n_tmpl - m0);
-            hipLaunchKernelGGL(k_bt_pass_a, dim3((unsigned)(BT_N2 * n1len / BT_TILE_A), (unsigned)nb),
-                               dim3(BT_THREADS_A), lds_a, c.stream, bank->d_W, d[det], s2[det], nf, bank->lo, n1len,
-                               n_time, m0, twN, (double2*)A);
+            hipLaunchKernelGGL(pass_a, dim3((unsigned)(BT_N2 * n1len / BT_TILE_A), (unsigned)nb), dim3(BT_THREADS_A), 0,
+                               c.stream, bank->d_W, d[det], s2[det], nf, bank->lo, n_time, m0, twN, (double2*)A);
             EXM_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_bt_pass_b, dim3((unsigned)(n1len / BT_COLS_B), (unsigned)nb), dim3(BT_THREADS_B),
                                lds_b, c.stream, (const double2*)A, n1len, n_time, m0, twN, lq, ph);
