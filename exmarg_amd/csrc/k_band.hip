@@ -29,7 +29,8 @@
 namespace exm {
 
 constexpr int WALK_THREADS = 1024;
-constexpr int BAND_THREADS = 1024;
+constexpr int BAND_THREADS = 1024;        // the piece form's workgroup
+constexpr int WAVE_THREADS = 1024;        // the wave form's workgroup (64 reads per wave)
 constexpr double FC_SCALE = 16777216.0;   // phase weights in fixed point, units of 2^-24
 constexpr uint32_t FC_MASK = 0x1FFFFFFu;  // 25 bits: the weight of the upper bin, 0 .. 2^24
 constexpr uint32_t DRAWN = 0x80000000u;   // walked time from the random draw (log weight lw_rand)
@@ -243,8 +244,9 @@ __host__ __device__ constexpr size_t piece_lds_bytes(int rowlen) {
 // copy one table row (slot floats, whole 64-float blocks) from global memory straight into LDS:
 // wave w moves blocks w, w + 16, ...; each lane one float of a block, no register staging.  The
 // loads complete asynchronously; the caller waits (s_waitcnt) before the barrier that publishes them.
-__device__ __forceinline__ void row_to_lds(const float* __restrict__ src, float* dst, int slot, int wid, int lane) {
-    for (int k = wid * 64; k < slot; k += BAND_THREADS) {
+__device__ __forceinline__ void row_to_lds(const float* __restrict__ src, float* dst, int slot, int wid, int lane,
+                                           int nthr) {
+    for (int k = wid * 64; k < slot; k += nthr) {
         __builtin_amdgcn_global_load_lds((const void*)(src + k + lane),
                                          (__attribute__((address_space(3))) void*)(dst + k), 4, 0, 0);
     }
@@ -339,14 +341,15 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     // rows b_start .. b_start + 2 (as far as the band and the table reach)
     const int b_top = min(b_end + 1, nb - 1);
     for (int q = 0; q < 3 && b_start + q <= b_top; ++q)
-        row_to_lds(Tr + (int64_t)(b_start + q) * rowlen, rows + ((b_start + q) % 3) * slot, slot, wid, lane);
+        row_to_lds(Tr + (int64_t)(b_start + q) * rowlen, rows + ((b_start + q) % 3) * slot, slot, wid, lane,
+                   BAND_THREADS);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
     for (int b = b_start; b <= b_end; ++b) {
         // row b + 2 into the slot row b - 1 left; it lands while row b is read
         if (b > b_start && b + 2 <= b_top)
-            row_to_lds(Tr + (int64_t)(b + 2) * rowlen, rows + ((b + 2) % 3) * slot, slot, wid, lane);
+            row_to_lds(Tr + (int64_t)(b + 2) * rowlen, rows + ((b + 2) % 3) * slot, slot, wid, lane, BAND_THREADS);
         // walk range and pieces of every read in this row
         int lo_p = 0, hi_p = 0;
         if (tid < nj && b >= my_first && b <= my_last) {
@@ -548,14 +551,14 @@ __device__ __forceinline__ double dpp_d(double old, double v) {
 // latency.  Merges are float64; their order follows lane positions, so a read's result depends on its
 // neighbours in the workgroup only through float64 rounding.
 template <int PN, typename V>
-__global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
+__global__ __launch_bounds__(WAVE_THREADS) void k_gather_wave(
     const uint2* __restrict__ stream, const float* __restrict__ wa, int nw, const float* __restrict__ lw_g,
     const float* __restrict__ T, int64_t tstride, int nphif, int nb, int nd, int ntau,
     const int32_t* __restrict__ jm, const int32_t* __restrict__ jtab, const double* __restrict__ joffb,
     const int32_t* __restrict__ tiles, const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds,
     double rdb, double offd, double rdd, double* __restrict__ om, double* __restrict__ os) {
     extern __shared__ __align__(16) unsigned char lds[];
-    constexpr int NWAVE = BAND_THREADS / 64;
+    constexpr int NWAVE = WAVE_THREADS / 64;
     const int rowlen = nphif * nd, slot = row_slot(rowlen);
     float* rows = (float*)lds;                                              // 3 ring slots
     int* band = (int*)(lds + piece_res_offset(rowlen));                     // band range of the tile
@@ -594,7 +597,8 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
     const int b_start = band[0], b_end = band[1];
     const int b_top = min(b_end + 1, nb - 1);
     for (int q = 0; q < 3 && b_start + q <= b_top; ++q)
-        row_to_lds(Tr + (int64_t)(b_start + q) * rowlen, rows + ((b_start + q) % 3) * slot, slot, wid, lane);
+        row_to_lds(Tr + (int64_t)(b_start + q) * rowlen, rows + ((b_start + q) % 3) * slot, slot, wid, lane,
+                   WAVE_THREADS);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
@@ -672,7 +676,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_wave(
     bool pre = false;                           // pa_ holds row b's first piece
     for (int b = b_start; b <= b_end; ++b) {
         if (b > b_start && b + 2 <= b_top)
-            row_to_lds(Tr + (int64_t)(b + 2) * rowlen, rows + ((b + 2) % 3) * slot, slot, wid, lane);
+            row_to_lds(Tr + (int64_t)(b + 2) * rowlen, rows + ((b + 2) % 3) * slot, slot, wid, lane, WAVE_THREADS);
         int f_lo, f_hi;
         bounds_of(b + 2, f_lo, f_hi);           // row b + 2's walk range, used at the end of this row
         const Layout N = layout(n_lo, n_hi);    // row b + 1
@@ -839,6 +843,7 @@ int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const 
     // the wave form unless the piece form is asked for; 6 walked times per piece in the wave form (measured
     // at 2000 templates x 2048 walked times x 53 nodes: 69.2 ms against 72.9 ms with 4 and 73.6 ms with 8)
     const bool wave = gather_mode() != 2;
+    static_assert(TILE_JOBS <= WAVE_THREADS && TILE_JOBS <= BAND_THREADS, "a workgroup holds every read of its tile");
     const int rowlen = st->nphif * st->nd;
     const size_t lds = wave ? piece_res_offset(rowlen) + 16 : piece_lds_bytes(rowlen);
     EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "three table rows and the piece results (%zu bytes) exceed the LDS", lds);
@@ -858,7 +863,8 @@ int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const 
                                                                               : k_gather_piece<8, double>);
     EXM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     ProfileScope prof("gather");
-    hipLaunchKernelGGL(kern, dim3(js.ntiles, st->ntau), dim3(BAND_THREADS), lds, cs, stream, wa, nw, lw_g,
+    hipLaunchKernelGGL(kern, dim3(js.ntiles, st->ntau), dim3(wave ? WAVE_THREADS : BAND_THREADS), lds, cs, stream,
+                       wa, nw, lw_g,
                        st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab, js.d_offb,
                        js.d_tiles, first_last, bounds, rdb, -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os);
     EXM_HIP(hipGetLastError());
