@@ -7,9 +7,9 @@
 // Weighting, transform and modulus/phase run in float64; the trajectories are rounded once to float32
 // on output.  The default transform is the band transform below (two passes that read only the band
 // and write the trajectories directly); exm_set_filter_transform(1), and lengths it does not take, use
-// hipFFT Z2Z batched over a chunk of templates, out of place from an input buffer whose zero padding
-// outside [lo, lo + n_freq) is written once (and again only when the geometry changes), so each chunk
-// writes only its n_freq weighted bins.  The float32 form
+// hipFFT (Z2Z, C2C in the float32 form) batched over a chunk of templates, out of place from an input
+// buffer whose zero padding outside [lo, lo + n_freq) is written once (and again only when the geometry
+// changes), so each chunk writes only its n_freq weighted bins.  The float32 form
 // (exm_set_filter_precision(1), part of the precision form of the batched analysis) keeps the
 // weighting and the modulus/phase in float64 but transforms in float32 (C2C), as the reference's GPU
 // path does (complex64, src/exmarg/search/bank.py:66-75).
@@ -87,7 +87,7 @@ __global__ void k_filter_post(const double2* __restrict__ z, int64_t n_time, int
 
 // ---------------------------------------------------------------- the band transform (float64)
 // The transform's input is zero outside the band [lo, lo + n_freq) (n_freq / n_time = 12 % at the
-// headline geometry), so the default float64 path does not write the zero-padded input nor read it:
+// headline geometry), so the default path does not write the zero-padded input nor read it:
 // with n_time = N1 * N2, N2 = 512, n = N2 n1 + n2 and k = k1 + N1 k2,
 //   A[n2][k1]     = w_N^{n2 k1} sum_{n1} x[N2 n1 + n2] w_N1^{n1 k1}     (pass A, one column per n2)
 //   z[k1 + N1 k2] = sum_{n2} A[n2][k1] w_N2^{n2 k2}                     (pass B, one column per k1)
@@ -111,17 +111,28 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// a complex value of the transform's type C (double2, or float2 for the float32 form) from two doubles
+template <typename C>
+__device__ __forceinline__ C cmake(double re, double im) {
+    if constexpr (sizeof(C) == 16) return make_double2(re, im);
+    else return make_float2((float)re, (float)im);
+}
 
 // One Stockham stage of radix R over ncol interleaved transforms of length L, buf[idx * ncol + c]:
 // thread t takes butterflies t + q nthr (q < 8 / R), butterfly b of column b % ncol and index
 // i = b / ncol < L / R; k = i mod p, inputs x_r = buf[i + r L / R] times w_L^{k r L / (R p)}, outputs
 // at j + r p with j = (i - k) R + k (p: the length of the transforms already formed).  ncol and p are
 // powers of two and L ncol = 8 nthr.
-template <int R, int L, int NCOL, int NTHR>
-__device__ __forceinline__ void stockham_stage(double2* buf, const double2* tw, int p, int tid) {
+template <int R, int L, int NCOL, int NTHR, typename C>
+__device__ __forceinline__ void stockham_stage(C* buf, const C* tw, int p, int tid) {
     static_assert(L * NCOL == 8 * NTHR, "every thread holds 8 values per stage");
     constexpr int NB = 8 / R, T = L / R;
-    double2 v[8];
+    C v[8];
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int b = tid + q * NTHR;
@@ -136,19 +147,19 @@ __device__ __forceinline__ void stockham_stage(double2* buf, const double2* tw, 
         const int b = tid + q * NTHR;
         const int c = b % NCOL, i = b / NCOL;
         const int k = i & (p - 1);
-        double2* x = v + q * R;
+        C* x = v + q * R;
         if (p > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) x[r] = cmul(x[r], tw[k * r * tstride]);
         }
         const int j = (i - k) * R + k;
         if constexpr (R == 4) {
-            const double2 s02 = cadd(x[0], x[2]), d02 = csub(x[0], x[2]);
-            const double2 s13 = cadd(x[1], x[3]), d13 = csub(x[1], x[3]);
+            const C s02 = cadd(x[0], x[2]), d02 = csub(x[0], x[2]);
+            const C s13 = cadd(x[1], x[3]), d13 = csub(x[1], x[3]);
             buf[j * NCOL + c] = cadd(s02, s13);
-            buf[(j + p) * NCOL + c] = make_double2(d02.x + d13.y, d02.y - d13.x);      // d02 - i d13
+            buf[(j + p) * NCOL + c] = C{d02.x + d13.y, d02.y - d13.x};      // d02 - i d13
             buf[(j + 2 * p) * NCOL + c] = csub(s02, s13);
-            buf[(j + 3 * p) * NCOL + c] = make_double2(d02.x - d13.y, d02.y + d13.x);  // d02 + i d13
+            buf[(j + 3 * p) * NCOL + c] = C{d02.x - d13.y, d02.y + d13.x};  // d02 + i d13
         } else {
             buf[j * NCOL + c] = cadd(x[0], x[1]);
             buf[(j + p) * NCOL + c] = csub(x[0], x[1]);
@@ -158,33 +169,38 @@ __device__ __forceinline__ void stockham_stage(double2* buf, const double2* tw, 
 }
 
 // the forward transforms of length L of the NCOL interleaved columns in buf
-template <int L, int NCOL, int NTHR>
-__device__ __forceinline__ void stockham(double2* buf, const double2* tw, int tid) {
+template <int L, int NCOL, int NTHR, typename C>
+__device__ __forceinline__ void stockham(C* buf, const C* tw, int tid) {
     int p = 1;
 #pragma unroll
     for (; p * 4 <= L; p *= 4) stockham_stage<4, L, NCOL, NTHR>(buf, tw, p, tid);
     if (p < L) stockham_stage<2, L, NCOL, NTHR>(buf, tw, p, tid);
 }
 
-// pass A: columns n2_0 .. n2_0 + TA - 1 (TA = BT_TILE_A / N1) of template m0 + blockIdx.y
-template <int N1>
+// pass A: columns n2_0 .. n2_0 + TA - 1 (TA = BT_TILE_A / N1) of template m0 + blockIdx.y, in the
+// arithmetic of C (double2; float2 for the float32 form, whose weighted bins are rounded to complex64 as
+// k_filter_pack32 rounds them)
+template <int N1, typename C>
 __global__ __launch_bounds__(BT_THREADS_A) void k_bt_pass_a(const float2* __restrict__ W, const double2* __restrict__ d,
                                                             const double* __restrict__ s2, int n_freq, int lo,
                                                             int64_t n_time, int m0, const double2* __restrict__ twN,
-                                                            double2* __restrict__ A) {
+                                                            C* __restrict__ A) {
     constexpr int TA = BT_TILE_A / N1;
-    __shared__ double2 buf[BT_TILE_A];                 // [n1][TA]
-    __shared__ double2 tw[N1];                         // w_N1^t = w_N^{512 t}
-    __shared__ double2 twl[BT_N2];                     // w_N^t, t < 512
+    __shared__ C buf[BT_TILE_A];                       // [n1][TA]
+    __shared__ C tw[N1];                               // w_N1^t = w_N^{512 t}
+    __shared__ C twl[BT_N2];                           // w_N^t, t < 512
     const int j = blockIdx.y, tid = threadIdx.x;
     const int n2_0 = blockIdx.x * TA;
-    for (int t = tid; t < N1; t += BT_THREADS_A) tw[t] = twN[(int64_t)t * BT_N2];
-    for (int t = tid; t < BT_N2; t += BT_THREADS_A) twl[t] = twN[t];
+    for (int t = tid; t < N1; t += BT_THREADS_A) {
+        const double2 w = twN[(int64_t)t * BT_N2];
+        tw[t] = cmake<C>(w.x, w.y);
+    }
+    for (int t = tid; t < BT_N2; t += BT_THREADS_A) twl[t] = cmake<C>(twN[t].x, twN[t].y);
     const float2* Wm = W + (int64_t)(m0 + j) * n_freq;
     for (int i = tid; i < BT_TILE_A; i += BT_THREADS_A) {
         const int c = i % TA, n1 = i / TA;
         const int f = n1 * BT_N2 + n2_0 + c - lo;
-        double2 v = make_double2(0.0, 0.0);
+        C v = cmake<C>(0.0, 0.0);
         if (f >= 0 && f < n_freq) {
             // w = conj(d) / sig2 (times 1/sig2, as numpy divides a complex by a real), times W
             const float2 wt = Wm[f];
@@ -192,14 +208,14 @@ __global__ __launch_bounds__(BT_THREADS_A) void k_bt_pass_a(const float2* __rest
             const double inv = 1.0 / s2[f];
             const double wr = dd.x * inv, wi = (-dd.y) * inv;
             const double ar = (double)wt.x, ai = (double)wt.y;
-            v = make_double2(ar * wr - ai * wi, ar * wi + ai * wr);
+            v = cmake<C>(ar * wr - ai * wi, ar * wi + ai * wr);
         }
         buf[i] = v;
     }
     __syncthreads();
     stockham<N1, TA, BT_THREADS_A>(buf, tw, tid);
     // times w_N^{n2 k1} = w_N^{512 u} w_N^{v} with n2 k1 = 512 u + v (n2 k1 < 512 N1)
-    double2* Aj = A + (int64_t)j * n_time;
+    C* Aj = A + (int64_t)j * n_time;
     for (int i = tid; i < BT_TILE_A; i += BT_THREADS_A) {
         const int c = i / N1, k1 = i % N1;
         const int n2 = n2_0 + c;
@@ -208,18 +224,20 @@ __global__ __launch_bounds__(BT_THREADS_A) void k_bt_pass_a(const float2* __rest
     }
 }
 
-// pass B: columns k1_0 .. k1_0 + 15 of template m0 + blockIdx.y, then ln|z| and arg z
-__global__ __launch_bounds__(BT_THREADS_B) void k_bt_pass_b(const double2* __restrict__ A, int n1len, int64_t n_time,
-                                                            int m0, const double2* __restrict__ twN,
-                                                            float* __restrict__ lq, float* __restrict__ ph) {
-    extern __shared__ double2 bsh[];
-    double2* buf = bsh;                                // [n2][16]
-    double2* tw = bsh + BT_COLS_B * BT_N2;             // w_512^t
+// pass B: columns k1_0 .. k1_0 + 15 of template m0 + blockIdx.y, then ln|z| and arg z (float64 from
+// the transform's values, as k_filter_post / k_filter_post32 form them)
+template <typename C>
+__global__ __launch_bounds__(BT_THREADS_B) void k_bt_pass_b(const C* __restrict__ A, int n1len, int64_t n_time, int m0,
+                                                            const double2* __restrict__ twN, float* __restrict__ lq,
+                                                            float* __restrict__ ph) {
+    extern __shared__ __align__(16) unsigned char bsh_raw[];
+    C* buf = (C*)bsh_raw;                              // [n2][16]
+    C* tw = buf + BT_COLS_B * BT_N2;                   // w_512^t
     const int j = blockIdx.y, tid = threadIdx.x;
     const int k1_0 = blockIdx.x * BT_COLS_B;
     const int64_t tstep = n_time / BT_N2;
-    for (int t = tid; t < BT_N2; t += BT_THREADS_B) tw[t] = twN[t * tstep];
-    const double2* Aj = A + (int64_t)j * n_time;
+    for (int t = tid; t < BT_N2; t += BT_THREADS_B) tw[t] = cmake<C>(twN[t * tstep].x, twN[t * tstep].y);
+    const C* Aj = A + (int64_t)j * n_time;
     for (int i = tid; i < BT_COLS_B * BT_N2; i += BT_THREADS_B) {
         const int c = i & (BT_COLS_B - 1), n2 = i / BT_COLS_B;
         buf[i] = Aj[(int64_t)n2 * n1len + k1_0 + c];
@@ -229,15 +247,17 @@ __global__ __launch_bounds__(BT_THREADS_B) void k_bt_pass_b(const double2* __res
     const int64_t row = (int64_t)(m0 + j) * n_time;
     for (int i = tid; i < BT_COLS_B * BT_N2; i += BT_THREADS_B) {
         const int c = i & (BT_COLS_B - 1), k2 = i / BT_COLS_B;
-        const double2 v = buf[i];
-        const double p = v.x * v.x + v.y * v.y;
+        const C v = buf[i];
+        const double x = (double)v.x, y = (double)v.y;
+        const double p = x * x + y * y;
         const int64_t o = row + k1_0 + c + (int64_t)n1len * k2;
         lq[o] = (float)(0.5 * ln_pos(p > 1e-38 ? p : 1e-38));
-        ph[o] = (float)atan2_f(v.y, v.x);
+        ph[o] = (float)atan2_f(y, x);
     }
 }
 
-constexpr size_t bt_lds_b() { return sizeof(double2) * (size_t)(BT_COLS_B * BT_N2 + BT_N2); }
+template <typename C>
+constexpr size_t bt_lds_b() { return sizeof(C) * (size_t)(BT_COLS_B * BT_N2 + BT_N2); }
 
 // whether the band transform takes transforms of length n_time (N1 = n_time / 512 a power of two in
 // [16, 2048])
@@ -313,32 +333,34 @@ static int fft_plan(int64_t n_time, int batch, bool f32, hipfftHandle* out) {
     return EXM_OK;
 }
 
-// the float64 band transform of every template for one segment's weighted data (d, s2 on the device)
+// the band transform of every template for one segment's weighted data (d, s2 on the device), in the
+// arithmetic of C
+template <typename C>
 static int filter_band(Bank* bank, const double2* const d[2], const double* const s2[2], Traj* traj) {
     Context& c = ctx();
     const int64_t n_time = bank->n_time;
     const int nf = bank->n_freq;
     const int n1len = (int)(n_time / BT_N2);
-    // templates per pass pair: A (16 n_time bytes per template) in the workspace, 64 MiB by default so
-    // that pass B reads it from the last-level cache
+    // templates per pass pair: A (sizeof(C) n_time bytes per template) in the workspace, 64 MiB by default
+    // so that pass B reads it from the last-level cache
     const int64_t ws = c.filter_bytes > 0 ? c.filter_bytes : (64ll << 20);
-    const int chunk = (int)std::min<int64_t>(bank->n_tmpl, std::max<int64_t>(1, ws / ((int64_t)sizeof(double2) * n_time)));
+    const int chunk = (int)std::min<int64_t>(bank->n_tmpl, std::max<int64_t>(1, ws / ((int64_t)sizeof(C) * n_time)));
     void* A = nullptr;
-    EXM_TRY(scratch(0, sizeof(double2) * (size_t)chunk * n_time, &A));
+    EXM_TRY(scratch(0, sizeof(C) * (size_t)chunk * n_time, &A));
     const double2* twN = nullptr;
     EXM_TRY(twiddles(n_time, &twN));
-    const size_t lds_b = bt_lds_b();
-    EXM_HIP(hipFuncSetAttribute((const void*)k_bt_pass_b, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
-    void (*pass_a)(const float2*, const double2*, const double*, int, int, int64_t, int, const double2*, double2*) = nullptr;
+    const size_t lds_b = bt_lds_b<C>();
+    EXM_HIP(hipFuncSetAttribute((const void*)k_bt_pass_b<C>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+    void (*pass_a)(const float2*, const double2*, const double*, int, int, int64_t, int, const double2*, C*) = nullptr;
     switch (n1len) {
-        case 16: pass_a = k_bt_pass_a<16>; break;
-        case 32: pass_a = k_bt_pass_a<32>; break;
-        case 64: pass_a = k_bt_pass_a<64>; break;
-        case 128: pass_a = k_bt_pass_a<128>; break;
-        case 256: pass_a = k_bt_pass_a<256>; break;
-        case 512: pass_a = k_bt_pass_a<512>; break;
-        case 1024: pass_a = k_bt_pass_a<1024>; break;
-        case 2048: pass_a = k_bt_pass_a<2048>; break;
+        case 16: pass_a = k_bt_pass_a<16, C>; break;
+        case 32: pass_a = k_bt_pass_a<32, C>; break;
+        case 64: pass_a = k_bt_pass_a<64, C>; break;
+        case 128: pass_a = k_bt_pass_a<128, C>; break;
+        case 256: pass_a = k_bt_pass_a<256, C>; break;
+        case 512: pass_a = k_bt_pass_a<512, C>; break;
+        case 1024: pass_a = k_bt_pass_a<1024, C>; break;
+        case 2048: pass_a = k_bt_pass_a<2048, C>; break;
         default: EXM_CHECK(false, EXM_ERR_ARG, "band transform of %d x 512 points", n1len);
     }
     for (int det = 0; det < 2; ++det) {
@@ -347,10 +369,10 @@ static int filter_band(Bank* bank, const double2* const d[2], const double* cons
         for (int m0 = 0; m0 < bank->n_tmpl; m0 += chunk) {
             const int nb = std::min(chunk, bank->n_tmpl - m0);
             hipLaunchKernelGGL(pass_a, dim3((unsigned)(BT_N2 * n1len / BT_TILE_A), (unsigned)nb), dim3(BT_THREADS_A), 0,
-                               c.stream, bank->d_W, d[det], s2[det], nf, bank->lo, n_time, m0, twN, (double2*)A);
+                               c.stream, bank->d_W, d[det], s2[det], nf, bank->lo, n_time, m0, twN, (C*)A);
             EXM_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_bt_pass_b, dim3((unsigned)(n1len / BT_COLS_B), (unsigned)nb), dim3(BT_THREADS_B),
-                               lds_b, c.stream, (const double2*)A, n1len, n_time, m0, twN, lq, ph);
+            hipLaunchKernelGGL(k_bt_pass_b<C>, dim3((unsigned)(n1len / BT_COLS_B), (unsigned)nb), dim3(BT_THREADS_B),
+                               lds_b, c.stream, (const C*)A, n1len, n_time, m0, twN, lq, ph);
             EXM_HIP(hipGetLastError());
         }
     }
@@ -375,10 +397,10 @@ int launch_filter(Bank* bank, const double* dH, const double* dL, const double* 
     EXM_HIP(hipMemcpyAsync(ds2H, s2H, sizeof(double) * nf, hipMemcpyHostToDevice, c.stream));
     EXM_HIP(hipMemcpyAsync(ds2L, s2L, sizeof(double) * nf, hipMemcpyHostToDevice, c.stream));
     ProfileScope prof("filter");
-    if (!f32 && c.filter_transform == 0 && band_transform_fits(n_time)) {
+    if (c.filter_transform == 0 && band_transform_fits(n_time)) {
         const double2* d[2] = {ddH, ddL};
         const double* s2[2] = {ds2H, ds2L};
-        return filter_band(bank, d, s2, traj);
+        return f32 ? filter_band<float2>(bank, d, s2, traj) : filter_band<double2>(bank, d, s2, traj);
     }
     // hipFFT: chunk of templates whose padded rows fit in the workspace (64 MiB by default: the
     // transform's input and output, 128 MiB together, stay in the 256 MB last-level cache; measured for

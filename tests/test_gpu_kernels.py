@@ -106,6 +106,37 @@ def test_band_transform_matches_fp64_oracle(hip, oracle, T, fmin, fmax):
         assert np.abs(dph).max() < 1e-6, k
 
 
+@pytest.mark.parametrize("T,fmin", [(1.0, 20.0), (16.0, 20.0), (2.0, 100.0)])
+def test_band_transform_float32(hip, oracle, T, fmin):
+    """the float32 band transform (float32 form of the filter) against the FP64 oracle with the float32
+    transform's bar (ln|z| and the phase within 2e-5 where |z| is above a hundredth of its maximum), and
+    against the float32 hipFFT path on the same bar"""
+    from exmarg_amd.search.bank import MassBank
+    seg = _segment(T=T, fmin=fmin, fmax=1024.0, seed=6)
+    bank_rows = synth.chirp_mass_grid_bank(30.0, 0.8, 0.1, 6)
+    bank = MassBank(bank_rows[:, 0], bank_rows[:, 1], bank_rows[:, 2], seg)
+    say("filter: float32 band transform, T=%g s, n_time %d" % (T, bank.n_time))
+    d = {I: seg["d_" + I][0] for I in "HL"}
+    s2 = {I: seg["sig2"] for I in "HL"}
+    hip.call("exm_set_filter_precision", 1)
+    try:
+        got = bank.trajectories(d, s2).to_host()
+        hip.call("exm_set_filter_transform", 1)
+        fft = bank.trajectories(d, s2).to_host()
+    finally:
+        hip.call("exm_set_filter_transform", 0)
+        hip.call("exm_set_filter_precision", 0)
+    ref = oracle.filter_fp64(bank.W, bank.lo, bank.n_time, d, s2)
+    for I in "HL":
+        lq, ph = ref["lq" + I], ref["ph" + I]
+        big = lq > lq.max(axis=1, keepdims=True) - np.log(1e2)
+        for g in (got, fft):
+            assert np.abs(g["lq" + I].astype(np.float64) - lq)[big].max() < 2e-5
+            dph = np.angle(np.exp(1j * (g["ph" + I].astype(np.float64) - ph)))
+            assert np.abs(dph[big]).max() < 2e-5
+        assert np.abs(got["lq" + I].astype(np.float64) - lq).max() > 0.0      # a different computation
+
+
 def test_filter_transform_choice_is_checked(hip):
     with pytest.raises(ValueError, match="filter transform"):
         hip.call("exm_set_filter_transform", 2)
