@@ -236,9 +236,10 @@ int exm_set_filter_workspace(int64_t bytes);
  * (src/exmarg/search/bank.py:66-75 keeps complex64); part of the precision form. */
 int exm_set_filter_precision(int p);
 
-/* The float64 matched filter's transform: 0 (default) the band transform, two passes that read only
- * the band's n_freq bins of each template and write ln|z| and arg z directly, wherever n_time / 512 is
- * a power of two in [16, 2048] (hipFFT otherwise); 1 hipFFT Z2Z always.  The same transform as
+/* The matched filter's transform: 0 (default) the band transform, two passes that read only the band's
+ * n_freq bins of each template and write ln|z| and arg z directly, in the arithmetic that
+ * exm_set_filter_precision chooses, wherever n_time / 512 is a power of two in [16, 2048] (hipFFT
+ * otherwise); 1 hipFFT (Z2Z, or C2C in float32) always.  The same transform as
  * src/exmarg/search/bank.py:70-75 (xp.fft.fft of the zero-padded band); no reference counterpart for
  * the choice. */
 int exm_set_filter_transform(int t);
