@@ -94,7 +94,7 @@ __global__ void k_filter_post(const double2* __restrict__ z, int64_t n_time, int
 // with w_L = e^{-2 pi i / L}.  Pass A reads only the band bins (the weighting of k_filter_pack fused
 // into its load) and writes A (16 n_time bytes per template); pass B reads A and writes ln|z| and
 // arg z (the arithmetic of k_filter_post).  Each pass transforms its columns in LDS by Stockham stages
-// (radix 4, a last radix-2 stage when the length is 2 * 4^s), several columns interleaved so that
+// (radix 8, a last radix-4 or radix-2 stage for the remaining factor), several columns interleaved so that
 // consecutive lanes touch consecutive addresses; every stage reads its values into registers, waits
 // for the workgroup, then writes its outputs in place.  Twiddles come from one table of w_N^t, t < N,
 // built on the host to ~1 ulp.
@@ -153,7 +153,27 @@ __device__ __forceinline__ void stockham_stage(C* buf, const C* tw, int p, int t
             for (int r = 1; r < R; ++r) x[r] = cmul(x[r], tw[k * r * tstride]);
         }
         const int j = (i - k) * R + k;
-        if constexpr (R == 4) {
+        if constexpr (R == 8) {
+            // a_k = x_k + x_{k+4}, b_k = (x_k - x_{k+4}) w_8^k; the even outputs are DFT4(a), the odd DFT4(b)
+            using T = decltype(x[0].x);
+            const T r2 = (T)0.70710678118654752440;
+            const C a0 = cadd(x[0], x[4]), a1 = cadd(x[1], x[5]), a2 = cadd(x[2], x[6]), a3 = cadd(x[3], x[7]);
+            const C e1 = csub(x[1], x[5]), e2 = csub(x[2], x[6]), e3 = csub(x[3], x[7]);
+            const C b0 = csub(x[0], x[4]);
+            const C b1 = C{(e1.x + e1.y) * r2, (e1.y - e1.x) * r2};     // times (1 - i) / sqrt 2
+            const C b2 = C{e2.y, -e2.x};                               // times -i
+            const C b3 = C{(e3.y - e3.x) * r2, -(e3.x + e3.y) * r2};   // times (-1 - i) / sqrt 2
+            const C sa02 = cadd(a0, a2), da02 = csub(a0, a2), sa13 = cadd(a1, a3), da13 = csub(a1, a3);
+            const C sb02 = cadd(b0, b2), db02 = csub(b0, b2), sb13 = cadd(b1, b3), db13 = csub(b1, b3);
+            buf[j * NCOL + c] = cadd(sa02, sa13);
+            buf[(j + p) * NCOL + c] = cadd(sb02, sb13);
+            buf[(j + 2 * p) * NCOL + c] = C{da02.x + da13.y, da02.y - da13.x};
+            buf[(j + 3 * p) * NCOL + c] = C{db02.x + db13.y, db02.y - db13.x};
+            buf[(j + 4 * p) * NCOL + c] = csub(sa02, sa13);
+            buf[(j + 5 * p) * NCOL + c] = csub(sb02, sb13);
+            buf[(j + 6 * p) * NCOL + c] = C{da02.x - da13.y, da02.y + da13.x};
+            buf[(j + 7 * p) * NCOL + c] = C{db02.x - db13.y, db02.y + db13.x};
+        } else if constexpr (R == 4) {
             const C s02 = cadd(x[0], x[2]), d02 = csub(x[0], x[2]);
             const C s13 = cadd(x[1], x[3]), d13 = csub(x[1], x[3]);
             buf[j * NCOL + c] = cadd(s02, s13);
@@ -173,8 +193,9 @@ template <int L, int NCOL, int NTHR, typename C>
 __device__ __forceinline__ void stockham(C* buf, const C* tw, int tid) {
     int p = 1;
 #pragma unroll
-    for (; p * 4 <= L; p *= 4) stockham_stage<4, L, NCOL, NTHR>(buf, tw, p, tid);
-    if (p < L) stockham_stage<2, L, NCOL, NTHR>(buf, tw, p, tid);
+    for (; p * 8 <= L; p *= 8) stockham_stage<8, L, NCOL, NTHR>(buf, tw, p, tid);
+    if (p * 4 == L) stockham_stage<4, L, NCOL, NTHR>(buf, tw, p, tid);
+    else if (p * 2 == L) stockham_stage<2, L, NCOL, NTHR>(buf, tw, p, tid);
 }
 
 // pass A: columns n2_0 .. n2_0 + TA - 1 (TA = BT_TILE_A / N1) of template m0 + blockIdx.y, in the
