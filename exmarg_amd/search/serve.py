@@ -13,6 +13,10 @@ Admission is by HBM budget: a slot holds at most `slot_templates` templates of w
 (template_bytes each); a request larger than a slot is refused.  The reference has no serving
 loop; it analyses one segment group at a time (src/exmarg/search/pe.py:289-346, pp.py:96-97).
 
+Banks are cached: requests with the same bank rows, segment length, band and noise curve share one
+device bank (and the node geometry and prior weights cached on it), least recently used banks beyond
+`bank_cache_bytes` of template factors being freed once no admitted request uses them.
+
     srv = Server(store)
     r = srv.submit(segment, bank_rows)          # bank rows (M_det, q, chi)
     while not r.done: srv.step()
@@ -20,6 +24,7 @@ loop; it analyses one segment group at a time (src/exmarg/search/pe.py:289-346, 
 """
 
 import collections
+import hashlib
 import time
 
 import numpy as np
@@ -61,12 +66,14 @@ class Server:
     max_request     (n_tmpl, n_time, n_freq) of the largest request expected: the work memory for
                     admitting it is reserved up front so that no admission allocates while a gather
                     runs
-    prior, ymin     distance prior of the node weights (pe_intrinsic)"""
+    prior, ymin     distance prior of the node weights (pe_intrinsic)
+    bank_cache_bytes  device bytes of template factors kept for banks no request uses (0: a bank is
+                    freed as soon as its last request finishes)"""
 
     SLOT_TEMPLATES = 32768          # default cap of a slot: larger batches only lengthen the wait of others
 
     def __init__(self, store, n_loud=1024, n_random=1024, slot_templates=None, reserve_bytes=16 << 30,
-                 max_request=None, prior="volume", ymin=0.1267, clock=time.perf_counter):
+                 max_request=None, prior="volume", ymin=0.1267, clock=time.perf_counter, bank_cache_bytes=8 << 30):
         self.store, self.prior, self.ymin, self.clock = store, prior, float(ymin), clock
         _store.upload(store)
         self.K, self.S = int(n_loud), int(n_random)
@@ -85,8 +92,11 @@ class Server:
         self.members = {0: [], 1: []}                   # admitted requests per slot
         self.queue = collections.deque()
         self.release = []                               # device objects freed once nothing is in flight
+        self.banks = collections.OrderedDict()          # bank key -> [MassBank, requests using it]
+        self.bank_cache_bytes = int(bank_cache_bytes)
         self.next_id = 0
-        self.stats = dict(batches=0, admitted=0, admitted_during_gather=0, finished=0, refused=0)
+        self.stats = dict(batches=0, admitted=0, admitted_during_gather=0, finished=0, refused=0, bank_builds=0,
+                          bank_hits=0)
 
     # ------------------------------------------------------------ client side
     def submit(self, segments, bank_rows, seed=0, t_submit=None):
@@ -131,6 +141,9 @@ class Server:
             self._finish(self.flying)
         for s in self.slots:
             s.free()
+        for bank, _ in self.banks.values():
+            self._free_bank(bank)
+        self.banks.clear()
         self._release()
 
     # ------------------------------------------------------------ internals
@@ -146,13 +159,13 @@ class Server:
         if batch.used + n > batch.cap:
             return False
         seg = req.seg
-        bank = mass_bank(req.rows, seg, "hip")
+        bank = self._bank(req.rows, seg)
         an = _pe._Analysis(bank, self.store, self.ymin, True)
         try:
             lnW = _pe.bank_weights(an, self.prior)
             an.start()
         except ValueError as e:
-            self.release.append(bank.handle)
+            self._unuse(bank)
             req.bank = None
             req.error = e
             req.t_done = self.clock()
@@ -180,14 +193,55 @@ class Server:
             req.ln_lambda = L
             req.ln_b = _pe._log_mean(L, _pe._cell_weights(req.bank))
             req.t_done = t
-            self.release.append(req.bank.handle)
-            self.release.append(req.an.plan.handle)
+            self._unuse(req.bank)
             req.an = None
             self.stats["finished"] += 1
         self.members[k] = []
         batch.reset()
         self.flying = None
         self._release()
+
+    @staticmethod
+    def _bank_key(rows, seg):
+        """what a bank depends on: its rows, the segment length, band and the noise curve of its norm"""
+        h = hashlib.sha1(np.ascontiguousarray(rows, np.float64).tobytes())
+        h.update(np.ascontiguousarray(seg["sig2_H"], np.float64).tobytes())
+        nu = np.asarray(seg["nu"])
+        return (h.hexdigest(), float(seg["T"]), float(seg["fmin"]), float(seg["fmax"]), int(nu[0]), int(nu.size))
+
+    def _bank(self, rows, seg):
+        """the cached bank of these rows and segment geometry, built on the device when absent"""
+        key = self._bank_key(rows, seg)
+        ent = self.banks.get(key)
+        if ent is None:
+            ent = [mass_bank(rows, seg, "hip"), 0]
+            ent[0]._cache_key = key
+            self.banks[key] = ent
+            self.stats["bank_builds"] += 1
+        else:
+            self.banks.move_to_end(key)
+            self.stats["bank_hits"] += 1
+        ent[1] += 1
+        return ent[0]
+
+    def _unuse(self, bank):
+        """one request less uses the bank; banks no request uses are freed, least recently used first,
+        while the unused ones hold more than bank_cache_bytes"""
+        self.banks[bank._cache_key][1] -= 1
+        idle = [(k, b) for k, (b, n) in self.banks.items() if n == 0]
+        held = sum(b.M.size * b.nu.size * 8 for _, b in idle)
+        for k, b in idle:
+            if held <= self.bank_cache_bytes:
+                break
+            held -= b.M.size * b.nu.size * 8
+            self._free_bank(b)
+            del self.banks[k]
+
+    def _free_bank(self, bank):
+        """free a bank and the read plans cached with it (pe._Analysis keeps one per store and ymin)"""
+        for plan, _ in bank.__dict__.get("_plans", {}).values():
+            self.release.append(plan.handle)
+        self.release.append(bank.handle)
 
     def _release(self):
         """free banks and read plans of finished requests (a free may wait for the device, so it is

@@ -92,3 +92,34 @@ def test_request_larger_than_a_slot_is_refused(store):
     srv.close()
     assert isinstance(big.error, MemoryError)
     assert small.error is None and np.isfinite(small.ln_b)
+
+
+def test_repeated_banks_are_cached(store):
+    """requests that repeat a bank (same rows, segment length, band and noise curve) share one device
+    bank: built once, results as with a bank of their own; with no cache budget a bank is shared only
+    while a request still uses it, and none is held once every request has ended"""
+    from exmarg_amd.search import Server
+    reqs = requests()
+    seg, rows, _ = reqs[0]
+    work = [(seg, rows, 5), reqs[1], (seg, rows, 6), (seg, rows, 5)]
+    say("serve: a bank repeated three times among four requests")
+    for budget, builds in ((8 << 30, 2), (0, None)):
+        srv = Server(store, n_loud=128, n_random=128, slot_templates=30, max_request=(24, 32768, 1000),
+                     bank_cache_bytes=budget)
+        handles = [srv.submit(s, r, seed=sd) for s, r, sd in work]
+        srv.run()
+        stats = dict(srv.stats)
+        held = len(srv.banks)
+        srv.close()
+        say("budget", budget, "stats", stats, "banks held", held)
+        assert stats["bank_builds"] + stats["bank_hits"] == 4
+        if budget:
+            assert stats["bank_builds"] == builds and held == 2
+        else:
+            assert stats["bank_builds"] >= 2 and held == 0
+        for h in handles:
+            assert h.error is None and np.isfinite(h.ln_b)
+        alone = serve(store, [(seg, rows, 5)], 30)[0][0]
+        for h in (handles[0], handles[3]):                  # same inputs as the request served alone
+            np.testing.assert_allclose(h.ln_lambda, alone.ln_lambda, rtol=0, atol=MIX_TOL)
+        assert abs(handles[2].ln_b - handles[0].ln_b) < 1.0       # another draw, the same bank
