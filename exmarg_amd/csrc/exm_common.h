@@ -183,6 +183,8 @@ struct Context {
     int filter_f32 = 0;                                     // matched-filter transform in float32
     int filter_transform = 0;                               // 0: band transform where it fits, 1: hipFFT
     std::unordered_map<int64_t, void*> bt_twiddles;         // device tables e^{-2 pi i t / n}, by n
+    hipStream_t filter_side = nullptr;                      // second stream of the band transform's batches
+    hipEvent_t filter_ev[2] = {nullptr, nullptr};           // fork and join of the two streams
     // geometry of the zero-padded transform input (scratch slot 9) whose padding is already zero
     void* filter_in = nullptr;
     int64_t filter_in_nt = 0, filter_in_rows = 0;

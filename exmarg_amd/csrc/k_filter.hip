@@ -363,11 +363,21 @@ static int filter_band(Bank* bank, const double2* const d[2], const double* cons
     const int nf = bank->n_freq;
     const int n1len = (int)(n_time / BT_N2);
     // templates per pass pair: A (sizeof(C) n_time bytes per template) in the workspace, 64 MiB by default
-    // so that pass B reads it from the last-level cache
+    // so that pass B reads it from the last-level cache.  Consecutive batches alternate between the
+    // library's stream and a second one, each with its own A, so that one batch's pass A overlaps the
+    // other's pass B (each pass alone leaves the CUs idle during its load and barrier phases).
     const int64_t ws = c.filter_bytes > 0 ? c.filter_bytes : (64ll << 20);
     const int chunk = (int)std::min<int64_t>(bank->n_tmpl, std::max<int64_t>(1, ws / ((int64_t)sizeof(C) * n_time)));
-    void* A = nullptr;
-    EXM_TRY(scratch(0, sizeof(C) * (size_t)chunk * n_time, &A));
+    const size_t abytes = (sizeof(C) * (size_t)chunk * n_time + 255) & ~(size_t)255;
+    void* Abuf = nullptr;
+    EXM_TRY(scratch(0, 2 * abytes, &Abuf));
+    if (!c.filter_side) EXM_HIP(hipStreamCreateWithFlags(&c.filter_side, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k)
+        if (!c.filter_ev[k]) EXM_HIP(hipEventCreateWithFlags(&c.filter_ev[k], hipEventDisableTiming));
+    hipStream_t streams[2] = {c.stream, c.filter_side};
+    C* As[2] = {(C*)Abuf, (C*)((char*)Abuf + abytes)};
+    EXM_HIP(hipEventRecord(c.filter_ev[0], c.stream));          // the side stream starts after the uploads
+    EXM_HIP(hipStreamWaitEvent(c.filter_side, c.filter_ev[0], 0));
     const double2* twN = nullptr;
     EXM_TRY(twiddles(n_time, &twN));
     const size_t lds_b = bt_lds_b<C>();
@@ -384,19 +394,24 @@ static int filter_band(Bank* bank, const double2* const d[2], const double* cons
         case 2048: pass_a = k_bt_pass_a<2048, C>; break;
         default: EXM_CHECK(false, EXM_ERR_ARG, "band transform of %d x 512 points", n1len);
     }
+    int q = 0;                                                  // batch count: stream and buffer q % 2
     for (int det = 0; det < 2; ++det) {
         float* lq = det == 0 ? traj->lqH() : traj->lqL();
         float* ph = det == 0 ? traj->phH() : traj->phL();
-        for (int m0 = 0; m0 < bank->n_tmpl; m0 += chunk) {
+        for (int m0 = 0; m0 < bank->n_tmpl; m0 += chunk, ++q) {
             const int nb = std::min(chunk, bank->n_tmpl - m0);
+            hipStream_t s = streams[q & 1];
+            C* A = As[q & 1];
             hipLaunchKernelGGL(pass_a, dim3((unsigned)(BT_N2 * n1len / BT_TILE_A), (unsigned)nb), dim3(BT_THREADS_A), 0,
-                               c.stream, bank->d_W, d[det], s2[det], nf, bank->lo, n_time, m0, twN, (C*)A);
+                               s, bank->d_W, d[det], s2[det], nf, bank->lo, n_time, m0, twN, A);
             EXM_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_bt_pass_b<C>, dim3((unsigned)(n1len / BT_COLS_B), (unsigned)nb), dim3(BT_THREADS_B),
-                               lds_b, c.stream, (const C*)A, n1len, n_time, m0, twN, lq, ph);
+                               lds_b, s, (const C*)A, n1len, n_time, m0, twN, lq, ph);
             EXM_HIP(hipGetLastError());
         }
     }
+    EXM_HIP(hipEventRecord(c.filter_ev[1], c.filter_side));      // later work on the library's stream
+    EXM_HIP(hipStreamWaitEvent(c.stream, c.filter_ev[1], 0));    // sees every batch
     return EXM_OK;
 }
 
