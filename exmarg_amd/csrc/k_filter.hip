@@ -85,19 +85,20 @@ __global__ void k_filter_post(const double2* __restrict__ z, int64_t n_time, int
     ph[o] = (float)atan2(v.y, v.x);
 }
 
-// ---------------------------------------------------------------- the band transform (float64)
+// ---------------------------------------------------------------- the band transform
 // The transform's input is zero outside the band [lo, lo + n_freq) (n_freq / n_time = 12 % at the
 // headline geometry), so the default path does not write the zero-padded input nor read it:
 // with n_time = N1 * N2, N2 = 512, n = N2 n1 + n2 and k = k1 + N1 k2,
 //   A[n2][k1]     = w_N^{n2 k1} sum_{n1} x[N2 n1 + n2] w_N1^{n1 k1}     (pass A, one column per n2)
 //   z[k1 + N1 k2] = sum_{n2} A[n2][k1] w_N2^{n2 k2}                     (pass B, one column per k1)
 // with w_L = e^{-2 pi i / L}.  Pass A reads only the band bins (the weighting of k_filter_pack fused
-// into its load) and writes A (16 n_time bytes per template); pass B reads A and writes ln|z| and
-// arg z (the arithmetic of k_filter_post).  Each pass transforms its columns in LDS by Stockham stages
-// (radix 8, a last radix-4 or radix-2 stage for the remaining factor), several columns interleaved so that
-// consecutive lanes touch consecutive addresses; every stage reads its values into registers, waits
-// for the workgroup, then writes its outputs in place.  Twiddles come from one table of w_N^t, t < N,
-// built on the host to ~1 ulp.
+// into its load) and writes A (one complex value per sample of every template); pass B reads A and
+// writes ln|z| and arg z (the arithmetic of k_filter_post).  The transform runs in float64 (double2),
+// or in float32 (float2) for the float32 form.  Each pass transforms its columns in LDS by Stockham
+// stages (radix 8, a last radix-4 or radix-2 stage for the remaining factor), several columns
+// interleaved so that consecutive lanes touch consecutive addresses; every stage reads its values into
+// registers, waits for the workgroup, then writes its outputs in place.  Twiddles come from one table
+// of w_N^t, t < N, built on the host to ~1 ulp.
 constexpr int BT_N2 = 512;
 constexpr int BT_TILE_A = 2048;                       // complex values per pass-A workgroup
 constexpr int BT_COLS_B = 16;                         // k1 columns per pass-B workgroup
