@@ -25,6 +25,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# vector-ALU issue peak of one MI355X: 256 CUs x 4 SIMDs x 16 lanes (a wave64 instruction every 4 cycles)
+# at the 2.4 GHz peak engine clock, in lane-instructions per second
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 TABLE_BYTES = 746_877_600        # one (87, 49, 200, 219) float32 table
 
 
@@ -655,6 +658,15 @@ def main():
         out["roofline"]["traffic"] = trf["traffic_bytes"]
         out["roofline"]["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this kernel on this "
                                              "workload, scaled by a 1 GiB copy (profiles/gather_traffic.json)")
+        if "valu_lane_instr_per_lookup" in trf:
+            # the gather's actual bound: vector-ALU issue (the measured lane-instructions per lookup at the
+            # timed lookup rate, against the issue peak)
+            per = trf["valu_lane_instr_per_lookup"]
+            out["roofline"]["valu"] = {"lane_instr_per_lookup": per, "achieved": per * lookups / g_s,
+                                       "peak": VALU_PEAK_LANE_OPS, "unit": "lane-instructions/s",
+                                       "frac": per * lookups / g_s / VALU_PEAK_LANE_OPS,
+                                       "source": "SQ_INSTS_VALU x 64 / lookups (profiles/counters_gather_wave_"
+                                                 "16s_2000tmpl_53nodes.txt)"}
     if f64_check is not None:
         out["float32_form_vs_float64"] = f64_check
     if not args.no_cpu:
