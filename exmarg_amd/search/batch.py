@@ -117,15 +117,27 @@ class SegmentBatch:
         self.handle.free()
 
 
+def release_kept(store):
+    """free the batch buffer pe_intrinsic_batch(..., keep_buffer=True) keeps with the store"""
+    kept = store.__dict__.pop("_batch", None)
+    if kept is not None:
+        kept[1].free()
+
+
 def pe_intrinsic_batch(items, store, prior="volume", ymin=0.1267, n_loud=1024, n_random=1024, seed=0,
-                       max_templates=None, max_bytes=None):
+                       max_templates=None, max_bytes=None, keep_buffer=False):
     """pe_intrinsic over many segment sets in batches that share one gather launch per batch.
 
     items  list of (segments, bank): anything read_segments takes and a mass_bank built on those
            segments (device 'hip'); banks may differ in length, band and templates
     other arguments as pe_intrinsic (src/exmarg/search/pe.py:289-346).
 
-    returns a list with one pe_intrinsic result dict per item."""
+    returns a list with one pe_intrinsic result dict per item.
+
+    keep_buffer  keep the batch's device buffer (sized from the free HBM unless max_templates / max_bytes
+           bound it) with the store for the next call with the same arguments, for callers that repeat
+           batches: allocating it anew after freeing it costs seconds.  The buffer holds that memory
+           until release_kept(store) or exmarg_amd.search.store.free(store)."""
     from . import store as _store
     _store.upload(store)
     prepared = []
@@ -138,7 +150,17 @@ def pe_intrinsic_batch(items, store, prior="volume", ymin=0.1267, n_loud=1024, n
     # work memory for admitting the largest segment first, so that the batch is sized on what is left
     big = max(prepared, key=lambda p: p[1].u.size * p[1].n_time)[1]
     _hip.call("exm_reserve", big.u.size, big.n_time, big.nu.size, int(n_loud) + int(n_random))
-    batch = SegmentBatch(store, n_loud, n_random, max_templates=max_templates, max_bytes=max_bytes)
+    key = (int(n_loud), int(n_random), max_templates, max_bytes)
+    kept = store.__dict__.get("_batch")
+    if keep_buffer and kept is not None and kept[0] == key and kept[1].dev is store.device:
+        batch = kept[1]
+        batch.reset()
+    else:
+        if keep_buffer:
+            release_kept(store)
+        batch = SegmentBatch(store, n_loud, n_random, max_templates=max_templates, max_bytes=max_bytes)
+        if keep_buffer:
+            store.__dict__["_batch"] = (key, batch)
     outs = [np.full((seg["rows"].size, bank.u.size), -np.inf) for seg, bank, _, _ in prepared]
     pending = []
 
@@ -164,7 +186,10 @@ def pe_intrinsic_batch(items, store, prior="volume", ymin=0.1267, n_loud=1024, n
         if pending:
             flush()
     finally:
-        batch.free()
+        if keep_buffer:
+            batch.reset()
+        else:
+            batch.free()
     results = []
     for (seg, bank, an, lnW), out in zip(prepared, outs):
         w = _pe._cell_weights(bank)

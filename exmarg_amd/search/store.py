@@ -200,8 +200,14 @@ def upload(store):
 
 
 def free(store):
-    """release the device copy of a store (src/exmarg/search/store.py:123-127)"""
-    if store is not None and store.device is not None:
+    """release the device copy of a store (src/exmarg/search/store.py:123-127), and the batch buffer
+    pe_intrinsic_batch keeps with it"""
+    if store is None:
+        return
+    kept = store.__dict__.pop("_batch", None)
+    if kept is not None:
+        kept[1].free()
+    if store.device is not None:
         store.device.free()
         store.device = None
 

@@ -129,4 +129,18 @@ def test_batch_of_uneven_segments_matches_single_calls(hip, oracle):
         ref = oracle.pe_intrinsic(read_segments(seg), b, store, tabs, n_loud=128, n_random=128, seed=4)
         np.testing.assert_allclose(r["ln_lambda"], ref["ln_lambda"], rtol=0, atol=LNL_TOL)
         np.testing.assert_allclose(r["ln_b"], ref["ln_b"], rtol=0, atol=LNB_TOL)
-    stm.free(st)
+    # the batch buffer kept with the store between calls: the same results, one buffer reused
+    from exmarg_amd.search.batch import release_kept
+    kw = dict(n_loud=128, n_random=128, seed=4, max_templates=3 * 8, keep_buffer=True)
+    k1 = pe_intrinsic_batch(items, st, **kw)
+    buf = st.__dict__["_batch"][1]
+    k2 = pe_intrinsic_batch(items, st, **kw)
+    assert st.__dict__["_batch"][1] is buf
+    for r, a, b in zip(res, k1, k2):
+        np.testing.assert_array_equal(a["ln_lambda"], r["ln_lambda"])
+        np.testing.assert_array_equal(b["ln_lambda"], r["ln_lambda"])
+    release_kept(st)
+    assert "_batch" not in st.__dict__
+    pe_intrinsic_batch(items[:1], st, **kw)
+    stm.free(st)                                          # frees the kept buffer with the store
+    assert "_batch" not in st.__dict__
