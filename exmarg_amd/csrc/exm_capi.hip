@@ -149,8 +149,23 @@ Batch::~Batch() {
 int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const double* offb, JobSet* js) {
     js->njobs = njobs;
     js->order.resize(njobs);
-    for (int64_t i = 0; i < njobs; ++i) js->order[i] = i;
-    std::stable_sort(js->order.begin(), js->order.end(), [&](int64_t a, int64_t b) { return tab[a] < tab[b]; });
+    // jobs grouped by table, keeping their order within a table: a counting sort over the table ids
+    // (a few hundred tables against millions of jobs), a stable comparison sort when the ids spread wide
+    int32_t tlo = 0, thi = -1;
+    if (njobs > 0) tlo = thi = tab[0];
+    for (int64_t i = 1; i < njobs; ++i) {
+        tlo = std::min(tlo, tab[i]);
+        thi = std::max(thi, tab[i]);
+    }
+    if (njobs > 0 && (int64_t)thi - tlo < (int64_t)1 << 20) {
+        std::vector<int64_t> start((size_t)(thi - tlo) + 2, 0);
+        for (int64_t i = 0; i < njobs; ++i) ++start[(size_t)(tab[i] - tlo) + 1];
+        for (size_t k = 1; k < start.size(); ++k) start[k] += start[k - 1];
+        for (int64_t i = 0; i < njobs; ++i) js->order[start[(size_t)(tab[i] - tlo)]++] = i;
+    } else {
+        for (int64_t i = 0; i < njobs; ++i) js->order[i] = i;
+        std::stable_sort(js->order.begin(), js->order.end(), [&](int64_t a, int64_t b) { return tab[a] < tab[b]; });
+    }
     std::vector<int32_t> sm(njobs), st(njobs), tiles;
     std::vector<double> so(njobs);
     for (int64_t i = 0; i < njobs; ++i) {
