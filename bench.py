@@ -230,6 +230,10 @@ def main_serve(args):
     from exmarg_amd.search import Server, store as stm
     _hip.init(0)
     info = _hip.device_info()
+    # the float32 form on request (--precision f32: float32 transform and interpolation)
+    prec = 1 if args.precision == "f32" else 0
+    _hip.call("exm_set_gather_precision", prec)
+    _hip.call("exm_set_filter_precision", prec)
     st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
     reqs = serve_requests(args.requests, args.seed, args.rate)
     srv = Server(st, n_loud=args.n_loud, n_random=args.n_random, max_request=(8000, 8 * 65536, 64257))
@@ -272,7 +276,8 @@ def main_serve(args):
         "admitted_during_gather": srv.stats["admitted_during_gather"],
         "templates_mean": float(n_tmpl.mean()), "gather_ms_total": g_ms, "filter_ms_total": f_ms,
         "gpu_gather_busy_frac": g_ms / 1e3 / wall,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": ("f32 transform and interpolation, f64 weighting, index arithmetic and log-sum-exp" if prec
+                  else "f64"), "data": "synthetic",
         "config": {"workload": "Poisson arrivals; T in {4,8,16,32,64} s at 20-1024 Hz; banks of 500-8000 templates "
                                "(log-uniform); 30%% injections SNR 6-40; 53 HBM-resident table nodes; "
                                "pe_intrinsic 'volume', n_loud=n_random=%d" % args.n_loud,
@@ -315,6 +320,9 @@ def main_stream(args):
     from exmarg_amd.search import pe_redshift_stream, stream, store as stm
     _hip.init(0)
     info = _hip.device_info()
+    prec = 1 if args.precision == "f32" else 0            # the float32 form on request
+    _hip.call("exm_set_gather_precision", prec)
+    _hip.call("exm_set_filter_precision", prec)
     st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
     masses = np.geomspace(10.0, 500.0, 3517)
     groups, truths = stream_groups(args.segments + args.group, args.group, args.seed)
@@ -347,7 +355,8 @@ def main_stream(args):
         "gather_ms_per_segment": g_ms / nseg, "source_frame_ms_per_segment": sf_ms / max(sf_n, 1),
         "ln_b": {"median": float(np.median(ln_b)), "max": float(ln_b.max())},
         "injections": [{"m1_level": a, "z_level": b, "snr": c} for a, b, c in levels],
-        "dtype": "f64", "data": "synthetic",
+        "dtype": ("f32 transform and interpolation, f64 weighting, index arithmetic and log-sum-exp" if prec
+                  else "f64"), "data": "synthetic",
         "config": {"workload": "8 s segments at 20-512 Hz, 5%% population injections (equal mass, zero spin), "
                                "noise variance = mean of 32 periodograms x 32/30 per group of %d; 3517-mass bank "
                                "10-500 Msun (geometric); 53 HBM-resident table nodes; pe_redshift on a 1600 x 1600 "
