@@ -45,6 +45,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle check")
     p.add_argument("--gather-kernel", type=int, default=0, help="0 automatic, 1 reference-order kernel, 2 band kernel")
     p.add_argument("--piece", type=int, default=0, help="walked times per piece of the band kernel (0 default)")
+    p.add_argument("--tile", type=int, default=0, help="reads per band-kernel workgroup (0 default)")
     p.add_argument("--precision", choices=["f64", "f32"], default="f64",
                    help="table interpolation of the band kernel (index arithmetic and log-sum-exp stay float64)")
     p.add_argument("--filter-mib", type=int, default=0, help="matched-filter workspace per transform (MiB, 0 default)")
@@ -597,6 +598,7 @@ def main():
     _hip.init(0)
     _hip.call("exm_set_gather_kernel", args.gather_kernel)
     _hip.call("exm_set_gather_piece", args.piece)
+    _hip.call("exm_set_gather_tile", args.tile)
     _hip.call("exm_set_gather_precision", 1 if args.precision == "f32" else 0)
     _hip.call("exm_set_filter_workspace", args.filter_mib << 20)
     _hip.call("exm_set_filter_precision", 1 if args.filter_precision == "f32" else 0)

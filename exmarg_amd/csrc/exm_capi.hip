@@ -173,19 +173,51 @@ int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const doub
         st[i] = tab[js->order[i]];
         so[i] = offb[js->order[i]];
     }
-    // tiles: runs of one table split evenly into pieces of at most TILE_JOBS jobs
+    // tiles: runs of one table split evenly into pieces of at most TILE_JOBS jobs.  Piece q of every
+    // table holds about the same templates, so the tiles of one piece read the same walk records:
+    // they are queued for one XCD (workgroups go to the 8 XCDs round-robin by workgroup index) and
+    // the queues interleaved, so that the workgroups of an XCD share their records in its L2.
+    constexpr int NXCD = 8;
+    std::vector<int64_t> run_a, run_n, run_p;
+    int64_t maxp = 0;
     for (int64_t a = 0; a < njobs;) {
         int64_t b = a;
         while (b < njobs && st[b] == st[a]) ++b;
         const int64_t n = b - a;
         const int64_t tile = g_ctx.tile ? g_ctx.tile : TILE_JOBS;
         const int64_t pieces = (n + tile - 1) / tile;
-        for (int64_t q = 0; q < pieces; ++q) {
-            const int64_t lo = a + n * q / pieces, hi = a + n * (q + 1) / pieces;
-            tiles.push_back((int32_t)lo);
-            tiles.push_back((int32_t)(hi - lo));
-        }
+        run_a.push_back(a);
+        run_n.push_back(n);
+        run_p.push_back(pieces);
+        maxp = std::max(maxp, pieces);
         a = b;
+    }
+    std::vector<std::vector<int32_t>> queue(NXCD);
+    for (int64_t q = 0; q < maxp; ++q)
+        for (size_t u = 0; u < run_a.size(); ++u) {
+            const int64_t pieces = run_p[u], n = run_n[u], a = run_a[u];
+            if (q >= pieces) continue;
+            int x;
+            if (maxp >= NXCD) {
+                x = (int)(q % NXCD);
+            } else {                                 // XCDs x = q, q + maxp, ... share piece q
+                const int nx = (int)((NXCD - 1 - q) / maxp) + 1;
+                x = (int)q + (int)maxp * (int)(u % nx);
+            }
+            const int64_t lo = a + n * q / pieces, hi = a + n * (q + 1) / pieces;
+            queue[x].push_back((int32_t)lo);
+            queue[x].push_back((int32_t)(hi - lo));
+        }
+    std::vector<size_t> head(NXCD, 0);
+    for (bool more = true; more;) {
+        more = false;
+        for (int x = 0; x < NXCD; ++x)
+            if (head[x] < queue[x].size()) {
+                tiles.push_back(queue[x][head[x]]);
+                tiles.push_back(queue[x][head[x] + 1]);
+                head[x] += 2;
+                more = true;
+            }
     }
     js->ntiles = (int)(tiles.size() / 2);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
