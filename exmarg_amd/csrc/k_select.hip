@@ -148,8 +148,15 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict_
     int kw = 0;
     while ((2 << kw) <= W) ++kw;
     const int off2 = W - (1 << kw);
-    const int64_t stride = (nt + NSAMP - 1) / NSAMP;             // one sample every stride times
-    const int nsamp = (int)((nt + stride - 1) / stride);          // <= NSAMP
+    // one sample every stride = 2^ssh times, the least power of two with NSAMP stride >= nt (a shift and
+    // a mask per key instead of a 64-bit division)
+    int ssh = 0;
+    while (((int64_t)NSAMP << ssh) < nt) ++ssh;
+    const int64_t stride = (int64_t)1 << ssh;
+    const int nsamp = (int)((nt + stride - 1) >> ssh);            // <= NSAMP
+    // lqH is read cyclically at t + tau: with |lags| < nt one conditional add or subtract wraps every
+    // index the tiles read (t0 + lag_lo + i lies in [lag_lo, nt + lag_hi)); a 64-bit remainder otherwise
+    const bool wrap_once = lag_lo > -nt && lag_hi < nt;
 
     // ---------------- A: keys and samples.  The lqH tile (with its lag halo) of the next tile is
     // requested into registers while this one is processed, and this tile's lqL before its doubling
@@ -161,8 +168,13 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict_
 #pragma unroll
         for (int k = 0; k < QH_PT; ++k) {
             const int i = tid + k * SEL_THREADS;
-            int64_t src = (t0 + lag_lo + i) % nt;
-            if (src < 0) src += nt;
+            int64_t src = t0 + lag_lo + i;
+            if (wrap_once) {
+                src = src < 0 ? src + nt : (src >= nt ? src - nt : src);
+            } else {
+                src %= nt;
+                if (src < 0) src += nt;
+            }
             qh_next[k] = i < nload ? qH[src] : 0.0f;
         }
     };
@@ -204,7 +216,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict_
                 const double sc = __dadd_rn(pexp(2.0 * (double)ql[k]), pexp(2.0 * (double)mx));
                 const uint64_t kk = (uint64_t)__double_as_longlong(sc);
                 key[t] = kk;
-                if (t % stride == 0) ckey[t / stride] = kk;
+                if ((t & (stride - 1)) == 0) ckey[t >> ssh] = kk;
             }
         }
         __syncthreads();
