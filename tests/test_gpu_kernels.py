@@ -347,6 +347,40 @@ def test_band_gather_clamps_and_many_reads(hip, oracle, kernel, piece):
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
 
 
+@pytest.fixture
+def tile(hip, request):
+    hip.call("exm_set_gather_tile", request.param)
+    yield request.param
+    hip.call("exm_set_gather_tile", 0)
+
+
+@pytest.mark.parametrize("kernel", [2, 3], indirect=True)
+@pytest.mark.parametrize("tile", [256, 512], indirect=True)
+def test_band_gather_tiles_over_xcd_queues(hip, oracle, kernel, tile):
+    # two tables cut into 9 and 3 tiles (256 reads per tile) or 5 and 2 (512): the tile list is built
+    # from per-XCD queues, piece q of every table in one queue (>= 8 pieces) or shared by several (< 8);
+    # every read must be gathered exactly once
+    from exmarg_amd.search import gather as g
+    say("gather: band kernel %d, tile %d, 2300 + 700 reads of two tables" % (kernel, tile))
+    s = _tiny_store("rough", (5, 49, 24, 17), n_nodes=2, seed=6)
+    nm, nt = 3, 4096
+    traj = _rand_traj(nm, nt, seed=12)
+    traj["lqH"] *= 3.0
+    traj["lqL"] *= 3.0
+    times_dev = g.select_times(traj, s.tau_idx, 300, 700, seed=2)
+    times = times_dev.to_host()
+    rng = np.random.default_rng(7)
+    n = 3000
+    ids = rng.integers(0, nm, n)
+    lna = rng.uniform(-10.0, 10.0, (n, 1))
+    tab = np.zeros((n, 1), np.int64)
+    tab[rng.permutation(n)[:700], 0] = 1
+    got = g.gather(traj, s, ids, lna, tab, times=times_dev)
+    ref = oracle.gather(traj, oracle.Tables(tables=s.tables, tau_idx=s.tau_idx), s.beta[0], s.delta[0], ids, lna,
+                        tab, times)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
+
+
 @pytest.mark.parametrize("form", [2, 3])
 @pytest.mark.parametrize("kind", ["smooth", "rough"])
 def test_band_gather_float32_form(hip, oracle, kind, form):
