@@ -88,20 +88,33 @@ __global__ __launch_bounds__(256) void k_gather_ref(
     os[e * ntau + r] = s;
 }
 
-// ln Lambda[e] = max_r om + ln sum_r os e^{om - max} - ln n_time
-__global__ void k_lse_lags(const double* __restrict__ om, const double* __restrict__ os, int64_t njobs, int ntau,
-                           const int32_t* __restrict__ jm, const double* __restrict__ ln_nt_g, double ln_nt,
-                           double* __restrict__ out) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= njobs) return;
-    const double ln_n_time = ln_nt_g ? ln_nt_g[jm[e]] : ln_nt;
+// ln Lambda[e] = max_r om + ln sum_r os e^{om - max} - ln n_time.  One wave per read: the lanes take the
+// lags (a read's om and os rows are contiguous, so the loads coalesce), the max and the sum are reduced
+// over the wave by lane shuffles (float64; the sum's order is the reduction tree's)
+__global__ __launch_bounds__(256) void k_lse_lags(const double* __restrict__ om, const double* __restrict__ os,
+                                                  int64_t njobs, int ntau, const int32_t* __restrict__ jm,
+                                                  const double* __restrict__ ln_nt_g, double ln_nt,
+                                                  double* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (e >= njobs) return;                             // the whole wave leaves together
     const double* a = om + e * ntau;
     const double* b = os + e * ntau;
     double mx = a[0];
-    for (int r = 1; r < ntau; ++r) mx = a[r] > mx ? a[r] : mx;
+    for (int r = lane; r < ntau; r += 64) mx = a[r] > mx ? a[r] : mx;
+#pragma unroll
+    for (int k = 32; k > 0; k >>= 1) {
+        const double o = __shfl_xor(mx, k, 64);
+        mx = o > mx ? o : mx;
+    }
     double tot = 0.0;
-    for (int r = 0; r < ntau; ++r) tot += b[r] * exp(a[r] - mx);
-    out[e] = mx + log(tot) - ln_n_time;
+    for (int r = lane; r < ntau; r += 64) tot += b[r] * exp(a[r] - mx);
+#pragma unroll
+    for (int k = 32; k > 0; k >>= 1) tot += __shfl_xor(tot, k, 64);
+    if (lane == 0) {
+        const double ln_n_time = ln_nt_g ? ln_nt_g[jm[e]] : ln_nt;
+        out[e] = mx + log(tot) - ln_n_time;
+    }
 }
 
 int launch_gather_ref(Traj* traj, Store* st, const JobSet& js, Times* times, double* om, double* os) {
@@ -137,7 +150,7 @@ int launch_gather(Traj* traj, Store* st, const JobSet& js, Times* times, double*
 
 int launch_lse(const JobSet& js, int ntau, const double* om, const double* os, const double* ln_nt_g, double ln_nt,
                double* d_out) {
-    hipLaunchKernelGGL(k_lse_lags, dim3((unsigned)((js.njobs + 255) / 256)), dim3(256), 0, cur_stream(), om, os,
+    hipLaunchKernelGGL(k_lse_lags, dim3((unsigned)((js.njobs + 3) / 4)), dim3(256), 0, cur_stream(), om, os,
                        js.njobs, ntau, js.d_m, ln_nt_g, ln_nt, d_out);
     EXM_HIP(hipGetLastError());
     return EXM_OK;
