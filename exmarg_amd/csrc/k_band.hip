@@ -175,6 +175,24 @@ __global__ __launch_bounds__(256) void k_walk_stream(const float* __restrict__ l
             stream[((int64_t)m * ntau + r) * nw + p0 + pl] = make_uint2(wcode[r * WS_STRIDE + pl], waH[r * WS_STRIDE + pl]);
 }
 
+// DPP lane moves within a wave64 (VALU, no LDS round trip): lane l receives the value of lane l - n of
+// its row of 16 lanes (row_shr:n), or of lane 15 / 31 of the rows below it (row_bcast:15 / :31); lanes
+// without a source lane, or outside the row and bank masks, receive `old`.  Combining each lane with
+// the moved value in the sequence shr 1, shr 2, shr 4 (banks 1-3), shr 8 (banks 2-3), bcast 15 (rows 1,
+// 3), bcast 31 (rows 2, 3), with `old` the identity of the combination, leaves at every lane the
+// combination over the lanes up to itself (an inclusive scan).
+constexpr int DPP_ROW_SHR = 0x110, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+
+template <int CTRL, int ROWM, int BANKM>
+__device__ __forceinline__ int dpp_i(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWM, BANKM, false);
+}
+
+template <int CTRL, int ROWM, int BANKM>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t old, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWM, BANKM, false);
+}
+
 // beta row of one walked time of one read: the clamped coordinate xb, the unclamped xbr and the row
 __device__ __forceinline__ int beta_row(float aL, double offb, double rdb, int nb, double& xb, double& xbr) {
     xbr = (double)aL * rdb + offb;
@@ -212,17 +230,19 @@ __global__ __launch_bounds__(256) void k_read_bounds(const float* __restrict__ w
     for (int c = 0; c < nw; c += 64) {
         const int p = c + lane;
         int run = p < nw ? beta_row(a[p], offb, rdb, nb, xb, xbr) : INT_MIN;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int y = __shfl_up(run, d, 64);
-            if (lane >= d) run = max(run, y);
-        }
+        // inclusive maximum scan over the lanes by DPP moves (INT_MIN the identity)
+        run = max(run, dpp_i<DPP_ROW_SHR | 1, 0xf, 0xf>(INT_MIN, run));
+        run = max(run, dpp_i<DPP_ROW_SHR | 2, 0xf, 0xf>(INT_MIN, run));
+        run = max(run, dpp_i<DPP_ROW_SHR | 4, 0xf, 0xe>(INT_MIN, run));
+        run = max(run, dpp_i<DPP_ROW_SHR | 8, 0xf, 0xc>(INT_MIN, run));
+        run = max(run, dpp_i<DPP_ROW_BCAST15, 0xa, 0xf>(INT_MIN, run));
+        run = max(run, dpp_i<DPP_ROW_BCAST31, 0xc, 0xf>(INT_MIN, run));
         run = max(run, carry);
         int prev = __shfl_up(run, 1, 64);
         if (lane == 0) prev = carry;
         if (p >= 1 && p < nw)
             for (int rr = prev + 1; rr <= min(run, last); ++rr) bj[rr - first] = (uint16_t)p;
-        carry = __shfl(run, 63, 64);
+        carry = __builtin_amdgcn_readlane(run, 63);
     }
     if (lane == 0) bj[last + 1 - first] = (uint16_t)nw;
 }
@@ -505,24 +525,6 @@ __device__ __forceinline__ void lse_merge(double& m, double& s, double m2, doubl
     const double e = (double)__expf(-(float)fabs(d));
     s = d > 0.0 ? fma(s, e, s2) : fma(s2, e, s);
     m = fmax(m, m2);
-}
-
-// DPP lane moves within a wave64 (VALU, no LDS round trip): lane l receives the value of lane l - n of
-// its row of 16 lanes (row_shr:n), or of lane 15 / 31 of the rows below it (row_bcast:15 / :31); lanes
-// without a source lane, or outside the row and bank masks, receive `old`.  Combining each lane with
-// the moved value in the sequence shr 1, shr 2, shr 4 (banks 1-3), shr 8 (banks 2-3), bcast 15 (rows 1,
-// 3), bcast 31 (rows 2, 3), with `old` the identity of the combination, leaves at every lane the
-// combination over the lanes up to itself (an inclusive scan).
-constexpr int DPP_ROW_SHR = 0x110, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
-
-template <int CTRL, int ROWM, int BANKM>
-__device__ __forceinline__ int dpp_i(int old, int v) {
-    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWM, BANKM, false);
-}
-
-template <int CTRL, int ROWM, int BANKM>
-__device__ __forceinline__ uint32_t dpp_u(uint32_t old, uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWM, BANKM, false);
 }
 
 // lane l receives lane l - n of its row of 16, or 0 where l - n lies outside the row (every lane
