@@ -4,11 +4,48 @@ their noise variance, the frequency grid and the band
 (src/exmarg/search/segments.py:1-54).
 """
 
-import zipfile
+import os
 
 import numpy as np
 
+from . import npzmember
+
 D0 = 10410.0              # reference distance of the stores (Mpc): y = D0 / D_L (segments.py:8)
+
+DETECTORS = ("H", "L")
+
+
+class _Source:
+    """the arrays of a segment set, from an .npz file or from a mapping of arrays.  From a file, the data
+    rows come from a memory map of the (uncompressed) member, so only the rows asked for are read."""
+
+    def __init__(self, segments):
+        self.path = os.fspath(segments) if isinstance(segments, (str, bytes, os.PathLike)) else None
+        if self.path is None:
+            self.arrays = segments
+            self.names = set(segments.keys())
+        else:
+            with np.load(self.path) as z:
+                self.names = {n for n in z.files}
+                self.arrays = {n: z[n] for n in self.names if not n.startswith("d_")}
+
+    def data(self, det):
+        name = "d_" + det
+        if self.path is None:
+            return np.asarray(self.arrays[name])
+        member = npzmember.locate(self.path, name + ".npy", compressed_ok=True)
+        if member is None:
+            with np.load(self.path) as z:
+                return z[name]
+        return member.mapped()
+
+    def noise(self, det):
+        """sig2 of one detector: its own sig2_<det> when present, else the common sig2"""
+        own = "sig2_" + det
+        return np.asarray(self.arrays[own if own in self.names else "sig2"], np.float64)
+
+    def scalar(self, name):
+        return float(np.asarray(self.arrays[name]))
 
 
 def read_segments(segments, rows=None):
@@ -19,36 +56,12 @@ def read_segments(segments, rows=None):
     rows      segment indices to keep (default all)
 
     returns a dict with the same keys, d_H, d_L as complex128 and sig2_H, sig2_L as float64."""
-    path = isinstance(segments, (str, bytes)) or hasattr(segments, "__fspath__")
-    z = np.load(segments) if path else segments
-    data = {I: _member_rows(segments, "d_" + I) if path else np.asarray(z["d_" + I]) for I in "HL"}
-    n = data["H"].shape[0]
-    rows = np.arange(n) if rows is None else np.asarray(rows, int)
-    out = {"rows": rows, "f": np.asarray(z["f"], float), "nu": np.asarray(z["nu"], np.int64),
-           "T": float(z["T"]), "fmin": float(z["fmin"]), "fmax": float(z["fmax"])}
-    for I in "HL":
-        out["d_" + I] = np.asarray(data[I][rows]).astype(complex)
-        key = "sig2_" + I if "sig2_" + I in z else "sig2"
-        out["sig2_" + I] = np.asarray(z[key], float)
+    src = _Source(segments)
+    d = {det: src.data(det) for det in DETECTORS}
+    keep = np.arange(d["H"].shape[0]) if rows is None else np.asarray(rows, int)
+    out = dict(rows=keep, f=np.asarray(src.arrays["f"], np.float64), nu=np.asarray(src.arrays["nu"], np.int64),
+               T=src.scalar("T"), fmin=src.scalar("fmin"), fmax=src.scalar("fmax"))
+    for det in DETECTORS:
+        out["d_" + det] = np.asarray(d[det][keep], dtype=np.complex128)
+        out["sig2_" + det] = src.noise(det)
     return out
-
-
-def _member_rows(path, name):
-    """one array of an npz file, memory-mapped when stored uncompressed (only the rows used are read)
-    (src/exmarg/search/segments.py:40-54)"""
-    with zipfile.ZipFile(path) as zf:
-        info = zf.getinfo(name + ".npy")
-    if info.compress_type != 0:
-        with np.load(path) as z:
-            return z[name]
-    with open(path, "rb") as fh:
-        fh.seek(info.header_offset)
-        head = fh.read(30)
-        nlen = int.from_bytes(head[26:28], "little")
-        elen = int.from_bytes(head[28:30], "little")
-        fh.seek(info.header_offset + 30 + nlen + elen)
-        ver = np.lib.format.read_magic(fh)
-        rd = np.lib.format.read_array_header_1_0 if ver == (1, 0) else np.lib.format.read_array_header_2_0
-        shape, fortran, dtype = rd(fh)
-        offset = fh.tell()
-    return np.memmap(path, dtype=dtype, mode="r", offset=offset, shape=shape, order="F" if fortran else "C")

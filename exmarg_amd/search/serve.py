@@ -114,21 +114,30 @@ class Server:
 
     # ------------------------------------------------------------ server loop
     def step(self):
-        """one pass of the loop: finish a completed gather, launch the filling slot when the device
-        has no gather, admit one queued request; returns whether anything happened"""
+        """one pass of the loop: finish a completed gather, admit every queued request that fits the
+        filling slot (a gather that ends meanwhile is finished at once), and launch the filling slot
+        when the device has no gather and the next queued request would not join it; returns whether
+        anything happened"""
         progress = False
+        while True:
+            moved = self._finish_done()
+            if self.flying is None and self.members[self.filling] and not (self.queue and self._fits(self.queue[0])):
+                self.slots[self.filling].launch()
+                self.flying, self.filling = self.filling, 1 - self.filling
+                self.stats["batches"] += 1
+                moved = True
+            if self.queue and self._admit(self.queue[0]):
+                self.queue.popleft()
+                moved = True
+            if not moved:
+                return progress
+            progress = True
+
+    def _finish_done(self):
         if self.flying is not None and self.slots[self.flying].done():
             self._finish(self.flying)
-            progress = True
-        if self.flying is None and self.members[self.filling]:
-            self.slots[self.filling].launch()
-            self.flying, self.filling = self.filling, 1 - self.filling
-            self.stats["batches"] += 1
-            progress = True
-        if self.queue and self._admit(self.queue[0]):
-            self.queue.popleft()
-            progress = True
-        return progress
+            return True
+        return False
 
     def run(self, until=None, poll=2e-4):
         """step until every submitted request has finished (or until() is true)"""
@@ -147,32 +156,51 @@ class Server:
         self._release()
 
     # ------------------------------------------------------------ internals
-    def _admit(self, req):
-        """admit the request into the filling slot if it fits; False leaves it queued"""
+    @staticmethod
+    def _size(req):
+        return req.rows.shape[0] if req.rows.ndim == 2 else req.rows.size
+
+    def _fits(self, req):
+        """whether the filling slot takes the request now (a request larger than any slot is taken, to
+        be refused)"""
         batch = self.slots[self.filling]
-        n = req.rows.shape[0] if req.rows.ndim == 2 else req.rows.size
+        n = self._size(req)
+        return n > batch.cap or batch.used + n <= batch.cap
+
+    def _refuse(self, req, err):
+        req.bank = req.an = None
+        req.error = err
+        req.t_done = self.clock()
+        self.stats["refused"] += 1
+        return True
+
+    def _admit(self, req):
+        """admit the request into the filling slot if it fits; False leaves it queued.  A request that
+        cannot be analysed (too large for a slot, a segment too short for the walk, a bank row the
+        waveform model rejects, a bank that reaches no node, a device error while it is filtered) is
+        finished at once with req.error set, so that it never holds up the requests behind it."""
+        batch = self.slots[self.filling]
+        n = self._size(req)
         if n > batch.cap:
-            req.error = MemoryError("request of %d templates exceeds a batch slot of %d" % (n, batch.cap))
-            req.t_done = self.clock()
-            self.stats["refused"] += 1
-            return True
+            return self._refuse(req, MemoryError("request of %d templates exceeds a batch slot of %d" % (n, batch.cap)))
         if batch.used + n > batch.cap:
             return False
         seg = req.seg
-        bank = self._bank(req.rows, seg)
-        an = _pe._Analysis(bank, self.store, self.ymin, True)
+        n_time = 8 * int(np.max(seg["nu"]))
+        if not 0 < self.K < n_time:
+            return self._refuse(req, ValueError("n_loud %d does not lie in (0, n_time = %d)" % (self.K, n_time)))
+        bank = None
         try:
+            bank = self._bank(req.rows, seg)
+            an = _pe._Analysis(bank, self.store, self.ymin, True)
             lnW = _pe.bank_weights(an, self.prior)
             an.start()
-        except ValueError as e:
-            self._unuse(bank)
-            req.bank = None
-            req.error = e
-            req.t_done = self.clock()
-            self.stats["refused"] += 1
-            return True
-        batch.add(an, {I: seg["d_" + I][0] for I in "HL"}, {I: seg["sig2_" + I] for I in "HL"},
-                  req.seed + int(seg["rows"][0]))
+            batch.add(an, {I: seg["d_" + I][0] for I in "HL"}, {I: seg["sig2_" + I] for I in "HL"},
+                      req.seed + int(seg["rows"][0]))
+        except Exception as e:          # noqa: BLE001 - the request carries its own error
+            if bank is not None:
+                self._unuse(bank)
+            return self._refuse(req, e)
         req.bank, req.an, req.lnW = bank, an, lnW
         req.t_admit = self.clock()
         self.members[self.filling].append(req)

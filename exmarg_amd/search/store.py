@@ -10,48 +10,26 @@ HBM holds the 40 GB store whole).
 import ctypes
 import glob
 import os
-import zipfile
+import re
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
 
 from .. import _hip, synth
+from . import npzmember
 
 NPHI = 96                 # phase bins of the full circle; the tables hold NPHI/2 + 1 (store.py:20)
 
 
-def _member_offset(path, name):
-    with zipfile.ZipFile(path) as zf:
-        info = zf.getinfo(name)
-    if info.compress_type != 0:
-        raise ValueError("member %s of %s is compressed" % (name, path))
-    with open(path, "rb") as fh:
-        fh.seek(info.header_offset)
-        head = fh.read(30)
-        nlen = int.from_bytes(head[26:28], "little")
-        elen = int.from_bytes(head[28:30], "little")
-        fh.seek(info.header_offset + 30 + nlen + elen)
-        ver = np.lib.format.read_magic(fh)
-        rd = np.lib.format.read_array_header_1_0 if ver == (1, 0) else np.lib.format.read_array_header_2_0
-        shape, fortran, dtype = rd(fh)
-        return fh.tell(), shape, fortran, dtype
-
-
 def read_npz_member(path, name):
     """one uncompressed .npy member of an npz, read sequentially (src/exmarg/search/store.py:23-36)"""
-    offset, shape, fortran, dtype = _member_offset(path, name)
-    with open(path, "rb") as fh:
-        fh.seek(offset)
-        arr = np.fromfile(fh, dtype=dtype, count=int(np.prod(shape)))
-    return arr.reshape(shape, order="F" if fortran else "C")
+    return npzmember.locate(path, name).read()
 
 
 def map_npz_member(path, name):
     """read-only memory map of one uncompressed .npy member (src/exmarg/search/store.py:39-52)"""
-    offset, shape, fortran, dtype = _member_offset(path, name)
-    return np.asarray(np.memmap(path, dtype=dtype, mode="r", offset=offset, shape=shape,
-                                order="F" if fortran else "C"))
+    return np.asarray(npzmember.locate(path, name).mapped())
 
 
 @dataclass
@@ -78,6 +56,36 @@ class RedshiftStore:
         return tuple(self.shape)
 
 
+_FILE_NAME = re.compile(r"bd_n(\d+)_s[0-9.]+\.npz$")
+
+
+@dataclass
+class _NodeFile:
+    """the grids and node values of one bd_*.npz file and where its table lies in the file"""
+    s: float
+    alpha_lo: float
+    alpha_hi: float
+    beta: np.ndarray
+    delta: np.ndarray
+    tau_idx: np.ndarray
+    n_max: int
+    table: npzmember.Member
+
+    @classmethod
+    def scan(cls, path):
+        m = _FILE_NAME.search(os.path.basename(path))
+        if m is None:
+            raise ValueError("%s is not named bd_n<order>_s<s>.npz" % path)
+        with np.load(path) as z:
+            alpha = np.asarray(z["alpha"], np.float64)
+            vals = dict(s=float(z["s_node"]), alpha_lo=float(alpha[0]), alpha_hi=float(alpha[-1]),
+                        beta=np.asarray(z["beta"]), delta=np.asarray(z["delta"]), tau_idx=np.asarray(z["tau_idx"]))
+        table = npzmember.locate(path, "T.npy")
+        if table.fortran or table.dtype != np.float32 or len(table.shape) != 4:
+            raise ValueError("%s: T must be a C-order float32 (ntau, nphif, nb, nd) array" % path)
+        return cls(n_max=int(m.group(1)), table=table, **vals)
+
+
 def load_redshift_store(store_dir, pinned=False, readers=6, mmap=False, host=True):
     """every node of the redshift store (src/exmarg/search/store.py:90-107).
 
@@ -86,35 +94,25 @@ def load_redshift_store(store_dir, pinned=False, readers=6, mmap=False, host=Tru
     upload() then streams the tables from the files into HBM with `readers` threads through pinned
     buffers (exm_store_load_files), never holding them in host memory.  pinned is accepted for the
     reference signature: uploads stage through the library's own pinned buffers."""
-    files = glob.glob(os.path.join(store_dir, "bd_*.npz"))
-    if not files:
+    nodes = sorted((_NodeFile.scan(p) for p in glob.glob(os.path.join(store_dir, "bd_*.npz"))),
+                   key=lambda nf: nf.s)
+    if not nodes:
         raise FileNotFoundError("no bd_*.npz in %s" % store_dir)
-    head = []
-    for f in files:
-        with np.load(f) as h:
-            head.append((float(h["s_node"]), float(h["alpha"][0]), float(h["alpha"][-1]),
-                         h["beta"], h["delta"], h["tau_idx"], f))
-    head.sort(key=lambda r: r[0])
-    s, alo, ahi, beta, delta, tau, files = zip(*head)
-    members = []
-    for f in files:
-        off, shape, fortran, dtype = _member_offset(f, "T.npy")
-        if fortran or np.dtype(dtype) != np.float32 or len(shape) != 4:
-            raise ValueError("%s: T must be a C-order float32 (ntau, nphif, nb, nd) array" % f)
-        members.append((f, off, tuple(shape)))
-    if len({m[2] for m in members}) != 1:
-        raise ValueError("the tables of a store must share their shape")
+    shapes = {nf.table.shape for nf in nodes}
+    if len(shapes) != 1:
+        raise ValueError("the tables of a store must share their shape, found %s" % sorted(shapes))
     if not host:
         tables = None
     elif mmap:
-        tables = [map_npz_member(f, "T.npy") for f in files]
+        tables = [np.asarray(nf.table.mapped()) for nf in nodes]
     else:
-        with ThreadPoolExecutor(max_workers=max(1, int(readers))) as ex:
-            tables = list(ex.map(lambda f: read_npz_member(f, "T.npy"), files))
-    st = RedshiftStore(s=np.array(s), alpha_lo=np.array(alo), alpha_hi=np.array(ahi), beta=list(beta),
-                       delta=list(delta), tau_idx=tau[0], tables=tables,
-                       n_max=int(os.path.basename(files[0]).split("_n")[1].split("_")[0]),
-                       shape=members[0][2], files=[(m[0], m[1]) for m in members])
+        with ThreadPoolExecutor(max_workers=max(1, int(readers))) as pool:
+            tables = list(pool.map(lambda nf: nf.table.read(), nodes))
+    st = RedshiftStore(s=np.array([nf.s for nf in nodes]), alpha_lo=np.array([nf.alpha_lo for nf in nodes]),
+                       alpha_hi=np.array([nf.alpha_hi for nf in nodes]), beta=[nf.beta for nf in nodes],
+                       delta=[nf.delta for nf in nodes], tau_idx=nodes[0].tau_idx, tables=tables,
+                       n_max=nodes[0].n_max, shape=nodes[0].table.shape,
+                       files=[(nf.table.path, nf.table.offset) for nf in nodes])
     st.readers = int(readers)
     return st
 

@@ -273,3 +273,111 @@ def test_to_source_frame_brute_force(oracle):
             Md = 2 * m1[i] * (1 + z[iz])
             exp_ = np.interp(Md, Mdet[g], col[g]) if g.sum() >= 2 and Mdet[g][0] <= Md <= Mdet[g][-1] else -np.inf
             assert out[i, iz] == pytest.approx(exp_) if np.isfinite(exp_) else out[i, iz] == -np.inf
+
+
+# ------------------------------------------------------------------ bank assembly pins
+def test_cell_weights_by_hand(oracle):
+    # np.gradient of the distinct values of an axis: end cells one-sided, inner cells centred
+    M = np.array([10.0, 20.0, 40.0])            # spacings 10, 15, 20
+    q = np.array([0.5, 1.0])                    # 0.5, 0.5
+    chi = np.array([-0.5, 0.0, 0.25])           # 0.5, 0.375, 0.25
+    grid = np.array(np.meshgrid(M, q, chi, indexing="ij")).reshape(3, -1).T
+    grid = grid[np.random.default_rng(6).permutation(grid.shape[0])]       # any row order
+    got = oracle.cell_weights(grid[:, 0], grid[:, 1], grid[:, 2])
+    gm = {10.0: 10.0, 20.0: 15.0, 40.0: 20.0}
+    gc = {-0.5: 0.5, 0.0: 0.375, 0.25: 0.25}
+    raw = np.array([gm[r[0]] * 0.5 * gc[r[2]] for r in grid])
+    np.testing.assert_allclose(got, raw / raw.sum(), rtol=1e-14)
+    # an axis that does not vary contributes nothing; a uniform grid weighs every cell alike
+    flat = oracle.cell_weights(grid[:, 0], grid[:, 1], np.zeros(grid.shape[0]))
+    np.testing.assert_allclose(flat, np.array([gm[r[0]] for r in grid]) / (2 * 3 * 45.0), rtol=1e-14)
+    uni = np.array(np.meshgrid([1.0, 2.0, 3.0], [0.2, 0.4], indexing="ij")).reshape(2, -1).T
+    np.testing.assert_allclose(oracle.cell_weights(uni[:, 0], uni[:, 1], np.zeros(6)), np.full(6, 1 / 6), rtol=1e-14)
+
+
+def _dl_derivative(oracle, z, h=1e-3):
+    """dD_L/dz by the five-point central stencil on the (pinned) dl_of_z"""
+    f = lambda zz: oracle.dl_of_z(zz)
+    return (f(z - 2 * h) - 8 * f(z - h) + 8 * f(z + h) - f(z + 2 * h)) / (12 * h)
+
+
+def test_redshift_prior_weights_against_quad(oracle):
+    # p(m1, z) ~ (1 + z) D_L^2 dD_L/dz, uniform in M_det = 2 m1 (1 + z) on mdet_range: the weight of a
+    # z column is the integral of that density over the column's cell, the weight of a m1 row its
+    # cell width; cells outside the M_det range weigh nothing
+    z = np.linspace(0.02, 8.0, 1600)
+    m1 = np.geomspace(3.0, 150.0, 60)
+    mdet = (10.0, 500.0)
+    w = oracle.pe_redshift_weights(m1, z, mdet)
+    p = lambda zz: (1.0 + zz) * oracle.dl_of_z(zz) ** 2 * _dl_derivative(oracle, np.atleast_1d(zz))
+    dz = np.gradient(z)
+    cells = np.array([integrate.quad(lambda x: float(p(x)[0]), zj - hj / 2, zj + hj / 2, epsrel=1e-10)[0]
+                      for zj, hj in zip(z[::97], dz[::97])])
+    Md = 2.0 * m1[:, None] * (1.0 + z[None, :])
+    inside = (Md >= mdet[0]) & (Md <= mdet[1])
+    assert (w[~inside] == 0).all() and (w[inside] > 0).all()
+    # the column profile: w[i, j] / (dm1_i) is the same for every row inside the range
+    prof = np.where(inside, w / np.gradient(m1)[:, None], np.nan)
+    col = np.nanmax(prof, axis=0)
+    np.testing.assert_allclose(np.where(inside, prof, col[None, :]), np.broadcast_to(col, prof.shape), rtol=1e-12)
+    # and proportional to the cell integral of the density (the ratio is one constant).  Interior
+    # columns only: the end columns take np.gradient's one-sided differences (first order), the
+    # convention of the reference's prior (src/exmarg/search/pe.py:250-255)
+    ratio = col[97::97] / cells[1:]
+    np.testing.assert_allclose(ratio / ratio[0], 1.0, rtol=1e-4)        # O(dz^2) cell rule
+    assert w.sum() == pytest.approx(1.0, abs=1e-12)
+
+
+def _assembly_case(oracle, prior, ymin):
+    """pe_intrinsic of the oracle on tables constant per node, e^{c_k} = A + B ln s_k, with every arrival
+    time walked: node k of template m then has Lambda = 87 (A + B ln s_k) at y = sqrt(s_k / u_m), which is
+    linear in ln y along each template (ln s_k = 2 ln y + ln u_m)"""
+    A, B = 10.0, 1.0
+    s = synth.S_NODES
+    nn = s.size
+    shape = (87, 49, 4, 5)
+    c = np.log(A + B * np.log(s)).astype(np.float32)
+    tabs = oracle.Tables(tables=[np.full(shape, ck, np.float32) for ck in c], tau_idx=synth.lag_indices(43))
+    beta, delta = tiny_grids(4, 5)
+    store = dict(s=s, alpha_lo=np.full(nn, 0.16), alpha_hi=np.full(nn, 4e4), beta=beta, delta=delta,
+                 tau_idx=synth.lag_indices(43))
+    rng = np.random.default_rng(12)
+    nf, lo, nt = 40, 10, 256
+    u = np.array([0.9, 5.0, 60.0, 800.0])
+    nm = u.size
+    bank = dict(W=(rng.standard_normal((nm, nf)) + 1j * rng.standard_normal((nm, nf))).astype(np.complex64),
+                u=u, M=np.array([10.0, 20.0, 40.0, 80.0]), q=np.ones(nm), chi=np.zeros(nm), lo=lo, n_time=nt)
+    seg = dict(d_H=rng.standard_normal((1, nf)) + 1j * rng.standard_normal((1, nf)),
+               d_L=rng.standard_normal((1, nf)) + 1j * rng.standard_normal((1, nf)),
+               sig2_H=np.ones(nf), sig2_L=np.ones(nf), rows=np.array([0]))
+    res = oracle.pe_intrinsic(seg, bank, store, tabs, prior=prior, ymin=ymin, n_loud=0)
+    return res, s, u, A, B, c
+
+
+@pytest.mark.parametrize("prior,ymin", [("volume", 0.1267), ("madau-dickinson", 0.0982)])
+def test_pe_assembly_closed_form(oracle, prior, ymin):
+    res, s, u, A, B, c = _assembly_case(oracle, prior, ymin)
+    density = oracle.distance_density(prior)
+    for m in range(u.size):
+        a = s / u[m]
+        alo = min(0.16, ymin * ymin)
+        valid = (np.sqrt(a) >= ymin) & (a > alo) & (a < 4e4)
+        ks = np.nonzero(valid)[0]
+        y_first, y_last = np.sqrt(a[ks[0]]), np.sqrt(a[ks[-1]])
+        lam_first = 87.0 * np.exp(np.float64(c[ks[0]]))
+        has_b = ks[0] == 0 or s[ks[0] - 1] / u[m] <= alo
+        y_lo = np.sqrt(alo * (1 + 1e-9)) if has_b else y_first
+        lam = lambda y: 87.0 * (A + B * (2.0 * np.log(y) + np.log(u[m])))
+        if density is None:
+            p = lambda y: 3.0 * ymin ** 3 * y ** -4
+        else:
+            p = lambda y: np.interp(np.log(y), density[0], density[1], left=0.0, right=0.0) / y
+        pts = list(np.sqrt(a[ks[1:-1]]))
+        total = integrate.quad(lambda y: p(y) * lam(y), y_first, y_last, points=pts or None, epsrel=1e-12,
+                               limit=500)[0]
+        if y_lo < y_first:
+            total += lam_first * integrate.quad(p, y_lo, y_first, epsrel=1e-12)[0]
+        tol = 1e-6 if density is None else 2e-5
+        assert res["ln_lambda"][0, m] == pytest.approx(np.log(total), abs=tol)
+    w = oracle.cell_weights(np.array([10.0, 20.0, 40.0, 80.0]), np.ones(4), np.zeros(4))
+    assert res["ln_b"][0] == pytest.approx(np.log(np.sum(w * np.exp(res["ln_lambda"][0]))), abs=1e-12)

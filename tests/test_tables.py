@@ -181,3 +181,29 @@ def test_device_sum_rejects_bad_ladders(hip):
         build_bd.sum_over_orders(col, row, lnq, [3, 1])
     with pytest.raises(ValueError, match="lnq"):
         build_bd.sum_over_orders(col, row, lnq[:2], [1])
+
+
+def test_cli_indices_name_grid_nodes(tmp_path):
+    """a node index means the node at that position of the 53-node s grid (the reference's
+    build_bd.py:88-92), whichever nodes the stage-2 directory holds"""
+    from exmarg_amd import build_bd
+    s_grid = np.load(os.path.join(os.path.dirname(__file__), "golden", "s_grid.npz"))["s_nodes"]
+    kdir = tmp_path / "k"
+    kdir.mkdir()
+    for i in (7, 3, 40):                        # a partial directory
+        stem = kdir / ("kstore_n120_s%09.3f" % s_grid[i])
+        np.savez(str(stem) + ".meta.npz", s_node=s_grid[i], lnq=np.zeros(3), tau_idx=np.zeros(1))
+    stems = build_bd.node_stems(str(kdir), [40, 3])
+    assert [os.path.basename(p) for p in stems] == ["kstore_n120_s%09.3f" % s_grid[i] for i in (40, 3)]
+    assert [os.path.basename(p) for p in build_bd.node_stems(str(kdir))] == \
+        ["kstore_n120_s%09.3f" % s_grid[i] for i in (3, 7, 40)]
+    with pytest.raises(FileNotFoundError, match="node 5"):
+        build_bd.node_stems(str(kdir), [5])
+    with pytest.raises(IndexError):
+        build_bd.node_stems(str(kdir), [53])
+    node = build_bd.Stage2Node.open(stems[1])
+    assert node.n_full == 120 and node.s == pytest.approx(s_grid[3])
+    assert build_bd.store_path("out", 7, 120, node.s).startswith(os.path.join("out_n007", "bd_n007_s"))
+    assert build_bd.truncation_orders(120, [5, 1, 5]) == [1, 5]
+    with pytest.raises(ValueError, match="above"):
+        build_bd.truncation_orders(10, [11])
