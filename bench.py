@@ -144,13 +144,16 @@ def cpu_leg(args, seg, bank, store, res, an_geometry, seed_row):
     lnl_ref = mx + np.log(np.exp(v - mx).sum(axis=0))
     err = float(np.max(np.abs(lnl_ref - res["ln_lambda"][0][chk])))
 
-    # -- CPU baseline: filter + selection of a template sample, gather rate on the tables of 4 nodes in RAM
+    # -- CPU baseline in the reference's own CPU arithmetic (src/exmarg/search/bank.py:66-75 complex64
+    # numpy filter, single-threaded as numpy's FFT is; gather.py:396-420 float32 score ranked by
+    # argpartition; gather.py:32-100 float64 gather with (template, lag) jobs over the threads as the
+    # reference's prange): filter + selection of a template sample, gather rate on 4 nodes in RAM
     samp = np.linspace(0, nm - 1, args.cpu_templates).astype(int)
     t0 = time.perf_counter()
-    trs = O.filter_fp64(bank.W[samp], bank.lo, nt, d, s2)
+    trs = O.filter_ref(bank.W[samp], bank.lo, nt, d, s2)
     t_filter = time.perf_counter() - t0
     t0 = time.perf_counter()
-    tms = O.select_times(trs, store.tau_idx, K, S, 0, pick=full_pick[samp])
+    tms = O.select_times_ref(trs, store.tau_idx, K, S, 0)
     t_select = time.perf_counter() - t0
     counts = valid.sum(axis=1)
     nodes = list(np.argsort(-counts)[:4])
@@ -161,27 +164,28 @@ def cpu_leg(args, seg, bank, store, res, an_geometry, seed_row):
     tabs_mem = O.Tables(tables=[t if t is not None else host_tables[nodes[0]] for t in host_tables],
                         tau_idx=store.tau_idx)
     t0 = time.perf_counter()
-    O.gather(trs, tabs_mem, beta, delta, ids, lnas, tabs, tms)
+    O.gather(trs, tabs_mem, beta, delta, ids, lnas, tabs, tms, order="reference")
     t_gather = time.perf_counter() - t0
     lookups_sample = ids.size * store.tau_idx.size * (K + S)
     cpu_lookup_rate = lookups_sample / t_gather
     n_jobs_all = int(valid.sum() + (bnode >= 0).sum())
     lookups_all = n_jobs_all * store.tau_idx.size * (K + S)
     t_seg = (t_filter + t_select) * nm / samp.size + lookups_all / cpu_lookup_rate
-    sample = ("filter+selection of %d of %d templates (scaled x%.1f) and the gather of those templates on the "
-              "4 most-used nodes (%d reads x 87 lags x %d times, tables in RAM), scaled to all %d reads"
-              % (samp.size, nm, nm / samp.size, ids.size, K + S, n_jobs_all))
+    sample = ("the reference's CPU arithmetic (complex64 numpy filter, float32 score with argpartition, float64 "
+              "OpenMP gather over (template, lag) jobs): filter+selection of %d of %d templates (scaled x%.1f) "
+              "and the gather of those templates on the 4 most-used nodes (%d reads x 87 lags x %d times, tables "
+              "in RAM), scaled to all %d reads" % (samp.size, nm, nm / samp.size, ids.size, K + S, n_jobs_all))
     # the same on one core: selection of a quarter of the sample, gather of it on one node
     O.set_threads(1)
     try:
         q = samp[: max(1, samp.size // 4)]
-        trq = O.filter_fp64(bank.W[q], bank.lo, nt, d, s2)
+        trq = O.filter_ref(bank.W[q], bank.lo, nt, d, s2)
         t0 = time.perf_counter()
-        tq = O.select_times(trq, store.tau_idx, K, S, 0, pick=full_pick[q])
+        tq = O.select_times_ref(trq, store.tau_idx, K, S, 0)
         t_sel1 = time.perf_counter() - t0
         ids1, tabs1, lnas1, _ = jobs_for(q, nodes={nodes[0]})
         t0 = time.perf_counter()
-        O.gather(trq, tabs_mem, beta, delta, ids1, lnas1, tabs1, tq)
+        O.gather(trq, tabs_mem, beta, delta, ids1, lnas1, tabs1, tq, order="reference")
         rate1 = ids1.size * store.tau_idx.size * (K + S) / (time.perf_counter() - t0)
     finally:
         O.set_threads(threads)
@@ -688,7 +692,9 @@ def main():
         an_geometry = geometry
         cpu, err, nchk = cpu_leg(args, seg, bank, st, res, an_geometry, 0)
         out["cpu_baseline"] = cpu
-        out["oracle_check"] = {"templates": nchk, "max_abs_dln_lambda": err, "bound_on_dln_b": err}
+        out["oracle_check"] = {"templates": nchk, "max_abs_dln_lambda_sample": err,
+                               "full_bank": "tests/test_gpu_configs.py::test_config2_full_bank (every template's ln "
+                                            "Lambda and ln B against the FP64 oracle)"}
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

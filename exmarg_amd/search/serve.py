@@ -39,8 +39,9 @@ from .segments import read_segments
 class Request:
     """one segment (a single row) analysed with its own bank"""
 
-    def __init__(self, rid, seg, rows, seed, t_submit):
+    def __init__(self, rid, seg, rows, seed, t_submit, factors=None):
         self.id, self.seg, self.rows, self.seed = rid, seg, rows, int(seed)
+        self.factors = factors
         self.t_submit = t_submit
         self.t_admit = self.t_done = None
         self.ln_lambda = self.ln_b = None
@@ -99,11 +100,12 @@ class Server:
                           bank_hits=0)
 
     # ------------------------------------------------------------ client side
-    def submit(self, segments, bank_rows, seed=0, t_submit=None):
-        """queue one segment (the first row of `segments`) with its bank rows (M_det, q, chi)"""
+    def submit(self, segments, bank_rows, seed=0, t_submit=None, factors=None):
+        """queue one segment (the first row of `segments`) with its bank rows (M_det, q, chi); factors
+        (W, u) computed elsewhere replace the device TaylorF2 of the bank (mass_bank's factors)"""
         seg = read_segments(segments, rows=[0])
         rows = np.asarray(bank_rows, float)
-        req = Request(self.next_id, seg, rows, seed, self.clock() if t_submit is None else t_submit)
+        req = Request(self.next_id, seg, rows, seed, self.clock() if t_submit is None else t_submit, factors)
         self.next_id += 1
         self.queue.append(req)
         return req
@@ -191,7 +193,7 @@ class Server:
             return self._refuse(req, ValueError("n_loud %d does not lie in (0, n_time = %d)" % (self.K, n_time)))
         bank = None
         try:
-            bank = self._bank(req.rows, seg)
+            bank = self._bank(req.rows, seg, req.factors)
             an = _pe._Analysis(bank, self.store, self.ymin, True)
             lnW = _pe.bank_weights(an, self.prior)
             an.start()
@@ -230,19 +232,23 @@ class Server:
         self._release()
 
     @staticmethod
-    def _bank_key(rows, seg):
-        """what a bank depends on: its rows, the segment length, band and the noise curve of its norm"""
+    def _bank_key(rows, seg, factors=None):
+        """what a bank depends on: its rows, the segment length, band and the noise curve of its norm (and
+        the factors when given)"""
         h = hashlib.sha1(np.ascontiguousarray(rows, np.float64).tobytes())
         h.update(np.ascontiguousarray(seg["sig2_H"], np.float64).tobytes())
+        if factors is not None:
+            h.update(np.ascontiguousarray(factors[0], np.complex64).tobytes())
+            h.update(np.ascontiguousarray(factors[1], np.float64).tobytes())
         nu = np.asarray(seg["nu"])
         return (h.hexdigest(), float(seg["T"]), float(seg["fmin"]), float(seg["fmax"]), int(nu[0]), int(nu.size))
 
-    def _bank(self, rows, seg):
+    def _bank(self, rows, seg, factors=None):
         """the cached bank of these rows and segment geometry, built on the device when absent"""
-        key = self._bank_key(rows, seg)
+        key = self._bank_key(rows, seg, factors)
         ent = self.banks.get(key)
         if ent is None:
-            ent = [mass_bank(rows, seg, "hip"), 0]
+            ent = [mass_bank(rows, seg, "hip", factors=factors), 0]
             ent[0]._cache_key = key
             self.banks[key] = ent
             self.stats["bank_builds"] += 1

@@ -14,10 +14,18 @@ computes (src/exmarg/search/gather.py:32-100 under numba).  Arrival-time
 ranking uses the float64 score of oracle.c (oracle_pexp) with ties to the
 smaller time.
 
+Reference-semantics mode (the reference's own device='cpu' arithmetic, for comparison with the
+reference's behaviour and as the CPU baseline): filter_ref keeps the matched filter in complex64 with
+numpy's single-precision FFT, select_times_ref ranks a float32 score with numpy's float32 exp and
+np.argpartition; the gather is the same float64 C gather (numba's promotion in the reference).
+pe_intrinsic(..., semantics="ref") runs the whole chain that way.
+
 Functions and the reference lines they follow:
   filter_fp64          src/exmarg/search/bank.py:56-78  (MassBank.trajectories)
+  filter_ref           the same in complex64 / float32 as numpy >= 2 evaluates it
   select_times         src/exmarg/search/gather.py:374-420
   select_times_np      the same in numpy (lexsort), for small inputs
+  select_times_ref     src/exmarg/search/gather.py:396-420 with its float32 score and argpartition
   gather               src/exmarg/search/gather.py:114-140 + _lse_lags :103-107
   node_geometry        src/exmarg/search/pe.py:58-84
   distance_weights     src/exmarg/search/pe.py:260-286 (+ priors.py:31-38)
@@ -63,9 +71,9 @@ def lib():
         L.oracle_pexp.argtypes = [f64]
         L.oracle_synth_fill.argtypes = [u64, i64, i32, i32, i32, i32, i32, P, P]
         L.oracle_gather.argtypes = [P, P, P, P, i64, P, P, i64, i32, P, i32, P, i32, i32, i32,
-                                    P, f64, f64, f64, P, P, i64, i32, P, P]
+                                    P, f64, f64, f64, P, P, i64, i32, P, P, i32]
         L.oracle_gather_synth.argtypes = [P, P, P, P, i64, P, P, i64, i32, P, i32, P, u64, i32, i32, i32, i32,
-                                          P, f64, f64, f64, P, P, i64, i32, P, P]
+                                          P, f64, f64, f64, P, P, i64, i32, P, P, i32]
         L.oracle_select.argtypes = [P, P, i64, i64, i32, i32, i32, i32, P, f32, P, P]
         L.oracle_scores.argtypes = [P, P, i64, i32, i32, P]
         L.oracle_set_threads.argtypes = [i32]
@@ -83,26 +91,59 @@ def _p(a):
 
 
 # ------------------------------------------------------------------ filter
-def filter_fp64(W, lo, n_time, d, sig2):
+def filter_fp64(W, lo, n_time, d, sig2, threads=None):
     """trajectories of one segment: dict lqH, phH, lqL, phL of float32 (n_template, n_time).
 
     W (n_template, n_freq) complex64; d, sig2 dicts {'H','L'} of (n_freq,) arrays.
-    z = FFT_N[pad(W conj(d)/sig2)] in complex128 (forward, unnormalized);
-    lq = 1/2 ln max(|z|^2, 1e-38), ph = atan2(Im z, Re z), each rounded once to float32."""
+    z = FFT_N[pad(W conj(d)/sig2)] in complex128 (forward, unnormalized, scipy.fft's pocketfft);
+    lq = 1/2 ln max(|z|^2, 1e-38), ph = atan2(Im z, Re z), each rounded once to float32.
+    Blocks of 64 templates run on `threads` host threads (default OMP_NUM_THREADS, else at most 16)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from scipy import fft as sfft
     W = np.asarray(W)
     nm, nf = W.shape
     out = {}
+    nthr = max(1, int(threads or os.environ.get("OMP_NUM_THREADS") or min(16, os.cpu_count() or 1)))
     for I in "HL":
         w = np.conj(np.asarray(d[I], np.complex128)) / np.asarray(sig2[I], np.float64)
         lq = np.empty((nm, n_time), np.float32)
         ph = np.empty((nm, n_time), np.float32)
-        for a in range(0, nm, 64):
+
+        def block(a, w=w, lq=lq, ph=ph):
             b = min(a + 64, nm)
             pad = np.zeros((b - a, n_time), np.complex128)
             pad[:, lo:lo + nf] = W[a:b].astype(np.complex128) * w[None, :]
-            z = np.fft.fft(pad, axis=-1)
+            z = sfft.fft(pad, axis=-1, overwrite_x=True)
             lq[a:b] = (0.5 * np.log(np.maximum(z.real ** 2 + z.imag ** 2, 1e-38))).astype(np.float32)
             ph[a:b] = np.arctan2(z.imag, z.real).astype(np.float32)
+
+        with ThreadPoolExecutor(max_workers=nthr) as pool:
+            list(pool.map(block, range(0, nm, 64)))
+        out["lq" + I], out["ph" + I] = lq, ph
+    return out
+
+
+def filter_ref(W, lo, n_time, d, sig2, chunk=1024):
+    """trajectories in the reference's CPU arithmetic (src/exmarg/search/bank.py:66-75): the weighted
+    data rounded to complex64, the product with W and the FFT in complex64 (numpy >= 2 keeps single
+    precision), |z|^2, the 1e-38 floor, ln and atan2 in float32"""
+    W = np.asarray(W, np.complex64)
+    nm, nf = W.shape
+    out = {}
+    for I in "HL":
+        w = (np.conj(np.asarray(d[I], np.complex128)) / np.asarray(sig2[I], np.float64)).astype(np.complex64)
+        lq = np.empty((nm, n_time), np.float32)
+        ph = np.empty((nm, n_time), np.float32)
+        for a in range(0, nm, chunk):
+            b = min(a + chunk, nm)
+            pad = np.zeros((b - a, n_time), np.complex64)
+            pad[:, lo:lo + nf] = W[a:b] * w[None, :]
+            z = np.fft.fft(pad, axis=-1)
+            if z.dtype != np.complex64:
+                raise RuntimeError("numpy.fft returned %s: the reference semantics need numpy >= 2" % z.dtype)
+            re, im = z.real, z.imag
+            lq[a:b] = np.float32(0.5) * np.log(np.maximum(re * re + im * im, np.float32(1e-38)))
+            ph[a:b] = np.arctan2(im, re)
         out["lq" + I], out["ph" + I] = lq, ph
     return out
 
@@ -179,6 +220,55 @@ def select_times_np(traj, tau_idx, n_loud, n_random, seed):
     return sel, lw
 
 
+def window_max(lq, lo, hi):
+    """max over tau in [lo, hi] of lq[:, (t + tau) mod n_time], exact (any evaluation order gives the
+    same float32 maximum)"""
+    lq = np.asarray(lq, np.float32)
+    nt = lq.shape[1]
+    ext = np.concatenate([lq, lq[:, :hi - lo]], axis=1)
+    win = np.lib.stride_tricks.sliding_window_view(ext, hi - lo + 1, axis=1)[:, :nt].max(axis=-1)
+    return np.roll(win, -lo, axis=1)
+
+
+def select_times_ref(traj, tau_idx, n_loud, n_random, seed, block=256):
+    """select_times with the reference's own ranking (src/exmarg/search/gather.py:396-420): the score
+    e^{2 lqL} + e^{2 max lqH} in float32 (numpy's float32 exp), the n_loud largest by np.argpartition
+    (whichever of a tie it returns), the drawn times indexing the rest in time order.
+    returns (sel, lw) like select_times."""
+    K, S = int(n_loud), int(n_random)
+    lqH = np.asarray(traj["lqH"], np.float32)
+    lqL = np.asarray(traj["lqL"], np.float32)
+    nm, nt = lqH.shape
+    lo, hi = lag_range(tau_idx)
+    pick = draw_picks(nm, nt, K, S, seed)
+    two = np.float32(2.0)
+    sel = np.empty((nm, K + S), np.int32)
+    for a in range(0, nm, block):
+        b = min(a + block, nm)
+        score = np.exp(two * lqL[a:b]) + np.exp(two * window_max(lqH[a:b], lo, hi))
+        top = np.argpartition(-score, K, axis=1)[:, :K]
+        keep = np.ones((b - a, nt), bool)
+        np.put_along_axis(keep, top, False, axis=1)
+        rest = np.nonzero(keep)[1].reshape(b - a, nt - K)
+        sel[a:b] = np.concatenate([top, np.take_along_axis(rest, pick[a:b], axis=1)], axis=1)
+    lw = np.zeros((nm, K + S), np.float32)
+    lw[:, K:] = random_log_weight(nt, K, S)
+    order = np.argsort(sel, axis=1, kind="stable")
+    return (np.ascontiguousarray(np.take_along_axis(sel, order, axis=1)),
+            np.ascontiguousarray(np.take_along_axis(lw, order, axis=1)))
+
+
+def top_set_differences(traj, tau_idx, n_loud, n_random, seed):
+    """per template, the number of times in the top-n_loud set of the float64 ranking (select_times)
+    that the reference's float32 ranking (select_times_ref) leaves out"""
+    a = select_times(traj, tau_idx, n_loud, n_random, seed)
+    b = select_times_ref(traj, tau_idx, n_loud, n_random, seed)
+    out = np.empty(a[0].shape[0], np.int64)
+    for m in range(out.size):
+        out[m] = np.setdiff1d(a[0][m][a[1][m] == 0], b[0][m][b[1][m] == 0]).size
+    return out
+
+
 # ------------------------------------------------------------------ gather
 def synth_table(seed, node, shape, tau_idx, kind="smooth"):
     """one synthetic table in host memory (reference layout), by the C recipe"""
@@ -190,17 +280,31 @@ def synth_table(seed, node, shape, tau_idx, kind="smooth"):
 
 
 class Tables:
-    """the tables a gather reads: host arrays (reference layout) or a synthetic recipe"""
+    """the tables a gather reads: host arrays (reference layout) or a synthetic recipe.  With a recipe and
+    materialize=True, each gather fills the tables it reads into host memory first (one pass of
+    oracle_synth_fill per table, OpenMP) and reads them from there: the same values, read at memory speed
+    instead of recomputed per lookup."""
 
-    def __init__(self, tables=None, synth=None, shape=None, tau_idx=None):
+    def __init__(self, tables=None, synth=None, shape=None, tau_idx=None, materialize=False):
         self.tables = tables
         self.synth = synth            # dict(seed, kind, nodes) with nodes the node id of each table
+        self.materialize = bool(materialize)
         if tables is not None:
             self.arrs = [np.ascontiguousarray(t, np.float32) for t in tables]
             self.shape = self.arrs[0].shape
         else:
             self.shape = tuple(shape)
         self.tau_idx = np.ascontiguousarray(tau_idx, np.int64)
+
+    def filled(self, tab_ids):
+        """Tables in host memory holding (at their own indices) the tables of tab_ids"""
+        used = np.unique(np.asarray(tab_ids))
+        kind = self.synth.get("kind", "smooth")
+        arrs = {int(k): synth_table(self.synth["seed"], int(self.synth["nodes"][k]), self.shape, self.tau_idx, kind)
+                for k in used}
+        first = arrs[int(used[0])]
+        ntab = len(self.synth["nodes"])
+        return Tables(tables=[arrs.get(k, first) for k in range(ntab)], tau_idx=self.tau_idx)
 
 
 def lse_lags(om, os_, n_time):
@@ -209,8 +313,14 @@ def lse_lags(om, os_, n_time):
     return mx + np.log(tot) - np.log(n_time)
 
 
-def gather_pairs(traj, tabs, beta, delta, mass_ids, lna, tab_ids, times=None):
-    """(om, os) (NS, ntau, nsel) running max and sum of every (slot, lag, template)"""
+def gather_pairs(traj, tabs, beta, delta, mass_ids, lna, tab_ids, times=None, order="slab"):
+    """(om, os) (NS, ntau, nsel) running max and sum of every (slot, lag, template).  order 'slab' runs
+    the threads over the templates of one (slot, lag) at a time (cache-friendly, the checker's order);
+    'reference' hands (template, lag) jobs to the threads as the reference's prange does (the CPU
+    baseline's order).  Both give the same values."""
+    if order not in ("slab", "reference"):
+        raise ValueError("order is 'slab' or 'reference', not %r" % (order,))
+    slab = 1 if order == "slab" else 0
     lqH, phH, lqL, phL = (np.ascontiguousarray(traj[k], np.float32) for k in ("lqH", "phH", "lqL", "phL"))
     n_time = lqH.shape[1]
     ids = np.ascontiguousarray(mass_ids, np.int64)
@@ -229,24 +339,27 @@ def gather_pairs(traj, tabs, beta, delta, mass_ids, lna, tab_ids, times=None):
         nw, use = sel.shape[1], 1
     om = np.empty((ns, ntau, nsel))
     os_ = np.empty((ns, ntau, nsel))
+    if tabs.tables is None and tabs.materialize:
+        tabs = tabs.filled(tab)
     L = lib()
     if tabs.tables is not None:
         ptrs = (ctypes.c_void_p * len(tabs.arrs))(*[a.ctypes.data for a in tabs.arrs])
         L.oracle_gather(_p(lqH), _p(phH), _p(lqL), _p(phL), n_time, _p(ids), _p(tab), nsel, ns,
                         _p(tabs.tau_idx), ntau, ptrs, nphif, nb, nd, _p(offb), rdb, offd, rdd,
-                        _p(sel), _p(lw), nw, use, _p(om), _p(os_))
+                        _p(sel), _p(lw), nw, use, _p(om), _p(os_), slab)
     else:
         nodes = np.ascontiguousarray(tabs.synth["nodes"], np.int64)
         L.oracle_gather_synth(_p(lqH), _p(phH), _p(lqL), _p(phL), n_time, _p(ids), _p(tab), nsel, ns,
                               _p(tabs.tau_idx), ntau, _p(nodes), int(tabs.synth["seed"]),
                               1 if tabs.synth.get("kind", "smooth") == "rough" else 0,
-                              nphif, nb, nd, _p(offb), rdb, offd, rdd, _p(sel), _p(lw), nw, use, _p(om), _p(os_))
+                              nphif, nb, nd, _p(offb), rdb, offd, rdd, _p(sel), _p(lw), nw, use, _p(om), _p(os_),
+                              slab)
     return om, os_
 
 
-def gather(traj, tabs, beta, delta, mass_ids, lna, tab_ids, times=None):
+def gather(traj, tabs, beta, delta, mass_ids, lna, tab_ids, times=None, order="slab"):
     """ln Lambda (NS, nsel) float64 (src/exmarg/search/gather.py:114-140)"""
-    om, os_ = gather_pairs(traj, tabs, beta, delta, mass_ids, lna, tab_ids, times)
+    om, os_ = gather_pairs(traj, tabs, beta, delta, mass_ids, lna, tab_ids, times, order)
     return lse_lags(om, os_, traj["lqH"].shape[1])
 
 
@@ -375,12 +488,19 @@ def cell_weights(M, q, chi):
     return cell / cell.sum()
 
 
-def analyze_segment(seg_row, bank, store, tabs, geometry, n_loud, n_random, seed, nodes_per_launch=6):
+def analyze_segment(seg_row, bank, store, tabs, geometry, n_loud, n_random, seed, nodes_per_launch=6,
+                    semantics="fp64"):
     """trajectories, times and lamN of one segment row.
-    seg_row: dict d_H, d_L (n_freq,), sig2_H, sig2_L, row (int); bank: dict W, lo, n_time."""
-    traj = filter_fp64(bank["W"], bank["lo"], bank["n_time"], {"H": seg_row["d_H"], "L": seg_row["d_L"]},
-                       {"H": seg_row["sig2_H"], "L": seg_row["sig2_L"]})
-    times = (select_times(traj, store["tau_idx"], n_loud, n_random, seed + int(seg_row["row"]))
+    seg_row: dict d_H, d_L (n_freq,), sig2_H, sig2_L, row (int); bank: dict W, lo, n_time.
+    semantics 'fp64' (the FP64 oracle) or 'ref' (the reference's CPU arithmetic: filter_ref and
+    select_times_ref)."""
+    if semantics not in ("fp64", "ref"):
+        raise ValueError("semantics is 'fp64' or 'ref', not %r" % (semantics,))
+    filt = filter_ref if semantics == "ref" else filter_fp64
+    select = select_times_ref if semantics == "ref" else select_times
+    traj = filt(bank["W"], bank["lo"], bank["n_time"], {"H": seg_row["d_H"], "L": seg_row["d_L"]},
+                {"H": seg_row["sig2_H"], "L": seg_row["sig2_L"]})
+    times = (select(traj, store["tau_idx"], n_loud, n_random, seed + int(seg_row["row"]))
              if n_loud > 0 else None)
     nn = len(store["s"])
     groups = [list(range(g, min(g + nodes_per_launch, nn))) for g in range(0, nn, nodes_per_launch)]
@@ -388,7 +508,7 @@ def analyze_segment(seg_row, bank, store, tabs, geometry, n_loud, n_random, seed
 
 
 def pe_intrinsic(seg, bank, store, tabs, prior="volume", ymin=0.1267, n_loud=1024, n_random=1024, seed=0,
-                 rows=None):
+                 rows=None, semantics="fp64"):
     """dict ln_lambda (n_seg, nm), ln_b (n_seg,), lamN (n_seg, nn+1, nm) (src/exmarg/search/pe.py:289-346).
     seg: the read_segments dict; bank: dict W, u, M, q, chi, lo, n_time; store: dict s, alpha_lo, alpha_hi,
     beta, delta, tau_idx."""
@@ -400,7 +520,7 @@ def pe_intrinsic(seg, bank, store, tabs, prior="volume", ymin=0.1267, n_loud=102
     for j, e in enumerate(rows):
         row = dict(d_H=seg["d_H"][e], d_L=seg["d_L"][e], sig2_H=seg["sig2_H"], sig2_L=seg["sig2_L"],
                    row=seg["rows"][e] if "rows" in seg else e)
-        _, _, lamN = analyze_segment(row, bank, store, tabs, geometry, n_loud, n_random, seed)
+        _, _, lamN = analyze_segment(row, bank, store, tabs, geometry, n_loud, n_random, seed, semantics=semantics)
         v = lnW + lamN
         mx = v.max(axis=0)
         out[j] = mx + np.log(np.exp(v - mx[None, :]).sum(axis=0))
@@ -484,6 +604,28 @@ def pe_redshift_weights(m1, z, mdet_range):
     inside = (Md >= mdet_range[0]) & (Md <= mdet_range[1])
     w = np.where(inside, np.exp(lnp)[None, :], 0.0) * np.gradient(m1)[:, None] * np.gradient(z)[None, :]
     return w / w.sum()
+
+
+def pe_redshift(seg, bank, store, tabs, zmax=8.0, nz=1600, m1_range=(3.0, 150.0), nm1=1600, ymin=0.1267,
+                mdet_range=(10.0, 500.0), n_loud=1024, n_random=1024, seed=0, rows=None, semantics="fp64"):
+    """dict ln_lambda (n_seg, nm1, nz), ln_b (n_seg,) (src/exmarg/search/pe.py:203-257): node values per
+    segment, the source-frame grid by to_source_frame and ln B by log_mean with pe_redshift_weights.
+    bank: dict W, u, M, lo, n_time; seg and store as pe_intrinsic."""
+    z = np.linspace(0.02, zmax, int(nz))
+    m1 = np.geomspace(m1_range[0], m1_range[1], int(nm1))
+    yq = D0 / dl_of_z(z)
+    w = pe_redshift_weights(m1, z, mdet_range)
+    geometry = node_geometry(store["s"], store["alpha_lo"], store["alpha_hi"], bank["u"], ymin)
+    rows = np.arange(np.asarray(seg["d_H"]).shape[0]) if rows is None else np.asarray(rows)
+    out = np.full((rows.size, m1.size, z.size), -np.inf)
+    ln_b = np.empty(rows.size)
+    for j, e in enumerate(rows):
+        row = dict(d_H=seg["d_H"][e], d_L=seg["d_L"][e], sig2_H=seg["sig2_H"], sig2_L=seg["sig2_L"],
+                   row=seg["rows"][e] if "rows" in seg else e)
+        _, _, lamN = analyze_segment(row, bank, store, tabs, geometry, n_loud, n_random, seed, semantics=semantics)
+        out[j] = to_source_frame(lamN, geometry[2], np.asarray(bank["M"], float), yq, m1, z)
+        ln_b[j] = log_mean(out[j], w)
+    return dict(m1=m1, z=z, ln_lambda=out, ln_b=ln_b)
 
 
 # ---------------------------------------------------------------- table build, stage 3

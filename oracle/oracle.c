@@ -106,80 +106,102 @@ static inline float tval(const table_src* ts, int64_t tab, int64_t idx, int nphi
     return ts->tables[tab][idx];
 }
 
+/* one (template j, lag r, slot n) of the gather: the online (max, sum) over the walked times */
+static void gather_one(const float* lqH, const float* phH, const float* lqL, const float* phL, int64_t n_time,
+                       const int64_t* mass_ids, const int64_t* tab_ids, int64_t nsel, int ns, int64_t j, int r, int n,
+                       const int64_t* tau_idx, int ntau, const table_src* ts, int nphif, int nb, int nd,
+                       const double* offbs, double rdb, double offd, double rdd,
+                       const int32_t* sel, const float* lw, int64_t n_walk, int use_sel,
+                       double* out_m, double* out_s) {
+    const double rph = NPHI / 6.283185307179586;
+    int64_t mid = mass_ids[j];
+    const float* qH = lqH + mid * n_time;
+    const float* pH = phH + mid * n_time;
+    const float* qL = lqL + mid * n_time;
+    const float* pL = phL + mid * n_time;
+    int64_t tau0 = ((tau_idx[r] % n_time) + n_time) % n_time;
+    int64_t tab = tab_ids[j * ns + n];
+    double m = -1e300, s = 0.0;
+    for (int64_t k = 0; k < n_walk; ++k) {
+        int64_t t = use_sel ? (int64_t)sel[mid * n_walk + k] : k;
+        int64_t i1 = t + tau0;
+        if (i1 >= n_time) i1 -= n_time;
+        float dph = pH[i1] - pL[t];               /* float32 - float32 stays float32 */
+        double x3 = (double)dph * rph;
+        if (x3 < 0.0) x3 += NPHI;
+        if (x3 >= NPHI) x3 -= NPHI;
+        int c0 = (int)x3;
+        if (c0 >= NPHI) c0 = NPHI - 1;
+        else if (c0 < 0) c0 = 0;
+        double fc = x3 - c0;
+        int c1 = (c0 + 1) % NPHI;
+        double aH = qH[i1], aL = qL[t];
+        double xbr = aL * rdb + offbs[j * ns + n];
+        double xb = xbr;
+        if (xb < 0.0) xb = 0.0;
+        else if (xb > nb - 1 - 1e-6) xb = nb - 1 - 1e-6;
+        double aLc = aL + (xb - xbr) / rdb;
+        double xd = (aH - aLc) * rdd + offd;
+        if (xd < 0.0) xd = 0.0;
+        else if (xd > nd - 1 - 1e-6) xd = nd - 1 - 1e-6;
+        int d0 = (int)xd;
+        double fdf = xd - d0, fd0 = 1.0 - fdf;
+        int d1 = d0 + 1 < nd ? d0 + 1 : nd - 1;
+        int b0 = (int)xb;
+        double fbf = xb - b0, fb0 = 1.0 - fbf;
+        int b1 = b0 + 1 < nb ? b0 + 1 : nb - 1;
+        double acc = 0.0;
+        for (int cc = 0; cc < 2; ++cc) {
+            int ci = cc ? c1 : c0;
+            if (ci > NPHI / 2) ci = NPHI - ci;
+            double fci = cc ? fc : 1.0 - fc;
+            int64_t rc = ((int64_t)r * nphif + ci) * nb;
+            double t00 = tval(ts, tab, (rc + b0) * nd + d0, nphif, nb, nd, tau_idx);
+            double t01 = tval(ts, tab, (rc + b0) * nd + d1, nphif, nb, nd, tau_idx);
+            double t10 = tval(ts, tab, (rc + b1) * nd + d0, nphif, nb, nd, tau_idx);
+            double t11 = tval(ts, tab, (rc + b1) * nd + d1, nphif, nb, nd, tau_idx);
+            double v = fb0 * (fd0 * t00 + fdf * t01) + fbf * (fd0 * t10 + fdf * t11);
+            acc += fci * v;
+        }
+        if (use_sel) acc += (double)lw[mid * n_walk + k];
+        if (acc > m) {
+            s = s * exp(m - acc) + 1.0;
+            m = acc;
+        } else {
+            s += exp(acc - m);
+        }
+    }
+    out_m[((int64_t)n * ntau + r) * nsel + j] = m;
+    out_s[((int64_t)n * ntau + r) * nsel + j] = s;
+}
+
+/* every (template, lag, slot).  slab_order 0: the reference's job split, one job per (template, lag)
+ * pair handed to the threads (gather.py:41, prange over nsel * ntau), the slots inside a job;
+ * slab_order 1: slot by slot and lag by lag, the threads sharing the templates, so that they read one
+ * lag slab of one table at a time (the same values and operations, a cache-friendly order) */
 static void gather_core(const float* lqH, const float* phH, const float* lqL, const float* phL, int64_t n_time,
                         const int64_t* mass_ids, const int64_t* tab_ids, int64_t nsel, int ns,
                         const int64_t* tau_idx, int ntau, const table_src* ts, int nphif, int nb, int nd,
                         const double* offbs, double rdb, double offd, double rdd,
                         const int32_t* sel, const float* lw, int64_t n_walk_sel, int use_sel,
-                        double* out_m, double* out_s) {
-    const double rph = NPHI / 6.283185307179586;
+                        double* out_m, double* out_s, int slab_order) {
     const int64_t n_walk = use_sel ? n_walk_sel : n_time;
+    if (!slab_order) {
 #pragma omp parallel for schedule(dynamic, 4)
-    for (int64_t job = 0; job < nsel * ntau; ++job) {
-        int64_t j = job / ntau;
-        int r = (int)(job % ntau);
-        int64_t mid = mass_ids[j];
-        const float* qH = lqH + mid * n_time;
-        const float* pH = phH + mid * n_time;
-        const float* qL = lqL + mid * n_time;
-        const float* pL = phL + mid * n_time;
-        int64_t tau0 = ((tau_idx[r] % n_time) + n_time) % n_time;
-        for (int n = 0; n < ns; ++n) {
-            int64_t tab = tab_ids[j * ns + n];
-            double m = -1e300, s = 0.0;
-            for (int64_t k = 0; k < n_walk; ++k) {
-                int64_t t = use_sel ? (int64_t)sel[mid * n_walk + k] : k;
-                int64_t i1 = t + tau0;
-                if (i1 >= n_time) i1 -= n_time;
-                float dph = pH[i1] - pL[t];               /* float32 - float32 stays float32 */
-                double x3 = (double)dph * rph;
-                if (x3 < 0.0) x3 += NPHI;
-                if (x3 >= NPHI) x3 -= NPHI;
-                int c0 = (int)x3;
-                if (c0 >= NPHI) c0 = NPHI - 1;
-                else if (c0 < 0) c0 = 0;
-                double fc = x3 - c0;
-                int c1 = (c0 + 1) % NPHI;
-                double aH = qH[i1], aL = qL[t];
-                double xbr = aL * rdb + offbs[j * ns + n];
-                double xb = xbr;
-                if (xb < 0.0) xb = 0.0;
-                else if (xb > nb - 1 - 1e-6) xb = nb - 1 - 1e-6;
-                double aLc = aL + (xb - xbr) / rdb;
-                double xd = (aH - aLc) * rdd + offd;
-                if (xd < 0.0) xd = 0.0;
-                else if (xd > nd - 1 - 1e-6) xd = nd - 1 - 1e-6;
-                int d0 = (int)xd;
-                double fdf = xd - d0, fd0 = 1.0 - fdf;
-                int d1 = d0 + 1 < nd ? d0 + 1 : nd - 1;
-                int b0 = (int)xb;
-                double fbf = xb - b0, fb0 = 1.0 - fbf;
-                int b1 = b0 + 1 < nb ? b0 + 1 : nb - 1;
-                double acc = 0.0;
-                for (int cc = 0; cc < 2; ++cc) {
-                    int ci = cc ? c1 : c0;
-                    if (ci > NPHI / 2) ci = NPHI - ci;
-                    double fci = cc ? fc : 1.0 - fc;
-                    int64_t rc = ((int64_t)r * nphif + ci) * nb;
-                    double t00 = tval(ts, tab, (rc + b0) * nd + d0, nphif, nb, nd, tau_idx);
-                    double t01 = tval(ts, tab, (rc + b0) * nd + d1, nphif, nb, nd, tau_idx);
-                    double t10 = tval(ts, tab, (rc + b1) * nd + d0, nphif, nb, nd, tau_idx);
-                    double t11 = tval(ts, tab, (rc + b1) * nd + d1, nphif, nb, nd, tau_idx);
-                    double v = fb0 * (fd0 * t00 + fdf * t01) + fbf * (fd0 * t10 + fdf * t11);
-                    acc += fci * v;
-                }
-                if (use_sel) acc += (double)lw[mid * n_walk + k];
-                if (acc > m) {
-                    s = s * exp(m - acc) + 1.0;
-                    m = acc;
-                } else {
-                    s += exp(acc - m);
-                }
-            }
-            out_m[((int64_t)n * ntau + r) * nsel + j] = m;
-            out_s[((int64_t)n * ntau + r) * nsel + j] = s;
-        }
+        for (int64_t job = 0; job < nsel * ntau; ++job)
+            for (int n = 0; n < ns; ++n)
+                gather_one(lqH, phH, lqL, phL, n_time, mass_ids, tab_ids, nsel, ns, job / ntau, (int)(job % ntau), n,
+                           tau_idx, ntau, ts, nphif, nb, nd, offbs, rdb, offd, rdd, sel, lw, n_walk, use_sel, out_m,
+                           out_s);
+        return;
     }
+    for (int n = 0; n < ns; ++n)
+        for (int r = 0; r < ntau; ++r) {
+#pragma omp parallel for schedule(dynamic, 8)
+            for (int64_t j = 0; j < nsel; ++j)
+                gather_one(lqH, phH, lqL, phL, n_time, mass_ids, tab_ids, nsel, ns, j, r, n, tau_idx, ntau, ts, nphif,
+                           nb, nd, offbs, rdb, offd, rdd, sel, lw, n_walk, use_sel, out_m, out_s);
+        }
 }
 
 void oracle_gather(const float* lqH, const float* phH, const float* lqL, const float* phL, int64_t n_time,
@@ -187,10 +209,10 @@ void oracle_gather(const float* lqH, const float* phH, const float* lqL, const f
                    const int64_t* tau_idx, int ntau, const float* const* tables, int nphif, int nb, int nd,
                    const double* offbs, double rdb, double offd, double rdd,
                    const int32_t* sel, const float* lw, int64_t n_walk, int use_sel,
-                   double* out_m, double* out_s) {
+                   double* out_m, double* out_s, int slab_order) {
     table_src ts = {tables, NULL, 0, 0};
     gather_core(lqH, phH, lqL, phL, n_time, mass_ids, tab_ids, nsel, ns, tau_idx, ntau, &ts, nphif, nb, nd,
-                offbs, rdb, offd, rdd, sel, lw, n_walk, use_sel, out_m, out_s);
+                offbs, rdb, offd, rdd, sel, lw, n_walk, use_sel, out_m, out_s, slab_order);
 }
 
 void oracle_gather_synth(const float* lqH, const float* phH, const float* lqL, const float* phL, int64_t n_time,
@@ -198,10 +220,10 @@ void oracle_gather_synth(const float* lqH, const float* phH, const float* lqL, c
                          const int64_t* tau_idx, int ntau, const int64_t* node_of_table, uint64_t seed, int kind,
                          int nphif, int nb, int nd, const double* offbs, double rdb, double offd, double rdd,
                          const int32_t* sel, const float* lw, int64_t n_walk, int use_sel,
-                         double* out_m, double* out_s) {
+                         double* out_m, double* out_s, int slab_order) {
     table_src ts = {NULL, node_of_table, seed, kind};
     gather_core(lqH, phH, lqL, phL, n_time, mass_ids, tab_ids, nsel, ns, tau_idx, ntau, &ts, nphif, nb, nd,
-                offbs, rdb, offd, rdd, sel, lw, n_walk, use_sel, out_m, out_s);
+                offbs, rdb, offd, rdd, sel, lw, n_walk, use_sel, out_m, out_s, slab_order);
 }
 
 /* ------------------------------------------------------------ select_times */

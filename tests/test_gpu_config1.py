@@ -44,3 +44,22 @@ def test_config1_matches_oracle(config1, oracle, n_loud, n_random):
     assert np.all(np.isfinite(res["ln_lambda"]))
     assert err <= LNB_TOL
     assert abs(res["ln_b"][0] - ref["ln_b"][0]) <= LNB_TOL
+
+
+def test_config1_matches_reference_semantics(config1, oracle):
+    """the HIP path against the oracle in the reference's own CPU arithmetic (complex64 filter, float32
+    score ranked by argpartition, float64 gather), the agreement the reference's CPU/GPU check asks for"""
+    from exmarg_amd.search import pe_intrinsic, read_segments
+    seg, bank, st = config1
+    say("config 1: HIP against the reference-semantics oracle")
+    res = pe_intrinsic(seg, bank, st, n_loud=1024, n_random=1024, seed=0, resident=True)
+    b = dict(W=bank.W, u=bank.u, M=bank.M, q=bank.q, chi=bank.chi, lo=bank.lo, n_time=bank.n_time)
+    store = dict(s=st.s, alpha_lo=st.alpha_lo, alpha_hi=st.alpha_hi, beta=st.beta[0], delta=st.delta[0],
+                 tau_idx=st.tau_idx)
+    tabs = oracle.Tables(synth=dict(seed=0, kind="smooth", nodes=np.arange(st.s.size)), shape=st.table_shape(),
+                         tau_idx=st.tau_idx)
+    ref = oracle.pe_intrinsic(read_segments(seg), b, store, tabs, n_loud=1024, n_random=1024, seed=0,
+                              semantics="ref")
+    err = np.max(np.abs(res["ln_lambda"] - ref["ln_lambda"]))
+    say("ln B %.8f reference semantics %.8f; max |d ln Lambda| %.2e" % (res["ln_b"][0], ref["ln_b"][0], err))
+    assert abs(res["ln_b"][0] - ref["ln_b"][0]) <= LNB_TOL

@@ -381,3 +381,93 @@ def test_pe_assembly_closed_form(oracle, prior, ymin):
         assert res["ln_lambda"][0, m] == pytest.approx(np.log(total), abs=tol)
     w = oracle.cell_weights(np.array([10.0, 20.0, 40.0, 80.0]), np.ones(4), np.zeros(4))
     assert res["ln_b"][0] == pytest.approx(np.log(np.sum(w * np.exp(res["ln_lambda"][0]))), abs=1e-12)
+
+
+# ------------------------------------------------------------------ reference-semantics mode
+def test_filter_ref_single_bin_and_against_fp64(oracle):
+    # the complex64 filter reproduces the single-bin closed form to float32 precision and the complex128
+    # filter to the precision of a complex64 FFT
+    n_time, lo, nf, nu0 = 256, 10, 20, 13
+    W = np.zeros((1, nf), np.complex64)
+    W[0, nu0 - lo] = 0.5 - 0.25j
+    d = np.full(nf, 2.0 - 1.0j)
+    s2 = np.full(nf, 4.0)
+    traj = oracle.filter_ref(W, lo, n_time, {"H": d, "L": d}, {"H": s2, "L": s2})
+    assert all(v.dtype == np.float32 for v in traj.values())
+    w = complex(W[0, nu0 - lo]) * np.conj(2.0 - 1.0j) / 4.0
+    np.testing.assert_allclose(traj["lqH"][0], np.log(abs(w)), rtol=0, atol=2e-6)
+    rng = np.random.default_rng(1)
+    n_time, lo, nf = 2048, 40, 200
+    W = (rng.standard_normal((3, nf)) + 1j * rng.standard_normal((3, nf))).astype(np.complex64)
+    dd = {I: rng.standard_normal(nf) + 1j * rng.standard_normal(nf) for I in "HL"}
+    ss = {I: rng.uniform(0.5, 2, nf) for I in "HL"}
+    a = oracle.filter_ref(W, lo, n_time, dd, ss)
+    b = oracle.filter_fp64(W, lo, n_time, dd, ss)
+    loud = b["lqL"] > np.log(3.0)                   # away from the zeros of z
+    assert np.abs(a["lqL"] - b["lqL"])[loud].max() < 1e-5
+    dph = np.angle(np.exp(1j * (a["phL"].astype(float) - b["phL"])))[loud]
+    assert np.abs(dph).max() < 1e-5
+
+
+def test_window_max_brute_force(oracle):
+    rng = np.random.default_rng(3)
+    lq = rng.standard_normal((2, 50)).astype(np.float32)
+    got = oracle.window_max(lq, -4, 6)
+    ref = np.array([[max(lq[m, (t + k) % 50] for k in range(-4, 7)) for t in range(50)] for m in range(2)])
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_select_ref_ranks_the_float32_score(oracle):
+    nt, K, S = 400, 30, 50
+    rng = np.random.default_rng(9)
+    traj = {"lqH": rng.standard_normal((3, nt)).astype(np.float32),
+            "lqL": rng.standard_normal((3, nt)).astype(np.float32)}
+    tau = synth.lag_indices(3)
+    sel, lw = oracle.select_times_ref(traj, tau, K, S, seed=5)
+    pick = np.random.default_rng(5).integers(0, nt - K, size=(3, S))
+    for m in range(3):
+        mx = np.array([max(traj["lqH"][m, (t + k) % nt] for k in range(-3, 4)) for t in range(nt)], np.float32)
+        score = np.exp(np.float32(2) * traj["lqL"][m]) + np.exp(np.float32(2) * mx)
+        assert score.dtype == np.float32
+        top = sel[m][lw[m] == 0]
+        others = np.setdiff1d(np.arange(nt), top)
+        assert top.size == K and score[top].min() >= score[others].max()
+        np.testing.assert_array_equal(np.sort(others[pick[m]]), sel[m][lw[m] != 0])
+    # well separated scores: the float32 and the float64 rankings pick the same times
+    a = oracle.select_times(traj, tau, K, S, seed=5)
+    if (oracle.top_set_differences(traj, tau, K, S, seed=5) == 0).all():
+        np.testing.assert_array_equal(a[0], sel)
+
+
+def test_top_set_flips_on_config1_trajectories(oracle):
+    """how often the reference's float32 ranking leaves out a time of the float64 top set, on the
+    trajectories of the 16 s, 20-1024 Hz segment with its 64-template bank (n_loud 1024): a flip is a
+    near-tie at the K-th score, so it is rare"""
+    from exmarg_amd.search.segments import read_segments
+    seg = read_segments(synth.simulate_at_snr(1, 20261203, (30.0, 0.8, 0.1), 12.0, 16.0, 20.0, 1024.0)[0])
+    rows = synth.chirp_mass_grid_bank(30.0, 0.8, 0.1, 64)
+    W, u = synth.template_factors(rows[:, 0], rows[:, 1], rows[:, 2], seg)
+    lo, n_time = int(seg["nu"][0]), 8 * int(seg["nu"][-1])
+    traj = oracle.filter_fp64(W, lo, n_time, {I: seg["d_" + I][0] for I in "HL"},
+                              {I: seg["sig2_" + I] for I in "HL"})
+    flips = oracle.top_set_differences(traj, synth.lag_indices(43), 1024, 1024, seed=0)
+    print("top-1024 times left out by the float32 ranking, per template: total %d, max %d, templates %d of 64"
+          % (flips.sum(), flips.max(), (flips > 0).sum()))
+    assert flips.max() <= 4 and flips.sum() <= 32
+
+
+def test_gather_orders_agree_bit_for_bit(oracle):
+    # the checker's slab order and the reference's (template, lag) job order compute the same sums
+    traj = tiny_traj(seed=21)
+    T = table_from(lambda r, c, b, d: np.sin(0.7 * r + 0.3 * c) + 0.2 * b - 0.1 * d)
+    beta, delta = tiny_grids(T.shape[2], T.shape[3])
+    tau = synth.lag_indices(2)
+    tabs = oracle.Tables(tables=[T, T + 0.5], tau_idx=tau)
+    times = oracle.select_times(traj, tau, 8, 8, seed=2)
+    ids = np.arange(NM)
+    lna = np.array([[0.3, 0.1], [-1.2, 0.0], [2.0, 1.5]])
+    tab = np.array([[0, 1], [1, 0], [0, 1]])
+    a = oracle.gather_pairs(traj, tabs, beta, delta, ids, lna, tab, times, order="slab")
+    b = oracle.gather_pairs(traj, tabs, beta, delta, ids, lna, tab, times, order="reference")
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
