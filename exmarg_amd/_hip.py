@@ -47,6 +47,7 @@ SIGNATURES = {
     "exm_gather": [H, H, P, I64, P, P, I32, H, P],
     "exm_plan_create": [H, I32, I64, P, P, P, HP],
     "exm_plan_gather": [H, H, H, H, P],
+    "exm_plan_gather_noalloc": [H, H, H, H, P],
     "exm_batch_create": [H, I64, I32, HP],
     "exm_batch_add": [H, H, H, P, P, P, P, I32, I32, I32, I32, P, F32, ctypes.POINTER(ctypes.c_int)],
     "exm_batch_add_seeded": [H, H, H, P, P, P, P, I32, I32, I32, I32, P, F32, ctypes.POINTER(ctypes.c_int)],

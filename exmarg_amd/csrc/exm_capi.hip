@@ -109,6 +109,21 @@ void dev_free(void* p, size_t bytes) {
 
 int scratch(int slot, size_t bytes, void** out) {
     Context& c = g_ctx;
+    if (c.arena) {
+        Context::Arena& a = *c.arena;
+        if (a.slot_ptr[slot] && a.slot_size[slot] >= bytes) {
+            *out = a.slot_ptr[slot];
+            return EXM_OK;
+        }
+        const size_t need = (bytes + 255) & ~(size_t)255;
+        EXM_CHECK(need <= a.cap - a.used, EXM_ERR_NOMEM,
+                  "the no-allocation scratch of %zu bytes is full (%zu used, %zu more asked)", a.cap, a.used, need);
+        a.slot_ptr[slot] = a.base + a.used;
+        a.slot_size[slot] = need;
+        a.used += need;
+        *out = a.slot_ptr[slot];
+        return EXM_OK;
+    }
     if (c.scratch_size[slot] < bytes) {
         EXM_HIP(hipStreamSynchronize(c.stream));
         dev_free(c.scratch[slot], c.scratch_size[slot]);
@@ -315,6 +330,17 @@ static int get_plain(exm_handle h, Object** o) {
     return EXM_OK;
 }
 
+int check_times(Traj* traj, exm_handle htimes, Times** tm) {
+    *tm = nullptr;
+    if (htimes == 0) return EXM_OK;
+    Times* t = get<Times>(htimes, K_TIMES);
+    EXM_CHECK(t != nullptr, EXM_ERR_HANDLE, "handle %llu is not a Times", (unsigned long long)htimes);
+    EXM_CHECK(t->n_tmpl == traj->n_tmpl, EXM_ERR_ARG, "times are for %d templates, trajectories for %d",
+              t->n_tmpl, traj->n_tmpl);
+    *tm = t;
+    return EXM_OK;
+}
+
 }  // namespace exm
 
 using namespace exm;
@@ -344,6 +370,7 @@ int exm_init(int device) {
     c.num_cu = prop.multiProcessorCount;
     c.total_mem = (int64_t)prop.totalGlobalMem;
     snprintf(c.name, sizeof(c.name), "%s", prop.gcnArchName);
+    EXM_TRY(noalloc_init(device));
     c.device = device;
     return EXM_OK;
 }
@@ -742,16 +769,6 @@ int exm_times_read(exm_handle h, int32_t* sel, float* lw) {
 }
 
 // ---------------------------------------------------------------- gather
-static int check_times(Traj* traj, exm_handle htimes, Times** tm) {
-    *tm = nullptr;
-    if (htimes == 0) return EXM_OK;
-    Times* t = get<Times>(htimes, K_TIMES);
-    EXM_CHECK(t != nullptr, EXM_ERR_HANDLE, "handle %llu is not a Times", (unsigned long long)htimes);
-    EXM_CHECK(t->n_tmpl == traj->n_tmpl, EXM_ERR_ARG, "times are for %d templates, trajectories for %d",
-              t->n_tmpl, traj->n_tmpl);
-    *tm = t;
-    return EXM_OK;
-}
 
 int exm_gather(exm_handle htraj, exm_handle hstore, const int64_t* mass_ids, int64_t nsel, const double* lna,
                const int64_t* tab_ids, int ns, exm_handle htimes, double* out) {

@@ -192,6 +192,15 @@ struct Context {
     int num_cu = 0;
     int64_t total_mem = 0;
     char name[256] = {0};
+    // while set (inside the no-allocation entry point), scratch() carves its slots from this
+    // caller-independent device region instead of allocating: each slot once per call, 256-byte aligned
+    struct Arena {
+        char* base = nullptr;
+        size_t cap = 0, used = 0;
+        void* slot_ptr[NSCRATCH] = {};
+        size_t slot_size[NSCRATCH] = {};
+    };
+    Arena* arena = nullptr;
 };
 
 Context& ctx();
@@ -235,6 +244,13 @@ int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_
 int launch_draw(const uint64_t rng_state[4], int64_t n_out, int64_t high, int64_t* d_out);
 // ln Lambda of every job of js into d_out (table-sorted order)
 int launch_gather(Traj* traj, Store* store, const JobSet& js, Times* times, double* d_out);
+// scratch bytes launch_gather takes for these arguments (every slot rounded up to 256 bytes)
+size_t gather_scratch_bytes(const Traj* traj, const Store* store, const JobSet& js, const Times* times);
+size_t band_scratch_bytes(const Traj* traj, const Store* store, const JobSet& js, const Times* times);
+// the no-allocation scratch region of the current device (exm_noalloc.hip), found at exm_init
+int noalloc_init(int device);
+// the times object of a handle (NULL for handle 0: every arrival time), checked against the trajectories
+int check_times(Traj* traj, exm_handle htimes, Times** tm);
 int launch_gather_ref(Traj* traj, Store* store, const JobSet& js, Times* times, double* om, double* os);
 int launch_gather_band(Traj* traj, Store* store, const JobSet& js, Times* times, double* om, double* os);
 int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* stream_dst);

@@ -897,6 +897,15 @@ int launch_fill(double* dst, int64_t n, double v) {
     return EXM_OK;
 }
 
+// scratch of launch_gather_band: slot 4 (walk records), slot 5 (walk temporaries), slot 6 (row bounds)
+size_t band_scratch_bytes(const Traj* traj, const Store* st, const JobSet& js, const Times* times) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t nm = (size_t)traj->n_tmpl, nw = (size_t)times->n_walk;
+    const size_t slot4 = al(nm * nw * 4 + WALK_PAD_BYTES) + al(nm * 4) + sizeof(uint2) * nm * st->ntau * nw + WALK_PAD_BYTES;
+    const size_t slot5 = 2 * al(nm * nw * 4);
+    return al(slot4) + al(slot5) + al(piece_bounds_bytes(st, js.njobs));
+}
+
 int launch_gather_band(Traj* traj, Store* st, const JobSet& js, Times* times, double* om, double* os) {
     const int nm = traj->n_tmpl, nw = times->n_walk;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };

@@ -148,6 +148,13 @@ int launch_gather(Traj* traj, Store* st, const JobSet& js, Times* times, double*
     return launch_lse(js, st->ntau, om, os, nullptr, std::log((double)traj->n_time), d_out);
 }
 
+size_t gather_scratch_bytes(const Traj* traj, const Store* st, const JobSet& js, const Times* times) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t need = al(sizeof(double) * 2 * (size_t)js.njobs * st->ntau);          // slot 0: (max, sum) pairs
+    if (times != nullptr && times->n_walk <= 8192 && gather_mode() != 1) need += band_scratch_bytes(traj, st, js, times);
+    return need;
+}
+
 int launch_lse(const JobSet& js, int ntau, const double* om, const double* os, const double* ln_nt_g, double ln_nt,
                double* d_out) {
     hipLaunchKernelGGL(k_lse_lags, dim3((unsigned)((js.njobs + 3) / 4)), dim3(256), 0, cur_stream(), om, os,

@@ -157,6 +157,18 @@ int exm_plan_create(exm_handle store, int n_tmpl, int64_t njobs, const int32_t* 
  * the boundary reads of src/exmarg/search/pe.py:178-194 in one launch. */
 int exm_plan_gather(exm_handle plan, exm_handle traj, exm_handle store, exm_handle times, double* out);
 
+/* Bytes of the no-allocation scratch region (one per device, held from the library's load until the
+ * process ends). */
+#define EXM_NOALLOC_SCRATCH_BYTES (128u << 20)
+
+/* exm_plan_gather without any device allocation: the same kernels and results, every scratch buffer
+ * taken from the library's static per-device region of EXM_NOALLOC_SCRATCH_BYTES.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (tight serving loops, harnesses that
+ * forbid it).  A plan, trajectories and times whose gather needs more scratch than the region holds
+ * are refused with EXM_ERR_NOMEM before anything runs (at the reference geometry, 87 lags and 2048
+ * walked times, about 80 templates).  Calls on one device share the region: calls that overlap race. */
+int exm_plan_gather_noalloc(exm_handle plan, exm_handle traj, exm_handle store, exm_handle times, double* out);
+
 /* ---------------------------------------------------------------- batches
  * Many segments analysed together (no reference counterpart: the reference loops over segments in
  * src/exmarg/search/pe.py:186-194 with one launch and one device-to-host copy per node group and
