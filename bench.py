@@ -38,6 +38,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--n-loud", type=int, default=1024)
     p.add_argument("--n-random", type=int, default=1024)
+    p.add_argument("--config", type=int, choices=[1, 2], default=2,
+                   help="plain workload: 2 (2000-template grid bank, 53 nodes; the headline) or 1 (64-template "
+                        "chirp-mass grid, one node)")
     p.add_argument("--grid", default="20,10,10", help="bank grid (M, q, chi)")
     p.add_argument("--nodes", type=int, default=53)
     p.add_argument("--cpu-templates", type=int, default=32, help="templates in the CPU-baseline sample")
@@ -65,11 +68,25 @@ def parse():
 
 
 def workload(args):
+    """the segment and bank rows of the plain workload: config 2 (default) is a 2000-template grid bank over
+    every node; config 1 the 64-template chirp-mass grid of the reference's CPU-runnable case (its one
+    node is placed by plain_store)"""
     from exmarg_amd import synth
     seg, truth, snr = synth.simulate_at_snr(1, 20261203, (30.0, 0.8, 0.1), 12.0, 16.0, 20.0, 1024.0)
-    shape = tuple(int(v) for v in args.grid.split(","))
-    rows = synth.grid_bank(shape=shape)
+    if getattr(args, "config", 2) == 1:
+        rows = synth.chirp_mass_grid_bank(30.0, 0.8, 0.1, 64)
+    else:
+        rows = synth.grid_bank(shape=tuple(int(v) for v in args.grid.split(",")))
     return seg, rows, float(snr[0])
+
+
+def plain_store(args, bank):
+    """the tables of the plain workload: every node (config 2), or config 1's one node at s = 4 max u, so
+    that every template reaches it"""
+    from exmarg_amd.search import store as stm
+    if args.config == 1:
+        return stm.synthetic_store(seed=0, s_nodes=np.array([4.0 * bank.u.max()]))
+    return stm.synthetic_store(n_nodes=args.nodes, seed=0)
 
 
 def measured_traffic(kernel_ms):
@@ -607,8 +624,8 @@ def main():
     _hip.call("exm_set_filter_workspace", args.filter_mib << 20)
     _hip.call("exm_set_filter_precision", 1 if args.filter_precision == "f32" else 0)
     info = _hip.device_info()
-    st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
     bank = mass_bank(rows, seg, "hip")
+    st = plain_store(args, bank)
     kw = dict(prior="volume", ymin=0.1267, n_loud=args.n_loud, n_random=args.n_random, seed=0, resident=True)
     for _ in range(args.warmup):
         pe_intrinsic(seg, bank, st, **kw)
@@ -643,8 +660,9 @@ def main():
     F = bank.nu.size
     nm, nt = bank.u.size, bank.n_time
     seg_bytes = segment_bytes(F, nm, nt, n_nodes_used)
-    g_s = g_ms / 1e3 / max(g_n, 1)
-    n_jobs = int(valid.sum() + (bnode >= 0).sum())
+    g_s = g_ms / 1e3 / max(g_n, 1)                 # one gather launch
+    g_step = g_ms / 1e3 / args.steps                # every gather launch of a step (a walk longer than one
+    n_jobs = int(valid.sum() + (bnode >= 0).sum())  # launch takes, e.g. every arrival time, runs in slices)
     walk = args.n_loud + args.n_random if args.n_loud > 0 else nt
     lookups = n_jobs * st.tau_idx.size * walk
     achieved = seg_bytes / g_s / 1e9
@@ -658,22 +676,28 @@ def main():
                       n for n, v in (("transform", args.filter_precision), ("interpolation", args.precision))
                       if v == "f32")),
         "data": "synthetic",
-        "config": {"workload": "one 16 s segment, 20-1024 Hz, 2000-template (M_det, q, chi) bank, 53 HBM-resident "
-                               "table nodes, pe_intrinsic 'volume' prior, n_loud=n_random=1024",
+        "config": {"workload": "one 16 s segment, 20-1024 Hz, %s, pe_intrinsic 'volume' prior, %s"
+                               % ("2000-template (M_det, q, chi) bank, 53 HBM-resident table nodes" if args.config == 2
+                                  else "64-template chirp-mass grid bank, one table node",
+                                  "n_loud=%d, n_random=%d" % (args.n_loud, args.n_random) if args.n_loud > 0
+                                  else "every arrival time (n_loud=0)"),
                    "n_templates": nm, "n_time": nt, "n_freq": F, "nodes": n_nodes_used, "reads": n_jobs,
                    "injection_snr": round(snr, 3), "device": info["name"], "num_cu": info["num_cu"]},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "gather",
                      "bytes_per_launch": seg_bytes, "kernel_ms": 1e3 * g_s},
-        "stages_ms": {"filter": f_ms / max(f_n, 1), "select": s_ms / max(s_n, 1),
-                      "walk": w_ms / max(w_n, 1), "bounds": b_ms / max(b_n, 1), "gather": g_ms / max(g_n, 1)},
-        "lookups_per_s": lookups / g_s,
+        "stages_ms": {"filter": f_ms / args.steps, "select": s_ms / args.steps,
+                      "walk": w_ms / args.steps, "bounds": b_ms / args.steps, "gather": g_ms / args.steps},
+        "gather_launches_per_step": g_n / args.steps,
+        "lookups_per_step": lookups,
+        "lookups_per_s": lookups / g_step,
         "path_hbm_frac": seg_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS,
         "ln_b": float(res["ln_b"][0]),
     }
     trf = measured_traffic(1e3 * g_s)
-    if trf is not None and args.grid == "20,10,10" and args.nodes == 53 and walk == 2048 and args.precision == "f64":
+    if (trf is not None and args.config == 2 and args.grid == "20,10,10" and args.nodes == 53 and walk == 2048
+            and args.precision == "f64"):
         out["roofline"]["traffic"] = trf["traffic_bytes"]
         out["roofline"]["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this kernel on this "
                                              "workload, scaled by a 1 GiB copy (profiles/gather_traffic.json)")
@@ -688,7 +712,7 @@ def main():
                                                  "16s_2000tmpl_53nodes.txt)"}
     if f64_check is not None:
         out["float32_form_vs_float64"] = f64_check
-    if not args.no_cpu:
+    if not args.no_cpu and args.n_loud > 0 and args.config == 2:
         an_geometry = geometry
         cpu, err, nchk = cpu_leg(args, seg, bank, st, res, an_geometry, 0)
         out["cpu_baseline"] = cpu

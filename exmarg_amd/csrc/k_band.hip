@@ -47,18 +47,23 @@ __device__ __forceinline__ float order_key_float(uint32_t k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
+// One block per template: the walk's nw times in increasing ln|z_L| (an LDS bitonic sort of (key, index)
+// pairs).  The walk is positions off .. off + nw - 1 of the template's row of the selection (sel, lw with
+// row stride sel_stride), or, without a selection, the arrival times off .. off + nw - 1 themselves (log
+// weight 0): walks longer than one launch takes are cut into consecutive slices.
 __global__ __launch_bounds__(WALK_THREADS) void k_walk_sort(const float* __restrict__ lqL, int64_t nt,
                                                             const int32_t* __restrict__ sel,
-                                                            const float* __restrict__ lw, int nw, int nwp,
+                                                            const float* __restrict__ lw, int64_t sel_stride,
+                                                            int64_t off, int nw, int nwp,
                                                             float* __restrict__ wa, float* __restrict__ wl,
                                                             int32_t* __restrict__ wt) {
     extern __shared__ uint64_t skeys[];
     const int m = blockIdx.x;
     const int tid = threadIdx.x;
-    const int32_t* s = sel + (int64_t)m * nw;
+    const int32_t* s = sel ? sel + (int64_t)m * sel_stride + off : nullptr;
     const float* q = lqL + (int64_t)m * nt;
     for (int i = tid; i < nwp; i += WALK_THREADS)
-        skeys[i] = i < nw ? (((uint64_t)float_order_key(q[s[i]]) << 32) | (uint32_t)i) : ~0ull;
+        skeys[i] = i < nw ? (((uint64_t)float_order_key(q[s ? (int64_t)s[i] : off + i]) << 32) | (uint32_t)i) : ~0ull;
     __syncthreads();
     for (int size = 2; size <= nwp; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
@@ -78,9 +83,9 @@ __global__ __launch_bounds__(WALK_THREADS) void k_walk_sort(const float* __restr
     }
     for (int p = tid; p < nw; p += WALK_THREADS) {
         const int i = (int)(uint32_t)skeys[p];
-        const int t = s[i];
+        const int32_t t = s ? s[i] : (int32_t)(off + i);
         wa[(int64_t)m * nw + p] = q[t];
-        wl[(int64_t)m * nw + p] = lw[(int64_t)m * nw + i];
+        wl[(int64_t)m * nw + p] = s ? lw[(int64_t)m * sel_stride + off + i] : 0.0f;
         wt[(int64_t)m * nw + p] = t;
     }
 }
@@ -272,6 +277,14 @@ __device__ __forceinline__ void row_to_lds(const float* __restrict__ src, float*
     }
 }
 
+// Combine (m2, s2) into the running (m, s) of a log-sum-exp: one exp of the non-positive difference.
+__device__ __forceinline__ void lse_merge(double& m, double& s, double m2, double s2) {
+    const double d = m2 - m;
+    const double e = (double)__expf(-(float)fabs(d));
+    s = d > 0.0 ? fma(s, e, s2) : fma(s2, e, s);
+    m = fmax(m, m2);
+}
+
 __device__ __forceinline__ double vfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float vfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
@@ -322,7 +335,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     const float* __restrict__ T, int64_t tstride, int nphif, int nb, int nd, int ntau,
     const int32_t* __restrict__ jm, const int32_t* __restrict__ jtab, const double* __restrict__ joffb,
     const int32_t* __restrict__ tiles, const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds,
-    double rdb, double offd, double rdd, double* __restrict__ om, double* __restrict__ os) {
+    double rdb, double offd, double rdd, double* __restrict__ om, double* __restrict__ os, int accumulate) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int rowlen = nphif * nd, slot = row_slot(rowlen);
     float* rows = (float*)lds;                                              // 3 ring slots
@@ -498,8 +511,10 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
         __syncthreads();
     }
     if (tid < nj) {
-        om[(int64_t)(j0 + tid) * ntau + r] = amx;
-        os[(int64_t)(j0 + tid) * ntau + r] = asum;
+        const int64_t o = (int64_t)(j0 + tid) * ntau + r;
+        if (accumulate) lse_merge(amx, asum, om[o], os[o]);
+        om[o] = amx;
+        os[o] = asum;
     }
 }
 
@@ -519,13 +534,6 @@ __device__ __forceinline__ void load_piece(const float* __restrict__ wp, const u
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 __device__ __forceinline__ int shfl_i(int v, int src) { return __shfl(v, src, 64); }
 
-// Combine (m2, s2) into the running (m, s) of a log-sum-exp: one exp of the non-positive difference.
-__device__ __forceinline__ void lse_merge(double& m, double& s, double m2, double s2) {
-    const double d = m2 - m;
-    const double e = (double)__expf(-(float)fabs(d));
-    s = d > 0.0 ? fma(s, e, s2) : fma(s2, e, s);
-    m = fmax(m, m2);
-}
 
 // lane l receives lane l - n of its row of 16, or 0 where l - n lies outside the row (every lane
 // written, so no previous value is needed)
@@ -558,7 +566,7 @@ __global__ __launch_bounds__(WAVE_THREADS) void k_gather_wave(
     const float* __restrict__ T, int64_t tstride, int nphif, int nb, int nd, int ntau,
     const int32_t* __restrict__ jm, const int32_t* __restrict__ jtab, const double* __restrict__ joffb,
     const int32_t* __restrict__ tiles, const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds,
-    double rdb, double offd, double rdd, double* __restrict__ om, double* __restrict__ os) {
+    double rdb, double offd, double rdd, double* __restrict__ om, double* __restrict__ os, int accumulate) {
     extern __shared__ __align__(16) unsigned char lds[];
     constexpr int NWAVE = WAVE_THREADS / 64;
     const int rowlen = nphif * nd, slot = row_slot(rowlen);
@@ -784,28 +792,36 @@ __global__ __launch_bounds__(WAVE_THREADS) void k_gather_wave(
         n_hi = f_hi;
     }
     if (mine) {
-        om[((int64_t)j0 + jl) * ntau + r] = amx;
-        os[((int64_t)j0 + jl) * ntau + r] = asum;
+        const int64_t o = ((int64_t)j0 + jl) * ntau + r;
+        if (accumulate) lse_merge(amx, asum, om[o], os[o]);
+        om[o] = amx;
+        os[o] = asum;
     }
 }
 
-// walk records of one segment into wa_dst (n_tmpl, nw) and stream_dst (n_tmpl, ntau, nw)
-int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* stream_dst) {
+// walk records of one segment into wa_dst (n_tmpl, nw) and stream_dst (n_tmpl, ntau, nw): positions
+// off .. off + nw - 1 of every template's selection (times), or the arrival times off .. off + nw - 1
+// (times NULL)
+int launch_walk(Traj* traj, Times* times, int64_t off, int nw, Store* st, float* wa_dst, uint2* stream_dst) {
     Context& c = ctx();
-    const int nm = traj->n_tmpl, nw = times->n_walk;
-    EXM_CHECK(nw <= 65535, EXM_ERR_ARG, "walk of %d times exceeds the 16-bit row bounds", nw);
+    const int nm = traj->n_tmpl;
+    EXM_CHECK(nw > 0 && nw <= 65535, EXM_ERR_ARG, "walk of %d times exceeds the 16-bit row bounds", nw);
+    EXM_CHECK(off >= 0 && off + nw <= (times ? (int64_t)times->n_walk : traj->n_time), EXM_ERR_ARG,
+              "walk slice [%lld, %lld) outside the walk", (long long)off, (long long)(off + nw));
     int nwp = 1;
     while (nwp < nw) nwp <<= 1;
+    const size_t lds = sizeof(uint64_t) * nwp;
+    EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "walk slice of %d times exceeds the LDS sort", nw);
     const size_t walk_bytes = ((size_t)nm * nw * 4 + 255) & ~(size_t)255;
     void* wbuf = nullptr;
     EXM_TRY(scratch(5, 2 * walk_bytes, &wbuf));
     float* wl = (float*)wbuf;
     int32_t* wt = (int32_t*)((char*)wbuf + walk_bytes);
     ProfileScope prof("walk");
-    const size_t lds = sizeof(uint64_t) * nwp;
     EXM_HIP(hipFuncSetAttribute((const void*)k_walk_sort, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_walk_sort, dim3(nm), dim3(WALK_THREADS), lds, c.stream, traj->lqL(), traj->n_time,
-                       times->d_sel, times->d_lw, nw, nwp, wa_dst, wl, wt);
+                       times ? times->d_sel : nullptr, times ? times->d_lw : nullptr,
+                       times ? (int64_t)times->n_walk : (int64_t)0, off, nw, nwp, wa_dst, wl, wt);
     EXM_HIP(hipGetLastError());
     const size_t ws_lds = walk_stream_lds_bytes(st->ntau);
     EXM_CHECK(ws_lds <= 160 * 1024, EXM_ERR_ARG, "walk stream of %d lags exceeds the LDS", st->ntau);
@@ -816,6 +832,10 @@ int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* strea
     return EXM_OK;
 }
 
+int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* stream_dst) {
+    return launch_walk(traj, times, 0, times->n_walk, st, wa_dst, stream_dst);
+}
+
 // piece gather of every read of js; template ids index wa, stream and lw_g
 size_t piece_bounds_bytes(const Store* st, int64_t njobs) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -823,7 +843,7 @@ size_t piece_bounds_bytes(const Store* st, int64_t njobs) {
 }
 
 int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const float* lw_g, const JobSet& js,
-                 double* om, double* os, void* bounds_buf) {
+                 double* om, double* os, void* bounds_buf, bool accumulate) {
     Context& c = ctx();
     const hipStream_t cs = cur_stream();
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -868,7 +888,8 @@ int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const 
     hipLaunchKernelGGL(kern, dim3(js.ntiles, st->ntau), dim3(wave ? WAVE_THREADS : BAND_THREADS), lds, cs, stream,
                        wa, nw, lw_g,
                        st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab, js.d_offb,
-                       js.d_tiles, first_last, bounds, rdb, -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os);
+                       js.d_tiles, first_last, bounds, rdb, -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os,
+                       accumulate ? 1 : 0);
     EXM_HIP(hipGetLastError());
     return EXM_OK;
 }
@@ -897,17 +918,33 @@ int launch_fill(double* dst, int64_t n, double v) {
     return EXM_OK;
 }
 
+// walked times per launch of the band gather: the whole walk when it is short; longer walks (every
+// arrival time, or selections of more than 8192 times) in slices of a power of two of at most 8192 times
+// whose walk records take at most 4 GiB
+int band_slice(const Traj* traj, const Store* st, const Times* times) {
+    const int64_t total = times ? (int64_t)times->n_walk : traj->n_time;
+    if (total <= 8192) return (int)total;
+    const int64_t per = (int64_t)traj->n_tmpl * st->ntau * (int64_t)sizeof(uint2);
+    int c = 8192;
+    while (c > 256 && per * c > ((int64_t)4 << 30)) c >>= 1;
+    return c;
+}
+
 // scratch of launch_gather_band: slot 4 (walk records), slot 5 (walk temporaries), slot 6 (row bounds)
 size_t band_scratch_bytes(const Traj* traj, const Store* st, const JobSet& js, const Times* times) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t nm = (size_t)traj->n_tmpl, nw = (size_t)times->n_walk;
+    const size_t nm = (size_t)traj->n_tmpl, nw = (size_t)band_slice(traj, st, times);
     const size_t slot4 = al(nm * nw * 4 + WALK_PAD_BYTES) + al(nm * 4) + sizeof(uint2) * nm * st->ntau * nw + WALK_PAD_BYTES;
     const size_t slot5 = 2 * al(nm * nw * 4);
     return al(slot4) + al(slot5) + al(piece_bounds_bytes(st, js.njobs));
 }
 
+// the band gather of one segment: the walk (the selection, or every arrival time when times is NULL) in
+// slices of band_slice() times, each slice's (max, sum) per read and lag merged into om, os
 int launch_gather_band(Traj* traj, Store* st, const JobSet& js, Times* times, double* om, double* os) {
-    const int nm = traj->n_tmpl, nw = times->n_walk;
+    const int nm = traj->n_tmpl;
+    const int64_t total = times ? (int64_t)times->n_walk : traj->n_time;
+    const int nw = band_slice(traj, st, times);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t wa_bytes = al((size_t)nm * nw * 4 + WALK_PAD_BYTES), lw_bytes = al((size_t)nm * 4);
     const size_t st_bytes = sizeof(uint2) * (size_t)nm * st->ntau * nw + WALK_PAD_BYTES;
@@ -916,9 +953,13 @@ int launch_gather_band(Traj* traj, Store* st, const JobSet& js, Times* times, do
     float* wa = (float*)buf;
     float* lw_g = (float*)((char*)buf + wa_bytes);
     uint2* stream = (uint2*)((char*)buf + wa_bytes + lw_bytes);
-    EXM_TRY(launch_fill(lw_g, nm, times->lw_rand));
-    EXM_TRY(launch_walk(traj, times, st, wa, stream));
-    return launch_piece(st, stream, wa, nw, lw_g, js, om, os);
+    EXM_TRY(launch_fill(lw_g, nm, times ? times->lw_rand : 0.0f));
+    for (int64_t off = 0; off < total; off += nw) {
+        const int n = (int)std::min<int64_t>(nw, total - off);
+        EXM_TRY(launch_walk(traj, times, off, n, st, wa, stream));
+        EXM_TRY(launch_piece(st, stream, wa, n, lw_g, js, om, os, nullptr, off > 0));
+    }
+    return EXM_OK;
 }
 
 }  // namespace exm

@@ -138,8 +138,7 @@ int launch_gather(Traj* traj, Store* st, const JobSet& js, Times* times, double*
     double* om = pairs;
     double* os = pairs + (size_t)js.njobs * st->ntau;
     const int mode = gather_mode();
-    const bool band_ok = times != nullptr && times->n_walk <= 8192;
-    if (band_ok && mode != 1) {
+    if (mode != 1) {
         EXM_TRY(launch_gather_band(traj, st, js, times, om, os));
     } else {
         ProfileScope prof("gather");
@@ -151,7 +150,7 @@ int launch_gather(Traj* traj, Store* st, const JobSet& js, Times* times, double*
 size_t gather_scratch_bytes(const Traj* traj, const Store* st, const JobSet& js, const Times* times) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t need = al(sizeof(double) * 2 * (size_t)js.njobs * st->ntau);          // slot 0: (max, sum) pairs
-    if (times != nullptr && times->n_walk <= 8192 && gather_mode() != 1) need += band_scratch_bytes(traj, st, js, times);
+    if (gather_mode() != 1) need += band_scratch_bytes(traj, st, js, times);
     return need;
 }
 
