@@ -77,11 +77,17 @@ def config2():
 
 
 def config3():
-    out = {}
+    # one segment length at a time, each kept in config3.partial.npz until the file is complete
+    part = os.path.join(HERE, "config3.partial.npz")
+    out = dict(np.load(part)) if os.path.exists(part) else {}
     rows = synth.grid_bank(shape=(20, 10, 10))
     for seg, T in bench.batch_items(256, 7):
+        if "ln_b_%d" % int(T) in out:
+            continue
         lnl, lnb = intrinsic(one_row(seg, 0), rows, 0)
         out["ln_lambda_%d" % int(T)], out["ln_b_%d" % int(T)] = lnl, lnb
+        np.savez(part + ".tmp.npz", **out)
+        os.replace(part + ".tmp.npz", part)
     return out
 
 
@@ -122,6 +128,9 @@ def main(names):
         data = globals()[name]()
         np.savez(path + ".tmp.npz", **data)
         os.replace(path + ".tmp.npz", path)
+        part = os.path.join(HERE, name + ".partial.npz")
+        if os.path.exists(part):
+            os.remove(part)
         print(name, "written", flush=True)
 
 
