@@ -887,7 +887,8 @@ int exm_batch_create(exm_handle hstore, int64_t max_templates, int n_walk, exm_h
     const size_t wa_bytes = al(sizeof(float) * (size_t)max_templates * n_walk + WALK_PAD_BYTES);
     const size_t lw_bytes = al(sizeof(float) * (size_t)max_templates);
     const size_t nt_bytes = al(sizeof(double) * (size_t)max_templates);
-    b->bytes = s_bytes + wa_bytes + lw_bytes + nt_bytes;
+    const size_t ah_bytes = al(sizeof(float) * (size_t)max_templates * st->ntau * n_walk);
+    b->bytes = s_bytes + wa_bytes + lw_bytes + nt_bytes + ah_bytes;
     int rc = dev_alloc((void**)&b->d_stream, b->bytes);
     if (rc != EXM_OK) {
         delete b;
@@ -897,6 +898,7 @@ int exm_batch_create(exm_handle hstore, int64_t max_templates, int n_walk, exm_h
     b->d_wa = (float*)(base + s_bytes);
     b->d_lw = (float*)(base + s_bytes + wa_bytes);
     b->d_lnnt = (double*)(base + s_bytes + wa_bytes + lw_bytes);
+    b->d_aH = (float*)(base + s_bytes + wa_bytes + lw_bytes + nt_bytes);
     *out = register_object(b);
     return EXM_OK;
 }
@@ -944,7 +946,8 @@ static int batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, cons
     if (rc == EXM_OK && !pick) rc = device_draw(rng_state, (int64_t)bank->n_tmpl * S, nrest, &d_pick);
     if (rc == EXM_OK) rc = launch_select(&tr, lag_lo, lag_hi, K, S, pick, d_pick, lw_rand, &tm);
     if (rc == EXM_OK)
-        rc = launch_walk(&tr, &tm, st, b->d_wa + b->used * b->nw, b->d_stream + b->used * b->ntau * b->nw);
+        rc = launch_walk(&tr, &tm, st, b->d_wa + b->used * b->nw, b->d_stream + b->used * b->ntau * b->nw,
+                         b->d_aH + b->used * b->ntau * b->nw);
     if (rc == EXM_OK) rc = launch_fill(b->d_lw + b->used, bank->n_tmpl, lw_rand);
     if (rc == EXM_OK) rc = launch_fill(b->d_lnnt + b->used, bank->n_tmpl, std::log((double)bank->n_time));
     // the host-side trajectory and times objects only borrow scratch memory
@@ -1039,7 +1042,7 @@ int exm_batch_launch(exm_handle hbatch) {
         double* om = (double*)(w + bnd);
         double* os = om + (size_t)njobs * st->ntau;
         double* dout = (double*)(w + bnd + pairs);
-        int rc = launch_piece(st, b->d_stream, b->d_wa, b->nw, b->d_lw, b->js, om, os, w);
+        int rc = launch_piece(st, b->d_stream, b->d_aH, b->d_wa, b->nw, b->d_lw, b->js, om, os, w);
         if (rc == EXM_OK) rc = launch_lse(b->js, st->ntau, om, os, b->d_lnnt, 0.0, dout);
         if (rc != EXM_OK) {
             (void)hipStreamSynchronize(b->stream);

@@ -144,7 +144,8 @@ struct Batch : Object {
     exm_handle store = 0;
     int nw = 0, ntau = 0;
     int64_t cap = 0, used = 0;          // templates (over all admitted segments)
-    uint2* d_stream = nullptr;           // (cap, ntau, nw)
+    uint2* d_stream = nullptr;           // (cap, ntau, nw) walk records
+    float* d_aH = nullptr;               // (cap, ntau, nw) ln|z_H| of the records (rows where beta may clamp)
     float* d_wa = nullptr;               // (cap, nw)
     float* d_lw = nullptr;               // (cap) drawn-time log weight per template
     double* d_lnnt = nullptr;            // (cap) ln n_time per template
@@ -253,14 +254,15 @@ int noalloc_init(int device);
 int check_times(Traj* traj, exm_handle htimes, Times** tm);
 int launch_gather_ref(Traj* traj, Store* store, const JobSet& js, Times* times, double* om, double* os);
 int launch_gather_band(Traj* traj, Store* store, const JobSet& js, Times* times, double* om, double* os);
-int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* stream_dst);
-int launch_walk(Traj* traj, Times* times, int64_t off, int nw, Store* st, float* wa_dst, uint2* stream_dst);
+int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* stream_dst, float* aH_dst);
+int launch_walk(Traj* traj, Times* times, int64_t off, int nw, Store* st, float* wa_dst, uint2* stream_dst,
+                float* aH_dst);
 int band_slice(const Traj* traj, const Store* st, const Times* times);
 // bounds_buf: piece_bounds_bytes() of device memory for the row bounds (scratch slot 6 when null)
 size_t piece_bounds_bytes(const Store* st, int64_t njobs);
 // accumulate: merge this walk's (max, sum) into om, os instead of writing them
-int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const float* lw_g, const JobSet& js,
-                 double* om, double* os, void* bounds_buf = nullptr, bool accumulate = false);
+int launch_piece(Store* st, const uint2* stream, const float* aHs, const float* wa, int nw, const float* lw_g,
+                 const JobSet& js, double* om, double* os, void* bounds_buf = nullptr, bool accumulate = false);
 // LSE over the lags of every read; ln n_time per template (ln_nt_g) or one value for all (ln_nt)
 int launch_lse(const JobSet& js, int ntau, const double* om, const double* os, const double* ln_nt_g, double ln_nt,
                double* d_out);

@@ -90,44 +90,61 @@ __global__ __launch_bounds__(WALK_THREADS) void k_walk_sort(const float* __restr
     }
 }
 
-// One block per (template, 32 walked times).  The records are computed with the lanes over the lags, so
-// that the z_H reads of one walked time (at t + tau for consecutive lags) are contiguous, kept in LDS
-// (two planes of ntau rows, odd stride so that neither access pattern conflicts on the banks) and
-// written with the lanes over the walked times (one contiguous run per lag, two lags per wave).  The
-// small tile keeps the LDS per block at 24 KB at 87 lags, six blocks per CU (1.54 ms for 2000 templates
-// x 2048 walked times x 87 lags, against 2.07 ms with 64 walked times per block and 1.58 ms with 16).
+// One block per (template, 32 walked times).  For every lag and walked time it writes the walk record
+// (8 bytes; layout below) and ln|z_H(t + tau)| (4 bytes, read only in the rows where beta may be clamped).
+// The records are computed with the lanes over the lags, so that the z_H reads of one walked time (at
+// t + tau for consecutive lags) are contiguous, kept in LDS (three planes of ntau rows, odd stride so that
+// neither access pattern conflicts on the banks) and written with the lanes over the walked times (one
+// contiguous run per lag, two lags per wave).  The small tile keeps the LDS per block at 35 KB at 87 lags.
+//
+// Walk record, bits 0-63 of (x, y): the Delta fraction of the lookup in units of 2^-24 (bits 0-23), the
+// LDS offset of its lower (Delta, phase) corner in a table row, d0 * NPHIF + lower stored phase bin
+// (bits 24-37), the weight of the upper phase bin in units of 2^-24 (bits 38-62) and the drawn flag (bit
+// 63).  Delta = ln|z_H(t + tau)| - ln|z_L(t)| does not depend on the node as long as beta is not clamped,
+// which only happens in the first and the last row pair of a table: every other row reads the record as
+// it is; those two rows recompute Delta from ln|z_H| (k_gather_edge).
 constexpr int WS_TILE = 32, WS_STRIDE = WS_TILE + 1;
+constexpr uint32_t FD_MASK = 0xFFFFFFu;           // 24 bits of Delta fraction
+constexpr int REC_O_SHIFT = 24;                   // the corner offset starts at bit 24 (of 64)
+constexpr uint32_t REC_O_MASK = 0x3FFFu;          // 14 bits: rows of at most 16384 floats
+constexpr int REC_W_SHIFT = 6;                    // the phase weight starts at bit 38 = 32 + 6
+constexpr uint32_t REC_DRAWN = 0x80000000u;       // bit 63
 
-// LDS of the walk stream block: the block's walked times, their phi_L and flags, the lag offsets, then
-// the two record planes
+// LDS of the walk stream block: the block's walked times, their phi_L, ln|z_L| and flags, the lag
+// offsets, then the three planes
 __host__ __device__ constexpr size_t walk_stream_lds_bytes(int ntau) {
-    return 3 * sizeof(uint32_t) * WS_TILE + sizeof(int32_t) * ntau + 2 * sizeof(uint32_t) * ntau * WS_STRIDE;
+    return 4 * sizeof(uint32_t) * WS_TILE + sizeof(int32_t) * ntau + 3 * sizeof(uint32_t) * ntau * WS_STRIDE;
 }
 
 __global__ __launch_bounds__(256) void k_walk_stream(const float* __restrict__ lqH, const float* __restrict__ phH,
                                                      const float* __restrict__ phL, int64_t nt,
                                                      const int32_t* __restrict__ tau, int ntau,
                                                      const int32_t* __restrict__ wt, const float* __restrict__ wl,
-                                                     int nw, uint2* __restrict__ stream) {
+                                                     const float* __restrict__ wa, int nw, double rdd, double offd,
+                                                     int nd, uint2* __restrict__ stream, float* __restrict__ aHs) {
     extern __shared__ uint32_t wsh[];
     int32_t* wts = (int32_t*)wsh;                       // the block's walked times
     float* pLs = (float*)(wsh + WS_TILE);               // their phi_L
     uint32_t* drs = wsh + 2 * WS_TILE;                  // their drawn flags
-    int32_t* tau0 = (int32_t*)(wsh + 3 * WS_TILE);      // lag offsets in [0, nt)
-    uint32_t* wcode = wsh + 3 * WS_TILE + ntau;
-    uint32_t* waH = wcode + ntau * WS_STRIDE;
+    float* aLs = (float*)(wsh + 3 * WS_TILE);           // their ln|z_L|
+    int32_t* tau0 = (int32_t*)(wsh + 4 * WS_TILE);      // lag offsets in [0, nt)
+    uint32_t* rx = wsh + 4 * WS_TILE + ntau;
+    uint32_t* ry = rx + ntau * WS_STRIDE;
+    uint32_t* waH = ry + ntau * WS_STRIDE;
     const int m = blockIdx.x, p0 = blockIdx.y * WS_TILE;
     const int np = min(WS_TILE, nw - p0);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int64_t base = (int64_t)m * nt;
     const double rph = NPHI / 6.283185307179586;
+    const double xdmax = nd - 1 - 1e-6;
     for (int r = threadIdx.x; r < ntau; r += 256) tau0[r] = (int32_t)(((int64_t)tau[r] % nt + nt) % nt);
     if (threadIdx.x < np) {
         const int p = p0 + threadIdx.x;
         const int32_t t = wt[(int64_t)m * nw + p];
         wts[threadIdx.x] = t;
         pLs[threadIdx.x] = phL[base + t];
-        drs[threadIdx.x] = wl[(int64_t)m * nw + p] != 0.0f ? DRAWN : 0u;
+        aLs[threadIdx.x] = wa[(int64_t)m * nw + p];
+        drs[threadIdx.x] = wl[(int64_t)m * nw + p] != 0.0f ? REC_DRAWN : 0u;
     }
     __syncthreads();
     // per chunk of 64 lags: the wave's walked times' phi_H and ln|z_H| are all requested, then the
@@ -153,6 +170,9 @@ __global__ __launch_bounds__(256) void k_walk_stream(const float* __restrict__ l
         for (int k = 0; k < PPW; ++k) {
             const int pl = wid + 4 * k;
             if (!(rv && pl < np)) continue;
+            // phase (gather.py:54-65): the stored bins of c0 and c0 + 1 (mod 96) are adjacent, ci1 = ci0 + 1
+            // below the mirror and ci0 - 1 from it on; the record holds the lower bin of the pair and the
+            // weight of the upper one (fc, or 1 - fc when c0's stored bin is the upper)
             const float dph = __fsub_rn(phv[k], pLs[pl]);
             double x3 = (double)dph * rph;
             if (x3 < 0.0) x3 += NPHI;
@@ -162,22 +182,31 @@ __global__ __launch_bounds__(256) void k_walk_stream(const float* __restrict__ l
             const double fc = x3 - c0;
             uint32_t fq = (uint32_t)__double2uint_rn(fc * FC_SCALE);
             fq = fq > (1u << 24) - 1 ? (1u << 24) - 1 : fq;
-            // stored bins of c0 and c0 + 1 (mod 96) are adjacent: ci1 = ci0 + 1 below the mirror, ci0 - 1
-            // from it on.  The record holds the lower bin of the pair and the weight of the upper one (fc,
-            // or 1 - fc when c0's stored bin is the upper).
             const uint32_t ci0 = (uint32_t)(c0 > NPHI / 2 ? NPHI - c0 : c0);
             const bool down = c0 >= NPHI / 2;
             const uint32_t lo = down ? ci0 - 1 : ci0;
             const uint32_t wq = down ? (1u << 24) - fq : fq;
-            wcode[r * WS_STRIDE + pl] = drs[pl] | (lo << BIN_SHIFT) | wq;
+            // Delta with beta unclamped (gather.py:67-78): the coordinate, its column and its fraction in
+            // 24-bit fixed point (at most 2^24 - 1, so that column d0 + 1 stays in the row)
+            double xd = ((double)aHv[k] - (double)aLs[pl]) * rdd + offd;
+            xd = xd > 0.0 ? (xd < xdmax ? xd : xdmax) : 0.0;               // NaN -> 0
+            const int d0 = (int)xd;
+            uint32_t fdq = (uint32_t)__double2uint_rn((xd - d0) * FC_SCALE);
+            fdq = fdq > FD_MASK ? FD_MASK : fdq;
+            const uint32_t o = (uint32_t)d0 * NPHIF + lo;
+            rx[r * WS_STRIDE + pl] = fdq | (o << REC_O_SHIFT);
+            ry[r * WS_STRIDE + pl] = (o >> (32 - REC_O_SHIFT)) | (wq << REC_W_SHIFT) | drs[pl];
             waH[r * WS_STRIDE + pl] = __float_as_uint(aHv[k]);
         }
     }
     __syncthreads();
     const int pl = lane & (WS_TILE - 1);
     for (int r = 2 * wid + (lane >> 5); r < ntau; r += 8)
-        if (pl < np)
-            stream[((int64_t)m * ntau + r) * nw + p0 + pl] = make_uint2(wcode[r * WS_STRIDE + pl], waH[r * WS_STRIDE + pl]);
+        if (pl < np) {
+            const int64_t o = ((int64_t)m * ntau + r) * nw + p0 + pl;
+            stream[o] = make_uint2(rx[r * WS_STRIDE + pl], ry[r * WS_STRIDE + pl]);
+            aHs[o] = __uint_as_float(waH[r * WS_STRIDE + pl]);
+        }
 }
 
 // DPP lane moves within a wave64 (VALU, no LDS round trip): lane l receives the value of lane l - n of
@@ -260,6 +289,9 @@ __host__ __device__ constexpr size_t piece_res_offset(int rowlen) {
     return ((size_t)12 * row_slot(rowlen) + 15) & ~(size_t)15;
 }
 
+// byte offset of the edge kernel's band range after its two row slots
+__host__ __device__ constexpr size_t edge_band_offset(int rowlen) { return (size_t)8 * row_slot(rowlen); }
+
 // bytes of LDS the piece kernel uses for a row of rowlen floats
 __host__ __device__ constexpr size_t piece_lds_bytes(int rowlen) {
     return piece_res_offset(rowlen) + 2 * sizeof(double) * BAND_THREADS + sizeof(int) * BAND_THREADS +
@@ -294,37 +326,60 @@ struct BandGeom {
     int nd;
 };
 
-// One lookup of App. B (the reference's gather.py:45-98) with the walk record (aL, e) of a walked time
-// whose beta row is b (R0 row b, R1 row b + 1 in LDS): float64 index and fraction arithmetic, the
-// bilinear (beta, Delta) interpolation at both phase bins and the phase interpolation in V.  A row is
-// stored Delta-major, [Delta][phase bin], so the four values of a row that one lookup reads (Delta
-// columns d0 and d0 + 1, stored bins lo and lo + 1) sit at o, o + 1, o + NPHIF and o + NPHIF + 1 from
-// one address: two ds_read2 per row.
+// The bilinear (beta, Delta) interpolation at both phase bins and the phase interpolation of one lookup
+// from rows R0 (row b) and R1 (row b + 1).  A row is stored Delta-major, [Delta][phase bin], so the four
+// values of a row that one lookup reads (Delta columns d0 and d0 + 1, stored bins lo and lo + 1) sit at
+// o, o + 1, o + NPHIF and o + NPHIF + 1 from one address: two ds_read2 per row.
 template <typename V>
-__device__ __forceinline__ double band_lookup(const float* R0, const float* R1, const BandGeom& g, double bd,
-                                              double offb, double cj, double lw_rand, float aLf, uint2 e) {
-    const double aL = (double)aLf;
-    const uint32_t code = e.x;
-    // beta coordinate (clamped) and its fraction in row b
-    const double xb = fmin(fmax(fma(aL, g.rdb, offb), 0.0), g.xbmax);
-    const double fb = xb - bd;
-    // Delta coordinate: ln q_L moves with the beta clamp, ln q_H stays
-    // xd lies in [0, nd - 1 - 1e-6] (NaN included: fmax returns the number), so d0 = floor(xd) is in
-    // [0, nd - 2] and its fraction xd - d0 is exact
-    const double xd = fmin(fmax(fma(-g.qbd, xb, fma((double)__uint_as_float(e.y), g.rdd, cj)), 0.0), g.xdmax);
-    const int d0 = (int)xd;
-    const double fd = __builtin_amdgcn_fract(xd);
-    // weight of the upper bin of the pair (exact: at most 25 significant bits)
-    const V wc = (V)(code & FC_MASK) * (V)(1.0 / FC_SCALE);
-    const int o = (int)__umul24((uint32_t)d0, (uint32_t)NPHIF) + (int)((code >> BIN_SHIFT) & 0x3F);
-    const V fdv = (V)fd, fbv = (V)fb;
+__device__ __forceinline__ V interp(const float* R0, const float* R1, int o, V fd, V fb, V wc) {
     const V a00 = R0[o], a01 = R0[o + 1], a10 = R0[o + NPHIF], a11 = R0[o + NPHIF + 1];
     const V b00 = R1[o], b01 = R1[o + 1], b10 = R1[o + NPHIF], b11 = R1[o + NPHIF + 1];
     // Delta, then beta, at the lower (x0) and upper (x1) bin, then the phase
-    const V r0l = vfma(fdv, a10 - a00, a00), r0u = vfma(fdv, a11 - a01, a01);
-    const V r1l = vfma(fdv, b10 - b00, b00), r1u = vfma(fdv, b11 - b01, b01);
-    const V vl = vfma(fbv, r1l - r0l, r0l), vu = vfma(fbv, r1u - r0u, r0u);
-    return (double)vfma(wc, vu - vl, vl) + ((code & DRAWN) ? lw_rand : 0.0);
+    const V r0l = vfma(fd, a10 - a00, a00), r0u = vfma(fd, a11 - a01, a01);
+    const V r1l = vfma(fd, b10 - b00, b00), r1u = vfma(fd, b11 - b01, b01);
+    const V vl = vfma(fb, r1l - r0l, r0l), vu = vfma(fb, r1u - r0u, r0u);
+    return vfma(wc, vu - vl, vl);
+}
+
+// the walk record's fields
+__device__ __forceinline__ int rec_offset(uint2 e) {
+    return (int)(__builtin_amdgcn_alignbit(e.y, e.x, REC_O_SHIFT) & REC_O_MASK);
+}
+__device__ __forceinline__ uint32_t rec_fd(uint2 e) { return e.x & FD_MASK; }
+__device__ __forceinline__ uint32_t rec_wc(uint2 e) { return (e.y >> REC_W_SHIFT) & FC_MASK; }
+
+// One lookup of an interior row b (beta not clamped there): the record carries Delta's column and fraction
+// and the phase pair; only the beta fraction depends on the read, xb - b = aL rdb + (offb - b), formed in
+// float64 (cb = offb - b).  Returns the interpolated value (without the drawn-time log weight).
+template <typename V>
+__device__ __forceinline__ V lean_lookup(const float* R0, const float* R1, double rdb, double cb, float aLf,
+                                         uint2 e) {
+    const V fb = (V)fma((double)aLf, rdb, cb);
+    const V fd = (V)rec_fd(e) * (V)(1.0 / FC_SCALE);
+    const V wc = (V)rec_wc(e) * (V)(1.0 / FC_SCALE);
+    return interp<V>(R0, R1, rec_offset(e), fd, fb, wc);
+}
+
+// One lookup of App. B (the reference's gather.py:45-98) in any row, beta clamp included: the record's
+// phase pair with Delta recomputed from ln|z_L| (aLf) and ln|z_H| (aH) in float64.  Used for the first and
+// the last row pair of a table, the rows where beta may be clamped.
+template <typename V>
+__device__ __forceinline__ double exact_lookup(const float* R0, const float* R1, const BandGeom& g, double bd,
+                                               double offb, double lw_rand, float aLf, uint2 e, float aH) {
+    const double aL = (double)aLf;
+    // beta coordinate (clamped) and its fraction in row b
+    const double xbr = fma(aL, g.rdb, offb);
+    const double xb = fmin(fmax(xbr, 0.0), g.xbmax);
+    const double fb = xb - bd;
+    // Delta coordinate: ln q_L moves with the beta clamp, ln q_H stays; xd lies in [0, nd - 1 - 1e-6]
+    // (NaN included: fmax returns the number), so d0 = floor(xd) is in [0, nd - 2]
+    const double aLc = aL + (xb - xbr) / g.rdb;
+    const double xd = fmin(fmax(((double)aH - aLc) * g.rdd + g.offd, 0.0), g.xdmax);
+    const int d0 = (int)xd;
+    const double fd = xd - d0;
+    const int lo = rec_offset(e) % NPHIF;
+    const V v = interp<V>(R0, R1, d0 * NPHIF + lo, (V)fd, (V)fb, (V)rec_wc(e) * (V)(1.0 / FC_SCALE));
+    return (double)v + ((e.y & REC_DRAWN) ? lw_rand : 0.0);
 }
 
 // one workgroup per (tile of reads of one table, lag); PN walked times per piece; V the type of the
@@ -350,8 +405,6 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     const int j0 = tiles[2 * blockIdx.x], nj = tiles[2 * blockIdx.x + 1];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const float* Tr = T + (int64_t)jtab[j0] * tstride + (int64_t)r * nb * rowlen;
-    const double qbd = rdd / rdb;     // Delta coordinate per beta coordinate
-    const BandGeom geo{rdb, offd, rdd, qbd, nb - 1 - 1e-6, nd - 1 - 1e-6, nd};
 
     int my_first = INT_MAX, my_last = INT_MIN;
     if (tid < nj) {
@@ -370,7 +423,8 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
         atomicMax(&wsum[17], my_last);
     }
     __syncthreads();
-    const int b_start = wsum[16], b_end = wsum[17];
+    // the interior rows of the band (rows 0 and nb - 2, where beta may be clamped, are k_gather_edge's)
+    const int b_start = max(wsum[16], 1), b_end = min(wsum[17], nb - 3);
     // rows b_start .. b_start + 2 (as far as the band and the table reach)
     const int b_top = min(b_end + 1, nb - 1);
     for (int q = 0; q < 3 && b_start + q <= b_top; ++q)
@@ -431,8 +485,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
                 const int pa = (int)plo[jj] + (base + tid - qex[jj]) * PN;
                 const int pe = min(pa + PN, (int)phi[jj]);
                 const int m = jm[job];
-                const double offb = joffb[job];
-                const double cj = offd + offb * qbd;
+                const double cb = joffb[job] - bd;
                 const double lw_rand = (double)lw_g[m];
                 const float* wam = wa + (int64_t)m * nw;
                 const uint2* sm = stream + ((int64_t)m * ntau + r) * nw;
@@ -454,8 +507,9 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
                     }
                     // value of lookup k of the chunk (-1e300 past the piece's end)
                     auto lookup = [&](int k) -> double {
-                        return pa + c0 + k < pe ? band_lookup<V>(R0, R1, geo, bd, offb, cj, lw_rand, aLv[k], ev[k])
-                                                : -1e300;
+                        const double v = (double)lean_lookup<V>(R0, R1, rdb, cb, aLv[k], ev[k]) +
+                                         ((ev[k].y & REC_DRAWN) ? lw_rand : 0.0);
+                        return pa + c0 + k < pe ? v : -1e300;
                     };
                     if constexpr (sizeof(V) == 4) {
                         // independent lookups (their LDS reads and interpolations overlap), then the
@@ -578,8 +632,6 @@ __global__ __launch_bounds__(WAVE_THREADS) void k_gather_wave(
     const int jl = wid + NWAVE * lane;                                      // this lane's read in the tile
     const bool mine = jl < nj;
     const float* Tr = T + (int64_t)jtab[j0] * tstride + (int64_t)r * nb * rowlen;
-    const double qbd = rdd / rdb;
-    const BandGeom geo{rdb, offd, rdd, qbd, nb - 1 - 1e-6, nd - 1 - 1e-6, nd};
 
     // this lane's read: rows, template, offsets
     int my_first = INT_MAX, my_last = INT_MIN, my_m = 0;
@@ -604,7 +656,8 @@ __global__ __launch_bounds__(WAVE_THREADS) void k_gather_wave(
         atomicMax(&band[1], my_last);
     }
     __syncthreads();
-    const int b_start = band[0], b_end = band[1];
+    // the interior rows of the band (rows 0 and nb - 2, where beta may be clamped, are k_gather_edge's)
+    const int b_start = max(band[0], 1), b_end = min(band[1], nb - 3);
     const int b_top = min(b_end + 1, nb - 1);
     for (int q = 0; q < 3 && b_start + q <= b_top; ++q)
         row_to_lds(Tr + (int64_t)(b_start + q) * rowlen, rows + ((b_start + q) % 3) * slot, slot, wid, lane,
@@ -698,7 +751,9 @@ __global__ __launch_bounds__(WAVE_THREADS) void k_gather_wave(
         // One round: the next piece (this row's next round, or row b + 1's first) is fetched into nxt,
         // then cur is computed and merged.  The round has no branches, so the compiler can overlap the
         // fetch's lane shuffles with the lookups: every slot of a piece is looked up, unused slots with
-        // the record of phase bin 0 (addresses inside the row) and their values discarded.
+        // the records that follow the piece in memory (the read's later walked times, the next lag's or
+        // template's, or the zeroed padding after the last; their offsets lie inside a row) and their
+        // values discarded.
         auto round = [&](int base, Piece& cur, Piece& nxt) __attribute__((always_inline)) {
             const bool same_row = base + 64 < Qw;
             const Layout F{same_row ? S.lo : N.lo, same_row ? S.hi : N.hi, same_row ? S.q : N.q,
@@ -708,34 +763,57 @@ __global__ __launch_bounds__(WAVE_THREADS) void k_gather_wave(
             double mx = -1e300, sm = 0.0;
             {
                 const int nuse = cur.pe - cur.pa;   // slots of the piece that hold its read's walked times
-                const double cj = offd + cur.offb * qbd, lw = (double)cur.lw;
-                auto lookup = [&](int k) -> double {
-                    const bool use = k < nuse;
-                    const uint2 e = make_uint2(use ? cur.ev[k].x : 0u, cur.ev[k].y);
-                    const double a = band_lookup<V>(R0, R1, geo, bd, cur.offb, cj, lw, cur.aL[k], e);
-                    return use ? a : -1e300;
-                };
-                if constexpr (PN <= 6) {
-                    // the piece's values first, then its (max, sum); the max, only the reference of the
-                    // sum, is taken in float; -inf values leave the sum unchanged
-                    double av[PN];
-#pragma unroll
-                    for (int k = 0; k < PN; ++k) av[k] = lookup(k);
-                    float mf = (float)av[0];
-#pragma unroll
-                    for (int k = 1; k < PN; ++k) mf = fmaxf(mf, (float)av[k]);
-                    mx = (double)mf;
-#pragma unroll
-                    for (int k = 0; k < PN; ++k) sm += (double)__expf((float)(av[k] - mx));
-                } else {
-                    // a running (max, sum) over the lookups (fewer live registers for long pieces)
+                const double cb = cur.offb - bd;
+                if constexpr (sizeof(V) == 4) {
+                    // float32 form: the piece's values in float, their max (the reference of the sum), then
+                    // the exps, a drawn time's weighted by e^{lw} (at most n_time / n_random), summed in
+                    // float64; slots past the piece's end are -inf and add nothing
+                    float av[PN];
 #pragma unroll
                     for (int k = 0; k < PN; ++k) {
-                        const double a = lookup(k);
-                        const double dlt = a - mx;
-                        const double ex = (double)__expf(-(float)fabs(dlt));
-                        sm = dlt > 0.0 ? fma(sm, ex, 1.0) : sm + ex;
-                        mx = dlt > 0.0 ? a : mx;
+                        const float v = lean_lookup<float>(R0, R1, rdb, cb, cur.aL[k], cur.ev[k]);
+                        av[k] = k < nuse ? v : -INFINITY;
+                    }
+                    float mf = av[0];
+#pragma unroll
+                    for (int k = 1; k < PN; ++k) mf = fmaxf(mf, av[k]);
+                    constexpr float LOG2E = 1.4426950408889634f;
+                    const float ew = __expf(cur.lw), ml = mf * LOG2E;
+#pragma unroll
+                    for (int k = 0; k < PN; ++k) {
+                        const float ex = __builtin_amdgcn_exp2f(fmaf(av[k], LOG2E, -ml));
+                        sm += (double)((cur.ev[k].y & REC_DRAWN) ? ex * ew : ex);
+                    }
+                    mx = (double)mf;
+                } else {
+                    const double lw = (double)cur.lw;
+                    auto lookup = [&](int k) -> double {
+                        const double a = (double)lean_lookup<V>(R0, R1, rdb, cb, cur.aL[k], cur.ev[k]) +
+                                         ((cur.ev[k].y & REC_DRAWN) ? lw : 0.0);
+                        return k < nuse ? a : -1e300;
+                    };
+                    if constexpr (PN <= 6) {
+                        // the piece's values first, then its (max, sum); the max, only the reference of the
+                        // sum, is taken in float; -inf values leave the sum unchanged
+                        double av[PN];
+#pragma unroll
+                        for (int k = 0; k < PN; ++k) av[k] = lookup(k);
+                        float mf = (float)av[0];
+#pragma unroll
+                        for (int k = 1; k < PN; ++k) mf = fmaxf(mf, (float)av[k]);
+                        mx = (double)mf;
+#pragma unroll
+                        for (int k = 0; k < PN; ++k) sm += (double)__expf((float)(av[k] - mx));
+                    } else {
+                        // a running (max, sum) over the lookups (fewer live registers for long pieces)
+#pragma unroll
+                        for (int k = 0; k < PN; ++k) {
+                            const double a = lookup(k);
+                            const double dlt = a - mx;
+                            const double ex = (double)__expf(-(float)fabs(dlt));
+                            sm = dlt > 0.0 ? fma(sm, ex, 1.0) : sm + ex;
+                            mx = dlt > 0.0 ? a : mx;
+                        }
                     }
                 }
             }
@@ -799,10 +877,96 @@ __global__ __launch_bounds__(WAVE_THREADS) void k_gather_wave(
     }
 }
 
+// The first and the last row pair of a table (rows 0 and nb - 2), where beta may be clamped: each lookup
+// recomputes its Delta coordinate from ln|z_L| and ln|z_H| (exact_lookup).  One workgroup per (tile of
+// reads of one table, lag); lane j takes read j of the tile and walks its times in those rows one after
+// the other (about 0.1 % of the lookups fall there, so their balance over the lanes does not matter).
+// The two rows of a pair are copied to LDS together; the read's (max, sum) is merged into om, os, which
+// the interior rows' gather has written.
+template <typename V>
+__global__ __launch_bounds__(WAVE_THREADS) void k_gather_edge(
+    const uint2* __restrict__ stream, const float* __restrict__ aHs, const float* __restrict__ wa, int nw,
+    const float* __restrict__ lw_g, const float* __restrict__ T, int64_t tstride, int nphif, int nb, int nd,
+    int ntau, const int32_t* __restrict__ jm, const int32_t* __restrict__ jtab, const double* __restrict__ joffb,
+    const int32_t* __restrict__ tiles, const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds,
+    double rdb, double offd, double rdd, double* __restrict__ om, double* __restrict__ os) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int rowlen = nphif * nd, slot = row_slot(rowlen);
+    float* rows = (float*)lds;                                              // 2 row slots
+    int* band = (int*)(lds + edge_band_offset(rowlen));
+    const int r = blockIdx.y;
+    const int j0 = tiles[2 * blockIdx.x], nj = tiles[2 * blockIdx.x + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const bool mine = tid < nj;
+    int my_first = INT_MAX, my_last = INT_MIN;
+    if (mine) {
+        my_first = first_last[2 * ((int64_t)j0 + tid)];
+        my_last = first_last[2 * ((int64_t)j0 + tid) + 1];
+    }
+    if (tid == 0) {
+        band[0] = INT_MAX;
+        band[1] = INT_MIN;
+    }
+    __syncthreads();
+    if (mine) {
+        atomicMin(&band[0], my_first);
+        atomicMax(&band[1], my_last);
+    }
+    __syncthreads();
+    const bool low = band[0] == 0, high = band[1] >= nb - 2;
+    if (!low && !high) return;                                              // the whole workgroup
+    const float* Tr = T + (int64_t)jtab[j0] * tstride + (int64_t)r * nb * rowlen;
+    const BandGeom geo{rdb, offd, rdd, rdd / rdb, nb - 1 - 1e-6, nd - 1 - 1e-6, nd};
+    int m = 0;
+    double offb = 0.0, lw = 0.0;
+    if (mine) {
+        m = jm[(int64_t)j0 + tid];
+        offb = joffb[(int64_t)j0 + tid];
+        lw = (double)lw_g[m];
+    }
+    const uint16_t* bj = bounds + ((int64_t)j0 + (mine ? tid : 0)) * (nb + 1);
+    const int64_t rec = ((int64_t)m * ntau + r) * nw;
+    double amx = -1e300, asum = 0.0;
+    bool any = false;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (!(pass == 0 ? low : high)) continue;
+        const int b = pass == 0 ? 0 : nb - 2;
+        if (pass == 1 && low) __syncthreads();                              // every lane is done with pass 0
+        row_to_lds(Tr + (int64_t)b * rowlen, rows, slot, wid, lane, WAVE_THREADS);
+        row_to_lds(Tr + (int64_t)(b + 1) * rowlen, rows + slot, slot, wid, lane, WAVE_THREADS);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (mine && b >= my_first && b <= my_last) {
+            const int p0 = bj[b - my_first], p1 = bj[b + 1 - my_first];
+            for (int p = p0; p < p1; ++p) {
+                const double a = exact_lookup<V>(rows, rows + slot, geo, (double)b, offb, lw, wa[(int64_t)m * nw + p],
+                                                 stream[rec + p], aHs[rec + p]);
+                const double d = a - amx;
+                const double e = (double)__expf(-(float)fabs(d));
+                asum = d > 0.0 ? fma(asum, e, 1.0) : asum + e;
+                amx = fmax(amx, a);
+                any = true;
+            }
+        }
+    }
+    if (any) {
+        const int64_t o = ((int64_t)j0 + tid) * ntau + r;
+        lse_merge(amx, asum, om[o], os[o]);
+        om[o] = amx;
+        os[o] = asum;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ dst, int64_t n, uint32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
 // walk records of one segment into wa_dst (n_tmpl, nw) and stream_dst (n_tmpl, ntau, nw): positions
 // off .. off + nw - 1 of every template's selection (times), or the arrival times off .. off + nw - 1
 // (times NULL)
-int launch_walk(Traj* traj, Times* times, int64_t off, int nw, Store* st, float* wa_dst, uint2* stream_dst) {
+int launch_walk(Traj* traj, Times* times, int64_t off, int nw, Store* st, float* wa_dst, uint2* stream_dst,
+                float* aH_dst) {
     Context& c = ctx();
     const int nm = traj->n_tmpl;
     EXM_CHECK(nw > 0 && nw <= 65535, EXM_ERR_ARG, "walk of %d times exceeds the 16-bit row bounds", nw);
@@ -825,15 +989,24 @@ int launch_walk(Traj* traj, Times* times, int64_t off, int nw, Store* st, float*
     EXM_HIP(hipGetLastError());
     const size_t ws_lds = walk_stream_lds_bytes(st->ntau);
     EXM_CHECK(ws_lds <= 160 * 1024, EXM_ERR_ARG, "walk stream of %d lags exceeds the LDS", st->ntau);
+    EXM_CHECK((int64_t)st->nd * NPHIF <= (int64_t)REC_O_MASK + 1, EXM_ERR_ARG,
+              "table rows of %d x %d values exceed the walk record's 14-bit offset", st->nd, NPHIF);
     EXM_HIP(hipFuncSetAttribute((const void*)k_walk_stream, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ws_lds));
     hipLaunchKernelGGL(k_walk_stream, dim3(nm, (nw + WS_TILE - 1) / WS_TILE), dim3(256), ws_lds, c.stream, traj->lqH(), traj->phH(),
-                       traj->phL(), traj->n_time, st->d_tau, st->ntau, wt, wl, nw, stream_dst);
+                       traj->phL(), traj->n_time, st->d_tau, st->ntau, wt, wl, wa_dst, nw, 1.0 / st->ddelta,
+                       -st->delta0 / st->ddelta, st->nd, stream_dst, aH_dst);
+    EXM_HIP(hipGetLastError());
+    // the padding after the records holds valid (zero) records: a piece may read up to a piece past the
+    // last walked time of a read
+    const int64_t n_rec = (int64_t)nm * st->ntau * nw;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c.stream, (uint32_t*)(stream_dst + n_rec),
+                       (int64_t)(WALK_PAD_BYTES / 4), 0u);
     EXM_HIP(hipGetLastError());
     return EXM_OK;
 }
 
-int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* stream_dst) {
-    return launch_walk(traj, times, 0, times->n_walk, st, wa_dst, stream_dst);
+int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* stream_dst, float* aH_dst) {
+    return launch_walk(traj, times, 0, times->n_walk, st, wa_dst, stream_dst, aH_dst);
 }
 
 // piece gather of every read of js; template ids index wa, stream and lw_g
@@ -842,8 +1015,8 @@ size_t piece_bounds_bytes(const Store* st, int64_t njobs) {
     return al(sizeof(int32_t) * 2 * (size_t)njobs) + al(sizeof(uint16_t) * (size_t)njobs * (st->nb + 1));
 }
 
-int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const float* lw_g, const JobSet& js,
-                 double* om, double* os, void* bounds_buf, bool accumulate) {
+int launch_piece(Store* st, const uint2* stream, const float* aHs, const float* wa, int nw, const float* lw_g,
+                 const JobSet& js, double* om, double* os, void* bounds_buf, bool accumulate) {
     Context& c = ctx();
     const hipStream_t cs = cur_stream();
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -891,6 +1064,15 @@ int launch_piece(Store* st, const uint2* stream, const float* wa, int nw, const 
                        js.d_tiles, first_last, bounds, rdb, -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os,
                        accumulate ? 1 : 0);
     EXM_HIP(hipGetLastError());
+    // rows 0 and nb - 2 (beta possibly clamped), merged into the interior rows' (max, sum)
+    auto edge = c.f32_interp ? k_gather_edge<float> : k_gather_edge<double>;
+    const size_t elds = edge_band_offset(rowlen) + 16;
+    EXM_CHECK(elds <= 160 * 1024, EXM_ERR_ARG, "two table rows (%zu bytes) exceed the LDS", elds);
+    EXM_HIP(hipFuncSetAttribute((const void*)edge, hipFuncAttributeMaxDynamicSharedMemorySize, (int)elds));
+    hipLaunchKernelGGL(edge, dim3(js.ntiles, st->ntau), dim3(WAVE_THREADS), elds, cs, stream, aHs, wa, nw, lw_g,
+                       st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab, js.d_offb,
+                       js.d_tiles, first_last, bounds, rdb, -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os);
+    EXM_HIP(hipGetLastError());
     return EXM_OK;
 }
 
@@ -934,7 +1116,8 @@ int band_slice(const Traj* traj, const Store* st, const Times* times) {
 size_t band_scratch_bytes(const Traj* traj, const Store* st, const JobSet& js, const Times* times) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t nm = (size_t)traj->n_tmpl, nw = (size_t)band_slice(traj, st, times);
-    const size_t slot4 = al(nm * nw * 4 + WALK_PAD_BYTES) + al(nm * 4) + sizeof(uint2) * nm * st->ntau * nw + WALK_PAD_BYTES;
+    const size_t slot4 = al(nm * nw * 4 + WALK_PAD_BYTES) + al(nm * 4) + al(sizeof(uint2) * nm * st->ntau * nw + WALK_PAD_BYTES) +
+                         sizeof(float) * nm * st->ntau * nw;
     const size_t slot5 = 2 * al(nm * nw * 4);
     return al(slot4) + al(slot5) + al(piece_bounds_bytes(st, js.njobs));
 }
@@ -947,17 +1130,19 @@ int launch_gather_band(Traj* traj, Store* st, const JobSet& js, Times* times, do
     const int nw = band_slice(traj, st, times);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t wa_bytes = al((size_t)nm * nw * 4 + WALK_PAD_BYTES), lw_bytes = al((size_t)nm * 4);
-    const size_t st_bytes = sizeof(uint2) * (size_t)nm * st->ntau * nw + WALK_PAD_BYTES;
+    const size_t st_bytes = al(sizeof(uint2) * (size_t)nm * st->ntau * nw + WALK_PAD_BYTES);
+    const size_t ah_bytes = sizeof(float) * (size_t)nm * st->ntau * nw;
     void* buf = nullptr;
-    EXM_TRY(scratch(4, wa_bytes + lw_bytes + st_bytes, &buf));
+    EXM_TRY(scratch(4, wa_bytes + lw_bytes + st_bytes + ah_bytes, &buf));
     float* wa = (float*)buf;
     float* lw_g = (float*)((char*)buf + wa_bytes);
     uint2* stream = (uint2*)((char*)buf + wa_bytes + lw_bytes);
+    float* aHs = (float*)((char*)buf + wa_bytes + lw_bytes + st_bytes);
     EXM_TRY(launch_fill(lw_g, nm, times ? times->lw_rand : 0.0f));
     for (int64_t off = 0; off < total; off += nw) {
         const int n = (int)std::min<int64_t>(nw, total - off);
-        EXM_TRY(launch_walk(traj, times, off, n, st, wa, stream));
-        EXM_TRY(launch_piece(st, stream, wa, n, lw_g, js, om, os, nullptr, off > 0));
+        EXM_TRY(launch_walk(traj, times, off, n, st, wa, stream, aHs));
+        EXM_TRY(launch_piece(st, stream, aHs, wa, n, lw_g, js, om, os, nullptr, off > 0));
     }
     return EXM_OK;
 }

@@ -18,7 +18,7 @@ from . import pe as _pe
 from .gather import lag_range, rng_state
 from .segments import read_segments
 
-RECORD_BYTES = 8          # bytes of walk record per template, lag and walked time
+RECORD_BYTES = 12         # bytes per template, lag and walked time: the walk record and ln|z_H|
 
 
 def template_bytes(ntau, n_walk, reads=54, nb=200):
