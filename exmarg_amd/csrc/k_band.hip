@@ -289,9 +289,6 @@ __host__ __device__ constexpr size_t piece_res_offset(int rowlen) {
     return ((size_t)12 * row_slot(rowlen) + 15) & ~(size_t)15;
 }
 
-// byte offset of the edge kernel's band range after its two row slots
-__host__ __device__ constexpr size_t edge_band_offset(int rowlen) { return (size_t)8 * row_slot(rowlen); }
-
 // bytes of LDS the piece kernel uses for a row of rowlen floats
 __host__ __device__ constexpr size_t piece_lds_bytes(int rowlen) {
     return piece_res_offset(rowlen) + 2 * sizeof(double) * BAND_THREADS + sizeof(int) * BAND_THREADS +
@@ -878,79 +875,46 @@ __global__ __launch_bounds__(WAVE_THREADS) void k_gather_wave(
 }
 
 // The first and the last row pair of a table (rows 0 and nb - 2), where beta may be clamped: each lookup
-// recomputes its Delta coordinate from ln|z_L| and ln|z_H| (exact_lookup).  One workgroup per (tile of
-// reads of one table, lag); lane j takes read j of the tile and walks its times in those rows one after
-// the other (about 0.1 % of the lookups fall there, so their balance over the lanes does not matter).
-// The two rows of a pair are copied to LDS together; the read's (max, sum) is merged into om, os, which
+// recomputes its Delta coordinate from ln|z_L| and ln|z_H| (exact_lookup).  One thread per (read, lag)
+// walks the read's times in those rows, reading the table values from memory (about 0.1 % of the lookups
+// fall there; most threads find none and return at once), and merges its (max, sum) into om, os, which
 // the interior rows' gather has written.
 template <typename V>
-__global__ __launch_bounds__(WAVE_THREADS) void k_gather_edge(
+__global__ __launch_bounds__(256) void k_gather_edge(
     const uint2* __restrict__ stream, const float* __restrict__ aHs, const float* __restrict__ wa, int nw,
     const float* __restrict__ lw_g, const float* __restrict__ T, int64_t tstride, int nphif, int nb, int nd,
     int ntau, const int32_t* __restrict__ jm, const int32_t* __restrict__ jtab, const double* __restrict__ joffb,
-    const int32_t* __restrict__ tiles, const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds,
-    double rdb, double offd, double rdd, double* __restrict__ om, double* __restrict__ os) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    const int rowlen = nphif * nd, slot = row_slot(rowlen);
-    float* rows = (float*)lds;                                              // 2 row slots
-    int* band = (int*)(lds + edge_band_offset(rowlen));
+    int64_t njobs, const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds, double rdb,
+    double offd, double rdd, double* __restrict__ om, double* __restrict__ os) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int r = blockIdx.y;
-    const int j0 = tiles[2 * blockIdx.x], nj = tiles[2 * blockIdx.x + 1];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const bool mine = tid < nj;
-    int my_first = INT_MAX, my_last = INT_MIN;
-    if (mine) {
-        my_first = first_last[2 * ((int64_t)j0 + tid)];
-        my_last = first_last[2 * ((int64_t)j0 + tid) + 1];
-    }
-    if (tid == 0) {
-        band[0] = INT_MAX;
-        band[1] = INT_MIN;
-    }
-    __syncthreads();
-    if (mine) {
-        atomicMin(&band[0], my_first);
-        atomicMax(&band[1], my_last);
-    }
-    __syncthreads();
-    const bool low = band[0] == 0, high = band[1] >= nb - 2;
-    if (!low && !high) return;                                              // the whole workgroup
-    const float* Tr = T + (int64_t)jtab[j0] * tstride + (int64_t)r * nb * rowlen;
+    if (j >= njobs) return;
+    const int first = first_last[2 * j], last = first_last[2 * j + 1];
+    const bool low = first == 0, high = last >= nb - 2;
+    if (!low && !high) return;
+    const int rowlen = nphif * nd;
+    const float* Tr = T + (int64_t)jtab[j] * tstride + (int64_t)r * nb * rowlen;
     const BandGeom geo{rdb, offd, rdd, rdd / rdb, nb - 1 - 1e-6, nd - 1 - 1e-6, nd};
-    int m = 0;
-    double offb = 0.0, lw = 0.0;
-    if (mine) {
-        m = jm[(int64_t)j0 + tid];
-        offb = joffb[(int64_t)j0 + tid];
-        lw = (double)lw_g[m];
-    }
-    const uint16_t* bj = bounds + ((int64_t)j0 + (mine ? tid : 0)) * (nb + 1);
+    const int m = jm[j];
+    const double offb = joffb[j], lw = (double)lw_g[m];
+    const uint16_t* bj = bounds + j * (nb + 1);
     const int64_t rec = ((int64_t)m * ntau + r) * nw;
     double amx = -1e300, asum = 0.0;
-    bool any = false;
     for (int pass = 0; pass < 2; ++pass) {
-        if (!(pass == 0 ? low : high)) continue;
         const int b = pass == 0 ? 0 : nb - 2;
-        if (pass == 1 && low) __syncthreads();                              // every lane is done with pass 0
-        row_to_lds(Tr + (int64_t)b * rowlen, rows, slot, wid, lane, WAVE_THREADS);
-        row_to_lds(Tr + (int64_t)(b + 1) * rowlen, rows + slot, slot, wid, lane, WAVE_THREADS);
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (mine && b >= my_first && b <= my_last) {
-            const int p0 = bj[b - my_first], p1 = bj[b + 1 - my_first];
-            for (int p = p0; p < p1; ++p) {
-                const double a = exact_lookup<V>(rows, rows + slot, geo, (double)b, offb, lw, wa[(int64_t)m * nw + p],
-                                                 stream[rec + p], aHs[rec + p]);
-                const double d = a - amx;
-                const double e = (double)__expf(-(float)fabs(d));
-                asum = d > 0.0 ? fma(asum, e, 1.0) : asum + e;
-                amx = fmax(amx, a);
-                any = true;
-            }
+        if (b < first || b > last || (pass == 1 && nb - 2 == 0)) continue;
+        const float* R0 = Tr + (int64_t)b * rowlen;
+        for (int p = bj[b - first]; p < bj[b + 1 - first]; ++p) {
+            const double a = exact_lookup<V>(R0, R0 + rowlen, geo, (double)b, offb, lw, wa[(int64_t)m * nw + p],
+                                             stream[rec + p], aHs[rec + p]);
+            const double d = a - amx;
+            const double e = (double)__expf(-(float)fabs(d));
+            asum = d > 0.0 ? fma(asum, e, 1.0) : asum + e;
+            amx = fmax(amx, a);
         }
     }
-    if (any) {
-        const int64_t o = ((int64_t)j0 + tid) * ntau + r;
+    if (amx > -1e300) {
+        const int64_t o = j * ntau + r;
         lse_merge(amx, asum, om[o], os[o]);
         om[o] = amx;
         os[o] = asum;
@@ -1066,12 +1030,9 @@ int launch_piece(Store* st, const uint2* stream, const float* aHs, const float* 
     EXM_HIP(hipGetLastError());
     // rows 0 and nb - 2 (beta possibly clamped), merged into the interior rows' (max, sum)
     auto edge = c.f32_interp ? k_gather_edge<float> : k_gather_edge<double>;
-    const size_t elds = edge_band_offset(rowlen) + 16;
-    EXM_CHECK(elds <= 160 * 1024, EXM_ERR_ARG, "two table rows (%zu bytes) exceed the LDS", elds);
-    EXM_HIP(hipFuncSetAttribute((const void*)edge, hipFuncAttributeMaxDynamicSharedMemorySize, (int)elds));
-    hipLaunchKernelGGL(edge, dim3(js.ntiles, st->ntau), dim3(WAVE_THREADS), elds, cs, stream, aHs, wa, nw, lw_g,
-                       st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab, js.d_offb,
-                       js.d_tiles, first_last, bounds, rdb, -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os);
+    hipLaunchKernelGGL(edge, dim3((unsigned)((js.njobs + 255) / 256), st->ntau), dim3(256), 0, cs, stream, aHs, wa, nw,
+                       lw_g, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab, js.d_offb,
+                       js.njobs, first_last, bounds, rdb, -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os);
     EXM_HIP(hipGetLastError());
     return EXM_OK;
 }
