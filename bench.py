@@ -63,6 +63,8 @@ def parse():
     p.add_argument("--group", type=int, default=32, help="stream: segments sharing one spectrum estimate")
     p.add_argument("--rate", type=float, default=50.0, help="serve: mean arrival rate (requests/s)")
     p.add_argument("--requests", type=int, default=40, help="serve: requests in the timed run")
+    p.add_argument("--serve-requests", type=int, default=24,
+                   help="plain: requests of the serving workload run after the headline steps (0: none)")
     p.add_argument("--seed", type=int, default=7, help="serve: seed of the request stream")
     return p.parse_args()
 
@@ -246,19 +248,13 @@ def serve_requests(n, seed, rate):
     return out
 
 
-def main_serve(args):
-    """the request loop under Poisson arrivals: one JSON line with throughput and latency"""
+def serve_run(st, n_requests, rate, seed, n_loud=1024, n_random=1024):
+    """the request loop on the resident store under Poisson arrivals (serve_requests): throughput and
+    latency of the requests, after one warm-up request per segment length"""
     from exmarg_amd import _hip
-    from exmarg_amd.search import Server, store as stm
-    _hip.init(0)
-    info = _hip.device_info()
-    # the float32 form on request (--precision f32: float32 transform and interpolation)
-    prec = 1 if args.precision == "f32" else 0
-    _hip.call("exm_set_gather_precision", prec)
-    _hip.call("exm_set_filter_precision", prec)
-    st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
-    reqs = serve_requests(args.requests, args.seed, args.rate)
-    srv = Server(st, n_loud=args.n_loud, n_random=args.n_random, max_request=(8000, 8 * 65536, 64257))
+    from exmarg_amd.search import Server
+    reqs = serve_requests(n_requests, seed, rate)
+    srv = Server(st, n_loud=n_loud, n_random=n_random, max_request=(8000, 8 * 65536, 64257))
     # warm-up: one small request per segment length (FFT plans, kernel loads)
     for T in (4.0, 8.0, 16.0, 32.0, 64.0):
         for t_arr, seg, rows in reqs:
@@ -287,24 +283,39 @@ def main_serve(args):
     lat = np.array([h.latency for h in ok])
     span = max(h.t_done for h in ok) - t0 - reqs[0][0]
     n_tmpl = np.array([h.rows.shape[0] for h in ok])
-    out = {
-        "metric": "serving: requests/s and p50/p99 latency under Poisson arrivals",
-        "value": len(ok) / span, "unit": "requests/s", "n_gpus": 1, "higher_is_better": True,
-        "offered_rate": args.rate, "requests": len(handles), "finished": len(ok),
-        "refused": len(handles) - len(ok),
-        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                      "max": float(lat.max())},
-        "wall_s": wall, "batches": srv.stats["batches"],
-        "admitted_during_gather": srv.stats["admitted_during_gather"],
-        "templates_mean": float(n_tmpl.mean()), "gather_ms_total": g_ms, "filter_ms_total": f_ms,
-        "gpu_gather_busy_frac": g_ms / 1e3 / wall,
-        "dtype": ("f32 transform and interpolation, f64 weighting, index arithmetic and log-sum-exp" if prec
-                  else "f64"), "data": "synthetic",
-        "config": {"workload": "Poisson arrivals; T in {4,8,16,32,64} s at 20-1024 Hz; banks of 500-8000 templates "
-                               "(log-uniform); 30%% injections SNR 6-40; 53 HBM-resident table nodes; "
-                               "pe_intrinsic 'volume', n_loud=n_random=%d" % args.n_loud,
-                   "device": info["name"], "num_cu": info["num_cu"]},
-    }
+    return {"value": len(ok) / span, "unit": "requests/s", "offered_rate": rate, "requests": len(handles),
+            "finished": len(ok), "refused": len(handles) - len(ok),
+            "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "max": float(lat.max())},
+            "wall_s": wall, "batches": srv.stats["batches"],
+            "admitted_during_gather": srv.stats["admitted_during_gather"],
+            "templates_mean": float(n_tmpl.mean()), "gather_ms_total": g_ms, "filter_ms_total": f_ms,
+            "gpu_gather_busy_frac": g_ms / 1e3 / wall}
+
+
+SERVE_WORKLOAD = ("Poisson arrivals; T in {4,8,16,32,64} s at 20-1024 Hz; banks of 500-8000 templates (log-uniform); "
+                  "30%% injections SNR 6-40; 53 HBM-resident table nodes; pe_intrinsic 'volume', n_loud=n_random=%d")
+
+
+def main_serve(args):
+    """the request loop under Poisson arrivals: one JSON line with throughput and latency"""
+    from exmarg_amd import _hip
+    from exmarg_amd.search import store as stm
+    _hip.init(0)
+    info = _hip.device_info()
+    # the float32 form on request (--precision f32: float32 transform and interpolation)
+    prec = 1 if args.precision == "f32" else 0
+    _hip.call("exm_set_gather_precision", prec)
+    _hip.call("exm_set_filter_precision", prec)
+    st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
+    res = serve_run(st, args.requests, args.rate, args.seed, args.n_loud, args.n_random)
+    out = {"metric": "serving: requests/s and p50/p99 latency under Poisson arrivals", "n_gpus": 1,
+           "higher_is_better": True}
+    out.update(res)
+    out.update({"dtype": ("f32 transform and interpolation, f64 weighting, index arithmetic and log-sum-exp" if prec
+                          else "f64"), "data": "synthetic",
+                "config": {"workload": SERVE_WORKLOAD % args.n_loud, "device": info["name"],
+                           "num_cu": info["num_cu"]}})
     print(json.dumps(out), flush=True)
 
 
@@ -712,6 +723,11 @@ def main():
                                                  "16s_2000tmpl_53nodes.txt)"}
     if f64_check is not None:
         out["float32_form_vs_float64"] = f64_check
+    if args.serve_requests > 0 and args.config == 2 and args.nodes == 53:
+        # the serving form on the same resident store (config 4's request mix), reported beside the headline
+        sv = serve_run(st, args.serve_requests, args.rate, args.seed, args.n_loud, args.n_random)
+        sv["workload"] = SERVE_WORKLOAD % args.n_loud
+        out["serving"] = sv
     if not args.no_cpu and args.n_loud > 0 and args.config == 2:
         an_geometry = geometry
         cpu, err, nchk = cpu_leg(args, seg, bank, st, res, an_geometry, 0)

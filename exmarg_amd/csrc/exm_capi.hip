@@ -35,11 +35,18 @@ int ensure_init() {
 
 // Device memory: blocks are rounded up (2 MiB above 16 MiB) and freed blocks are kept in a pool by
 // size, up to a quarter of the device, so that the per-call objects of repeated analyses (trajectories,
-// times, read lists) reuse memory instead of mapping and unmapping it every call.  A failed hipMalloc
-// releases the pool and tries once more.
-static std::multimap<size_t, void*> g_pool;              // free blocks by size
+// times, read lists) reuse memory instead of mapping and unmapping it every call.  Frees are stream-
+// ordered: a freed block carries an event recorded on the stream that issued its last work, and is handed
+// out again only once that event has completed, so that a free never waits for unrelated work (a batch's
+// gather running on its own stream).  A failed hipMalloc releases the pool and tries once more.
+struct PooledBlock {
+    void* p;
+    hipEvent_t ev;
+};
+static std::multimap<size_t, PooledBlock> g_pool;        // free blocks by size
 static size_t g_pool_bytes = 0;
 static std::unordered_map<void*, size_t> g_blocks;        // the size of every block handed out
+static std::vector<hipEvent_t> g_spare_events;
 
 static size_t block_size(size_t bytes) {
     if (bytes == 0) bytes = 256;
@@ -48,7 +55,11 @@ static size_t block_size(size_t bytes) {
 }
 
 static void pool_release() {
-    for (auto& kv : g_pool) (void)hipFree(kv.second);
+    (void)hipDeviceSynchronize();
+    for (auto& kv : g_pool) {
+        (void)hipFree(kv.second.p);
+        g_spare_events.push_back(kv.second.ev);
+    }
     g_pool.clear();
     g_pool_bytes = 0;
 }
@@ -62,9 +73,19 @@ int dev_alloc(void** p, size_t bytes) {
     const size_t slack = bytes > ((size_t)16 << 20) ? bytes / 4 : 0;
     if (it != g_pool.end() && it->first <= bytes + slack) {
         const size_t got = it->first;
-        *p = it->second;
+        const PooledBlock blk = it->second;
         g_pool.erase(it);
         g_pool_bytes -= got;
+        // the work that used the block before it was freed has finished
+        const hipError_t e = hipEventSynchronize(blk.ev);
+        g_spare_events.push_back(blk.ev);
+        if (e != hipSuccess) {
+            (void)hipFree(blk.p);
+            set_error("waiting for a pooled block: %s", hipGetErrorString(e));
+            *p = nullptr;
+            return EXM_ERR_HIP;
+        }
+        *p = blk.p;
         g_mem_in_use += (int64_t)got;
         g_blocks[*p] = got;
         return EXM_OK;
@@ -72,7 +93,6 @@ int dev_alloc(void** p, size_t bytes) {
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess && !g_pool.empty()) {
         (void)hipGetLastError();
-        (void)hipDeviceSynchronize();
         pool_release();
         e = hipMalloc(p, bytes);
     }
@@ -97,14 +117,27 @@ void dev_free(void* p, size_t bytes) {
         bytes = block_size(bytes);
     }
     g_mem_in_use -= (int64_t)bytes;
-    // a pooled block may be handed out at once: the work that used it must have finished
-    (void)hipDeviceSynchronize();
     if (g_ctx.total_mem > 0 && g_pool_bytes + bytes <= (size_t)g_ctx.total_mem / 4) {
-        g_pool.emplace(bytes, p);
-        g_pool_bytes += bytes;
-    } else {
-        (void)hipFree(p);
+        // the block's last work ran on the gather stream in use (a batch's inside its launch) or the
+        // library stream, which also waits for the filter's side stream at the end of every filter
+        hipEvent_t ev = nullptr;
+        if (!g_spare_events.empty()) {
+            ev = g_spare_events.back();
+            g_spare_events.pop_back();
+        } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            ev = nullptr;
+        }
+        if (ev && hipEventRecord(ev, cur_stream()) == hipSuccess) {
+            g_pool.emplace(bytes, PooledBlock{p, ev});
+            g_pool_bytes += bytes;
+            return;
+        }
+        (void)hipGetLastError();
+        if (ev) g_spare_events.push_back(ev);
     }
+    (void)hipDeviceSynchronize();
+    (void)hipFree(p);
 }
 
 int scratch(int slot, size_t bytes, void** out) {
@@ -125,7 +158,6 @@ int scratch(int slot, size_t bytes, void** out) {
         return EXM_OK;
     }
     if (c.scratch_size[slot] < bytes) {
-        EXM_HIP(hipStreamSynchronize(c.stream));
         dev_free(c.scratch[slot], c.scratch_size[slot]);
         c.scratch[slot] = nullptr;
         c.scratch_size[slot] = 0;
