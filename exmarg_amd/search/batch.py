@@ -117,8 +117,24 @@ class SegmentBatch:
         self.handle.free()
 
 
+class _Slots:
+    """the two batch slots of pe_intrinsic_batch (kept with the store between calls on request)"""
+
+    def __init__(self, slots):
+        self.slots = slots
+        self.dev = slots[0].dev
+
+    def reset(self):
+        for b in self.slots:
+            b.reset()
+
+    def free(self):
+        for b in self.slots:
+            b.free()
+
+
 def release_kept(store):
-    """free the batch buffer pe_intrinsic_batch(..., keep_buffer=True) keeps with the store"""
+    """free the batch buffers pe_intrinsic_batch(..., keep_buffer=True) keeps with the store"""
     kept = store.__dict__.pop("_batch", None)
     if kept is not None:
         kept[1].free()
@@ -134,9 +150,13 @@ def pe_intrinsic_batch(items, store, prior="volume", ymin=0.1267, n_loud=1024, n
 
     returns a list with one pe_intrinsic result dict per item.
 
-    keep_buffer  keep the batch's device buffer (sized from the free HBM unless max_templates / max_bytes
-           bound it) with the store for the next call with the same arguments, for callers that repeat
-           batches: allocating it anew after freeing it costs seconds.  The buffer holds that memory
+    The device memory (sized from the free HBM unless max_templates / max_bytes bound it) is split into
+    two batch slots: while the gather of one slot runs on its own stream, the next segments are filtered,
+    their times selected and walked into the other, so that producing walk records and consuming them
+    overlap on the device.
+
+    keep_buffer  keep the slots with the store for the next call with the same arguments, for callers that
+           repeat batches: allocating them anew after freeing them costs seconds.  They hold that memory
            until release_kept(store) or exmarg_amd.search.store.free(store)."""
     from . import store as _store
     _store.upload(store)
@@ -147,49 +167,73 @@ def pe_intrinsic_batch(items, store, prior="volume", ymin=0.1267, n_loud=1024, n
         lnW = _pe.bank_weights(an, prior)
         an.start()
         prepared.append((seg, bank, an, lnW))
-    # work memory for admitting the largest segment first, so that the batch is sized on what is left
+    # work memory for admitting the largest segment first, so that the slots are sized on what is left
     big = max(prepared, key=lambda p: p[1].u.size * p[1].n_time)[1]
     _hip.call("exm_reserve", big.u.size, big.n_time, big.nu.size, int(n_loud) + int(n_random))
     key = (int(n_loud), int(n_random), max_templates, max_bytes)
     kept = store.__dict__.get("_batch")
     if keep_buffer and kept is not None and kept[0] == key and kept[1].dev is store.device:
-        batch = kept[1]
-        batch.reset()
+        slots = kept[1]
+        slots.reset()
     else:
         if keep_buffer:
             release_kept(store)
-        batch = SegmentBatch(store, n_loud, n_random, max_templates=max_templates, max_bytes=max_bytes)
+        half_t = None if max_templates is None else max(1, int(max_templates) // 2)
+        if max_bytes is None:
+            free, _ = _hip.mem_info()
+            half_b = (free - (8 << 30)) // 2
+        else:
+            half_b = int(max_bytes) // 2
+        first = SegmentBatch(store, n_loud, n_random, max_templates=half_t, max_bytes=half_b)
+        slots = _Slots([first, SegmentBatch(store, n_loud, n_random, max_templates=first.cap)])
         if keep_buffer:
-            store.__dict__["_batch"] = (key, batch)
+            store.__dict__["_batch"] = (key, slots)
     outs = [np.full((seg["rows"].size, bank.u.size), -np.inf) for seg, bank, _, _ in prepared]
-    pending = []
+    members = {0: [], 1: []}
+    state = dict(cur=0, flying=None)
 
-    def flush():
-        for (i, j), lamN in zip(pending, batch.run()):
-            lnW = prepared[i][3]
-            v = lnW + lamN
+    def finish(k):
+        for (i, j), lamN in zip(members[k], slots.slots[k].collect()):
+            v = prepared[i][3] + lamN
             mx = v.max(axis=0)
             outs[i][j] = mx + np.log(np.exp(v - mx[None, :]).sum(axis=0))
-        pending.clear()
-        batch.reset()
+        members[k] = []
+        slots.slots[k].reset()
+
+    def launch():
+        """launch the filling slot, then finish the one launched before it (it is filled next)"""
+        k = state["cur"]
+        slots.slots[k].launch()
+        prev, state["flying"], state["cur"] = state["flying"], k, 1 - k
+        if prev is not None:
+            finish(prev)
 
     try:
         for i, (seg, bank, an, lnW) in enumerate(prepared):
-            if bank.u.size > batch.cap:
-                raise MemoryError("a bank of %d templates exceeds the batch capacity %d" % (bank.u.size, batch.cap))
+            if bank.u.size > slots.slots[0].cap:
+                raise MemoryError("a bank of %d templates exceeds the batch slot capacity %d"
+                                  % (bank.u.size, slots.slots[0].cap))
             for j in range(seg["rows"].size):
-                if not batch.fits(bank):
-                    flush()
-                batch.add(an, {I: seg["d_" + I][j] for I in "HL"}, {I: seg["sig2_" + I] for I in "HL"},
-                          seed + int(seg["rows"][j]))
-                pending.append((i, j))
-        if pending:
-            flush()
+                if not slots.slots[state["cur"]].fits(bank):
+                    launch()
+                slots.slots[state["cur"]].add(an, {I: seg["d_" + I][j] for I in "HL"},
+                                              {I: seg["sig2_" + I] for I in "HL"}, seed + int(seg["rows"][j]))
+                members[state["cur"]].append((i, j))
+        if members[state["cur"]]:
+            launch()
+        if state["flying"] is not None:
+            finish(state["flying"])
+            state["flying"] = None
     finally:
+        if state["flying"] is not None:          # an error left a launch in flight: wait for it
+            try:
+                slots.slots[state["flying"]].collect()
+            except Exception:                    # noqa: BLE001 - the original error is the one raised
+                pass
         if keep_buffer:
-            batch.reset()
+            slots.reset()
         else:
-            batch.free()
+            slots.free()
     results = []
     for (seg, bank, an, lnW), out in zip(prepared, outs):
         w = _pe._cell_weights(bank)

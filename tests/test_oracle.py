@@ -3,6 +3,7 @@ brute force on tiny inputs and the golden vectors held in the reference's data f
 so that a plausible mistake in the oracle (a dropped term, a wrong sign or index, a transposed operand)
 fails it."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -471,3 +472,16 @@ def test_gather_orders_agree_bit_for_bit(oracle):
     b = oracle.gather_pairs(traj, tabs, beta, delta, ids, lna, tab, times, order="reference")
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_oracle_under_sanitizers():
+    """every public function of the C oracle built with AddressSanitizer and UndefinedBehaviorSanitizer
+    and run on inputs that reach its edges (oracle/sanitize_check.c), with closed-form value checks"""
+    import shutil
+    import subprocess
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle")
+    if shutil.which("make") is None or shutil.which("gcc") is None:
+        pytest.skip("make and gcc are needed to build the sanitized oracle")
+    r = subprocess.run(["make", "-s", "-C", here, "sanitize"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "all checks passed" in r.stdout
