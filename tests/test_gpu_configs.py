@@ -44,7 +44,7 @@ def say(*a):
 def golden(name):
     path = os.path.join(GOLDEN, name + ".npz")
     if not os.path.exists(path):
-        pytest.skip("fixture %s not generated yet: python tests/golden/make_config_fixtures.py %s" % (path, name))
+        pytest.fail("fixture %s missing: python tests/golden/make_config_fixtures.py %s" % (path, name))
     return np.load(path)
 
 
