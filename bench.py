@@ -334,7 +334,7 @@ def stream_groups(n, group, seed, n_blocks=32, frac_inj=0.05, mdet_range=(10.0, 
         for _ in range(k):
             if rng.uniform() < frac_inj:
                 m1, z = stream.draw_population(rng, mdet_range)
-                params.append((2.0 * m1 * (1.0 + z), m1, z, synth.D0 / float(cosmology.luminosity_distance(z))))
+                params.append((2.0 * m1 * (1.0 + z), m1, z, synth.D0 / float(np.ravel(cosmology.luminosity_distance(z))[0])))
             else:
                 params.append(None)
         it_d, it_y = iter(params), iter(params)
@@ -346,8 +346,34 @@ def stream_groups(n, group, seed, n_blocks=32, frac_inj=0.05, mdet_range=(10.0, 
     return groups, truths
 
 
+def stream_strain(groups, fs=4096.0, n_blocks=32, seed=0):
+    """the stream workload as strain: each group's segments as consecutive blocks of band-limited strain
+    sampled at fs (x = irfft of d fs on the band bins, so that the front end gives d back), and n_blocks
+    neighbouring noise-only blocks per detector drawn with the true noise variance (aLIGO fit), from
+    which the front end estimates the group's noise variance"""
+    from exmarg_amd import synth
+    rng = np.random.default_rng(seed)
+    out = []
+    for seg in groups:
+        T = float(seg["T"])
+        n = int(round(T * fs))
+        nu = np.asarray(seg["nu"])
+        sig2 = T * synth.psd_ajith(nu / T) / 2.0
+        strain, noise = {}, {}
+        for det in "HL":
+            Y = np.zeros((seg["d_" + det].shape[0], n // 2 + 1), complex)
+            Y[:, nu] = seg["d_" + det] * fs
+            strain[det] = np.fft.irfft(Y, n, axis=1).ravel()
+            Y = np.zeros((n_blocks, n // 2 + 1), complex)
+            Y[:, nu] = fs * np.sqrt(sig2 / 2.0) * (rng.standard_normal((n_blocks, nu.size))
+                                                   + 1j * rng.standard_normal((n_blocks, nu.size)))
+            noise[det] = np.fft.irfft(Y, n, axis=1).ravel()
+        out.append((strain, noise, T, float(seg["fmin"]), float(seg["fmax"])))
+    return out
+
+
 def main_stream(args):
-    """pe_redshift over a stream of 8 s segments: one JSON line with segments/s"""
+    """pe_redshift over a stream of 8 s segments given as strain: one JSON line with segments/s"""
     from exmarg_amd import _hip
     from exmarg_amd.priors import madau_dickinson_ymin
     from exmarg_amd.search import pe_redshift_stream, stream, store as stm
@@ -358,16 +384,29 @@ def main_stream(args):
     _hip.call("exm_set_filter_precision", prec)
     st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
     masses = np.geomspace(10.0, 500.0, 3517)
+    fs = 4096.0
     groups, truths = stream_groups(args.segments + args.group, args.group, args.seed)
-    warm, groups, truths = groups[:1], groups[1:], truths[1:]
+    strains = stream_strain(groups, fs)
+    warm, strains, truths = strains[:1], strains[1:], truths[1:]
     kw = dict(zmax=10.0, nz=1600, m1_range=(2.0, 200.0), nm1=1600, mdet_range=(10.0, 500.0),
               ymin=madau_dickinson_ymin(), n_loud=args.n_loud, n_random=args.n_random)
-    for _ in pe_redshift_stream(warm, masses, st, **kw):
+    front_s = [0.0]
+
+    def front(items):
+        """the segments and the noise estimate of each group from its strain, on the device"""
+        for x, noise, T, fmin, fmax in items:
+            t1 = time.perf_counter()
+            g = stream.segments_from_strain(x, noise, fs, T, fmin, fmax)
+            front_s[0] += time.perf_counter() - t1
+            yield g
+
+    for _ in pe_redshift_stream(front(warm), masses, st, **kw):
         pass
+    front_s[0] = 0.0
     inj = {(g, j) for g, tr in enumerate(truths) for j, (p, _) in enumerate(tr) if p is not None}
     _hip.profile(True)
     t0 = time.perf_counter()
-    res = list(pe_redshift_stream(groups, masses, st, keep=lambda g, row: (g, row) in inj, **kw))
+    res = list(pe_redshift_stream(front(strains), masses, st, keep=lambda g, row: (g, row) in inj, **kw))
     wall = time.perf_counter() - t0
     g_ms, g_n = _hip.profile_get("gather")
     sf_ms, sf_n = _hip.profile_get("source_frame")
@@ -386,12 +425,15 @@ def main_stream(args):
         "value": nseg / wall, "unit": "segments/s", "n_gpus": 1, "higher_is_better": True,
         "segments": nseg, "wall_s": wall, "ms_per_segment": 1e3 * wall / nseg,
         "gather_ms_per_segment": g_ms / nseg, "source_frame_ms_per_segment": sf_ms / max(sf_n, 1),
+        "front_end_ms_per_segment": 1e3 * front_s[0] / nseg,
         "ln_b": {"median": float(np.median(ln_b)), "max": float(ln_b.max())},
         "injections": [{"m1_level": a, "z_level": b, "snr": c} for a, b, c in levels],
         "dtype": ("f32 transform and interpolation, f64 weighting, index arithmetic and log-sum-exp" if prec
                   else "f64"), "data": "synthetic",
-        "config": {"workload": "8 s segments at 20-512 Hz, 5%% population injections (equal mass, zero spin), "
-                               "noise variance = mean of 32 periodograms x 32/30 per group of %d; 3517-mass bank "
+        "config": {"workload": "8 s segments at 20-512 Hz given as strain sampled at 4096 Hz, 5%% population "
+                               "injections (equal mass, zero spin); segments and noise variance (mean of the "
+                               "periodograms of 32 neighbouring 8 s noise blocks x 32/30, per group of %d) formed "
+                               "on the device from the strain; 3517-mass bank "
                                "10-500 Msun (geometric); 53 HBM-resident table nodes; pe_redshift on a 1600 x 1600 "
                                "(m1, z) grid, z <= 10, ymin %.4f, n_loud=n_random=%d"
                                % (args.group, madau_dickinson_ymin(), args.n_loud),

@@ -72,6 +72,8 @@ SIGNATURES = {
     "exm_copy_calibration": [I64, ctypes.POINTER(ctypes.c_double)],
     "exm_tables_sum_orders": [P, P, I32, I32, I32, I32, P, P, I32, P, I32, P],
     "exm_source_frame": [P, P, I32, I32, P, P, P, I32, P, I32, P, P, P],
+    "exm_strain_segments": [P, I32, I64, F64, I32, I32, P],
+    "exm_strain_psd": [P, I32, I64, F64, I32, I32, F64, P],
 }
 
 

@@ -42,13 +42,14 @@ def _node_geometry(bank, st, ymin):
     u = np.asarray(bank.u, float)
     nn, nm = s.size, u.size
     alo = np.minimum(np.asarray(st.alpha_lo, float), ymin * ymin)
-    a = s[:, None] / u[None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):   # u = 0 (no power in the band): never valid
+        a = s[:, None] / u[None, :]
+        s_prev_a = np.concatenate([[0.0], s[:-1]])[:, None] / u[None, :]
     valid = (np.sqrt(a) >= ymin) & (a > alo[:, None]) & (a < np.asarray(st.alpha_hi, float)[:, None])
     lna = np.where(valid, np.log(np.where(valid, a, 1.0)), 0.0)
     y = np.zeros((nn + 1, nm))
     y[:nn] = np.where(valid, np.sqrt(a), 0.0)
-    s_prev = np.concatenate([[0.0], s[:-1]])
-    low = valid & (s_prev[:, None] / u[None, :] <= alo[:, None])
+    low = valid & (s_prev_a <= alo[:, None])
     bnode = np.where(low, np.arange(nn)[:, None], -1).max(axis=0)
     blna = np.zeros(nm)
     has = bnode >= 0

@@ -16,13 +16,14 @@ next fills), and every segment's node values go through the device source-frame 
 the (m1, z) grid and ln B).  The population posterior of pp.py:143-161 (credible levels of m1 and z)
 is computed for the rows a caller marks.
 
-Time-domain front end (the segment convention of src/exmarg/pp.py:38-72: d = dt * rfft(x) on the bins
-ceil(fmin T) .. floor(fmax T), E|d|^2 = sig2 = T S / 2): segment_from_strain, periodograms_from_strain.
+Time-domain front end on the device (the segment convention of src/exmarg/pp.py:38-72: d = dt rfft(x) on
+the bins ceil(fmin T) .. floor(fmax T), E|d|^2 = sig2 = T S / 2): segment_from_strain, psd_from_strain and
+segments_from_strain (a segment set with its estimated noise variance from strain series).
 """
 
 import numpy as np
 
-from .. import cosmology
+from .. import _hip, cosmology
 from ..priors import madau_dickinson_ymin, population_m1_z
 from . import pe as _pe, store as _store
 from .bank import mass_bank
@@ -39,18 +40,24 @@ def noise_periodograms(sig2, n_blocks, rng):
     return sig2 / 2.0 * (g[0] ** 2 + g[1] ** 2)
 
 
+_SCALES = {"n-2": lambda N: N / (N - 2.0), "n-1": lambda N: N / (N - 1.0), "none": lambda N: 1.0}
+
+
+def _scale(scale, N):
+    if scale not in _SCALES:
+        raise ValueError("scale must be 'n-2', 'n-1' or 'none'")
+    return _SCALES[scale](N)
+
+
 def psd_estimate(periodograms, scale="n-2"):
-    """the noise variance per bin from N periodograms: their mean times N/(N-2) (scale 'n-2', which keeps
-    the noise mean of the likelihood ratio), N/(N-1) ('n-1', unbiased 1/S) or 1 ('none')
-    (notebooks/07_estimated_psd.ipynb cell 2: the mean is S g with g ~ Gamma(N, 1/N))"""
+    """the noise variance per bin from N periodograms a caller already holds: their mean times N/(N-2)
+    (scale 'n-2', which keeps the noise mean of the likelihood ratio), N/(N-1) ('n-1', unbiased 1/S) or 1
+    ('none') (notebooks/07_estimated_psd.ipynb cell 2: the mean is S g with g ~ Gamma(N, 1/N)).  From
+    strain data, psd_from_strain forms the periodograms and their mean on the device."""
     P = np.asarray(periodograms, float)
     if P.ndim != 2 or P.shape[0] < 3:
         raise ValueError("need (N >= 3, n_freq) periodograms")
-    N = P.shape[0]
-    factor = {"n-2": N / (N - 2.0), "n-1": N / (N - 1.0), "none": 1.0}
-    if scale not in factor:
-        raise ValueError("scale must be 'n-2', 'n-1' or 'none'")
-    return P.mean(axis=0) * factor[scale]
+    return P.mean(axis=0) * _scale(scale, P.shape[0])
 
 
 def frequency_bins(T, fmin, fmax):
@@ -58,29 +65,65 @@ def frequency_bins(T, fmin, fmax):
     return np.arange(int(np.ceil(fmin * T)), int(np.floor(fmax * T)) + 1)
 
 
-def segment_from_strain(x, fs, fmin, fmax):
-    """frequency-domain segment of the time series x (n_det, n) sampled at fs: d = dt rfft(x) on the bins
-    of the band, so that white noise of one-sided density S gives E|d|^2 = T S / 2"""
-    x = np.atleast_2d(np.asarray(x, float))
-    n = x.shape[1]
+def _band(n, fs, fmin, fmax):
     T = n / float(fs)
     nu = frequency_bins(T, fmin, fmax)
-    if nu[-1] > n // 2:
-        raise ValueError("fmax is above the Nyquist frequency")
-    d = np.fft.rfft(x, axis=1)[:, nu] / fs
+    if nu.size == 0:
+        raise ValueError("no frequency bin of a %g s block lies in [%g, %g] Hz" % (T, fmin, fmax))
+    if nu[0] < 0 or nu[-1] > n // 2:
+        raise ValueError("the band [%g, %g] Hz is not inside [0, fs / 2] = [0, %g] Hz" % (fmin, fmax, fs / 2.0))
+    return nu, T
+
+
+def segment_from_strain(x, fs, fmin, fmax):
+    """frequency-domain segment of each row of the time series x (n_row, n) sampled at fs, on the device:
+    d = dt rfft(x) on the bins of the band, so that white noise of one-sided density S gives
+    E|d|^2 = T S / 2 (the convention of src/exmarg/pp.py:38-72).  returns d (n_row, n_freq) complex128,
+    nu and T."""
+    x = _hip.contiguous(np.atleast_2d(x), np.float64)
+    if x.ndim != 2:
+        raise ValueError("x must be one series or rows of equally long series")
+    n = x.shape[1]
+    nu, T = _band(n, fs, fmin, fmax)
+    d = np.empty((x.shape[0], nu.size), np.complex128)
+    _hip.call("exm_strain_segments", _hip.ptr(x), x.shape[0], n, float(fs), int(nu[0]), nu.size, _hip.ptr(d))
     return d, nu, T
 
 
-def periodograms_from_strain(x, fs, T, fmin, fmax):
-    """periodograms |d|^2 of the consecutive blocks of T seconds of the time series x (1-d), on the bins of
-    the band: the input of psd_estimate"""
-    x = np.asarray(x, float)
+def psd_from_strain(x, fs, T, fmin, fmax, scale="n-2"):
+    """the noise variance per bin of the band estimated from the consecutive blocks of T seconds of the
+    series x (1-d; a partial last block is dropped), on the device: the mean of their periodograms
+    |d|^2 (d as segment_from_strain) times the factor of psd_estimate (notebooks/07_estimated_psd.ipynb
+    cell 5, there with simulated periodograms)"""
+    x = np.asarray(x, np.float64).ravel()
     n = int(round(T * fs))
-    nb = x.size // n
-    if nb < 1:
-        raise ValueError("the series is shorter than one block")
-    d, _, _ = segment_from_strain(x[:nb * n].reshape(nb, n), fs, fmin, fmax)
-    return np.abs(d) ** 2
+    N = x.size // n if n > 0 else 0
+    if N < 3:
+        raise ValueError("need at least 3 blocks of %g s, the series holds %d" % (T, N))
+    nu, _ = _band(n, fs, fmin, fmax)
+    blocks = _hip.contiguous(x[:N * n], np.float64)
+    out = np.empty(nu.size, np.float64)
+    _hip.call("exm_strain_psd", _hip.ptr(blocks), N, n, float(fs), int(nu[0]), nu.size, float(_scale(scale, N)),
+              _hip.ptr(out))
+    return out
+
+
+def segments_from_strain(strain, noise, fs, T, fmin, fmax, scale="n-2"):
+    """a segment set (the dict read_segments takes) from strain data: strain[I] (1-d) cut into consecutive
+    blocks of T seconds, one segment each, and the noise variance sig2_I estimated from the blocks of
+    noise[I] (psd_from_strain), for I in 'H', 'L'"""
+    out = {}
+    for det in ("H", "L"):
+        x = np.asarray(strain[det], np.float64).ravel()
+        n = int(round(T * fs))
+        if x.size < n:
+            raise ValueError("the %s series is shorter than one segment" % det)
+        k = x.size // n
+        d, nu, T_ = segment_from_strain(x[:k * n].reshape(k, n), fs, fmin, fmax)
+        out["d_" + det] = d
+        out["sig2_" + det] = psd_from_strain(noise[det], fs, T, fmin, fmax, scale)
+    out.update(f=nu / T_, nu=nu, T=T_, fmin=float(fmin), fmax=float(fmax))
+    return out
 
 
 # ---------------------------------------------------------------- population

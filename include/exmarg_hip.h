@@ -270,6 +270,21 @@ int exm_get_gather_tuning(int* kernel, int* piece, int* tile, int* precision);
 int exm_source_frame(const double* lamN, const double* yN, int nrow, int nm, const double* Mdet, const double* yq,
                      const double* m1, int nm1, const double* z, int nz, const double* w, double* out, double* ln_b);
 
+/* ---------------------------------------------------------------- time-domain front end
+ * The frequency-domain segment of each of n_rows time series x (n_rows, n) float64 sampled at fs:
+ * d (n_rows, n_freq) complex128 = rfft(x)[nu0 .. nu0 + n_freq - 1] / fs, the segment convention of
+ * src/exmarg/pp.py:38-72 (bins ceil(fmin T) .. floor(fmax T), E|d|^2 = T S / 2 for white noise of
+ * one-sided density S).  Replaces the host transform a caller of read_segments
+ * (src/exmarg/search/segments.py:11-54) would run on strain data.  nu0 + n_freq - 1 <= n / 2. */
+int exm_strain_segments(const double* x, int n_rows, int64_t n, double fs, int nu0, int n_freq, double* d);
+
+/* The noise variance per bin from n_blocks neighbouring blocks x (n_blocks, n) float64: sig2 (n_freq)
+ * = scale / n_blocks * sum_b |rfft(x_b)[nu0 + k] / fs|^2, the blocks added in order.  scale N/(N-2)
+ * gives the estimate of notebooks/07_estimated_psd.ipynb cell 5 (there formed from simulated
+ * periodograms), which keeps the noise mean of the likelihood ratio. */
+int exm_strain_psd(const double* x, int n_blocks, int64_t n, double fs, int nu0, int n_freq, double scale,
+                   double* sig2);
+
 /* ---------------------------------------------------------------- table build, stage 3
  * The sum over the orders of one node (scripts/build_tables/build_bd.py:52-79, build_node): col and
  * row are the last column (q2 = q_max) and last row (q1 = q_max) of every (q1, q2) block of the

@@ -37,6 +37,8 @@ Functions and the reference lines they follow:
   dl_of_z              src/exmarg/search/pe.py:26-28 (astropy Planck18 restated: flat LCDM with
                        photons and one massive neutrino species, the Komatsu et al. 2011
                        Eq. 26 fitting function for the neutrino density)
+  segment_from_strain  src/exmarg/pp.py:38-72 (d = dt rfft(x) on the band bins)
+  psd_from_strain      notebooks/07_estimated_psd.ipynb cell 5 (mean periodogram times N/(N-2))
   bd_grids             scripts/build_tables/build_bd.py:52-60 (beta and Delta grids of stage 3)
   bd_sum_orders        scripts/build_tables/build_bd.py:61-76 (stage 3: the sum over the orders)
 """
@@ -626,6 +628,24 @@ def pe_redshift(seg, bank, store, tabs, zmax=8.0, nz=1600, m1_range=(3.0, 150.0)
         out[j] = to_source_frame(lamN, geometry[2], np.asarray(bank["M"], float), yq, m1, z)
         ln_b[j] = log_mean(out[j], w)
     return dict(m1=m1, z=z, ln_lambda=out, ln_b=ln_b)
+
+
+# ---------------------------------------------------------------- stream front end
+def segment_from_strain(x, fs, nu0, n_freq):
+    """d = rfft(x)[nu0 : nu0 + n_freq] / fs of each row of x, complex128 (numpy's pocketfft): the
+    segment convention of src/exmarg/pp.py:38-72 (E|d|^2 = T S / 2 for white noise of one-sided density S)"""
+    x = np.atleast_2d(np.asarray(x, np.float64))
+    return np.fft.rfft(x, axis=1)[:, nu0:nu0 + n_freq] / fs
+
+
+def psd_from_strain(blocks, fs, nu0, n_freq, scale):
+    """scale / N * sum over the N blocks (rows), in order, of |d_b|^2: the mean periodogram estimate of
+    notebooks/07_estimated_psd.ipynb cell 5 (there from simulated periodograms), scale N/(N-2)"""
+    d = segment_from_strain(blocks, fs, nu0, n_freq)
+    acc = np.zeros(n_freq)
+    for b in range(d.shape[0]):
+        acc += d[b].real * d[b].real + d[b].imag * d[b].imag
+    return acc * (scale / d.shape[0])
 
 
 # ---------------------------------------------------------------- table build, stage 3

@@ -485,3 +485,42 @@ def test_oracle_under_sanitizers():
     r = subprocess.run(["make", "-s", "-C", here, "sanitize"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
     assert "all checks passed" in r.stdout
+
+
+def test_strain_segment_closed_form(oracle):
+    """a tone A cos(2 pi k t / T + phi) on bin k gives d_k = A T / 2 e^{i phi} and nothing on the other
+    bins (for tones on distinct bins of the band, odd and even lengths)"""
+    for n, fs in ((8192, 1024.0), (6001, 750.0)):
+        T = n / fs
+        t = np.arange(n) / fs
+        tones = {37: (1.5, 0.3), 38: (0.25, -2.0), 900: (3.0, 1.1)}
+        x = sum(A * np.cos(2 * np.pi * k * t / T + ph) for k, (A, ph) in tones.items())
+        d = oracle.segment_from_strain(x, fs, 30, 1000)[0]
+        want = np.zeros(1000, complex)
+        for k, (A, ph) in tones.items():
+            want[k - 30] = A * T / 2 * np.exp(1j * ph)
+        np.testing.assert_allclose(d, want, rtol=0, atol=1e-11 * T)
+
+
+def test_strain_white_noise_normalization(oracle):
+    """white noise of one-sided density S gives E|d|^2 = T S / 2 on every bin of the band"""
+    rng = np.random.default_rng(1)
+    fs, T, S = 1024.0, 4.0, 3e-3
+    x = rng.normal(0.0, np.sqrt(S * fs / 2.0), (400, int(fs * T)))
+    d = oracle.segment_from_strain(x, fs, 80, 1121)
+    np.testing.assert_allclose(np.mean(np.abs(d) ** 2), T * S / 2.0, rtol=1e-2)
+
+
+def test_strain_psd_closed_form(oracle):
+    """blocks carrying tones of amplitude A_b on bin k: the estimate on k is scale / N sum_b (A_b T / 2)^2,
+    and the tone-free bins stay at zero"""
+    n, fs, N = 4096, 512.0, 5
+    T = n / fs
+    t = np.arange(n) / fs
+    amps = np.array([1.0, 2.0, 0.5, 3.0, 1.5])
+    blocks = np.stack([A * np.cos(2 * np.pi * 100 * t / T + 0.7 * b) for b, A in enumerate(amps)])
+    scale = N / (N - 2.0)
+    p = oracle.psd_from_strain(blocks, fs, 90, 20, scale)
+    want = np.zeros(20)
+    want[10] = scale / N * np.sum((amps * T / 2) ** 2)
+    np.testing.assert_allclose(p, want, rtol=1e-12, atol=1e-20 * T * T)

@@ -1,5 +1,5 @@
-"""Host side of the segment stream: the periodogram estimate of the noise variance, the time-domain
-front end and the credible levels (no device)."""
+"""Host side of the segment stream: the periodogram estimate of the noise variance from periodograms a
+caller holds, and the credible levels (no device)."""
 import numpy as np
 import pytest
 
@@ -26,20 +26,6 @@ def test_psd_estimate_factor_and_distribution():
         stream.psd_estimate(P, scale="median")
     with pytest.raises(ValueError):
         stream.psd_estimate(P[:2])
-
-
-def test_segment_from_strain_normalization():
-    """white noise of one-sided density S gives E|d|^2 = T S / 2 on every bin of the band"""
-    rng = np.random.default_rng(1)
-    fs, T, S = 1024.0, 4.0, 3e-3
-    sigma = np.sqrt(S * fs / 2.0)                  # one-sided density S = 2 sigma^2 / fs
-    x = rng.normal(0.0, sigma, (400, int(fs * T)))
-    d, nu, T_ = stream.segment_from_strain(x, fs, 20.0, 300.0)
-    assert T_ == T and nu[0] == 80 and nu[-1] == 1200
-    np.testing.assert_allclose(np.mean(np.abs(d) ** 2), T * S / 2.0, rtol=1e-2)
-    P = stream.periodograms_from_strain(x[:8].ravel(), fs, T, 20.0, 300.0)
-    assert P.shape == (8, nu.size)
-    np.testing.assert_allclose(P, np.abs(d[:8]) ** 2, rtol=1e-12)
 
 
 def test_credible_level_and_population():
