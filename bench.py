@@ -120,7 +120,6 @@ def cpu_leg(args, seg, bank, store, res, an_geometry, seed_row):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import exm_oracle as O
     from exmarg_amd.search import read_segments
-    from exmarg_amd.search.pe import _bank_weights
     O.lib()
     rseg = read_segments(seg)
     d = {I: rseg["d_" + I][0] for I in "HL"}
@@ -157,7 +156,7 @@ def cpu_leg(args, seg, bank, store, res, an_geometry, seed_row):
     lamN = np.full((nn + 1, chk.size), -np.inf)
     for v, (k, j) in zip(vals, rows):
         lamN[k, j] = v
-    lnW = _bank_weights(yN, "volume", 0.1267, bank)[:, chk]
+    lnW = O.bank_weights(yN, "volume", 0.1267)[:, chk]
     v = lnW + lamN
     mx = v.max(axis=0)
     lnl_ref = mx + np.log(np.exp(v - mx).sum(axis=0))

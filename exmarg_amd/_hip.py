@@ -48,6 +48,9 @@ SIGNATURES = {
     "exm_plan_create": [H, I32, I64, P, P, P, HP],
     "exm_plan_gather": [H, H, H, H, P],
     "exm_plan_gather_noalloc": [H, H, H, H, P],
+    "exm_distance_weights": [P, I32, I32, I32, F64, P, P, I32, P],
+    "exm_plan_set_weights": [H, P, P],
+    "exm_plan_gather_lse": [H, H, H, H, P, P],
     "exm_batch_create": [H, I64, I32, HP],
     "exm_batch_add": [H, H, H, P, P, P, P, I32, I32, I32, I32, P, F32, ctypes.POINTER(ctypes.c_int)],
     "exm_batch_add_seeded": [H, H, H, P, P, P, P, I32, I32, I32, I32, P, F32, ctypes.POINTER(ctypes.c_int)],
@@ -56,6 +59,7 @@ SIGNATURES = {
     "exm_reserve": [I32, I64, I32, I32],
     "exm_batch_poll": [H, ctypes.POINTER(ctypes.c_int)],
     "exm_batch_collect": [H, P],
+    "exm_batch_collect_lse": [H, P, P],
     "exm_batch_reset": [H],
     "exm_batch_info": [H, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)],
     "exm_set_gather_kernel": [I32],

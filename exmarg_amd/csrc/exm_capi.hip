@@ -178,7 +178,10 @@ Times::~Times() {
     dev_free(d_sel, sizeof(int32_t) * (size_t)n_tmpl * n_walk);
     dev_free(d_lw, sizeof(float) * (size_t)n_tmpl * n_walk);
 }
-Plan::~Plan() { jobset_free(&js); }
+Plan::~Plan() {
+    jobset_free(&js);
+    if (d_w) dev_free(d_w, w_bytes);
+}
 Batch::~Batch() {
     if (done) {
         (void)hipEventSynchronize(done);
@@ -993,6 +996,14 @@ static int batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, cons
         b->h_tab.push_back(plan->h_tab[e]);
         b->h_offb.push_back(plan->h_offb[e]);
     }
+    // the plan's node weights go with its reads (a segment admitted without them leaves the batch
+    // without a device assembly: exm_batch_collect still gives the read values)
+    b->weights = b->weights && plan->has_weights;
+    if (b->weights) {
+        b->h_lnw.insert(b->h_lnw.end(), plan->h_lnw.begin(), plan->h_lnw.end());
+        b->h_cellw.insert(b->h_cellw.end(), plan->h_cellw.begin(), plan->h_cellw.end());
+    }
+    b->seg_tmpl.push_back((int32_t)b->used);
     b->seg_reads.push_back((int64_t)plan->h_m.size());
     b->used += bank->n_tmpl;
     if (slot) *slot = (int)b->seg_reads.size() - 1;
@@ -1058,7 +1069,9 @@ int exm_batch_launch(exm_handle hbatch) {
         auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
         const size_t bnd = al(piece_bounds_bytes(st, njobs));
         const size_t pairs = al(sizeof(double) * 2 * (size_t)njobs * st->ntau);
-        const size_t need = bnd + pairs + sizeof(double) * (size_t)njobs;
+        const size_t lse_at = bnd + pairs + ((sizeof(double) * (size_t)njobs + 255) & ~(size_t)255);
+        const int nseg = (int)b->seg_tmpl.size();
+        const size_t need = lse_at + (b->weights ? lse_device_bytes(njobs, b->used, nseg) : 0);
         if (need > b->work_bytes) {
             if (b->d_work) dev_free(b->d_work, b->work_bytes);
             b->d_work = nullptr;
@@ -1076,6 +1089,17 @@ int exm_batch_launch(exm_handle hbatch) {
         double* dout = (double*)(w + bnd + pairs);
         int rc = launch_piece(st, b->d_stream, b->d_aH, b->d_wa, b->nw, b->d_lw, b->js, om, os, w);
         if (rc == EXM_OK) rc = launch_lse(b->js, st->ntau, om, os, b->d_lnnt, 0.0, dout);
+        // with every segment's node weights: ln Lambda per template and ln B per segment on the device
+        b->lse_launched = false;
+        if (rc == EXM_OK && b->weights) {
+            lse_layout(njobs, (int)b->used, b->h_m.data(), b->js.order, b->h_lnw.data(), &b->lay);
+            b->h_segs.assign(b->seg_tmpl.begin(), b->seg_tmpl.end());
+            b->h_segs.push_back((int32_t)b->used);
+            b->lse.carve(w + lse_at, njobs, b->used, nseg);
+            rc = b->lse.upload(b->lay, b->h_cellw.data(), b->h_segs.data(), b->stream);
+            if (rc == EXM_OK) rc = b->lse.launch(dout, b->stream);
+            b->lse_launched = rc == EXM_OK;
+        }
         if (rc != EXM_OK) {
             (void)hipStreamSynchronize(b->stream);
             batch_release_jobs(b);
@@ -1127,6 +1151,35 @@ int exm_batch_collect(exm_handle hbatch, double* out) {
             for (int64_t i = 0; i < njobs; ++i) out[b->js.order[i]] = sorted[i];
         }
     }
+    b->lse_launched = false;
+    batch_release_jobs(b);
+    return rc;
+}
+
+int exm_batch_collect_lse(exm_handle hbatch, double* ln_lambda, double* ln_b) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(Batch, b, hbatch, K_BATCH);
+    EXM_CHECK(b->in_flight, EXM_ERR_STATE, "batch not launched");
+    const int64_t njobs = (int64_t)b->h_m.size();
+    EXM_CHECK(b->lse_launched || njobs == 0, EXM_ERR_STATE,
+              "a segment of the batch was admitted with a plan without node weights (exm_plan_set_weights); "
+              "collect the read values with exm_batch_collect");
+    EXM_CHECK((ln_lambda && ln_b) || b->used == 0, EXM_ERR_ARG, "NULL argument");
+    b->in_flight = false;
+    int rc = hipEventSynchronize(b->done) == hipSuccess ? EXM_OK : EXM_ERR_HIP;
+    if (rc != EXM_OK) set_error("batch gather failed");
+    if (rc == EXM_OK && njobs > 0) {
+        if (hipMemcpyAsync(ln_lambda, b->lse.L, sizeof(double) * (size_t)b->used, hipMemcpyDeviceToHost, b->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(ln_b, b->lse.ln_b, sizeof(double) * (size_t)b->lse.nseg, hipMemcpyDeviceToHost, b->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(b->stream) != hipSuccess) {
+            set_error("batch result copy failed");
+            rc = EXM_ERR_HIP;
+        }
+    }
+    b->lse_launched = false;
     batch_release_jobs(b);
     return rc;
 }
@@ -1147,6 +1200,11 @@ int exm_batch_reset(exm_handle hbatch) {
     b->h_tab.clear();
     b->h_offb.clear();
     b->seg_reads.clear();
+    b->weights = true;
+    b->h_lnw.clear();
+    b->h_cellw.clear();
+    b->seg_tmpl.clear();
+    b->lse_launched = false;
     return EXM_OK;
 }
 

@@ -129,12 +129,47 @@ struct JobSet {
 int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const double* offb, JobSet* js);
 void jobset_free(JobSet* js);
 
+// The float64 assembly of results over reads (k_assemble.hip): per template, the positions of its reads
+// among the table-sorted values in the caller's read order (CSR), ln w of every read by sorted position.
+struct LseLayout {
+    std::vector<int32_t> off;     // (n_tmpl + 1)
+    std::vector<int32_t> pos;     // (njobs) sorted positions, template by template
+    std::vector<double> lnw;      // (njobs) ln w by sorted position
+};
+void lse_layout(int64_t njobs, int n_tmpl, const int32_t* m, const std::vector<int64_t>& order, const double* lnw,
+                LseLayout* out);
+size_t lse_device_bytes(int64_t njobs, int64_t n_tmpl, int nseg);
+int check_node_weights(int n_tmpl, int64_t njobs, const double* lnw, const double* cellw);
+// the layout on the device, the cell weights, the segments' template ranges and the outputs, carved from
+// one block of lse_device_bytes
+struct LseDevice {
+    int32_t* off = nullptr;
+    int32_t* pos = nullptr;
+    double* lnw = nullptr;
+    double* cell = nullptr;
+    int32_t* seg = nullptr;
+    double* L = nullptr;
+    double* ln_b = nullptr;
+    int64_t nm = 0;
+    int nseg = 0;
+    void carve(void* base, int64_t njobs, int64_t n_tmpl, int nseg);
+    int upload(const LseLayout& lay, const double* cellw, const int32_t* segs, hipStream_t s) const;
+    // ln Lambda per template from the table-sorted read values val, then ln B per segment
+    int launch(const double* val, hipStream_t s) const;
+};
+
 // A reusable job set for exm_plan_gather, with the reads in the caller's order kept on the host.
 struct Plan : Object {
     int n_tmpl = 0;
     JobSet js;
     std::vector<int32_t> h_m, h_tab;
     std::vector<double> h_offb;
+    // node weights (exm_plan_set_weights): ln w per read in the caller's order, cell weight per template
+    bool has_weights = false;
+    std::vector<double> h_lnw, h_cellw;
+    void* d_w = nullptr;
+    size_t w_bytes = 0;
+    LseDevice lse;
     ~Plan();
 };
 
@@ -160,6 +195,15 @@ struct Batch : Object {
     JobSet js;
     void* d_work = nullptr;              // first/last rows, row bounds, (max, sum) pairs, results
     size_t work_bytes = 0;
+    // node weights of every admitted segment (from its plan), for exm_batch_collect_lse: ln w per read in
+    // admission order, cell weight per template, first template of every segment
+    bool weights = true;
+    std::vector<double> h_lnw, h_cellw;
+    std::vector<int32_t> seg_tmpl;
+    LseLayout lay;                       // the launch's uploads read these until the batch is collected
+    std::vector<int32_t> h_segs;
+    LseDevice lse;
+    bool lse_launched = false;
     ~Batch();
 };
 

@@ -108,6 +108,19 @@ class SegmentBatch:
             o += n
         return res
 
+    def collect_lse(self):
+        """wait for the launched gather; (ln Lambda (n_template), ln B) of every admitted segment, in admission
+        order, assembled on the device (every segment's plan carries node weights: _Analysis.use_weights)"""
+        L = np.empty(max(self.used, 1), np.float64)
+        ln_b = np.empty(max(len(self.entries), 1), np.float64)
+        _hip.call("exm_batch_collect_lse", int(self.handle), _hip.ptr(L), _hip.ptr(ln_b))
+        res, o = [], 0
+        for k, a in enumerate(self.entries):
+            n = a.bank.u.size
+            res.append((L[o:o + n].copy(), float(ln_b[k])))
+            o += n
+        return res
+
     def reset(self):
         _hip.call("exm_batch_reset", int(self.handle))
         self.used = 0
@@ -164,9 +177,8 @@ def pe_intrinsic_batch(items, store, prior="volume", ymin=0.1267, n_loud=1024, n
     for segments, bank in items:
         seg = read_segments(segments)
         an = _pe._Analysis(bank, store, ymin, True)
-        lnW = _pe.bank_weights(an, prior)
-        an.start()
-        prepared.append((seg, bank, an, lnW))
+        _pe.bank_weights(an, prior)
+        prepared.append((seg, bank, an.start().use_weights(prior)))
     # work memory for admitting the largest segment first, so that the slots are sized on what is left
     big = max(prepared, key=lambda p: p[1].u.size * p[1].n_time)[1]
     _hip.call("exm_reserve", big.u.size, big.n_time, big.nu.size, int(n_loud) + int(n_random))
@@ -188,15 +200,15 @@ def pe_intrinsic_batch(items, store, prior="volume", ymin=0.1267, n_loud=1024, n
         slots = _Slots([first, SegmentBatch(store, n_loud, n_random, max_templates=first.cap)])
         if keep_buffer:
             store.__dict__["_batch"] = (key, slots)
-    outs = [np.full((seg["rows"].size, bank.u.size), -np.inf) for seg, bank, _, _ in prepared]
+    outs = [np.full((seg["rows"].size, bank.u.size), -np.inf) for seg, bank, _ in prepared]
+    lnbs = [np.full(seg["rows"].size, -np.inf) for seg, _, _ in prepared]
     members = {0: [], 1: []}
     state = dict(cur=0, flying=None)
 
     def finish(k):
-        for (i, j), lamN in zip(members[k], slots.slots[k].collect()):
-            v = prepared[i][3] + lamN
-            mx = v.max(axis=0)
-            outs[i][j] = mx + np.log(np.exp(v - mx[None, :]).sum(axis=0))
+        for (i, j), (L, lnb) in zip(members[k], slots.slots[k].collect_lse()):
+            outs[i][j] = L
+            lnbs[i][j] = lnb
         members[k] = []
         slots.slots[k].reset()
 
@@ -209,7 +221,7 @@ def pe_intrinsic_batch(items, store, prior="volume", ymin=0.1267, n_loud=1024, n
             finish(prev)
 
     try:
-        for i, (seg, bank, an, lnW) in enumerate(prepared):
+        for i, (seg, bank, an) in enumerate(prepared):
             if bank.u.size > slots.slots[0].cap:
                 raise MemoryError("a bank of %d templates exceeds the batch slot capacity %d"
                                   % (bank.u.size, slots.slots[0].cap))
@@ -234,9 +246,5 @@ def pe_intrinsic_batch(items, store, prior="volume", ymin=0.1267, n_loud=1024, n
             slots.reset()
         else:
             slots.free()
-    results = []
-    for (seg, bank, an, lnW), out in zip(prepared, outs):
-        w = _pe._cell_weights(bank)
-        results.append(dict(M=bank.M, q=bank.q, chi=bank.chi, u=bank.u, ln_lambda=out,
-                            ln_b=np.array([_pe._log_mean(L, w) for L in out]), rows=seg["rows"]))
-    return results
+    return [dict(M=bank.M, q=bank.q, chi=bank.chi, u=bank.u, ln_lambda=out, ln_b=lnb, rows=seg["rows"])
+            for (seg, bank, _), out, lnb in zip(prepared, outs, lnbs)]

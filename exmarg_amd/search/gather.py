@@ -126,8 +126,28 @@ class Plan:
                                       _hip.ptr(self.job_m), _hip.ptr(self.job_tab), _hip.ptr(self.job_lna),
                                       kind="plan")
 
+    weights_key = None
+
     def run(self, traj, dev, times=None):
         out = np.empty(self.job_m.size, np.float64)
         _hip.call("exm_plan_gather", int(self.handle), int(traj.handle), int(dev.handle),
                   int(times.handle) if times is not None else 0, _hip.ptr(out))
         return out
+
+    def set_weights(self, lnw, cellw):
+        """ln w of every read (the distance weight of its point) and the template average weights, for
+        run_lse and for batches admitting this plan"""
+        lnw = _hip.contiguous(lnw, np.float64)
+        cellw = _hip.contiguous(cellw, np.float64)
+        if lnw.shape != self.job_m.shape or cellw.shape != (self.n_template,):
+            raise ValueError("weights of shape %s / %s for %d reads and %d templates"
+                             % (lnw.shape, cellw.shape, self.job_m.size, self.n_template))
+        _hip.call("exm_plan_set_weights", int(self.handle), _hip.ptr(lnw), _hip.ptr(cellw))
+
+    def run_lse(self, traj, dev, times=None):
+        """(ln Lambda (n_template), ln B) of one segment, assembled on the device"""
+        out = np.empty(self.n_template, np.float64)
+        ln_b = np.empty(1, np.float64)
+        _hip.call("exm_plan_gather_lse", int(self.handle), int(traj.handle), int(dev.handle),
+                  int(times.handle) if times is not None else 0, _hip.ptr(out), _hip.ptr(ln_b))
+        return out, float(ln_b[0])

@@ -5,9 +5,9 @@ Parameter estimation of single segments with the stored tables on the GPU
 pe_intrinsic  (M_det, q, chi) on a template bank, the distance integrated inside the tables
 pe_redshift   (m1_source, z) at equal mass and zero spin, the distance a parameter through y = D0 / D_L(z)
 
-Per segment the device runs the matched filter, the arrival-time selection and one gather of
-every (template, node) pair plus the boundary points; the host assembles the node values with the
-distance-prior weights (float64) and averages over the bank.
+Per segment the device runs the matched filter, the arrival-time selection, one gather of every
+(template, node) pair plus the boundary points, and the float64 assembly: the distance-prior weights
+(once per bank), the log-sum-exp over each template's points and the average over the bank.
 """
 
 import itertools
@@ -59,77 +59,53 @@ def _node_geometry(bank, st, ymin):
     return valid, lna, y, bnode, blna
 
 
-def _sorted_points(yN):
-    """per template: the rows with y > 0 sorted by y, as (order, count) with order (n_rows, nm)"""
-    Y = np.where(yN > 0, yN, np.inf)
-    order = np.argsort(Y, axis=0, kind="stable")
-    count = (yN > 0).sum(axis=0)
-    return order, count
+_PRIOR_KIND = {"volume": 0, "madau-dickinson": 1}
 
 
-def _interval_weights(x, ok, density, ymin):
-    """weights of the piecewise-linear integration on consecutive points x[i], x[i+1] (columns) where
-    ok[i] marks a valid interval; returns w with the shape of x"""
-    a, b = x[:-1], x[1:]
-    h = np.where(ok, b - a, 1.0)
-    a = np.where(ok, a, 0.0)
-    b = np.where(ok, b, 1.0)
+def _device_weights(yN, prior, ymin):
+    """lnW (n_rows, nm) of exm_distance_weights for the points yN (y <= 0: not reached)"""
+    if prior not in _PRIOR_KIND:
+        raise ValueError("prior must be 'volume' or 'madau-dickinson', got %r" % (prior,))
+    density = distance_density(prior)
+    yN = _hip.contiguous(yN, np.float64)
+    lnW = np.empty(yN.shape, np.float64)
     if density is None:
-        c, k = 3.0 * ymin ** 3, 3.0
-        ea, eb = np.exp(-k * a), np.exp(-k * b)
-        I0 = (ea - eb) / k
-        I1 = (a * ea - b * eb) / k + (ea - eb) / k ** 2
-        left = c * (b * I0 - I1) / h
-        right = c * (I1 - a * I0) / h
+        _hip.call("exm_distance_weights", _hip.ptr(yN), yN.shape[0], yN.shape[1], 0, float(ymin), None, None, 0,
+                  _hip.ptr(lnW))
     else:
-        lny, p = density
-        step = (b - a) / 400.0
-        t = a[..., None] + np.arange(401) * step[..., None]
-        t[..., -1] = b
-        f = np.interp(t.ravel(), lny, p, left=0.0, right=0.0).reshape(t.shape)
-        left = np.trapezoid(f * (b[..., None] - t) / h[..., None], t, axis=-1)
-        right = np.trapezoid(f * (t - a[..., None]) / h[..., None], t, axis=-1)
-    w = np.zeros(x.shape)
-    w[1:] += np.where(ok, right, 0.0)
-    w[:-1] += np.where(ok, left, 0.0)
-    return w
+        lnx, px = (_hip.contiguous(a, np.float64) for a in density)
+        _hip.call("exm_distance_weights", _hip.ptr(yN), yN.shape[0], yN.shape[1], 1, float(ymin), _hip.ptr(lnx),
+                  _hip.ptr(px), lnx.size, _hip.ptr(lnW))
+    return lnW
 
 
 def distance_weights(y, prior, ymin):
     """ln of the weights that integrate the distance prior against a likelihood ratio linear in x = ln y
     between the points y (ascending): sum_j w_j Lambda(y_j) = integral dy p(y) Lambda(y) over
-    [y_0, y_last] (src/exmarg/search/pe.py:260-286).
+    [y_0, y_last] (src/exmarg/search/pe.py:260-286), computed on the device in float64.
 
     prior  'volume': p(y) = 3 ymin^3 y^-4, integrated exactly on every interval;
            'madau-dickinson': the density in ln y of exmarg_amd.priors, trapezoid rule on 401 points
            inside every interval"""
-    density = distance_density(prior)
-    x = np.log(np.asarray(y, float))
-    ok = np.ones(max(x.size - 1, 0), bool)
-    w = _interval_weights(x, ok, density, ymin)
-    with np.errstate(divide="ignore"):
-        return np.log(w)
+    y = np.asarray(y, float)
+    if prior not in _PRIOR_KIND:
+        raise ValueError("prior must be 'volume' or 'madau-dickinson', got %r" % (prior,))
+    if y.ndim != 1 or np.any(y <= 0) or np.any(np.diff(y) <= 0):
+        raise ValueError("y must hold positive ascending points")
+    if y.size < 2:                         # no interval: every weight zero
+        return np.full(y.size, -np.inf)
+    return _device_weights(y[:, None], prior, ymin)[:, 0]
 
 
 def _bank_weights(yN, prior, ymin, bank):
     """lnW (nn+1, nm): distance weights of every template over the points it reaches
-    (src/exmarg/search/pe.py:321-327), all templates at once"""
-    density = distance_density(prior)
-    order, count = _sorted_points(yN)
+    (src/exmarg/search/pe.py:321-327), all templates at once on the device"""
+    count = (yN > 0).sum(axis=0)
     bad = np.nonzero(count < 2)[0]
     if bad.size:
         m = int(bad[0])
         raise ValueError("template %d (M %.2f, u %.3f) reaches fewer than two store nodes" % (m, bank.M[m], bank.u[m]))
-    Ys = np.take_along_axis(yN, order, axis=0)
-    rank = np.arange(yN.shape[0])[:, None]
-    x = np.log(np.where(rank < count[None, :], Ys, 1.0))
-    ok = (rank[1:] < count[None, :])
-    w = _interval_weights(x, ok, density, ymin)
-    lnW = np.full(yN.shape, -np.inf)
-    with np.errstate(divide="ignore"):
-        lw = np.where(rank < count[None, :], np.log(np.where(rank < count[None, :], w, 1.0)), -np.inf)
-    np.put_along_axis(lnW, order, lw, axis=0)
-    return lnW
+    return _device_weights(yN, prior, ymin)
 
 
 def bank_weights(an, prior):
@@ -140,16 +116,6 @@ def bank_weights(an, prior):
     if key not in cache:
         cache[key] = _bank_weights(an.geometry[2], prior, an.ymin, an.bank)
     return cache[key]
-
-
-def _log_mean(lam, w):
-    """ln sum_i w_i exp(lam_i) for normalized weights w, lam = -inf allowed (src/exmarg/search/pe.py:135-142)"""
-    lam, w = np.asarray(lam), np.asarray(w)
-    fin = np.isfinite(lam) & (w > 0)
-    if not bool(fin.any()):
-        return -np.inf
-    mx = np.where(fin, lam, -np.inf).max()
-    return float(mx + np.log(np.sum(np.where(fin, w * np.exp(np.where(fin, lam, mx) - mx), 0.0))))
 
 
 def _cell_weights(bank):
@@ -205,21 +171,47 @@ class _Analysis:
         self.plan, self.row = cache[key]
         return self
 
-    def node_values(self, seg, j, n_loud, n_random, seed):
-        """lamN (nn+1, nm) of segment j of seg (src/exmarg/search/pe.py:168-200)"""
-        bank = self.bank
-        traj = bank.trajectories({I: seg["d_" + I][j] for I in "HL"}, {I: seg["sig2_" + I] for I in "HL"})
+    def use_weights(self, prior):
+        """give the plan's reads the distance weights of prior and the bank's cell weights, for the
+        device assembly of ln Lambda and ln B"""
+        lnW = bank_weights(self, prior)
+        if self.plan.weights_key != prior:
+            self.plan.set_weights(lnW[self.row, self.plan.job_m], _cell_weights(self.bank))
+            self.plan.weights_key = prior
+        return self
+
+    def _filtered(self, seg, j, n_loud, n_random, seed):
+        """trajectories and arrival times of segment j of seg on the device"""
+        traj = self.bank.trajectories({I: seg["d_" + I][j] for I in "HL"}, {I: seg["sig2_" + I] for I in "HL"})
         times = None
         if n_loud > 0:
             times = _gather.select_times(traj, self.store.tau_idx, n_loud, n_random, seed + int(seg["rows"][j]))
-        vals = self.plan.run(traj, self.dev, times)
-        if times is not None:
-            times.free()
-        traj.free()
+        return traj, times
+
+    def node_values(self, seg, j, n_loud, n_random, seed):
+        """lamN (nn+1, nm) of segment j of seg (src/exmarg/search/pe.py:168-200)"""
+        traj, times = self._filtered(seg, j, n_loud, n_random, seed)
+        try:
+            vals = self.plan.run(traj, self.dev, times)
+        finally:
+            if times is not None:
+                times.free()
+            traj.free()
         nn = self.geometry[0].shape[0]
-        lamN = np.full((nn + 1, bank.u.size), -np.inf)
+        lamN = np.full((nn + 1, self.bank.u.size), -np.inf)
         lamN[self.row, self.plan.job_m] = vals
         return lamN
+
+    def ln_lambda(self, seg, j, n_loud, n_random, seed):
+        """(ln Lambda (nm), ln B) of segment j of seg, assembled on the device with the weights of
+        use_weights (src/exmarg/search/pe.py:320-346)"""
+        traj, times = self._filtered(seg, j, n_loud, n_random, seed)
+        try:
+            return self.plan.run_lse(traj, self.dev, times)
+        finally:
+            if times is not None:
+                times.free()
+            traj.free()
 
     def close(self):
         if self.dev is not None and not self.resident and not self.was_resident:
@@ -246,18 +238,15 @@ def pe_intrinsic(segments, bank, store, rows=None, prior="volume", ymin=0.1267, 
     returns dict: M, q, chi, u (n_template,), ln_lambda (n_segment, n_template), ln_b (n_segment,), rows."""
     seg = read_segments(segments, rows)
     an = _Analysis(bank, store, ymin, resident)
-    lnW = bank_weights(an, prior)
-    an.start()
+    bank_weights(an, prior)                 # the host-side checks before the tables go up
+    an.start().use_weights(prior)
     try:
         out = np.full((seg["rows"].size, bank.u.size), -np.inf)
+        ln_b = np.full(seg["rows"].size, -np.inf)
         for j in range(seg["rows"].size):
-            v = lnW + an.node_values(seg, j, n_loud, n_random, seed)
-            mx = v.max(axis=0)
-            out[j] = mx + np.log(np.exp(v - mx[None, :]).sum(axis=0))
+            out[j], ln_b[j] = an.ln_lambda(seg, j, n_loud, n_random, seed)
     finally:
         an.close()
-    w = _cell_weights(bank)
-    ln_b = np.array([_log_mean(L, w) for L in out])
     return dict(M=bank.M, q=bank.q, chi=bank.chi, u=bank.u, ln_lambda=out, ln_b=ln_b, rows=seg["rows"])
 
 

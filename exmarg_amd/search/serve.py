@@ -45,7 +45,7 @@ class Request:
         self.t_submit = t_submit
         self.t_admit = self.t_done = None
         self.ln_lambda = self.ln_b = None
-        self.bank = self.an = self.lnW = None
+        self.bank = self.an = None
         self.error = None
 
     @property
@@ -195,15 +195,15 @@ class Server:
         try:
             bank = self._bank(req.rows, seg, req.factors)
             an = _pe._Analysis(bank, self.store, self.ymin, True)
-            lnW = _pe.bank_weights(an, self.prior)
-            an.start()
+            _pe.bank_weights(an, self.prior)
+            an.start().use_weights(self.prior)
             batch.add(an, {I: seg["d_" + I][0] for I in "HL"}, {I: seg["sig2_" + I] for I in "HL"},
                       req.seed + int(seg["rows"][0]))
         except Exception as e:          # noqa: BLE001 - the request carries its own error
             if bank is not None:
                 self._unuse(bank)
             return self._refuse(req, e)
-        req.bank, req.an, req.lnW = bank, an, lnW
+        req.bank, req.an = bank, an
         req.t_admit = self.clock()
         self.members[self.filling].append(req)
         self.stats["admitted"] += 1
@@ -213,15 +213,11 @@ class Server:
 
     def _finish(self, k):
         batch = self.slots[k]
-        lamNs = batch.collect()
+        results = batch.collect_lse()
         t = self.clock()
-        for req, lamN in zip(self.members[k], lamNs):
-            v = req.lnW + lamN
-            mx = v.max(axis=0)
-            with np.errstate(invalid="ignore"):
-                L = mx + np.log(np.exp(v - mx[None, :]).sum(axis=0))
+        for req, (L, lnb) in zip(self.members[k], results):
             req.ln_lambda = L
-            req.ln_b = _pe._log_mean(L, _pe._cell_weights(req.bank))
+            req.ln_b = lnb
             req.t_done = t
             self._unuse(req.bank)
             req.an = None

@@ -157,6 +157,28 @@ int exm_plan_create(exm_handle store, int n_tmpl, int64_t njobs, const int32_t* 
  * the boundary reads of src/exmarg/search/pe.py:178-194 in one launch. */
 int exm_plan_gather(exm_handle plan, exm_handle traj, exm_handle store, exm_handle times, double* out);
 
+/* The distance-prior weights of every template over the points it reaches, float64 on the device:
+ * yN (nrow, nm) the points y = sqrt(a) of each template (y <= 0: not reached; nrow <= 64, every
+ * template at least two points); lnW (nrow, nm) receives ln w (-inf where not reached) with
+ * sum_j w_j Lambda(y_j) = integral p(y) Lambda(y) dy for Lambda piecewise linear in ln y.
+ * prior 0: p = 3 ymin^3 y^-4 integrated exactly per interval; prior 1: the density p(ln y) tabulated
+ * on lnx (ntab, ascending; zero outside), trapezoid rule on 401 points per interval.  Replaces
+ * src/exmarg/search/pe.py:260-286 (distance_weights) as pe_intrinsic applies it per template
+ * (pe.py:321-327). */
+int exm_distance_weights(const double* yN, int nrow, int nm, int prior, double ymin, const double* lnx,
+                         const double* px, int ntab, double* lnW);
+
+/* The node weights of a plan's reads: lnw (njobs) ln w of each read in the plan's order (the distance
+ * weight of its node, -inf allowed), cellw (n_tmpl) the template average weights (summing to one).
+ * Kept with the plan for exm_plan_gather_lse and for batches admitting it. */
+int exm_plan_set_weights(exm_handle plan, const double* lnw, const double* cellw);
+
+/* exm_plan_gather with the result assembled on the device in float64: ln_lambda (n_tmpl) = LSE over
+ * each template's reads of ln w + ln Lambda_read (src/exmarg/search/pe.py:333-336, in node order) and
+ * ln_b (1) = ln sum_m c_m e^{ln_lambda_m} over the finite entries (pe.py:135-142, _log_mean). */
+int exm_plan_gather_lse(exm_handle plan, exm_handle traj, exm_handle store, exm_handle times, double* ln_lambda,
+                        double* ln_b);
+
 /* Bytes of the no-allocation scratch region (one per device, held from the library's load until the
  * process ends). */
 #define EXM_NOALLOC_SCRATCH_BYTES (128u << 20)
@@ -208,6 +230,10 @@ int exm_batch_launch(exm_handle batch);
 int exm_reserve(int n_tmpl, int64_t n_time, int n_freq, int n_walk);
 int exm_batch_poll(exm_handle batch, int* done);
 int exm_batch_collect(exm_handle batch, double* out);
+/* As exm_batch_collect, with every admitted segment's result assembled on the device as
+ * exm_plan_gather_lse does (every segment's plan must carry node weights): ln_lambda (templates
+ * used, admission order) and ln_b (one per segment). */
+int exm_batch_collect_lse(exm_handle batch, double* ln_lambda, double* ln_b);
 
 /* Empty the batch (keeps its memory). */
 int exm_batch_reset(exm_handle batch);
