@@ -4,23 +4,27 @@
 // For every template, beta = ln a + 2 ln|z_L(t)| of a walked time t is increasing in ln|z_L(t)| at
 // every node and lag.  So, once per segment:
 //   k_walk_sort    sorts each template's walked times by ln|z_L|;
-//   k_walk_stream  writes, for every (template, lag) and walked time in that order, the pair of adjacent
-//                  stored phase bins that phi_H(t + tau) - phi_L(t) falls between (bins mirrored to
-//                  [0, 48]; the lower one is stored), the weight of the upper bin, a drawn-time flag and
-//                  ln|z_H(t + tau)|: 8 bytes per lookup, read contiguously by the gather;
+//   k_walk_stream  writes, for every (template, lag) and walked time in that order, one 8-byte record:
+//                  the Delta column and fraction (24-bit fixed point) of the lookup with beta unclamped,
+//                  which do not depend on the node; the pair of adjacent stored phase bins that
+//                  phi_H(t + tau) - phi_L(t) falls between (bins mirrored to [0, 48]; the record holds
+//                  the LDS offset of the lower bin in the Delta column) and the weight of the upper bin
+//                  (25-bit fixed point); a drawn-time flag.  ln|z_H(t + tau)| goes to a side array that
+//                  only the rows where beta may be clamped read;
 //   k_read_bounds  finds, for every read (template, table, ln a), where each beta row starts in the
 //                  sorted walk.
-// k_gather_piece then runs one workgroup per (table, lag, tile of at most 1024 reads of that table).
-// It sweeps the beta rows in increasing order with a ring of three rows (every phase bin and Delta
-// column, 3 x 43 KB at the reference geometry) in LDS.  At row b the lookups of the tile whose beta
-// row is b are cut into pieces of PIECE consecutive walked times of one read; the pieces are dealt to
-// the lanes in rounds of one piece per lane, each lane writes its piece's (max, sum) to LDS, and the
-// lane that owns a read merges that read's pieces in piece order.  Every table value a lookup reads
-// comes from LDS, every table row is read from HBM once per workgroup, and a read's result does not
-// depend on which other reads share its workgroup (the merge order is fixed by the read alone).
+// The gather (k_gather_wave, or the piece form k_gather_piece) then runs one workgroup per (table, lag,
+// tile of at most 1024 reads of that table) over the interior rows 1 .. nb - 3.  It sweeps the beta rows
+// in increasing order with a ring of three rows (every phase bin and Delta column, 3 x 43 KB at the
+// reference geometry) in LDS; at row b the lookups of the tile whose beta row is b are cut into pieces
+// of consecutive walked times of one read, dealt to the lanes in rounds, and each read's pieces are
+// merged by the lane that owns the read.  Every table value a lookup reads comes from LDS and every
+// table row is read from HBM once per workgroup.  k_gather_edge adds the lookups of rows 0 and nb - 2,
+// where beta may be clamped, with the Delta coordinate recomputed from ln|z_L| and ln|z_H| exactly.
 //
-// Arithmetic: index and fraction arithmetic, the interpolation and every sum in float64 (the MI355X
-// runs float64 FMAs at the float32 rate); exp of non-positive differences in float32.
+// Arithmetic (float64 form): the beta fraction, the interpolation and every sum in float64 (the MI355X
+// runs float64 FMAs at the float32 rate), exp of non-positive differences in float32; the float32 form
+// interpolates in float32.
 #include <climits>
 #include <cmath>
 
