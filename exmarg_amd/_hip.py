@@ -74,6 +74,7 @@ SIGNATURES = {
     "exm_profile_reset": [],
     "exm_profile_get": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)],
     "exm_copy_calibration": [I64, ctypes.POINTER(ctypes.c_double)],
+    "exm_debug_fail_alloc": [I64],
     "exm_tables_sum_orders": [P, P, I32, I32, I32, I32, P, P, I32, P, I32, P],
     "exm_source_frame": [P, P, I32, I32, P, P, P, I32, P, I32, P, P, P],
     "exm_strain_segments": [P, I32, I64, F64, I32, I32, P],

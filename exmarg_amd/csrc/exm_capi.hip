@@ -64,7 +64,15 @@ static void pool_release() {
     g_pool_bytes = 0;
 }
 
+// allocation-failure injection (exm_debug_fail_alloc): the allocation after this many more fails once
+static int64_t g_fail_after = -1;
+
 int dev_alloc(void** p, size_t bytes) {
+    if (g_fail_after >= 0 && g_fail_after-- == 0) {
+        set_error("device allocation of %zu bytes failed: injected failure", bytes);
+        *p = nullptr;
+        return EXM_ERR_NOMEM;
+    }
     bytes = block_size(bytes);
     // the smallest pooled block that holds the request: of the same size, or for large requests up to a
     // quarter larger (work buffers whose size varies from call to call then reuse one block instead of
@@ -1182,6 +1190,12 @@ int exm_batch_collect_lse(exm_handle hbatch, double* ln_lambda, double* ln_b) {
     b->lse_launched = false;
     batch_release_jobs(b);
     return rc;
+}
+
+int exm_debug_fail_alloc(int64_t after) {
+    LOCK;
+    g_fail_after = after < 0 ? -1 : after;
+    return EXM_OK;
 }
 
 int exm_batch_run(exm_handle hbatch, double* out) {

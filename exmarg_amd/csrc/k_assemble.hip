@@ -131,7 +131,7 @@ constexpr int LM_THREADS = 256;
 
 // ln_b[g] = ln sum c_m e^{L_m} over the templates seg[g] .. seg[g+1] with finite L and c > 0 (-inf if
 // none); one workgroup per segment, a fixed strided split and tree
-__global__ void __launch_bounds__(LM_THREADS) k_log_mean(const double* __restrict__ L, const double* __restrict__ c,
+__global__ void __launch_bounds__(LM_THREADS) k_bank_mean(const double* __restrict__ L, const double* __restrict__ c,
                                                          const int32_t* __restrict__ seg, double* __restrict__ ln_b) {
     __shared__ double red[LM_THREADS];
     const int g = blockIdx.x, t = threadIdx.x;
@@ -216,7 +216,7 @@ int LseDevice::launch(const double* val, hipStream_t s) const {
         EXM_HIP(hipGetLastError());
     }
     if (nseg > 0) {
-        hipLaunchKernelGGL(k_log_mean, dim3((unsigned)nseg), dim3(LM_THREADS), 0, s, L, cell, seg, ln_b);
+        hipLaunchKernelGGL(k_bank_mean, dim3((unsigned)nseg), dim3(LM_THREADS), 0, s, L, cell, seg, ln_b);
         EXM_HIP(hipGetLastError());
     }
     return EXM_OK;

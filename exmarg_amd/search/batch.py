@@ -197,7 +197,11 @@ def pe_intrinsic_batch(items, store, prior="volume", ymin=0.1267, n_loud=1024, n
         else:
             half_b = int(max_bytes) // 2
         first = SegmentBatch(store, n_loud, n_random, max_templates=half_t, max_bytes=half_b)
-        slots = _Slots([first, SegmentBatch(store, n_loud, n_random, max_templates=first.cap)])
+        try:
+            slots = _Slots([first, SegmentBatch(store, n_loud, n_random, max_templates=first.cap)])
+        except BaseException:
+            first.free()
+            raise
         if keep_buffer:
             store.__dict__["_batch"] = (key, slots)
     outs = [np.full((seg["rows"].size, bank.u.size), -np.inf) for seg, bank, _ in prepared]

@@ -185,7 +185,12 @@ class _Analysis:
         traj = self.bank.trajectories({I: seg["d_" + I][j] for I in "HL"}, {I: seg["sig2_" + I] for I in "HL"})
         times = None
         if n_loud > 0:
-            times = _gather.select_times(traj, self.store.tau_idx, n_loud, n_random, seed + int(seg["rows"][j]))
+            try:
+                times = _gather.select_times(traj, self.store.tau_idx, n_loud, n_random,
+                                             seed + int(seg["rows"][j]))
+            except BaseException:
+                traj.free()
+                raise
         return traj, times
 
     def node_values(self, seg, j, n_loud, n_random, seed):

@@ -331,6 +331,11 @@ int exm_profile_reset(void);
 /* total milliseconds and launch count of stage `name` since the last reset (synchronizes). */
 int exm_profile_get(const char* name, double* total_ms, int64_t* count);
 
+/* Failure injection for tests of the error paths: the device allocation after `after` more
+ * successful ones (pooled reuse included) fails once with EXM_ERR_NOMEM, as a refused hipMalloc
+ * would; a negative value clears it.  No reference counterpart. */
+int exm_debug_fail_alloc(int64_t after);
+
 /* Copy `bytes` (a multiple of 16) between two fresh device buffers with a plain copy kernel and
  * return its time: the kernel of known traffic that calibrates the memory-traffic counters of a
  * profiler.  No reference counterpart. */
