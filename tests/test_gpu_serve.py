@@ -123,3 +123,29 @@ def test_repeated_banks_are_cached(store):
         for h in (handles[0], handles[3]):                  # same inputs as the request served alone
             np.testing.assert_allclose(h.ln_lambda, alone.ln_lambda, rtol=0, atol=MIX_TOL)
         assert abs(handles[2].ln_b - handles[0].ln_b) < 1.0       # another draw, the same bank
+
+
+def test_bad_requests_do_not_hold_up_others(store):
+    """a segment too short for the walk (n_time <= n_loud) and a bank row the waveform model rejects
+    (q > 1), queued ahead of a valid request, are refused with their own errors; the valid request is
+    served, and the refused ones hold no bank"""
+    from exmarg_amd.search import Server
+    reqs = requests()
+    short, _, _ = synth.simulate(1, 45, lambda r: (30.0, 0.8, 0.1), lambda r: 0.0, 0.25, 20.0, 64.0)
+    bad_rows = np.array([[30.0, 1.5, 0.1], [31.0, 0.8, 0.1]])
+    srv = Server(store, n_loud=128, n_random=128, slot_templates=30, max_request=(24, 32768, 1000),
+                 bank_cache_bytes=0)
+    try:
+        h_short = srv.submit(short, reqs[1][1], seed=1)
+        h_bad = srv.submit(reqs[1][0], bad_rows, seed=1)
+        h_good = srv.submit(*reqs[1][:2], seed=1)
+        srv.run()
+        held = len(srv.banks)
+        stats = dict(srv.stats)
+    finally:
+        srv.close()
+    say("short:", h_short.error, "| bad row:", h_bad.error, "| stats", stats)
+    assert isinstance(h_short.error, ValueError) and "n_loud" in str(h_short.error)
+    assert isinstance(h_bad.error, ValueError)
+    assert h_good.error is None and np.isfinite(h_good.ln_b)
+    assert stats["refused"] == 2 and stats["finished"] == 1 and held == 0
