@@ -46,7 +46,7 @@ def parse():
     p.add_argument("--cpu-templates", type=int, default=32, help="templates in the CPU-baseline sample")
     p.add_argument("--check-templates", type=int, default=8, help="templates checked against the oracle")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle check")
-    p.add_argument("--gather-kernel", type=int, default=0, help="0 automatic, 1 reference-order kernel, 2 band kernel")
+    p.add_argument("--gather-kernel", type=int, default=0, help="0 automatic, 1 reference-order kernel, 2-4 band kernel (piece, wave, scheduled form)")
     p.add_argument("--piece", type=int, default=0, help="walked times per piece of the band kernel (0 default)")
     p.add_argument("--tile", type=int, default=0, help="reads per band-kernel workgroup (0 default)")
     p.add_argument("--precision", choices=["f64", "f32"], default="f64",
@@ -694,6 +694,7 @@ def main():
     s_ms, s_n = _hip.profile_get("select")
     w_ms, w_n = _hip.profile_get("walk")
     b_ms, b_n = _hip.profile_get("bounds")
+    sc_ms, sc_n = _hip.profile_get("schedule")
     _hip.profile(False)
     f64_check = None
     if args.precision == "f32" or args.filter_precision == "f32":
@@ -740,7 +741,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "gather",
                      "bytes_per_launch": seg_bytes, "kernel_ms": 1e3 * g_s},
         "stages_ms": {"filter": f_ms / args.steps, "select": s_ms / args.steps,
-                      "walk": w_ms / args.steps, "bounds": b_ms / args.steps, "gather": g_ms / args.steps},
+                      "walk": w_ms / args.steps, "bounds": b_ms / args.steps, "schedule": sc_ms / args.steps,
+                      "gather": g_ms / args.steps},
         "gather_launches_per_step": g_n / args.steps,
         "lookups_per_step": lookups,
         "lookups_per_s": lookups / g_step,

@@ -1075,7 +1075,7 @@ int exm_batch_launch(exm_handle hbatch) {
         StreamScope scope(b->stream);
         EXM_TRY(jobset_build(njobs, b->h_m.data(), b->h_tab.data(), b->h_offb.data(), &b->js));
         auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-        const size_t bnd = al(piece_bounds_bytes(st, njobs));
+        const size_t bnd = al(piece_bounds_bytes(st, b->js, b->nw));
         const size_t pairs = al(sizeof(double) * 2 * (size_t)njobs * st->ntau);
         const size_t lse_at = bnd + pairs + ((sizeof(double) * (size_t)njobs + 255) & ~(size_t)255);
         const int nseg = (int)b->seg_tmpl.size();
@@ -1095,6 +1095,7 @@ int exm_batch_launch(exm_handle hbatch) {
         double* om = (double*)(w + bnd);
         double* os = om + (size_t)njobs * st->ntau;
         double* dout = (double*)(w + bnd + pairs);
+        b->res_off = bnd + pairs;
         int rc = launch_piece(st, b->d_stream, b->d_aH, b->d_wa, b->nw, b->d_lw, b->js, om, os, w);
         if (rc == EXM_OK) rc = launch_lse(b->js, st->ntau, om, os, b->d_lnnt, 0.0, dout);
         // with every segment's node weights: ln Lambda per template and ln B per segment on the device
@@ -1148,7 +1149,7 @@ int exm_batch_collect(exm_handle hbatch, double* out) {
     if (rc != EXM_OK) set_error("batch gather failed");
     if (rc == EXM_OK && njobs > 0) {
         auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-        const size_t off = al(piece_bounds_bytes(st, njobs)) + al(sizeof(double) * 2 * (size_t)njobs * st->ntau);
+        const size_t off = b->res_off;
         std::vector<double> sorted(njobs);
         if (hipMemcpyAsync(sorted.data(), (char*)b->d_work + off, sizeof(double) * njobs, hipMemcpyDeviceToHost,
                            b->stream) != hipSuccess ||
@@ -1234,8 +1235,9 @@ int exm_batch_info(exm_handle hbatch, int64_t* templates_used, int64_t* reads, i
 
 int exm_set_gather_kernel(int which) {
     LOCK;
-    EXM_CHECK(which >= 0 && which <= 3, EXM_ERR_ARG,
-              "gather kernel must be 0 (auto), 1 (reference), 2 (band, piece form) or 3 (band, wave form)");
+    EXM_CHECK(which >= 0 && which <= 4, EXM_ERR_ARG,
+              "gather kernel must be 0 (auto), 1 (reference), 2 (band, piece form), 3 (band, wave form) or 4 "
+              "(band, scheduled form)");
     g_gather_mode = which;
     return EXM_OK;
 }

@@ -195,6 +195,7 @@ struct Batch : Object {
     JobSet js;
     void* d_work = nullptr;              // first/last rows, row bounds, (max, sum) pairs, results
     size_t work_bytes = 0;
+    size_t res_off = 0;                  // where the launch put the results in d_work
     // node weights of every admitted segment (from its plan), for exm_batch_collect_lse: ln w per read in
     // admission order, cell weight per template, first template of every segment
     bool weights = true;
@@ -302,8 +303,9 @@ int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* strea
 int launch_walk(Traj* traj, Times* times, int64_t off, int nw, Store* st, float* wa_dst, uint2* stream_dst,
                 float* aH_dst);
 int band_slice(const Traj* traj, const Store* st, const Times* times);
-// bounds_buf: piece_bounds_bytes() of device memory for the row bounds (scratch slot 6 when null)
-size_t piece_bounds_bytes(const Store* st, int64_t njobs);
+// bounds_buf: piece_bounds_bytes() of device memory for the row bounds and the band kernel's schedule
+// (scratch slot 6 when null); its size depends on the band kernel form and piece length set at the call
+size_t piece_bounds_bytes(const Store* st, const JobSet& js, int nw);
 // accumulate: merge this walk's (max, sum) into om, os instead of writing them
 int launch_piece(Store* st, const uint2* stream, const float* aHs, const float* wa, int nw, const float* lw_g,
                  const JobSet& js, double* om, double* os, void* bounds_buf = nullptr, bool accumulate = false);

@@ -310,10 +310,41 @@ __device__ __forceinline__ void row_to_lds(const float* __restrict__ src, float*
     }
 }
 
-// Combine (m2, s2) into the running (m, s) of a log-sum-exp: one exp of the non-positive difference.
+// e^x over the whole float64 range (x clamped to +-1100, beyond which the result is 0 or inf anyway):
+// 2^n (ldexp) for the nearest integer n of x log2(e), times e^t, t = (x log2(e) - n) ln 2, |t| <= 0.347.
+// Float64 (V double): e^t by its Taylor polynomial to degree 12 (truncation < 0.347^13 / 13! < 2e-16).
+// Float32 form: 2^f by the float32 exp2 (1 ulp).
+template <typename V>
+__device__ __forceinline__ double exp_wide(double x) {
+    const double y = fmin(fmax(x, -1100.0), 1100.0) * 1.4426950408889634;
+    const double n = __builtin_rint(y);
+    if constexpr (sizeof(V) == 4) {
+        return ldexp((double)__builtin_amdgcn_exp2f((float)(y - n)), (int)n);
+    } else {
+        const double t = (y - n) * 0.6931471805599453;
+        constexpr double C[13] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040, 1.0 / 40320,
+                                  1.0 / 362880, 1.0 / 3628800, 1.0 / 39916800, 1.0 / 479001600};
+        double p = C[12];
+#pragma unroll
+        for (int k = 11; k >= 0; --k) p = fma(p, t, C[k]);
+        return ldexp(p, (int)n);
+    }
+}
+
+// Combine (m2, s2) into the running (m, s) of a log-sum-exp: one exp of the non-positive difference,
+// in float32 (the per-round merges of the wave and piece forms).
 __device__ __forceinline__ void lse_merge(double& m, double& s, double m2, double s2) {
     const double d = m2 - m;
     const double e = (double)__expf(-(float)fabs(d));
+    s = d > 0.0 ? fma(s, e, s2) : fma(s2, e, s);
+    m = fmax(m, m2);
+}
+
+// The same merge with the exp in float64 (exp_wide): for merging whole results (edge rows into the interior
+// rows', a walk slice into the previous ones), whose maxima may lie far apart while both sums matter.
+__device__ __forceinline__ void lse_merge_exact(double& m, double& s, double m2, double s2) {
+    const double d = m2 - m;
+    const double e = exp_wide<double>(-fabs(d));
     s = d > 0.0 ? fma(s, e, s2) : fma(s2, e, s);
     m = fmax(m, m2);
 }
@@ -567,7 +598,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_gather_piece(
     }
     if (tid < nj) {
         const int64_t o = (int64_t)(j0 + tid) * ntau + r;
-        if (accumulate) lse_merge(amx, asum, om[o], os[o]);
+        if (accumulate) lse_merge_exact(amx, asum, om[o], os[o]);
         om[o] = amx;
         os[o] = asum;
     }
@@ -872,9 +903,324 @@ __global__ __launch_bounds__(WAVE_THREADS) void k_gather_wave(
     }
     if (mine) {
         const int64_t o = ((int64_t)j0 + jl) * ntau + r;
-        if (accumulate) lse_merge(amx, asum, om[o], os[o]);
+        if (accumulate) lse_merge_exact(amx, asum, om[o], os[o]);
         om[o] = amx;
         os[o] = asum;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// The scheduled form of the band kernel.
+//
+// Which walked times of which read fall in which beta row does not depend on the lag, so the pieces a
+// workgroup deals (PN consecutive walked times of one read in one row) are listed once per launch, for
+// every tile and interior row, before the gather: k_sched_count counts the pieces of every (tile, row),
+// k_sched_emit writes their descriptors in row order and, within a row, in read order.  The gather
+// (k_gather_sched, one workgroup per (tile, lag) as the other forms) then sweeps the rows with the same
+// three-row LDS ring, and its waves take the pieces of the current row in chunks of 64 from a counter
+// in LDS, so they stay busy until the row's last chunk.  Each read's sum is kept in LDS against a fixed
+// reference: the value of its loudest walked time (the last of its sorted walk, found before the sweep
+// with the exact lookup), so that a piece contributes s e^{m - M} (its own (max, sum) rescaled to the
+// read's reference M) by one float64 LDS atomic add, and no lane has to meet the other pieces of its
+// read.  M is a value of the read's own sum, so the sum is at least ~1 wherever M's time is an interior
+// lookup; its terms exceed e^709 only if the read's values span more than ~700 (an overflow the final
+// step detects, flagging the (read, lag) for k_gather_edge to recompute exactly).
+//
+// Piece descriptor, 32 bits: the read's index in its tile (bits 0-9), the slots used minus one (bits
+// 10-12) and the walk position of its first slot (bits 13-31).
+constexpr int SCHED_THREADS = 1024;
+constexpr int SD_USE_SHIFT = 10, SD_POS_SHIFT = 13;
+constexpr uint32_t SD_READ_MASK = 0x3FFu;
+
+// pieces a read may hold in the interior rows at most: each row's run ends in at most one partial piece
+__host__ __device__ inline int64_t sched_cap_per_read(int nw, int pn, int nb) { return (nw + pn - 1) / pn + nb; }
+
+// walk range of read j in interior row b (empty outside the read's rows and outside 1 .. nb - 3)
+__device__ __forceinline__ void interior_run(const int32_t* __restrict__ first_last,
+                                             const uint16_t* __restrict__ bounds, int64_t j, int nb, int b,
+                                             int& lo, int& hi) {
+    lo = hi = 0;
+    const int first = first_last[2 * j], last = first_last[2 * j + 1];
+    if (b >= 1 && b <= nb - 3 && b >= first && b <= last) {
+        const uint16_t* bj = bounds + j * (nb + 1);
+        lo = bj[b - first];
+        hi = bj[b + 1 - first];
+    }
+}
+
+// sum over the workgroup of one int per thread (every thread receives it); red holds NWAVE ints
+template <int NWAVE>
+__device__ __forceinline__ int block_sum(int v, int* red) {
+#pragma unroll
+    for (int k = 32; k > 0; k >>= 1) v += __shfl_xor(v, k, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int s = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVE; ++w) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+// pieces of every (tile, row): cnt[t * (nb + 1) + b]
+template <int PN>
+__global__ __launch_bounds__(SCHED_THREADS) void k_sched_count(const int32_t* __restrict__ tiles,
+                                                               const int32_t* __restrict__ first_last,
+                                                               const uint16_t* __restrict__ bounds, int nb,
+                                                               int32_t* __restrict__ cnt) {
+    __shared__ int red[SCHED_THREADS / 64];
+    const int t = blockIdx.x, b = blockIdx.y;
+    const int j0 = tiles[2 * t], nj = tiles[2 * t + 1];
+    int q = 0;
+    if ((int)threadIdx.x < nj) {
+        int lo, hi;
+        interior_run(first_last, bounds, (int64_t)j0 + threadIdx.x, nb, b, lo, hi);
+        q = (hi - lo + PN - 1) / PN;
+    }
+    const int total = block_sum<SCHED_THREADS / 64>(q, red);
+    if (threadIdx.x == 0) cnt[(int64_t)t * (nb + 1) + b] = total;
+}
+
+// descriptors of every (tile, row): the tile's pieces start at desc + j0 * cap (its first read's index
+// times the per-read capacity), row b's at rowoff[t * (nb + 1) + b] after that, rowoff[.. + nb] = total
+template <int PN>
+__global__ __launch_bounds__(SCHED_THREADS) void k_sched_emit(const int32_t* __restrict__ tiles,
+                                                              const int32_t* __restrict__ first_last,
+                                                              const uint16_t* __restrict__ bounds, int nb,
+                                                              const int32_t* __restrict__ cnt, int64_t cap,
+                                                              int32_t* __restrict__ rowoff,
+                                                              uint32_t* __restrict__ desc) {
+    __shared__ int red[SCHED_THREADS / 64];
+    __shared__ int wex[SCHED_THREADS / 64];
+    const int t = blockIdx.x, b = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int j0 = tiles[2 * t], nj = tiles[2 * t + 1];
+    const int32_t* ct = cnt + (int64_t)t * (nb + 1);
+    // where row b starts: the pieces of the rows before it (nb < 1024 rows, one per thread)
+    const int start = block_sum<SCHED_THREADS / 64>(tid < b ? ct[tid] : 0, red);
+    int lo = 0, hi = 0;
+    if (tid < nj) interior_run(first_last, bounds, (int64_t)j0 + tid, nb, b, lo, hi);
+    const int q = (hi - lo + PN - 1) / PN;
+    // exclusive sum of q over the workgroup: within the wave by shuffles, then over the waves
+    int x = q;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wex[wid] = x;
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < wid; ++w) before += wex[w];
+    const int ex = before + x - q;
+    uint32_t* out = desc + (int64_t)j0 * cap + start + ex;
+    for (int i = 0; i < q; ++i) {
+        const int pa = lo + i * PN;
+        const int use = min(PN, hi - pa);
+        out[i] = (uint32_t)tid | ((uint32_t)(use - 1) << SD_USE_SHIFT) | ((uint32_t)pa << SD_POS_SHIFT);
+    }
+    if (tid == 0) {
+        rowoff[(int64_t)t * (nb + 1) + b] = start;
+        if (b == nb - 1) rowoff[(int64_t)t * (nb + 1) + nb] = start + ct[b];
+    }
+}
+
+// bytes of LDS the scheduled kernel uses for a row of rowlen floats: the ring, then per read of the tile
+// its ln-a offset and sum (float64), template and drawn-time weight (int, float) and reference (float),
+// then the chunk counters and the band range
+__host__ __device__ constexpr size_t sched_lds_bytes(int rowlen) {
+    return piece_res_offset(rowlen) + (size_t)TILE_JOBS * (8 + 8 + 8 + 4) + 8 * sizeof(int);
+}
+
+template <int PN, typename V>
+__global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
+    const uint2* __restrict__ stream, const float* __restrict__ aHs, const float* __restrict__ wa, int nw,
+    const float* __restrict__ lw_g, const float* __restrict__ T, int64_t tstride, int nphif, int nb, int nd,
+    int ntau, const int32_t* __restrict__ jm, const int32_t* __restrict__ jtab, const double* __restrict__ joffb,
+    const int32_t* __restrict__ tiles, const int32_t* __restrict__ first_last, const uint32_t* __restrict__ desc,
+    int64_t cap, const int32_t* __restrict__ rowoff, double rdb, double offd, double rdd, double* __restrict__ om,
+    double* __restrict__ os, uint8_t* __restrict__ flags, int accumulate) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int rowlen = nphif * nd, slot = row_slot(rowlen);
+    float* rows = (float*)lds;                                              // 3 ring slots
+    double* s_offb = (double*)(lds + piece_res_offset(rowlen));
+    double* s_acc = s_offb + TILE_JOBS;
+    int2* s_mlw = (int2*)(s_acc + TILE_JOBS);                               // template, drawn-time weight
+    float* s_M = (float*)(s_mlw + TILE_JOBS);
+    int* s_ctr = (int*)(s_M + TILE_JOBS);                                   // [0..2] chunks taken, [4..5] band
+    const int r = blockIdx.y, t = blockIdx.x;
+    const int j0 = tiles[2 * t], nj = tiles[2 * t + 1];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const float* Tr = T + (int64_t)jtab[j0] * tstride + (int64_t)r * nb * rowlen;
+    const BandGeom geo{rdb, offd, rdd, rdd / rdb, nb - 1 - 1e-6, nd - 1 - 1e-6, nd};
+    if (tid == 0) {
+        s_ctr[4] = INT_MAX;
+        s_ctr[5] = INT_MIN;
+    }
+    __syncthreads();
+    if (tid < nj) {
+        const int64_t j = (int64_t)j0 + tid;
+        const int first = first_last[2 * j], last = first_last[2 * j + 1];
+        const int m = jm[j];
+        const double offb = joffb[j];
+        const float lw = lw_g[m];
+        s_offb[tid] = offb;
+        s_acc[tid] = 0.0;
+        s_mlw[tid] = make_int2(m, __float_as_int(lw));
+        // the reference: the value at the loudest walked time (row `last`, the exact lookup from memory)
+        const int64_t e = ((int64_t)m * ntau + r) * nw + nw - 1;
+        const float* R0 = Tr + (int64_t)last * rowlen;
+        const double M = exact_lookup<V>(R0, R0 + rowlen, geo, (double)last, offb, (double)lw,
+                                         wa[(int64_t)m * nw + nw - 1], stream[e], aHs[e]);
+        s_M[tid] = (M > -1e30 && M < 1e30) ? (float)M : 0.0f;
+        atomicMin(&s_ctr[4], first);
+        atomicMax(&s_ctr[5], last);
+    }
+    __syncthreads();
+    // the interior rows of the band (rows 0 and nb - 2, where beta may be clamped, are k_gather_edge's)
+    const int b_start = max(s_ctr[4], 1), b_end = min(s_ctr[5], nb - 3);
+    if (b_start <= b_end) {
+        const int b_top = min(b_end + 1, nb - 1);
+        const int wid = tid >> 6;
+        if (tid == 0) {
+            s_ctr[b_start % 3] = 0;
+            s_ctr[(b_start + 1) % 3] = 0;
+        }
+        for (int q = 0; q < 3 && b_start + q <= b_top; ++q)
+            row_to_lds(Tr + (int64_t)(b_start + q) * rowlen, rows + ((b_start + q) % 3) * slot, slot, wid, lane,
+                       SCHED_THREADS);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        const int32_t* ro = rowoff + (int64_t)t * (nb + 1);
+        const uint32_t* dt = desc + (int64_t)j0 * cap;
+        // a chunk: 64 consecutive pieces of one row, one per lane (lanes past the row's last piece use
+        // none of their slots)
+        struct Chunk {
+            int row;
+            int jl, nuse;
+            double offb;
+            float lw, M;
+            float aL[PN];
+            uint2 ev[PN];
+        };
+        // the next chunk of row b (its index from the row's counter), or -1 once the row is dealt
+        auto take = [&](int b) -> int {
+            int c = 0;
+            if (lane == 0) c = atomicAdd(&s_ctr[b % 3], 1);
+            c = __builtin_amdgcn_readfirstlane(c);
+            return c * 64 < ro[b + 1] - ro[b] ? c : -1;
+        };
+        auto fetch = [&](int b, int c, Chunk& P) {
+            const int Q = ro[b + 1] - ro[b];
+            const int i = c * 64 + lane;
+            const bool act = i < Q;
+            const uint32_t d = act ? dt[ro[b] + i] : 0u;
+            P.row = b;
+            P.jl = (int)(d & SD_READ_MASK);
+            P.nuse = act ? (int)((d >> SD_USE_SHIFT) & 7u) + 1 : 0;
+            const int pa = (int)(d >> SD_POS_SHIFT);
+            const int2 mlw = s_mlw[P.jl];
+            P.offb = s_offb[P.jl];
+            P.lw = __int_as_float(mlw.y);
+            P.M = s_M[P.jl];
+            load_piece<PN>(wa + (int64_t)mlw.x * nw + pa, stream + ((int64_t)mlw.x * ntau + r) * nw + pa, P.aL, P.ev);
+        };
+        // one chunk's lookups: each piece's (max, sum), rescaled to its read's reference and added to the
+        // read's sum
+        auto compute = [&](const Chunk& P) __attribute__((always_inline)) {
+            const int b = P.row;
+            const int sb = b % 3;
+            const float* R0 = rows + sb * slot;
+            const float* R1 = rows + (sb + 1 == 3 ? 0 : sb + 1) * slot;
+            const double cb = P.offb - (double)b;
+            const int nuse = P.nuse;
+            double mx, sm = 0.0;
+            if constexpr (sizeof(V) == 4) {
+                float av[PN];
+#pragma unroll
+                for (int k = 0; k < PN; ++k) {
+                    const float v = lean_lookup<float>(R0, R1, rdb, cb, P.aL[k], P.ev[k]);
+                    av[k] = k < nuse ? v : -INFINITY;
+                }
+                float mf = av[0];
+#pragma unroll
+                for (int k = 1; k < PN; ++k) mf = fmaxf(mf, av[k]);
+                constexpr float LOG2E = 1.4426950408889634f;
+                const float ew = __expf(P.lw), ml = mf * LOG2E;
+#pragma unroll
+                for (int k = 0; k < PN; ++k) {
+                    const float ex = __builtin_amdgcn_exp2f(fmaf(av[k], LOG2E, -ml));
+                    sm += (double)((P.ev[k].y & REC_DRAWN) ? ex * ew : ex);
+                }
+                mx = (double)mf;
+            } else {
+                const double lw = (double)P.lw;
+                double av[PN];
+#pragma unroll
+                for (int k = 0; k < PN; ++k) {
+                    const double a = (double)lean_lookup<V>(R0, R1, rdb, cb, P.aL[k], P.ev[k]) +
+                                     ((P.ev[k].y & REC_DRAWN) ? lw : 0.0);
+                    av[k] = k < nuse ? a : -1e300;
+                }
+                float mf = (float)av[0];
+#pragma unroll
+                for (int k = 1; k < PN; ++k) mf = fmaxf(mf, (float)av[k]);
+                mx = (double)mf;
+#pragma unroll
+                for (int k = 0; k < PN; ++k) sm += (double)__expf((float)(av[k] - mx));
+            }
+            if (nuse > 0) atomicAdd(&s_acc[P.jl], sm * exp_wide<V>(mx - (double)P.M));
+        };
+
+        Chunk cur, nxt;
+        bool have = false;                      // cur holds a chunk of row b
+        for (int b = b_start; b <= b_end; ++b) {
+            // row b + 2 into the slot row b - 1 left; it lands while row b is read.  Row b + 2's counter
+            // (row b - 1's, last used before the barrier that ended row b - 1) restarts for it.
+            if (b > b_start && b + 2 <= b_top)
+                row_to_lds(Tr + (int64_t)(b + 2) * rowlen, rows + ((b + 2) % 3) * slot, slot, wid, lane,
+                           SCHED_THREADS);
+            if (tid == 0) s_ctr[(b + 2) % 3] = 0;
+            if (!have) {
+                const int c = take(b);
+                if (c >= 0) {
+                    fetch(b, c, cur);
+                    have = true;
+                }
+            }
+            while (have) {
+                // the next chunk (this row's, or once it is dealt, row b + 1's) is requested before this
+                // one is computed
+                int nb_ = b, c = take(b);
+                if (c < 0 && b + 1 <= b_end) {
+                    nb_ = b + 1;
+                    c = take(b + 1);
+                }
+                if (c >= 0) fetch(nb_, c, nxt);
+                compute(cur);
+                have = c >= 0;
+                if (have) cur = nxt;
+                if (have && nb_ != b) break;    // cur already holds row b + 1's first chunk
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
+    }
+    if (tid < nj) {
+        const int64_t o = ((int64_t)j0 + tid) * ntau + r;
+        const double S = s_acc[tid];
+        const bool bad = !(S < INFINITY);       // a term beyond float64 (or a NaN input): recomputed
+        // the sum as (M + ln S, 1): the log-sum itself, so that the merges after it (edge rows, slices,
+        // lags) rescale terms of comparable size
+        double mx = -1e300, sm = 0.0;
+        if (!bad && S > 0.0) {
+            mx = (double)s_M[tid] + log(S);
+            sm = 1.0;
+        }
+        if (accumulate) lse_merge_exact(mx, sm, om[o], os[o]);
+        om[o] = mx;
+        os[o] = sm;
+        flags[o] = bad ? 1 : 0;
     }
 }
 
@@ -882,20 +1228,22 @@ __global__ __launch_bounds__(WAVE_THREADS) void k_gather_wave(
 // recomputes its Delta coordinate from ln|z_L| and ln|z_H| (exact_lookup).  One thread per (read, lag)
 // walks the read's times in those rows, reading the table values from memory (about 0.1 % of the lookups
 // fall there; most threads find none and return at once), and merges its (max, sum) into om, os, which
-// the interior rows' gather has written.
+// the interior rows' gather has written.  A (read, lag) the scheduled form flagged (flags, when given)
+// has its interior rows recomputed here the same way.
 template <typename V>
 __global__ __launch_bounds__(256) void k_gather_edge(
     const uint2* __restrict__ stream, const float* __restrict__ aHs, const float* __restrict__ wa, int nw,
     const float* __restrict__ lw_g, const float* __restrict__ T, int64_t tstride, int nphif, int nb, int nd,
     int ntau, const int32_t* __restrict__ jm, const int32_t* __restrict__ jtab, const double* __restrict__ joffb,
     int64_t njobs, const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds, double rdb,
-    double offd, double rdd, double* __restrict__ om, double* __restrict__ os) {
+    double offd, double rdd, double* __restrict__ om, double* __restrict__ os, const uint8_t* __restrict__ flags) {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int r = blockIdx.y;
     if (j >= njobs) return;
     const int first = first_last[2 * j], last = first_last[2 * j + 1];
     const bool low = first == 0, high = last >= nb - 2;
-    if (!low && !high) return;
+    const bool redo = flags != nullptr && flags[j * ntau + r] != 0;
+    if (!low && !high && !redo) return;
     const int rowlen = nphif * nd;
     const float* Tr = T + (int64_t)jtab[j] * tstride + (int64_t)r * nb * rowlen;
     const BandGeom geo{rdb, offd, rdd, rdd / rdb, nb - 1 - 1e-6, nd - 1 - 1e-6, nd};
@@ -904,22 +1252,24 @@ __global__ __launch_bounds__(256) void k_gather_edge(
     const uint16_t* bj = bounds + j * (nb + 1);
     const int64_t rec = ((int64_t)m * ntau + r) * nw;
     double amx = -1e300, asum = 0.0;
-    for (int pass = 0; pass < 2; ++pass) {
-        const int b = pass == 0 ? 0 : nb - 2;
+    // rows 0 and nb - 2, then (flagged) every interior row of the read
+    const int n_int = redo ? max(0, min(last, nb - 3) - max(first, 1) + 1) : 0;
+    for (int pass = 0; pass < 2 + n_int; ++pass) {
+        const int b = pass == 0 ? 0 : pass == 1 ? nb - 2 : max(first, 1) + pass - 2;
         if (b < first || b > last || (pass == 1 && nb - 2 == 0)) continue;
         const float* R0 = Tr + (int64_t)b * rowlen;
         for (int p = bj[b - first]; p < bj[b + 1 - first]; ++p) {
             const double a = exact_lookup<V>(R0, R0 + rowlen, geo, (double)b, offb, lw, wa[(int64_t)m * nw + p],
                                              stream[rec + p], aHs[rec + p]);
             const double d = a - amx;
-            const double e = (double)__expf(-(float)fabs(d));
+            const double e = exp_wide<double>(-fabs(d));
             asum = d > 0.0 ? fma(asum, e, 1.0) : asum + e;
             amx = fmax(amx, a);
         }
     }
     if (amx > -1e300) {
         const int64_t o = j * ntau + r;
-        lse_merge(amx, asum, om[o], os[o]);
+        lse_merge_exact(amx, asum, om[o], os[o]);
         om[o] = amx;
         os[o] = asum;
     }
@@ -978,21 +1328,60 @@ int launch_walk(Traj* traj, Times* times, Store* st, float* wa_dst, uint2* strea
 }
 
 // piece gather of every read of js; template ids index wa, stream and lw_g
-size_t piece_bounds_bytes(const Store* st, int64_t njobs) {
-    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    return al(sizeof(int32_t) * 2 * (size_t)njobs) + al(sizeof(uint16_t) * (size_t)njobs * (st->nb + 1));
+// band kernel form of launch_piece: the scheduled form (the default) unless the piece or wave form is
+// asked for (exm_set_gather_kernel 2 or 3)
+enum class BandForm { Sched, Wave, Piece };
+static BandForm band_form() {
+    const int mode = gather_mode();
+    return mode == 2 ? BandForm::Piece : mode == 3 ? BandForm::Wave : BandForm::Sched;
 }
+
+// walked times per piece of the form that launch_piece runs (defaults measured at 2000 templates x 2048
+// walked times x 53 nodes: the scheduled form 46.7 ms with 8 against 47.4 ms with 6 and 52.5 ms with 4;
+// the wave form 69.2 ms with 6 against 72.9 ms with 4 and 73.6 ms with 8)
+static int band_piece_len(BandForm f) {
+    const int p = ctx().piece;
+    return p ? p : (f == BandForm::Piece ? PIECE : f == BandForm::Sched ? 8 : 6);
+}
+
+// Work memory of launch_piece beside the walk: first/last rows and row bounds of every read, and for the
+// scheduled form the recompute flags (one byte per read and lag), the pieces per (tile, row), the row
+// offsets and the piece descriptors (at most sched_cap_per_read per read).
+struct PieceWork {
+    size_t fl, bnd, flags, cnt, rowoff, desc, total;
+};
+static PieceWork piece_work(const Store* st, const JobSet& js, int nw) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    PieceWork w{};
+    const size_t nj = (size_t)js.njobs;
+    w.fl = 0;
+    w.bnd = al(sizeof(int32_t) * 2 * nj);
+    size_t end = w.bnd + al(sizeof(uint16_t) * nj * (st->nb + 1));
+    const BandForm f = band_form();
+    if (f == BandForm::Sched) {
+        const size_t rows = (size_t)js.ntiles * (st->nb + 1);
+        w.flags = end;
+        w.cnt = w.flags + al(nj * st->ntau);
+        w.rowoff = w.cnt + al(sizeof(int32_t) * rows);
+        w.desc = w.rowoff + al(sizeof(int32_t) * rows);
+        end = w.desc + al(sizeof(uint32_t) * nj * (size_t)sched_cap_per_read(nw, band_piece_len(f), st->nb));
+    }
+    w.total = end;
+    return w;
+}
+
+size_t piece_bounds_bytes(const Store* st, const JobSet& js, int nw) { return piece_work(st, js, nw).total; }
 
 int launch_piece(Store* st, const uint2* stream, const float* aHs, const float* wa, int nw, const float* lw_g,
                  const JobSet& js, double* om, double* os, void* bounds_buf, bool accumulate) {
     Context& c = ctx();
     const hipStream_t cs = cur_stream();
-    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t fl_bytes = al(sizeof(int32_t) * 2 * (size_t)js.njobs);
+    const PieceWork pw = piece_work(st, js, nw);
     void* bbuf = bounds_buf;
-    if (!bbuf) EXM_TRY(scratch(6, piece_bounds_bytes(st, js.njobs), &bbuf));
-    int32_t* first_last = (int32_t*)bbuf;
-    uint16_t* bounds = (uint16_t*)((char*)bbuf + fl_bytes);
+    if (!bbuf) EXM_TRY(scratch(6, pw.total, &bbuf));
+    char* wb = (char*)bbuf;
+    int32_t* first_last = (int32_t*)(wb + pw.fl);
+    uint16_t* bounds = (uint16_t*)(wb + pw.bnd);
     const double rdb = 2.0 / st->dbeta;
     {
         ProfileScope prof("bounds");
@@ -1003,40 +1392,81 @@ int launch_piece(Store* st, const uint2* stream, const float* aHs, const float* 
     EXM_CHECK(piece_res_offset(st->nphif * st->nd) % 16 == 0, EXM_ERR_ARG, "piece results misaligned");
     EXM_CHECK(row_slot(st->nphif * st->nd) - st->nphif * st->nd <= TABLE_PAD_FLOATS, EXM_ERR_ARG,
               "row copy overruns the table padding");
-    // the wave form unless the piece form is asked for; 6 walked times per piece in the wave form (measured
-    // at 2000 templates x 2048 walked times x 53 nodes: 69.2 ms against 72.9 ms with 4 and 73.6 ms with 8)
-    const bool wave = gather_mode() != 2;
-    static_assert(TILE_JOBS <= WAVE_THREADS && TILE_JOBS <= BAND_THREADS, "a workgroup holds every read of its tile");
+    const BandForm form = band_form();
+    static_assert(TILE_JOBS <= WAVE_THREADS && TILE_JOBS <= BAND_THREADS && TILE_JOBS <= SCHED_THREADS &&
+                      TILE_JOBS <= (int)SD_READ_MASK + 1,
+                  "a workgroup holds every read of its tile");
     const int rowlen = st->nphif * st->nd;
-    const size_t lds = wave ? piece_res_offset(rowlen) + 16 : piece_lds_bytes(rowlen);
-    EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "three table rows and the piece results (%zu bytes) exceed the LDS", lds);
-    const int pn = c.piece ? c.piece : (wave ? 6 : PIECE);
-    EXM_CHECK(!wave || pn != 16, EXM_ERR_ARG, "the wave form of the band kernel takes pieces of 4, 6 or 8 walked times");
-    EXM_CHECK(wave || pn != 6, EXM_ERR_ARG, "the piece form of the band kernel takes pieces of 4, 8 or 16 walked times");
-    using Kern = decltype(&k_gather_piece<8, double>);
-    Kern kern;
-    if (wave)
-        kern = c.f32_interp ? (pn == 4 ? k_gather_wave<4, float> : pn == 6 ? k_gather_wave<6, float> : k_gather_wave<8, float>)
-                            : (pn == 4 ? k_gather_wave<4, double> : pn == 6 ? k_gather_wave<6, double>
-                                                                            : k_gather_wave<8, double>);
-    else
-        kern = c.f32_interp ? (pn == 4 ? k_gather_piece<4, float> : pn == 16 ? k_gather_piece<16, float>
-                                                                             : k_gather_piece<8, float>)
-                            : (pn == 4 ? k_gather_piece<4, double> : pn == 16 ? k_gather_piece<16, double>
-                                                                              : k_gather_piece<8, double>);
-    EXM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    ProfileScope prof("gather");
-    hipLaunchKernelGGL(kern, dim3(js.ntiles, st->ntau), dim3(wave ? WAVE_THREADS : BAND_THREADS), lds, cs, stream,
-                       wa, nw, lw_g,
-                       st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab, js.d_offb,
-                       js.d_tiles, first_last, bounds, rdb, -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os,
-                       accumulate ? 1 : 0);
-    EXM_HIP(hipGetLastError());
-    // rows 0 and nb - 2 (beta possibly clamped), merged into the interior rows' (max, sum)
+    const size_t lds = form == BandForm::Sched  ? sched_lds_bytes(rowlen)
+                       : form == BandForm::Wave ? piece_res_offset(rowlen) + 16
+                                                : piece_lds_bytes(rowlen);
+    EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "three table rows and the per-read state (%zu bytes) exceed the LDS", lds);
+    const int pn = band_piece_len(form);
+    EXM_CHECK(form == BandForm::Piece || pn != 16, EXM_ERR_ARG,
+              "the scheduled and wave forms of the band kernel take pieces of 4, 6 or 8 walked times");
+    EXM_CHECK(form != BandForm::Piece || pn != 6, EXM_ERR_ARG,
+              "the piece form of the band kernel takes pieces of 4, 8 or 16 walked times");
+    const double offd = -st->delta0 / st->ddelta, rdd = 1.0 / st->ddelta;
+    uint8_t* flags = nullptr;
+    if (form == BandForm::Sched) {
+        // the pieces of every (tile, interior row), listed once for all lags
+        EXM_CHECK(st->nb <= SCHED_THREADS, EXM_ERR_ARG, "the schedule takes at most %d beta rows", SCHED_THREADS);
+        EXM_CHECK(nw < (1 << (32 - SD_POS_SHIFT)), EXM_ERR_ARG, "walk of %d times exceeds the piece descriptor", nw);
+        const int64_t cap = sched_cap_per_read(nw, pn, st->nb);
+        EXM_CHECK(cap * TILE_JOBS < INT32_MAX, EXM_ERR_ARG, "a tile's pieces exceed the 32-bit row offsets");
+        flags = (uint8_t*)(wb + pw.flags);
+        int32_t* cnt = (int32_t*)(wb + pw.cnt);
+        int32_t* rowoff = (int32_t*)(wb + pw.rowoff);
+        uint32_t* desc = (uint32_t*)(wb + pw.desc);
+        if (js.ntiles > 0) {
+            ProfileScope prof("schedule");
+            const dim3 grid(js.ntiles, st->nb);
+            auto kc = pn == 4 ? k_sched_count<4> : pn == 6 ? k_sched_count<6> : k_sched_count<8>;
+            auto ke = pn == 4 ? k_sched_emit<4> : pn == 6 ? k_sched_emit<6> : k_sched_emit<8>;
+            hipLaunchKernelGGL(kc, grid, dim3(SCHED_THREADS), 0, cs, js.d_tiles, first_last, bounds, st->nb, cnt);
+            EXM_HIP(hipGetLastError());
+            hipLaunchKernelGGL(ke, grid, dim3(SCHED_THREADS), 0, cs, js.d_tiles, first_last, bounds, st->nb, cnt, cap,
+                               rowoff, desc);
+            EXM_HIP(hipGetLastError());
+        }
+        using SKern = decltype(&k_gather_sched<6, double>);
+        SKern kern = c.f32_interp ? (pn == 4 ? k_gather_sched<4, float> : pn == 6 ? k_gather_sched<6, float>
+                                                                                 : k_gather_sched<8, float>)
+                                  : (pn == 4 ? k_gather_sched<4, double> : pn == 6 ? k_gather_sched<6, double>
+                                                                                   : k_gather_sched<8, double>);
+        EXM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        ProfileScope prof("gather");
+        hipLaunchKernelGGL(kern, dim3(js.ntiles, st->ntau), dim3(SCHED_THREADS), lds, cs, stream, aHs, wa, nw, lw_g,
+                           st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab, js.d_offb,
+                           js.d_tiles, first_last, desc, cap, rowoff, rdb, offd, rdd, om, os, flags,
+                           accumulate ? 1 : 0);
+        EXM_HIP(hipGetLastError());
+    } else {
+        const bool wave = form == BandForm::Wave;
+        using Kern = decltype(&k_gather_piece<8, double>);
+        Kern kern;
+        if (wave)
+            kern = c.f32_interp ? (pn == 4 ? k_gather_wave<4, float> : pn == 6 ? k_gather_wave<6, float> : k_gather_wave<8, float>)
+                                : (pn == 4 ? k_gather_wave<4, double> : pn == 6 ? k_gather_wave<6, double>
+                                                                                : k_gather_wave<8, double>);
+        else
+            kern = c.f32_interp ? (pn == 4 ? k_gather_piece<4, float> : pn == 16 ? k_gather_piece<16, float>
+                                                                                 : k_gather_piece<8, float>)
+                                : (pn == 4 ? k_gather_piece<4, double> : pn == 16 ? k_gather_piece<16, double>
+                                                                                  : k_gather_piece<8, double>);
+        EXM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        ProfileScope prof("gather");
+        hipLaunchKernelGGL(kern, dim3(js.ntiles, st->ntau), dim3(wave ? WAVE_THREADS : BAND_THREADS), lds, cs, stream,
+                           wa, nw, lw_g, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab,
+                           js.d_offb, js.d_tiles, first_last, bounds, rdb, offd, rdd, om, os, accumulate ? 1 : 0);
+        EXM_HIP(hipGetLastError());
+    }
+    // rows 0 and nb - 2 (beta possibly clamped), merged into the interior rows' (max, sum), and the
+    // interior rows of any (read, lag) the scheduled form flagged
     auto edge = c.f32_interp ? k_gather_edge<float> : k_gather_edge<double>;
     hipLaunchKernelGGL(edge, dim3((unsigned)((js.njobs + 255) / 256), st->ntau), dim3(256), 0, cs, stream, aHs, wa, nw,
                        lw_g, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab, js.d_offb,
-                       js.njobs, first_last, bounds, rdb, -st->delta0 / st->ddelta, 1.0 / st->ddelta, om, os);
+                       js.njobs, first_last, bounds, rdb, offd, rdd, om, os, flags);
     EXM_HIP(hipGetLastError());
     return EXM_OK;
 }
@@ -1084,7 +1514,7 @@ size_t band_scratch_bytes(const Traj* traj, const Store* st, const JobSet& js, c
     const size_t slot4 = al(nm * nw * 4 + WALK_PAD_BYTES) + al(nm * 4) + al(sizeof(uint2) * nm * st->ntau * nw + WALK_PAD_BYTES) +
                          sizeof(float) * nm * st->ntau * nw;
     const size_t slot5 = 2 * al(nm * nw * 4);
-    return al(slot4) + al(slot5) + al(piece_bounds_bytes(st, js.njobs));
+    return al(slot4) + al(slot5) + al(piece_bounds_bytes(st, js, (int)nw));
 }
 
 // the band gather of one segment: the walk (the selection, or every arrival time when times is NULL) in

@@ -1,10 +1,10 @@
 """
 Autotuner of the band gather kernel.
 
-The band kernel has a form and two shape knobs: the form (3: each wave deals its own reads' pieces;
-2: pieces dealt over the whole workgroup), the piece length (walked times one lane reads in a row: 4,
-6 or 8 in the wave form, 4, 8 or 16 in the piece form) and the tile (reads per workgroup, 256, 512 or
-1024).  The best choice depends on the walk, the table geometry and how the reads spread over the
+The band kernel has a form and two shape knobs: the form (4: pieces listed once for every lag and
+dealt to the waves in chunks; 3: each wave deals its own reads' pieces; 2: pieces dealt over the whole
+workgroup), the piece length (walked times one lane reads in a row: 4, 6 or 8 in the scheduled and wave
+forms, 4, 8 or 16 in the piece form) and the tile (reads per workgroup, 256, 512 or 1024).  The best choice depends on the walk, the table geometry and how the reads spread over the
 beta rows, so it is measured, not modelled: tune_batch times every candidate on the caller's own
 filled batch (the gather is rerun, nothing is consumed) with the kernel's own HIP events, keeps the
 fastest and records it in a JSON cache keyed by device, table geometry, walk length and precision;
@@ -24,7 +24,7 @@ import os
 
 from . import _hip
 
-FORMS = (3, 2)
+FORMS = (4, 3, 2)
 PIECES = (4, 6, 8, 16)
 TILES = (256, 512, 1024)
 
@@ -85,8 +85,8 @@ def tune_batch(batch, repeats=2, forms=FORMS, pieces=PIECES, tiles=TILES):
     try:
         for f in forms:
             for p in pieces:
-                if (f == 3 and p == 16) or (f == 2 and p == 6):
-                    continue                          # wave form: 4, 6 or 8; piece form: 4, 8 or 16
+                if (f in (3, 4) and p == 16) or (f == 2 and p == 6):
+                    continue                          # scheduled, wave form: 4, 6 or 8; piece form: 4, 8 or 16
                 for t in tiles:
                     set_knobs(p, t, form=f)
                     batch.run()                      # warm (code objects, work memory)

@@ -240,16 +240,16 @@ int exm_batch_reset(exm_handle batch);
 
 int exm_batch_info(exm_handle batch, int64_t* templates_used, int64_t* reads, int* segments);
 
-/* Choose the gather kernel: 0 automatic (the band kernel in its wave form for a selection of at most
- * 8192 walked times per template, the reference-order kernel otherwise), 1 the reference-order
- * kernel, 2 the band kernel's piece form (pieces dealt over the whole workgroup), 3 its wave form
- * (each wave deals its own reads' pieces).  No reference counterpart (the reference has one kernel
- * per device). */
+/* Choose the gather kernel: 0 automatic (the band kernel in its scheduled form, walks longer than 8192
+ * times in slices), 1 the reference-order kernel, 2 the band kernel's piece form (pieces dealt over the
+ * whole workgroup), 3 its wave form (each wave deals its own reads' pieces), 4 its scheduled form (the
+ * pieces listed once for every lag, dealt to the waves in chunks, each read's sum kept in LDS against a
+ * fixed reference).  No reference counterpart (the reference has one kernel per device). */
 int exm_set_gather_kernel(int which);
 
-/* Walked times per piece of the band kernel: 4, 6 or 8 in its wave form, 4, 8 or 16 in its piece form
- * (0 restores the default: 6 in the wave form, 8 in the piece form).  Any length gives the same results
- * to rounding; the choice is a tuning knob.  No reference counterpart. */
+/* Walked times per piece of the band kernel: 4, 6 or 8 in its scheduled and wave forms, 4, 8 or 16 in its
+ * piece form (0 restores the default: 8 in the scheduled form, 6 in the wave form, 8 in the piece form).  Any
+ * length gives the same results to rounding; the choice is a tuning knob.  No reference counterpart. */
 int exm_set_gather_piece(int n);
 
 /* Arithmetic of the band kernel's table interpolation: 0 float64 (default), 1 float32.  Index and

@@ -8,6 +8,9 @@ import pytest
 from exmarg_amd import synth
 
 pytestmark = pytest.mark.gpu
+# repeated runs of one analysis: the band kernel's default form adds each read's pieces by float64
+# atomic adds in LDS, whose order varies from run to run, so repeats agree to float64 rounding
+REPEAT_RTOL = 1e-13
 
 
 def say(*a):
@@ -47,7 +50,7 @@ def test_refused_allocations_in_pe_intrinsic(hip, small):
             hip.call("exm_debug_fail_alloc", -1)
         assert _in_use(hip) == base, "device memory held after a refused allocation (k = %d)" % k
         again = pe_intrinsic(seg, bank, st, **kw)
-        np.testing.assert_array_equal(again["ln_lambda"], ref["ln_lambda"])
+        np.testing.assert_allclose(again["ln_lambda"], ref["ln_lambda"], rtol=REPEAT_RTOL, atol=0)
     say("pe_intrinsic: %d of 24 injection points hit an allocation" % failed)
     assert failed > 0
 
@@ -69,7 +72,7 @@ def test_refused_allocations_in_batches(hip, small):
             hip.call("exm_debug_fail_alloc", -1)
         assert _in_use(hip) == base, "device memory held after a refused allocation (k = %d)" % k
     again = pe_intrinsic_batch([(seg, bank)], st, **kw)
-    np.testing.assert_array_equal(again[0]["ln_lambda"], ref[0]["ln_lambda"])
+    np.testing.assert_allclose(again[0]["ln_lambda"], ref[0]["ln_lambda"], rtol=REPEAT_RTOL, atol=0)
     say("pe_intrinsic_batch: %d of 24 injection points hit an allocation" % failed)
     assert failed > 0
 

@@ -283,7 +283,7 @@ def _tiny_store(kind, shape, n_nodes=2, seed=1):
     return st.synthetic_store(n_nodes=n_nodes, seed=seed, kind=kind, shape=shape, host=True)
 
 
-KERNELS = {1: 1e-12, 2: 1e-6, 3: 1e-6}     # reference-order kernel: same operation order as the oracle; band kernel
+KERNELS = {1: 1e-12, 2: 1e-6, 3: 1e-6, 4: 1e-6}     # reference-order kernel: same operation order as the oracle; band kernel
 
 
 @pytest.fixture
@@ -293,7 +293,7 @@ def kernel(hip, request):
     hip.call("exm_set_gather_kernel", 0)
 
 
-@pytest.mark.parametrize("kernel", [1, 2, 3], indirect=True)
+@pytest.mark.parametrize("kernel", [1, 2, 3, 4], indirect=True)
 @pytest.mark.parametrize("kind", ["smooth", "rough"])
 @pytest.mark.parametrize("subsampled", [False, True])
 def test_gather_matches_oracle(hip, oracle, kernel, kind, subsampled):
@@ -325,7 +325,8 @@ def piece(hip, request):
     hip.call("exm_set_gather_piece", 0)
 
 
-@pytest.mark.parametrize("kernel,piece", [(2, 4), (2, 8), (2, 16), (3, 4), (3, 6), (3, 8)], indirect=True)
+@pytest.mark.parametrize("kernel,piece", [(2, 4), (2, 8), (2, 16), (3, 4), (3, 6), (3, 8), (4, 4), (4, 6), (4, 8)],
+                         indirect=True)
 def test_band_gather_clamps_and_many_reads(hip, oracle, kernel, piece):
     # beta clamped at both table edges, Delta clamped, more reads of one table than one tile holds
     from exmarg_amd.search import gather as g
@@ -354,7 +355,7 @@ def tile(hip, request):
     hip.call("exm_set_gather_tile", 0)
 
 
-@pytest.mark.parametrize("kernel", [2, 3], indirect=True)
+@pytest.mark.parametrize("kernel", [2, 3, 4], indirect=True)
 @pytest.mark.parametrize("tile", [256, 512], indirect=True)
 def test_band_gather_tiles_over_xcd_queues(hip, oracle, kernel, tile):
     # two tables cut into 9 and 3 tiles (256 reads per tile) or 5 and 2 (512): the tile list is built
@@ -381,7 +382,7 @@ def test_band_gather_tiles_over_xcd_queues(hip, oracle, kernel, tile):
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
 
 
-@pytest.mark.parametrize("form", [2, 3])
+@pytest.mark.parametrize("form", [2, 3, 4])
 @pytest.mark.parametrize("kind", ["smooth", "rough"])
 def test_band_gather_float32_form(hip, oracle, kind, form):
     """float32 table interpolation (index arithmetic and log-sum-exp in float64) stays within 2e-6
@@ -431,7 +432,7 @@ def test_gather_full_geometry_one_node(hip, oracle):
     st.free(s)
 
 
-@pytest.mark.parametrize("kernel", [2, 3], indirect=True)
+@pytest.mark.parametrize("kernel", [2, 3, 4], indirect=True)
 @pytest.mark.parametrize("walk", ["every_time", "long_selection"])
 def test_band_gather_in_slices(hip, oracle, kernel, walk):
     """walks longer than one band launch takes (every arrival time of 20000, or a selection of 10000
@@ -456,7 +457,7 @@ def test_band_gather_in_slices(hip, oracle, kernel, walk):
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
 
 
-@pytest.mark.parametrize("kernel,piece", [(2, 8), (3, 4), (3, 6), (3, 8)], indirect=True)
+@pytest.mark.parametrize("kernel,piece", [(2, 8), (3, 4), (3, 6), (3, 8), (4, 4), (4, 6), (4, 8)], indirect=True)
 @pytest.mark.parametrize("n_walk", [8192, 24])
 def test_band_gather_skewed_rows(hip, oracle, kernel, piece, n_walk):
     """reads whose whole walk falls in one clamped row (one read's pieces fill whole rounds of lanes)
@@ -484,7 +485,7 @@ def test_band_gather_skewed_rows(hip, oracle, kernel, piece, n_walk):
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
 
 
-@pytest.mark.parametrize("kernel,piece", [(2, 8), (3, 4), (3, 6), (3, 8)], indirect=True)
+@pytest.mark.parametrize("kernel,piece", [(2, 8), (3, 4), (3, 6), (3, 8), (4, 4), (4, 6), (4, 8)], indirect=True)
 def test_band_gather_negative_log_weight(hip, oracle, kernel, piece):
     """more random draws than times outside the top K: the drawn times' log weight ln((nt - K) / S)
     (gather.py:433-435) is negative, so the drawn terms weigh less than the loud ones"""
@@ -507,6 +508,46 @@ def test_band_gather_negative_log_weight(hip, oracle, kernel, piece):
     ref = oracle.gather(traj, oracle.Tables(tables=s.tables, tau_idx=s.tau_idx), s.beta[0], s.delta[0], ids, lna,
                         tab, times)
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("kernel", [3, 4], indirect=True)
+@pytest.mark.parametrize("loud", ["low_rows", "high_rows"])
+def test_band_gather_values_spanning_beyond_float64(hip, oracle, kernel, loud):
+    """table values 1500 above the rest in the lower (or the upper) half of the beta rows: a read's
+    terms then span e^1500, beyond float64.  The scheduled form sums each read against the value of its
+    loudest walked time; with the lower rows loud that reference lies ~1500 below the largest terms, the
+    sum overflows and the (read, lag) is flagged and recomputed exactly, and with the upper rows loud the
+    lower rows' terms vanish against it.  Both must match the oracle's online log-sum-exp."""
+    from exmarg_amd.search import gather as g
+    from exmarg_amd.search import store as st
+    say("gather: band kernel %d, values spanning 1500 (%s)" % (kernel, loud))
+    arr = synth.synthetic_store_arrays(n_nodes=1, seed=4, kind="rough", shape=(5, 49, 24, 17))
+    T = arr["tables"][0]
+    nb = T.shape[2]
+    rows = np.arange(nb) < nb // 2 if loud == "low_rows" else np.arange(nb) >= nb // 2
+    T = (T + np.where(rows, 1500.0, 0.0)[None, None, :, None]).astype(np.float32)
+    rs = st.RedshiftStore(s=arr["s"], alpha_lo=np.full(1, 0.16), alpha_hi=np.full(1, 4e4), beta=[arr["beta"]],
+                          delta=[arr["delta"]], tau_idx=arr["tau_idx"], tables=[T], n_max=120)
+    st.upload(rs)
+    try:
+        nm = 3
+        traj = _rand_traj(nm, 4096, seed=31)
+        traj["lqH"] *= 3.0
+        traj["lqL"] *= 3.0
+        times_dev = g.select_times(traj, rs.tau_idx, 300, 500, seed=4)
+        times = times_dev.to_host()
+        rng = np.random.default_rng(5)
+        nr = 60
+        ids = rng.integers(0, nm, nr)
+        lna = rng.uniform(-2.0, 2.0, (nr, 1))
+        tab = np.zeros((nr, 1), np.int64)
+        got = g.gather(traj, rs, ids, lna, tab, times=times_dev)
+        ref = oracle.gather(traj, oracle.Tables(tables=[T], tau_idx=rs.tau_idx), rs.beta[0], rs.delta[0], ids, lna,
+                            tab, times)
+        assert np.all(ref > 1000.0)
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
+    finally:
+        st.free(rs)
 
 
 def test_wave_form_rejects_pieces_of_16(hip):

@@ -6,6 +6,9 @@ import pytest
 from exmarg_amd import synth
 
 pytestmark = pytest.mark.gpu
+# repeated runs of one analysis: the band kernel's default form adds each read's pieces by float64
+# atomic adds in LDS, whose order varies from run to run, so repeats agree to float64 rounding
+REPEAT_RTOL = 1e-13
 
 LNB_TOL = 1e-5          # |ln B - ln B_oracle| bar of the analysis
 LNL_TOL = 1e-5          # per-template ln Lambda
@@ -137,8 +140,8 @@ def test_batch_of_uneven_segments_matches_single_calls(hip, oracle):
     k2 = pe_intrinsic_batch(items, st, **kw)
     assert st.__dict__["_batch"][1] is buf
     for r, a, b in zip(res, k1, k2):
-        np.testing.assert_array_equal(a["ln_lambda"], r["ln_lambda"])
-        np.testing.assert_array_equal(b["ln_lambda"], r["ln_lambda"])
+        np.testing.assert_allclose(a["ln_lambda"], r["ln_lambda"], rtol=REPEAT_RTOL, atol=0)
+        np.testing.assert_allclose(b["ln_lambda"], r["ln_lambda"], rtol=REPEAT_RTOL, atol=0)
     release_kept(st)
     assert "_batch" not in st.__dict__
     pe_intrinsic_batch(items[:1], st, **kw)

@@ -1,8 +1,10 @@
 """exm_plan_gather_noalloc (the plan gather with every scratch buffer in the library's static per-device
-region) against exm_plan_gather through the C ABI: the same results, bit for bit, on two calls in a row
-(the second with fewer templates, so that it runs over what the first left in the region), for a
-subsampled walk (band kernel) and an every-time walk (reference-order kernel); a gather larger than the
-region is refused before anything runs."""
+region) against exm_plan_gather through the C ABI: the same results on two calls in a row (the second
+with fewer templates, so that it runs over what the first left in the region), for a subsampled walk and
+an every-time walk (band kernel, in slices); a gather larger than the region is refused before anything
+runs.  The band kernel's default (scheduled) form adds each read's pieces with float64 atomic adds in
+LDS, whose order varies from run to run, so two runs agree to float64 rounding (relative 1e-13 here),
+not bit for bit."""
 import numpy as np
 import pytest
 
@@ -54,14 +56,14 @@ def test_noalloc_equals_plan_gather(hip, setup, n_loud):
         say("no-alloc gather: %d templates, %d reads, n_loud %d" % (n_templates, an.plan.job_m.size, n_loud))
         a, b = both(hip, an, traj, st, times)
         assert np.isfinite(a).all()
-        np.testing.assert_array_equal(b, a)
+        np.testing.assert_allclose(b, a, rtol=1e-13, atol=0)
         # the no-allocation call itself takes no pooled memory
         c = np.empty_like(b)
         before = hip.mem_in_use()
         hip.call("exm_plan_gather_noalloc", int(an.plan.handle), int(traj.handle), int(st.device.handle),
                  int(times.handle) if times is not None else 0, hip.ptr(c))
         assert hip.mem_in_use() == before
-        np.testing.assert_array_equal(c, a)
+        np.testing.assert_allclose(c, a, rtol=1e-13, atol=0)
         if times is not None:
             times.free()
         traj.free()
