@@ -751,7 +751,7 @@ def main():
     }
     trf = measured_traffic(1e3 * g_s)
     if (trf is not None and args.config == 2 and args.grid == "20,10,10" and args.nodes == 53 and walk == 2048
-            and args.precision == "f64"):
+            and args.precision == "f64" and args.gather_kernel in (0, 4) and args.piece in (0, 8)):
         out["roofline"]["traffic"] = trf["traffic_bytes"]
         out["roofline"]["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this kernel on this "
                                              "workload, scaled by a 1 GiB copy (profiles/gather_traffic.json)")
@@ -762,7 +762,7 @@ def main():
             out["roofline"]["valu"] = {"lane_instr_per_lookup": per, "achieved": per * lookups / g_s,
                                        "peak": VALU_PEAK_LANE_OPS, "unit": "lane-instructions/s",
                                        "frac": per * lookups / g_s / VALU_PEAK_LANE_OPS,
-                                       "source": "SQ_INSTS_VALU x 64 / lookups (profiles/counters_gather_wave_"
+                                       "source": "SQ_INSTS_VALU x 64 / lookups (profiles/counters_gather_sched_"
                                                  "16s_2000tmpl_53nodes.txt)"}
     if f64_check is not None:
         out["float32_form_vs_float64"] = f64_check

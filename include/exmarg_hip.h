@@ -188,7 +188,8 @@ int exm_plan_gather_lse(exm_handle plan, exm_handle traj, exm_handle store, exm_
  * the reference: for callers that cannot allocate device memory (tight serving loops, harnesses that
  * forbid it).  A plan, trajectories and times whose gather needs more scratch than the region holds
  * are refused with EXM_ERR_NOMEM before anything runs (at the reference geometry, 87 lags and 2048
- * walked times, about 80 templates).  Calls on one device share the region: calls that overlap race. */
+ * walked times, about 2.3 MB per template that reaches 46 nodes: about 55 such templates).  Calls on one
+ * device share the region: calls that overlap race. */
 int exm_plan_gather_noalloc(exm_handle plan, exm_handle traj, exm_handle store, exm_handle times, double* out);
 
 /* ---------------------------------------------------------------- batches
