@@ -1048,11 +1048,11 @@ int exm_reserve(int n_tmpl, int64_t n_time, int n_freq, int n_walk) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t nm = (size_t)n_tmpl, nt = (size_t)n_time, nw = (size_t)n_walk, nf = (size_t)n_freq;
     void* p = nullptr;
-    // filter and selection work (the selection's keys: 8 n_time bytes per template, at most 4 GiB; the
+    // filter and selection work (the selection's keys: 8 n_time bytes per template, at most SEL_KEY_BYTES; the
     // band transform's two buffers of one workspace each; plus one row), the filter's zero-padded input
     // (its workspace, at least one row), uploads, walk temporaries, batch admission
     const size_t fws = g_ctx.filter_bytes > 0 ? (size_t)g_ctx.filter_bytes : ((size_t)64 << 20);
-    const size_t keys = std::min<size_t>((size_t)1 << 32, 8 * nm * nt);
+    const size_t keys = std::min<size_t>((size_t)SEL_KEY_BYTES, 8 * nm * nt);
     const size_t bands = 2 * (std::max(std::min(fws, 16 * nm * nt), 16 * nt) + 256);
     EXM_TRY(scratch(0, std::max(std::max(keys, bands), 16 * nt) + 16 * nt, &p));
     EXM_TRY(scratch(9, std::max(fws, 16 * nt), &p));

@@ -73,6 +73,9 @@ constexpr int TABLE_PAD_FLOATS = 256;
 // Spare bytes after the walk buffers (walked ln|z_L| and the lag stream): the band kernel's wave form
 // reads every piece as PN consecutive records from one address, up to 7 records past a read's range.
 constexpr size_t WALK_PAD_BYTES = 64;
+// Largest key buffer of one selection launch (8 bytes per template and time): the templates of one launch
+// are as many as fit, so a bank of a few thousand templates is selected in one launch at every length.
+constexpr int64_t SEL_KEY_BYTES = (int64_t)1 << 34;
 struct Store : Object {
     int ntab = 0, ntau = 0, nphif = 0, nb = 0, nd = 0;
     int64_t tstride = 0;   // floats per table

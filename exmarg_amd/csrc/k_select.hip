@@ -384,10 +384,11 @@ int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_
     EXM_CHECK(Sp <= 2 * CAND && RADIX <= 2 * CAND, EXM_ERR_ARG, "selection of %d drawn times too large", S);
     const size_t lds = (sizeof(uint64_t) + sizeof(int32_t)) * CAND + sizeof(int32_t) * Kp + sizeof(int) * (SEL_WAVES + 4);
     EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "selection needs %zu bytes of LDS", lds);
-    // templates per launch: the keys of up to 4 GiB (4096 templates of 131072 times), so that a bank of a
-    // few thousand templates is one launch of several workgroups per CU slot and no launch ends on a
-    // mostly idle last wave (1 GiB, two launches of 1024 and 976 templates, took 2.79 ms against 2.39)
-    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(traj->n_tmpl, (1ll << 32) / (8 * nt)));
+    // templates per launch: the keys of up to SEL_KEY_BYTES (4096 templates of 524288 times), so that a
+    // bank of a few thousand templates is one launch of several workgroups per CU slot and no launch ends
+    // on a mostly idle last wave (at 131072 times, 1 GiB of keys in two launches of 1024 and 976 templates
+    // took 2.79 ms against 2.39 in one)
+    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(traj->n_tmpl, SEL_KEY_BYTES / (8 * nt)));
     uint64_t* keys = nullptr;
     EXM_TRY(scratch(0, sizeof(uint64_t) * (size_t)chunk * nt, (void**)&keys));
     // the draw: uploaded from the host (pick) or already on the device (d_pick)

@@ -550,6 +550,37 @@ def test_band_gather_values_spanning_beyond_float64(hip, oracle, kernel, loud):
         st.free(rs)
 
 
+def test_band_gather_repeat_agreement(hip):
+    """the wave form (kernel 3) adds in a fixed order and repeats bit for bit; the scheduled form (kernel 4)
+    adds with float64 atomics in the order waves reach them, so its repeats differ only in the last bits.
+    The two forms round differently (float32 exponentials of terms against different references), and
+    agree with each other well inside the oracle tolerance of 1e-6"""
+    from exmarg_amd.search import gather as g
+    say("gather: repeats of the wave and scheduled forms")
+    s = _tiny_store("rough", (9, 49, 40, 31), n_nodes=2, seed=6)
+    traj = _rand_traj(5, 8192, seed=40)
+    traj["lqH"] *= 2.0
+    traj["lqL"] *= 2.0
+    times = g.select_times(traj, s.tau_idx, 512, 700, seed=8)
+    rng = np.random.default_rng(9)
+    nr = 300
+    ids = rng.integers(0, 5, nr)
+    lna = rng.uniform(-3.0, 3.0, (nr, 2))
+    tab = rng.integers(0, 2, (nr, 2))
+    out = {}
+    try:
+        for k in (3, 4):
+            hip.call("exm_set_gather_kernel", k)
+            out[k] = [g.gather(traj, s, ids, lna, tab, times=times) for _ in range(3)]
+    finally:
+        hip.call("exm_set_gather_kernel", 0)
+    for r in out[3][1:]:
+        np.testing.assert_array_equal(r, out[3][0])
+    for r in out[4][1:]:
+        np.testing.assert_allclose(r, out[4][0], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(out[4][0], out[3][0], rtol=0, atol=1e-7)
+
+
 def test_wave_form_rejects_pieces_of_16(hip):
     from exmarg_amd.search import gather as g
     s = _tiny_store("smooth", (5, 49, 8, 9), n_nodes=1)
