@@ -10,8 +10,11 @@
 //   B  the K largest keys: the samples are sorted in LDS and a threshold taken below the expected
 //      K-th key; one pass collects every key at or above it (at most CAND of them) and an LDS bitonic
 //      sort by (key descending, time ascending) gives the K winners exactly (ties to the earlier
-//      time).  When the threshold catches fewer than K or more than CAND keys, an exact radix select
-//      (11-bit digits, LDS histogram) and an ordered compaction pass give the same winners;
+//      time).  From 262144 times on (one sample in 128 keys or fewer) the same pass counts the keys at
+//      or above four alternate thresholds, and when the threshold catches fewer than K or more than
+//      CAND keys a second pass collects at the highest alternate that holds K to CAND keys.  Failing
+//      that, an exact radix select (11-bit digits, LDS histogram) and an ordered compaction pass give
+//      the same winners;
 //   C  the winners in increasing t;
 //   D  the drawn times rest[pick] of the remaining times in time order,
 //      rest[p] = p + #{i : top[i] - i <= p}, sorted by a bitonic sort and merged with the winners.
@@ -25,6 +28,11 @@ constexpr int RADIX = 2048;
 constexpr int RADIX_PER_THREAD = RADIX / SEL_THREADS;
 constexpr int NSAMP = 2048;          // at most this many samples of the keys
 constexpr int CAND = 4096;           // candidates sorted in LDS
+// with sparse samples, the candidate threshold's expected count in units of K, and its alternates'
+// (increasing)
+constexpr double THR_F = 2.2;
+constexpr int NALT = 4;
+__device__ constexpr double ALT_F[NALT] = {1.3, 1.7, 2.8, 3.5};
 // The candidate arrays (CAND keys, then CAND times) also hold, while they are not in use, the sparse-
 // table tile of phase A (after the samples), the radix histogram and the drawn times of phase D.
 constexpr int TILE_FLOATS = ((CAND - NSAMP) * 8 + CAND * 4) / 4;   // room for the two tile halves
@@ -136,7 +144,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict_
     int32_t* picked = (int32_t*)ckey;                             // Sp, phase D only
     int32_t* top = ctime + CAND;                                  // Kp
     int* wsum = top + Kp;                                         // SEL_WAVES
-    int* misc = wsum + SEL_WAVES;                                 // 4
+    int* misc = wsum + SEL_WAVES;                                 // 4 + NALT
 
     const int tid = threadIdx.x;
     const int m = m0 + blockIdx.x;
@@ -229,34 +237,76 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(const float* __restrict_
     for (int i = tid; i < np2; i += SEL_THREADS) ctime[i] = i;
     __syncthreads();
     bitonic_desc_pairs(ckey, ctime, np2);
-    // expected count at or above sample r is (r + 1) stride; aim for 1.5 K plus a margin
-    const int64_t rr = (int64_t)(1.5 * K / (double)stride) + 8;
-    const uint64_t thr0 = rr < nsamp ? ckey[rr] : 0;
-    if (tid == 0) misc[2] = 0;
+    // expected count at or above sample r is (r + 1) stride.  With a sample in at most 64 keys, aim for
+    // 1.5 K plus eight samples (the count stays within K .. 2048, one bitonic sort of 2048 at short
+    // lengths); with sparser samples (one in 128 or more keys from 262144 times on, where the window
+    // maximum also repeats over neighbouring times) the count strays further, so aim for 2.2 K, midway
+    // between K and CAND, and count the keys at or above alternate thresholds at ALT_F[j] K (decreasing
+    // thresholds) in the same pass: one more pass at the highest alternate holding K to CAND keys costs
+    // far less than the exact radix select
+    const bool sparse = stride >= 128;
+    const int64_t rr = sparse ? (int64_t)(THR_F * K / (double)stride + 0.5) - 1
+                              : (int64_t)(1.5 * K / (double)stride) + 8;
+    const uint64_t thr0 = rr < nsamp ? ckey[rr < 0 ? 0 : rr] : 0;
+    uint64_t thr_alt[NALT];
+#pragma unroll
+    for (int j = 0; j < NALT; ++j) {
+        const int64_t r = (int64_t)(ALT_F[j] * K / (double)stride + 0.5) - 1;
+        thr_alt[j] = r < nsamp ? ckey[r < 0 ? 0 : r] : 0;
+    }
+    if (tid == 0) {
+        misc[2] = 0;
+        for (int j = 0; j < NALT; ++j) misc[4 + j] = 0;
+    }
     __syncthreads();
-    // eight keys per thread are requested before any is tested
-    constexpr int KB = 8;
-    for (int64_t t0 = 0; t0 < nt; t0 += KB * SEL_THREADS) {
-        uint64_t kb[KB];
+    // the keys at or above thr into the candidate arrays (their count into misc[2]); with count_alt, the
+    // number of keys at or above each alternate into misc[4 ..]; eight keys per thread are requested
+    // before any is tested
+    auto collect = [&](uint64_t thr, bool count_alt) {
+        constexpr int KB = 8;
+        int nalt[NALT] = {};
+        for (int64_t t0 = 0; t0 < nt; t0 += KB * SEL_THREADS) {
+            uint64_t kb[KB];
 #pragma unroll
-        for (int j = 0; j < KB; ++j) {
-            const int64_t t = t0 + j * SEL_THREADS + tid;
-            kb[j] = t < nt ? key[t] : 0;
-        }
+            for (int j = 0; j < KB; ++j) {
+                const int64_t t = t0 + j * SEL_THREADS + tid;
+                kb[j] = t < nt ? key[t] : 0;
+            }
 #pragma unroll
-        for (int j = 0; j < KB; ++j) {
-            const int64_t t = t0 + j * SEL_THREADS + tid;
-            if (t < nt && kb[j] >= thr0) {
-                const int pos = atomicAdd(&misc[2], 1);
-                if (pos < CAND) {
-                    ckey[pos] = kb[j];
-                    ctime[pos] = (int32_t)t;
+            for (int j = 0; j < KB; ++j) {
+                const int64_t t = t0 + j * SEL_THREADS + tid;
+                if (count_alt) {
+#pragma unroll
+                    for (int a = 0; a < NALT; ++a) nalt[a] += __popcll(__ballot(t < nt && kb[j] >= thr_alt[a]));
+                }
+                if (t < nt && kb[j] >= thr) {
+                    const int pos = atomicAdd(&misc[2], 1);
+                    if (pos < CAND) {
+                        ckey[pos] = kb[j];
+                        ctime[pos] = (int32_t)t;
+                    }
                 }
             }
         }
-    }
+        if (count_alt && (tid & 63) == 0)
+            for (int a = 0; a < NALT; ++a) atomicAdd(&misc[4 + a], nalt[a]);
+    };
+    collect(thr0, sparse);
     __syncthreads();
-    const int ncand = misc[2];
+    int ncand = misc[2];
+    if (sparse && (ncand < K || ncand > CAND)) {
+        int best = -1;
+        for (int a = 0; a < NALT && best < 0; ++a)
+            if (misc[4 + a] >= K && misc[4 + a] <= CAND) best = a;
+        if (best >= 0) {
+            __syncthreads();
+            if (tid == 0) misc[2] = 0;
+            __syncthreads();
+            collect(thr_alt[best], false);
+            __syncthreads();
+            ncand = misc[2];
+        }
+    }
     __syncthreads();
     if (ncand >= K && ncand <= CAND) {
         int nc2 = 1;
@@ -382,7 +432,7 @@ int launch_select(Traj* traj, int lag_lo, int lag_hi, int K, int S, const int64_
     const int span = lag_hi - lag_lo;
     EXM_CHECK(span + 1 <= TILE_FLOATS / 4, EXM_ERR_ARG, "selection window of %d lags too wide", span + 1);
     EXM_CHECK(Sp <= 2 * CAND && RADIX <= 2 * CAND, EXM_ERR_ARG, "selection of %d drawn times too large", S);
-    const size_t lds = (sizeof(uint64_t) + sizeof(int32_t)) * CAND + sizeof(int32_t) * Kp + sizeof(int) * (SEL_WAVES + 4);
+    const size_t lds = (sizeof(uint64_t) + sizeof(int32_t)) * CAND + sizeof(int32_t) * Kp + sizeof(int) * (SEL_WAVES + 4 + NALT);
     EXM_CHECK(lds <= 160 * 1024, EXM_ERR_ARG, "selection needs %zu bytes of LDS", lds);
     // templates per launch: the keys of up to SEL_KEY_BYTES (4096 templates of 524288 times), so that a
     // bank of a few thousand templates is one launch of several workgroups per CU slot and no launch ends

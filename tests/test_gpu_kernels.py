@@ -220,7 +220,8 @@ def _rand_traj(nm, nt, seed):
 
 
 @pytest.mark.parametrize("nm,nt,K,S,lag", [(5, 4096, 64, 100, 43), (3, 20000, 1024, 1024, 43), (4, 300, 299, 5, 3),
-                                           (2, 131072, 1024, 1024, 43), (2, 9000, 4000, 512, 1)])
+                                           (2, 131072, 1024, 1024, 43), (2, 9000, 4000, 512, 1),
+                                           (4, 524288, 1024, 1024, 43)])
 def test_select_matches_oracle(hip, oracle, nm, nt, K, S, lag):
     from exmarg_amd.search import gather as g
     say("select: nm=%d nt=%d K=%d S=%d" % (nm, nt, K, S))
@@ -232,20 +233,62 @@ def test_select_matches_oracle(hip, oracle, nm, nt, K, S, lag):
     np.testing.assert_array_equal(lw, rlw)
 
 
-@pytest.mark.parametrize("layout", ["off_sample", "one_template_flat"])
+def _sample_layout(nt, K, high, seed):
+    """template scores e^{2 lqL} (lqH far below) whose samples, the keys at every stride-th time (stride
+    the least power of two with 2048 stride >= nt), take the values v_i = 10 - 0.05 i (i < 64) and -5
+    elsewhere; for each (f, n) in high, n keys at times off the samples take the value v_rank(f), where
+    rank(f) is the sample rank at an expected count of f K as the candidate pass ranks them (2.2 K its
+    threshold, 1.3, 1.7, 2.8, 3.5 K the alternates)"""
+    stride = 1
+    while 2048 * stride < nt:
+        stride *= 2
+    rank = lambda f: int(f * K / stride + 0.5) - 1
+    lqL = np.full(nt, -5.0, np.float32)
+    v = (10.0 - 0.05 * np.arange(64)).astype(np.float32)
+    lqL[np.arange(64) * stride] = v
+    off = np.flatnonzero(np.arange(nt) % stride != 0)
+    np.random.default_rng(seed).shuffle(off)
+    o = 0
+    for f, n in high:
+        lqL[off[o:o + n]] = v[rank(f)]
+        o += n
+    return lqL, stride, rank
+
+
+@pytest.mark.parametrize("layout", ["off_sample", "one_template_flat", "retry_high", "retry_low"])
 def test_select_threshold_fallbacks(hip, oracle, layout):
     """scores arranged so that the sample threshold catches too many or too few keys: the exact
-    selection must give the oracle's times all the same"""
+    selection must give the oracle's times all the same.  retry_high: 3000 keys off the samples just above
+    the threshold push its count past the 4096 candidates while the highest alternate holds 1500 + 10;
+    retry_low: the threshold catches 18 keys and the third alternate 1500 + 22; both take the second pass
+    at an alternate"""
     from exmarg_amd.search import gather as g
     say("select: %s" % layout)
     nm, nt = 2, 65536
+    if layout.startswith("retry"):
+        nt = 262144                               # samples sparse enough for the alternates (stride 128)
     traj = _rand_traj(nm, nt, seed=4)
     if layout == "off_sample":
         # loud times only where no sample is taken (every time but multiples of 32)
         traj["lqL"][:, np.arange(nt) % 32 != 0] += 6.0
-    else:
+    elif layout == "one_template_flat":
         traj["lqL"][0] = 0.25                     # template 0: many equal scores around the threshold
         traj["lqH"][0] = -3.0
+    else:
+        K = 1024
+        if layout == "retry_high":
+            # 1500 keys at the value of sample rank(1.3) - 2, 3000 between rank(1.3) and the threshold's
+            spec = [(1.0, 1500), (1.7, 3000)]
+        else:
+            spec = [(2.75, 1500)]                 # between the threshold's sample and the third alternate's
+        lqL, stride, rank = _sample_layout(nt, K, spec, seed=11)
+        assert stride == 128
+        if layout == "retry_high":
+            assert rank(1.0) < rank(1.3) < rank(1.7) < rank(2.2)
+        else:
+            assert rank(2.2) < rank(2.75) <= rank(2.8)
+        traj["lqL"][0] = lqL
+        traj["lqH"][0] = -20.0
     tau = synth.lag_indices(43)
     sel, lw = g.select_times(traj, tau, 1024, 512, seed=3).to_host()
     rsel, rlw = oracle.select_times(traj, tau, 1024, 512, seed=3)
