@@ -914,17 +914,20 @@ __global__ __launch_bounds__(WAVE_THREADS) void k_gather_wave(
 //
 // Which walked times of which read fall in which beta row does not depend on the lag, so the pieces a
 // workgroup deals (PN consecutive walked times of one read in one row) are listed once per launch, for
-// every tile and interior row, before the gather: k_sched_count counts the pieces of every (tile, row),
+// every tile and row, before the gather: k_sched_count counts the pieces of every (tile, row),
 // k_sched_emit writes their descriptors in row order and, within a row, in read order.  The gather
 // (k_gather_sched, one workgroup per (tile, lag) as the other forms) then sweeps the rows with the same
 // three-row LDS ring, and its waves take the pieces of the current row in chunks of 64 from a counter
-// in LDS, so they stay busy until the row's last chunk.  Each read's sum is kept in LDS against a fixed
+// in LDS, so they stay busy until the row's last chunk.  Rows 0 and nb - 2, where beta may be clamped,
+// are swept too, their chunks with the exact lookup (Delta recomputed with the clamp, ln|z_H| read for
+// each walked time), so k_gather_edge only recomputes what this kernel flags.  Each read's sum is kept
+// in LDS against a fixed
 // reference: the value of its loudest walked time (the last of its sorted walk, found before the sweep
 // with the exact lookup), so that a piece contributes s e^{m - M} (its own (max, sum) rescaled to the
 // read's reference M) by one float64 LDS atomic add, and no lane has to meet the other pieces of its
-// read.  M is a value of the read's own sum, so the sum is at least ~1 wherever M's time is an interior
-// lookup; its terms exceed e^709 only if the read's values span more than ~700 (an overflow the final
-// step detects, flagging the (read, lag) for k_gather_edge to recompute exactly).
+// read.  M is a value of the read's own sum, so the sum is at least ~1; its terms exceed e^709 only if
+// the read's values span more than ~700 (an overflow the final step detects, flagging the (read, lag)
+// for k_gather_edge to recompute exactly).
 //
 // Piece descriptor, 32 bits: the read's index in its tile (bits 0-9), the slots used minus one (bits
 // 10-12) and the walk position of its first slot (bits 13-31).
@@ -932,16 +935,16 @@ constexpr int SCHED_THREADS = 1024;
 constexpr int SD_USE_SHIFT = 10, SD_POS_SHIFT = 13;
 constexpr uint32_t SD_READ_MASK = 0x3FFu;
 
-// pieces a read may hold in the interior rows at most: each row's run ends in at most one partial piece
+// pieces a read may hold at most: each row's run ends in at most one partial piece
 __host__ __device__ inline int64_t sched_cap_per_read(int nw, int pn, int nb) { return (nw + pn - 1) / pn + nb; }
 
-// walk range of read j in interior row b (empty outside the read's rows and outside 1 .. nb - 3)
-__device__ __forceinline__ void interior_run(const int32_t* __restrict__ first_last,
-                                             const uint16_t* __restrict__ bounds, int64_t j, int nb, int b,
-                                             int& lo, int& hi) {
+// walk range of read j in row b (empty outside the read's rows, and for b = nb - 1, which no walked time
+// reaches: rows run over 0 .. nb - 2)
+__device__ __forceinline__ void row_run(const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds,
+                                        int64_t j, int nb, int b, int& lo, int& hi) {
     lo = hi = 0;
     const int first = first_last[2 * j], last = first_last[2 * j + 1];
-    if (b >= 1 && b <= nb - 3 && b >= first && b <= last) {
+    if (b <= nb - 2 && b >= first && b <= last) {
         const uint16_t* bj = bounds + j * (nb + 1);
         lo = bj[b - first];
         hi = bj[b + 1 - first];
@@ -974,7 +977,7 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_sched_count(const int32_t* __
     int q = 0;
     if ((int)threadIdx.x < nj) {
         int lo, hi;
-        interior_run(first_last, bounds, (int64_t)j0 + threadIdx.x, nb, b, lo, hi);
+        row_run(first_last, bounds, (int64_t)j0 + threadIdx.x, nb, b, lo, hi);
         q = (hi - lo + PN - 1) / PN;
     }
     const int total = block_sum<SCHED_THREADS / 64>(q, red);
@@ -999,7 +1002,7 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_sched_emit(const int32_t* __r
     // where row b starts: the pieces of the rows before it (nb < 1024 rows, one per thread)
     const int start = block_sum<SCHED_THREADS / 64>(tid < b ? ct[tid] : 0, red);
     int lo = 0, hi = 0;
-    if (tid < nj) interior_run(first_last, bounds, (int64_t)j0 + tid, nb, b, lo, hi);
+    if (tid < nj) row_run(first_last, bounds, (int64_t)j0 + tid, nb, b, lo, hi);
     const int q = (hi - lo + PN - 1) / PN;
     // exclusive sum of q over the workgroup: within the wave by shuffles, then over the waves
     int x = q;
@@ -1077,15 +1080,14 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
         atomicMax(&s_ctr[5], last);
     }
     __syncthreads();
-    // the interior rows of the band (rows 0 and nb - 2, where beta may be clamped, are k_gather_edge's)
-    const int b_start = max(s_ctr[4], 1), b_end = min(s_ctr[5], nb - 3);
+    // every row of the band: rows 0 and nb - 2, where beta may be clamped, with the exact lookup
+    const int b_start = max(s_ctr[4], 0), b_end = min(s_ctr[5], nb - 2);
     if (b_start <= b_end) {
         const int b_top = min(b_end + 1, nb - 1);
         const int wid = tid >> 6;
-        if (tid == 0) {
-            s_ctr[b_start % 3] = 0;
-            s_ctr[(b_start + 1) % 3] = 0;
-        }
+        // the chunk counters of rows b_start .. b_start + 2 (every slot: with row 0 swept apart from the
+        // interior loop, no loop iteration restarts row 2's)
+        if (tid < 3) s_ctr[tid] = 0;
         for (int q = 0; q < 3 && b_start + q <= b_top; ++q)
             row_to_lds(Tr + (int64_t)(b_start + q) * rowlen, rows + ((b_start + q) % 3) * slot, slot, wid, lane,
                        SCHED_THREADS);
@@ -1097,7 +1099,7 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
         // none of their slots)
         struct Chunk {
             int row;
-            int jl, nuse;
+            int jl, nuse, pa;
             double offb;
             float lw, M;
             float aL[PN];
@@ -1119,6 +1121,7 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
             P.jl = (int)(d & SD_READ_MASK);
             P.nuse = act ? (int)((d >> SD_USE_SHIFT) & 7u) + 1 : 0;
             const int pa = (int)(d >> SD_POS_SHIFT);
+            P.pa = pa;
             const int2 mlw = s_mlw[P.jl];
             P.offb = s_offb[P.jl];
             P.lw = __int_as_float(mlw.y);
@@ -1172,9 +1175,45 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
             if (nuse > 0) atomicAdd(&s_acc[P.jl], sm * exp_wide<V>(mx - (double)P.M));
         };
 
+        // rows 0 and nb - 2, where beta may be clamped (about 0.1 % of the lookups): every chunk of row b,
+        // each lane's piece by the exact lookup (Delta recomputed with the clamp from ln|z_L| and ln|z_H|)
+        // in an online float64 log-sum-exp; rows b and b + 1 are in the ring
+        auto edge_row = [&](int b) {
+            const float* R0 = rows + (b % 3) * slot;
+            const float* R1 = rows + ((b + 1) % 3) * slot;
+            const int Q = ro[b + 1] - ro[b];
+            for (int c = take(b); c >= 0; c = take(b)) {
+                const int i = c * 64 + lane;
+                if (i < Q) {
+                    const uint32_t d = dt[ro[b] + i];
+                    const int jl = (int)(d & SD_READ_MASK), nuse = (int)((d >> SD_USE_SHIFT) & 7u) + 1;
+                    const int pa = (int)(d >> SD_POS_SHIFT);
+                    const int2 mlw = s_mlw[jl];
+                    const double offb = s_offb[jl], lw = (double)__int_as_float(mlw.y);
+                    const int64_t e0 = ((int64_t)mlw.x * ntau + r) * nw + pa;
+                    const float* wp = wa + (int64_t)mlw.x * nw + pa;
+                    double mx = -1e300, sm = 0.0;
+                    for (int k = 0; k < nuse; ++k) {
+                        const double a = exact_lookup<V>(R0, R1, geo, (double)b, offb, lw, wp[k], stream[e0 + k],
+                                                         aHs[e0 + k]);
+                        const double dd = a - mx;
+                        const double ex = exp_wide<double>(-fabs(dd));
+                        sm = dd > 0.0 ? fma(sm, ex, 1.0) : sm + ex;
+                        mx = fmax(mx, a);
+                    }
+                    atomicAdd(&s_acc[jl], sm * exp_wide<V>(mx - (double)s_M[jl]));
+                }
+            }
+        };
+        if (b_start == 0) {
+            edge_row(0);
+            __syncthreads();
+        }
+        // the interior rows 1 .. nb - 3
+        const int bi_start = max(b_start, 1), bi_end = min(b_end, nb - 3);
         Chunk cur, nxt;
         bool have = false;                      // cur holds a chunk of row b
-        for (int b = b_start; b <= b_end; ++b) {
+        for (int b = bi_start; b <= bi_end; ++b) {
             // row b + 2 into the slot row b - 1 left; it lands while row b is read.  Row b + 2's counter
             // (row b - 1's, last used before the barrier that ended row b - 1) restarts for it.
             if (b > b_start && b + 2 <= b_top)
@@ -1189,10 +1228,10 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
                 }
             }
             while (have) {
-                // the next chunk (this row's, or once it is dealt, row b + 1's) is requested before this
-                // one is computed
+                // the next chunk (this row's, or once it is dealt, row b + 1's when it is interior) is
+                // requested before this one is computed
                 int nb_ = b, c = take(b);
-                if (c < 0 && b + 1 <= b_end) {
+                if (c < 0 && b + 1 <= bi_end) {
                     nb_ = b + 1;
                     c = take(b + 1);
                 }
@@ -1204,6 +1243,13 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
             }
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
+        }
+        if (b_end == nb - 2 && nb - 2 > 0) {
+            // rows nb - 2 and nb - 1 are in the ring: preloaded, or loaded by the interior sweep's last
+            // iteration and landed at its barrier.  Its counter was restarted at the start or by the
+            // iteration of row nb - 4, and the interior sweep takes no chunk of it.
+            edge_row(nb - 2);
+            __syncthreads();                    // every read's sum complete
         }
     }
     if (tid < nj) {
@@ -1225,23 +1271,25 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
 }
 
 // The first and the last row pair of a table (rows 0 and nb - 2), where beta may be clamped: each lookup
-// recomputes its Delta coordinate from ln|z_L| and ln|z_H| (exact_lookup).  One thread per (read, lag)
-// walks the read's times in those rows, reading the table values from memory (about 0.1 % of the lookups
-// fall there; most threads find none and return at once), and merges its (max, sum) into om, os, which
-// the interior rows' gather has written.  A (read, lag) the scheduled form flagged (flags, when given)
-// has its interior rows recomputed here the same way.
+// recomputes its Delta coordinate from ln|z_L| and ln|z_H| (exact_lookup).  With edge_rows (after the
+// piece and wave forms, which sweep the interior rows only), one thread per (read, lag) walks the read's
+// times in those rows, reading the table values from memory (about 0.1 % of the lookups fall there; most
+// threads find none and return at once), and merges its (max, sum) into om, os, which the interior rows'
+// gather has written.  A (read, lag) the scheduled form flagged (flags, when given; that form sweeps
+// every row itself) has all its rows recomputed here the same way.
 template <typename V>
 __global__ __launch_bounds__(256) void k_gather_edge(
     const uint2* __restrict__ stream, const float* __restrict__ aHs, const float* __restrict__ wa, int nw,
     const float* __restrict__ lw_g, const float* __restrict__ T, int64_t tstride, int nphif, int nb, int nd,
     int ntau, const int32_t* __restrict__ jm, const int32_t* __restrict__ jtab, const double* __restrict__ joffb,
     int64_t njobs, const int32_t* __restrict__ first_last, const uint16_t* __restrict__ bounds, double rdb,
-    double offd, double rdd, double* __restrict__ om, double* __restrict__ os, const uint8_t* __restrict__ flags) {
+    double offd, double rdd, double* __restrict__ om, double* __restrict__ os, const uint8_t* __restrict__ flags,
+    int edge_rows) {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int r = blockIdx.y;
     if (j >= njobs) return;
     const int first = first_last[2 * j], last = first_last[2 * j + 1];
-    const bool low = first == 0, high = last >= nb - 2;
+    const bool low = edge_rows && first == 0, high = edge_rows && last >= nb - 2;
     const bool redo = flags != nullptr && flags[j * ntau + r] != 0;
     if (!low && !high && !redo) return;
     const int rowlen = nphif * nd;
@@ -1409,7 +1457,7 @@ int launch_piece(Store* st, const uint2* stream, const float* aHs, const float* 
     const double offd = -st->delta0 / st->ddelta, rdd = 1.0 / st->ddelta;
     uint8_t* flags = nullptr;
     if (form == BandForm::Sched) {
-        // the pieces of every (tile, interior row), listed once for all lags
+        // the pieces of every (tile, row), listed once for all lags
         EXM_CHECK(st->nb <= SCHED_THREADS, EXM_ERR_ARG, "the schedule takes at most %d beta rows", SCHED_THREADS);
         EXM_CHECK(nw < (1 << (32 - SD_POS_SHIFT)), EXM_ERR_ARG, "walk of %d times exceeds the piece descriptor", nw);
         const int64_t cap = sched_cap_per_read(nw, pn, st->nb);
@@ -1461,12 +1509,12 @@ int launch_piece(Store* st, const uint2* stream, const float* aHs, const float* 
                            js.d_offb, js.d_tiles, first_last, bounds, rdb, offd, rdd, om, os, accumulate ? 1 : 0);
         EXM_HIP(hipGetLastError());
     }
-    // rows 0 and nb - 2 (beta possibly clamped), merged into the interior rows' (max, sum), and the
-    // interior rows of any (read, lag) the scheduled form flagged
+    // the piece and wave forms: rows 0 and nb - 2 (beta possibly clamped), merged into the interior rows'
+    // (max, sum); the scheduled form: every row of any (read, lag) it flagged
     auto edge = c.f32_interp ? k_gather_edge<float> : k_gather_edge<double>;
     hipLaunchKernelGGL(edge, dim3((unsigned)((js.njobs + 255) / 256), st->ntau), dim3(256), 0, cs, stream, aHs, wa, nw,
                        lw_g, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab, js.d_offb,
-                       js.njobs, first_last, bounds, rdb, offd, rdd, om, os, flags);
+                       js.njobs, first_last, bounds, rdb, offd, rdd, om, os, flags, form == BandForm::Sched ? 0 : 1);
     EXM_HIP(hipGetLastError());
     return EXM_OK;
 }
