@@ -55,10 +55,14 @@ def parse():
     p.add_argument("--filter-precision", choices=["f64", "f32"], default="f64",
                    help="matched-filter transform (weighting, modulus and phase stay float64)")
     p.add_argument("--ingest-nodes", type=int, default=4, help="ingest: full-size tables written and read back")
-    p.add_argument("--mode", choices=["plain", "serve", "stream", "batch", "ingest", "tables"], default="plain",
+    p.add_argument("--mode", choices=["plain", "serve", "stream", "batch", "ingest", "tables", "published"],
+                   default="plain",
                    help="plain: pe_intrinsic on one segment per step (the headline); serve: the request loop "
                         "under Poisson arrivals; stream: pe_redshift over 8 s segments with an estimated spectrum; "
-                        "batch: 256 segments of uneven length in batched launches (float32 form, autotuned)")
+                        "batch: 256 segments of uneven length in batched launches (float32 form, autotuned); "
+                        "published: the reference's own timed workloads, beside its published times")
+    p.add_argument("--published", default="speed,twelve,three",
+                   help="published: which of the reference's timed workloads to run (speed, twelve, three)")
     p.add_argument("--segments", type=int, default=64, help="stream: segments in the timed run")
     p.add_argument("--group", type=int, default=32, help="stream: segments sharing one spectrum estimate")
     p.add_argument("--rate", type=float, default=50.0, help="serve: mean arrival rate (requests/s)")
@@ -652,6 +656,131 @@ def main_tables(args):
     print(json.dumps(line), flush=True)
 
 
+# The reference's own timings of the workloads below (its notebooks, NVIDIA hardware, its real tables of the
+# same geometry): seconds per segment, the GPU, where.
+PUBLISHED = {
+    "speed": (0.102, "NVIDIA L40S", "notebooks/05_speed.ipynb cell 7: 128 segments in 13 s, batches of 16"),
+    "twelve": (2.75, "NVIDIA L40S", "notebooks/05_speed.ipynb cell 3: 12 segments in 33.0 s (chunk=12, "
+                                    "block=64); 1.7 s per segment with the default block in "
+                                    "02_single_segments.ipynb cell 14"),
+    "three": (33.0, "GPU not named", "notebooks/02_single_segments.ipynb cell 5: 2 segments x 78,000 templates "
+                                     "in 66 s (29.9 s per segment over 8 segments, cell 11)"),
+}
+
+
+def reference_noise():
+    """S(f) of the noise curve in the reference's own segment files (sig2 = T S / 2 on their 8 s, 20-512 Hz
+    grid; tests/golden/segments_2p.npz holds two of those segments)"""
+    with np.load(os.path.join(ROOT, "tests", "golden", "segments_2p.npz")) as z:
+        f, S = z["f"].copy(), 2.0 * z["sig2"] / float(z["T"])
+    return lambda x: np.interp(x, f, S, left=np.inf, right=np.inf)
+
+
+def reads_of(bank, st, ymin=0.1267):
+    """(template, node) reads of one segment: the work of its gather, 87 lags x 2048 walked times each"""
+    from exmarg_amd.search.pe import _node_geometry
+    geometry = _node_geometry(bank, st, ymin)
+    return int(geometry[0].sum() + (geometry[3] >= 0).sum())
+
+
+def published_entry(name, seconds, n_seg, reads, n_lag, extra):
+    ref_s, gpu, where = PUBLISHED[name]
+    s = seconds / n_seg
+    return dict(extra, seconds_per_segment=s, segments_per_gpu_hour=3600.0 / s, segments=n_seg,
+                reads_per_segment=reads, lookups_per_segment=reads * n_lag * 2048,
+                reference={"seconds_per_segment": ref_s, "gpu": gpu, "source": where},
+                speedup_vs_reference=ref_s / s)
+
+
+def main_published(args):
+    """the reference's own timed workloads through the drop-in, one JSON line beside its published times:
+
+    speed   128 simulated 8 s segments (20-512 Hz, equal-mass injections M_det ~ U(20, 200), D_L log-uniform in
+            1-15 Gpc), the 120 masses geomspace(20, 200) of its paper, pe_redshift on the 1600 x 1600 grid in
+            batches of 16 segments sorted by SNR (05_speed.ipynb cell 7)
+    twelve  12 segments of 8 s with its 3517-mass bank, pe_redshift in one call (05_speed.ipynb cell 3): the two
+            segments of its data file in tests/golden, six times each
+    three   2 segments of 4 s (20-512 Hz) with its 78,000-template bank (120 masses geomspace(20, 200) x 26 q in
+            [0.3, 1] x 25 chi in [-0.6, 0.6]) in 10 pieces, pe_intrinsic 'volume' (02_single_segments.ipynb cell 5)
+
+    The segments' noise is the curve of the reference's segment files; the tables are the synthetic 53 nodes
+    of the same geometry (the work of a lookup does not depend on the table values)."""
+    from exmarg_amd import _hip, synth
+    from exmarg_amd.search import mass_bank, pe_intrinsic, pe_redshift, store as stm
+    _hip.init(0)
+    info = _hip.device_info()
+    st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
+    noise = reference_noise()
+    n_lag = st.tau_idx.size
+    which = [w.strip() for w in args.published.split(",") if w.strip()]
+    out = {"metric": "seconds per segment of the reference's own timed workloads (its notebooks), this drop-in "
+                     "on one MI355X beside the reference's published times",
+           "unit": "s/segment", "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+           "device": info["name"], "num_cu": info["num_cu"], "workloads": {}}
+    fast = dict(block=64, resident=True)
+    if "speed" in which:
+        draw = lambda rng: (rng.uniform(20.0, 200.0), 1.0, 0.0)                       # noqa: E731
+        draw_y = lambda rng: synth.D0 / np.exp(rng.uniform(np.log(1000.0), np.log(15000.0)))   # noqa: E731
+        seg, _, snr = synth.simulate(128, 20261020, draw, draw_y, 8.0, 20.0, 512.0, psd=noise)
+        bank = mass_bank(np.geomspace(20.0, 200.0, 120), seg, "hip")
+        pe_redshift(seg, bank, st, rows=[0], **fast)
+        order = np.argsort(snr)
+        batches = [np.sort(order[i:i + 16]) for i in range(0, order.size, 16)]
+        per = []
+        t0 = time.perf_counter()
+        for b in batches:
+            t1 = time.perf_counter()
+            pe_redshift(seg, bank, st, rows=b, chunk=16, **fast)
+            per.append((time.perf_counter() - t1) / b.size)
+        wall = time.perf_counter() - t0
+        out["workloads"]["speed"] = published_entry(
+            "speed", wall, seg["d_H"].shape[0], reads_of(bank, st), n_lag,
+            {"workload": "128 simulated 8 s segments at 20-512 Hz, SNR %.1f-%.1f; 120-mass equal-mass bank "
+                         "geomspace(20, 200); pe_redshift 1600 x 1600, batches of 16" % (snr.min(), snr.max()),
+             "batch_seconds_per_segment": {"min": min(per), "max": max(per)}})
+        print("speed: %.4f s per segment" % (wall / 128), file=sys.stderr, flush=True)
+    if "twelve" in which:
+        with np.load(os.path.join(ROOT, "tests", "golden", "segments_2p.npz")) as z:
+            two = {k: z[k] for k in z.files}
+        seg = dict(two, d_H=np.tile(two["d_H"], (6, 1)), d_L=np.tile(two["d_L"], (6, 1)))
+        bank = mass_bank(os.path.join(ROOT, "exmarg_amd", "data", "bank_2p.npy"), seg, "hip")
+        pe_redshift(seg, bank, st, rows=[0], **fast)
+        t0 = time.perf_counter()
+        res = pe_redshift(seg, bank, st, chunk=12, **fast)
+        wall = time.perf_counter() - t0
+        out["workloads"]["twelve"] = published_entry(
+            "twelve", wall, 12, reads_of(bank, st), n_lag,
+            {"workload": "12 segments of 8 s at 20-512 Hz (the reference's two_parameters rows 0 and 1, six times "
+                         "each); its 3517-mass bank; pe_redshift 1600 x 1600 in one call",
+             "ln_b": [float(v) for v in res["ln_b"][:2]]})
+        print("twelve: %.4f s per segment" % (wall / 12), file=sys.stderr, flush=True)
+    if "three" in which:
+        draw = lambda rng: (rng.uniform(25.0, 150.0), rng.uniform(0.35, 1.0), rng.uniform(-0.5, 0.5))  # noqa: E731
+        draw_y = lambda rng: synth.D0 / np.exp(rng.uniform(np.log(400.0), np.log(2000.0)))   # noqa: E731
+        seg, _, snr = synth.simulate(2, 20261021, draw, draw_y, 4.0, 20.0, 512.0, psd=noise)
+        axes = np.meshgrid(np.geomspace(20.0, 200.0, 120), np.linspace(0.3, 1.0, 26), np.linspace(-0.6, 0.6, 25),
+                           indexing="ij")
+        rows = np.stack([a.ravel() for a in axes], axis=1)
+        pieces = np.array_split(np.arange(rows.shape[0]), 10)
+        t0 = time.perf_counter()
+        banks = [mass_bank(rows[p], seg, "hip") for p in pieces]
+        t_bank = time.perf_counter() - t0
+        kw = dict(prior="volume", resident=True)
+        pe_intrinsic(seg, banks[0], st, rows=[0], **kw)
+        lnl = np.empty((2, rows.shape[0]))
+        t0 = time.perf_counter()
+        for p, b in zip(pieces, banks):
+            lnl[:, p] = pe_intrinsic(seg, b, st, rows=[0, 1], chunk=2, **kw)["ln_lambda"]
+        wall = time.perf_counter() - t0
+        out["workloads"]["three"] = published_entry(
+            "three", wall, 2, sum(reads_of(b, st) for b in banks), n_lag,
+            {"workload": "2 simulated 4 s segments at 20-512 Hz (SNR %.1f, %.1f); 78,000-template (M_det, q, chi) "
+                         "bank in 10 pieces; pe_intrinsic 'volume'" % tuple(snr),
+             "bank_build_s": t_bank})
+        print("three: %.3f s per segment" % (wall / 2), file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -666,6 +795,8 @@ def main():
         return main_ingest(args)
     if args.mode == "tables":
         return main_tables(args)
+    if args.mode == "published":
+        return main_published(args)
     if args.mode == "batch":
         if args.precision == "f64" and "--precision" not in " ".join(sys.argv):
             args.precision = "f32"

@@ -1331,12 +1331,12 @@ int exm_source_frame(const double* lamN, const double* yN, int nrow, int nm, con
     EXM_TRY(source_frame_limits(nrow, nm));
     for (int m = 1; m < nm; ++m)
         EXM_CHECK(Mdet[m] > Mdet[m - 1], EXM_ERR_ARG, "detector-frame masses must ascend (template %d)", m);
-    // device copies: lamN, yN, M, yq, m1, z, w | surf | out | ln_b (slot 8)
+    // device copies: lamN, yN, M, yq, m1, z, w | surf | out | ln_b (slot 8) | ln B partials
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t n_in = 2 * (size_t)nrow * nm + nm + nz + nm1 + nz;
     const size_t n_grid = (size_t)nm1 * nz;
     const size_t o_w = al(8 * n_in), o_surf = o_w + (w ? al(8 * n_grid) : 0), o_out = o_surf + al(8 * (size_t)nz * nm),
-                 o_lnb = o_out + al(8 * n_grid), total = o_lnb + 256;
+                 o_lnb = o_out + al(8 * n_grid), o_part = o_lnb + 256, total = o_part + al(16 * LOG_MEAN_BLOCKS);
     void* buf = nullptr;
     EXM_TRY(scratch(8, total, &buf));
     char* b = (char*)buf;
@@ -1356,7 +1356,8 @@ int exm_source_frame(const double* lamN, const double* yN, int nrow, int nm, con
     double* dlnb = (double*)(b + o_lnb);
     {
         ProfileScope prof("source_frame");
-        EXM_TRY(launch_source_frame(dL, dY, nrow, nm, dM, dyq, dm1, nm1, dz, nz, dw, (double*)(b + o_surf), dout, dlnb));
+        EXM_TRY(launch_source_frame(dL, dY, nrow, nm, dM, dyq, dm1, nm1, dz, nz, dw, (double*)(b + o_surf), dout, dlnb,
+                                    (double*)(b + o_part)));
     }
     if (out) EXM_HIP(hipMemcpyAsync(out, dout, 8 * n_grid, hipMemcpyDeviceToHost, s));
     if (w) EXM_HIP(hipMemcpyAsync(ln_b, dlnb, 8, hipMemcpyDeviceToHost, s));

@@ -327,8 +327,10 @@ int store_load_files(Store* st, int n, const char* const* paths, const int64_t* 
                      int readers, double* seconds);
 int launch_source_frame(const double* lamN, const double* yN, int nrow, int nm, const double* M, const double* yq,
                         const double* m1, int nm1, const double* z, int nz, const double* w, double* surf,
-                        double* out, double* ln_b);
+                        double* out, double* ln_b, double* part);
 int source_frame_limits(int nrow, int nm);
+// workgroups of the first stage of ln B on the source-frame grid; part holds 2 doubles per workgroup
+constexpr int LOG_MEAN_BLOCKS = 1024;
 int launch_taylorf2(const double* d_M, const double* d_q, const double* d_chi, int n_tmpl, int n_freq, int lo,
                     double T, double d0_mpc, const double* d_sig2H, float2* d_W, double* d_u);
 int launch_fill(double* dst, int64_t n, double v);

@@ -19,7 +19,7 @@ namespace {
 constexpr int SRC_ROWS = 64;        // node rows per template (53 nodes + the boundary row fit)
 constexpr int SURF_THREADS = 64;
 constexpr int OUT_THREADS = 256;
-constexpr int LSE_THREADS = 1024;
+constexpr int LM_THREADS = 256;
 
 // one wave per template: its finite points sorted by y (rank sort), then every yq
 __global__ __launch_bounds__(SURF_THREADS) void k_surface(const double* __restrict__ lamN, const double* __restrict__ yN,
@@ -122,35 +122,59 @@ __global__ __launch_bounds__(OUT_THREADS) void k_mass_interp(const double* __res
     }
 }
 
-// ln sum w e^L over the finite cells with w > 0: max, then the scaled sum (one workgroup)
-__global__ __launch_bounds__(LSE_THREADS) void k_log_mean(const double* __restrict__ L, const double* __restrict__ w,
-                                                          int64_t n, double* __restrict__ ln_b) {
-    __shared__ double red[LSE_THREADS / 64];
+// ln sum w e^L over the finite cells with w > 0, in two stages.  Stage 1: workgroup b takes the cells
+// [b per, (b + 1) per) and writes its (max, sum w e^(L - max)) to part[2 b], part[2 b + 1] ((-inf, 0) when
+// none of its cells counts).  Stage 2 (one workgroup): M = the largest partial max, ln_b = M + ln sum_b
+// part_sum_b e^(part_max_b - M), or -inf when no cell counts.
+__device__ __forceinline__ double block_reduce(double v, bool is_max, double* red) {
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    for (int d = 32; d > 0; d >>= 1) {
+        const double o = __shfl_xor(v, d, 64);
+        v = is_max ? fmax(v, o) : v + o;
+    }
+    __syncthreads();                    // red may still be read by the previous reduction
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    v = red[0];
+    for (int k = 1; k < LM_THREADS / 64; ++k) v = is_max ? fmax(v, red[k]) : v + red[k];
+    return v;
+}
+
+__global__ __launch_bounds__(LM_THREADS) void k_log_mean_part(const double* __restrict__ L,
+                                                              const double* __restrict__ w, int64_t n, int64_t per,
+                                                              double* __restrict__ part) {
+    __shared__ double red[LM_THREADS / 64];
+    const int64_t a = (int64_t)blockIdx.x * per, b = min(n, a + per);
     double mx = -INFINITY;
-    for (int64_t i = t; i < n; i += LSE_THREADS)
+    for (int64_t i = a + threadIdx.x; i < b; i += LM_THREADS)
         if (w[i] > 0.0 && isfinite(L[i])) mx = fmax(mx, L[i]);
-    for (int d = 32; d > 0; d >>= 1) mx = fmax(mx, __shfl_xor(mx, d, 64));
-    if (lane == 0) red[wid] = mx;
-    __syncthreads();
-    mx = -INFINITY;
-    for (int k = 0; k < LSE_THREADS / 64; ++k) mx = fmax(mx, red[k]);
-    __syncthreads();
+    mx = block_reduce(mx, true, red);
+    double s = 0.0;
+    if (isfinite(mx))
+        for (int64_t i = a + threadIdx.x; i < b; i += LM_THREADS)
+            if (w[i] > 0.0 && isfinite(L[i])) s += w[i] * exp(L[i] - mx);
+    s = block_reduce(s, false, red);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = mx;
+        part[2 * blockIdx.x + 1] = s;
+    }
+}
+
+__global__ __launch_bounds__(LM_THREADS) void k_log_mean(const double* __restrict__ part, int nblk,
+                                                         double* __restrict__ ln_b) {
+    __shared__ double red[LM_THREADS / 64];
+    double mx = -INFINITY;
+    for (int k = threadIdx.x; k < nblk; k += LM_THREADS) mx = fmax(mx, part[2 * k]);
+    mx = block_reduce(mx, true, red);
     if (!isfinite(mx)) {
-        if (t == 0) *ln_b = -INFINITY;
+        if (threadIdx.x == 0) *ln_b = -INFINITY;
         return;
     }
     double s = 0.0;
-    for (int64_t i = t; i < n; i += LSE_THREADS)
-        if (w[i] > 0.0 && isfinite(L[i])) s += w[i] * exp(L[i] - mx);
-    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
-    if (lane == 0) red[wid] = s;
-    __syncthreads();
-    if (t == 0) {
-        double tot = 0.0;
-        for (int k = 0; k < LSE_THREADS / 64; ++k) tot += red[k];
-        *ln_b = mx + log(tot);
-    }
+    for (int k = threadIdx.x; k < nblk; k += LM_THREADS)
+        if (isfinite(part[2 * k])) s += part[2 * k + 1] * exp(part[2 * k] - mx);
+    s = block_reduce(s, false, red);
+    if (threadIdx.x == 0) *ln_b = mx + log(s);
 }
 
 }  // namespace
@@ -163,7 +187,7 @@ int source_frame_limits(int nrow, int nm) {
 
 int launch_source_frame(const double* lamN, const double* yN, int nrow, int nm, const double* M, const double* yq,
                         const double* m1, int nm1, const double* z, int nz, const double* w, double* surf,
-                        double* out, double* ln_b) {
+                        double* out, double* ln_b, double* part) {
     EXM_TRY(source_frame_limits(nrow, nm));
     const hipStream_t s = cur_stream();
     hipLaunchKernelGGL(k_surface, dim3(nm), dim3(SURF_THREADS), 0, s, lamN, yN, nrow, nm, yq, nz, surf);
@@ -173,7 +197,12 @@ int launch_source_frame(const double* lamN, const double* yN, int nrow, int nm, 
     hipLaunchKernelGGL(k_mass_interp, dim3(nz), dim3(OUT_THREADS), lds, s, surf, nm, M, m1, nm1, z, nz, out);
     EXM_HIP(hipGetLastError());
     if (w) {
-        hipLaunchKernelGGL(k_log_mean, dim3(1), dim3(LSE_THREADS), 0, s, out, w, (int64_t)nm1 * nz, ln_b);
+        const int64_t n = (int64_t)nm1 * nz;
+        const int64_t per = (n + LOG_MEAN_BLOCKS - 1) / LOG_MEAN_BLOCKS;       // n >= 1, so per >= 1
+        const int nblk = (int)((n + per - 1) / per);
+        hipLaunchKernelGGL(k_log_mean_part, dim3(nblk), dim3(LM_THREADS), 0, s, out, w, n, per, part);
+        EXM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_log_mean, dim3(1), dim3(LM_THREADS), 0, s, part, nblk, ln_b);
         EXM_HIP(hipGetLastError());
     }
     return EXM_OK;

@@ -223,6 +223,60 @@ class _Analysis:
             _store.free(self.store)
 
 
+# Reads per table that one gather launch should reach before segments share it: below it a launch's
+# workgroups have few lookups per staged table row, and staging the rows, not the lookups, sets its time.
+READS_PER_TABLE = 4096
+# template-segments one shared launch may hold (2.14 MB of walk records each at 87 lags x 2048 times)
+SHARED_TEMPLATES = 8192
+
+
+def _segments_per_launch(an, n_seg, chunk, n_loud):
+    """how many segments share one gather launch: 1 unless the bank's reads per table are few, then up to
+    chunk (the reference's segments per launch, src/exmarg/search/pe.py:173-194) and as many as bring the
+    reads per table to READS_PER_TABLE, within SHARED_TEMPLATES template-segments"""
+    if n_loud <= 0 or n_seg < 2 or int(chunk) < 2:
+        return 1
+    n_tab = max(int(np.asarray(an.store.s).size), 1)
+    per_tab = max(an.plan.job_m.size / n_tab, 1.0)
+    want = int(np.ceil(READS_PER_TABLE / per_tab))
+    return max(1, min(int(chunk), n_seg, want, SHARED_TEMPLATES // max(an.bank.u.size, 1)))
+
+
+def _segment_results(an, seg, chunk, n_loud, n_random, seed, lse):
+    """(j, result) for every segment j of seg, in order: (ln Lambda, ln B) assembled on the device when lse,
+    else the node values lamN (n_node + 1, n_template).  Segments whose bank has few reads per table share
+    gather launches (a SegmentBatch of _segments_per_launch segments), so that the table rows one launch
+    stages serve the reads of all of them; otherwise, or when the device memory for a shared launch is not
+    free, each segment has a launch of its own."""
+    n = seg["rows"].size
+    g = _segments_per_launch(an, n, chunk, n_loud)
+    batch = None
+    if g > 1:
+        from .batch import SegmentBatch
+        try:
+            batch = SegmentBatch(an.store, n_loud, n_random, max_templates=g * an.bank.u.size)
+        except MemoryError:
+            batch = None
+    if batch is None:
+        for j in range(n):
+            yield j, (an.ln_lambda(seg, j, n_loud, n_random, seed) if lse
+                      else an.node_values(seg, j, n_loud, n_random, seed))
+        return
+    try:
+        sig2 = {I: seg["sig2_" + I] for I in "HL"}
+        for j0 in range(0, n, g):
+            js = range(j0, min(n, j0 + g))
+            for j in js:
+                batch.add(an, {I: seg["d_" + I][j] for I in "HL"}, sig2, seed + int(seg["rows"][j]))
+            batch.launch()
+            res = batch.collect_lse() if lse else batch.collect()
+            batch.reset()
+            for j, r in zip(js, res):
+                yield j, r
+    finally:
+        batch.free()
+
+
 def pe_intrinsic(segments, bank, store, rows=None, prior="volume", ymin=0.1267, chunk=4, nodes_per_launch=6,
                  block=256, resident=False, n_loud=1024, n_random=1024, seed=0):
     """ln Lambda of each template of the bank, the distance integrated over its prior
@@ -237,8 +291,9 @@ def pe_intrinsic(segments, bank, store, rows=None, prior="volume", ymin=0.1267, 
     resident          keep the tables on the device after the call
     n_loud, n_random  arrival times visited per template (select_times); n_loud = 0 visits all of them
     seed              seed of the random draw (plus the segment index)
-    chunk, nodes_per_launch, block are accepted for the reference signature: every node of a segment
-    is read in one launch.
+    chunk             segments per gather launch at most (see _segment_results); nodes_per_launch and
+                      block are accepted for the reference signature: every node of a segment is read in
+                      one launch.
 
     returns dict: M, q, chi, u (n_template,), ln_lambda (n_segment, n_template), ln_b (n_segment,), rows."""
     seg = read_segments(segments, rows)
@@ -248,8 +303,8 @@ def pe_intrinsic(segments, bank, store, rows=None, prior="volume", ymin=0.1267, 
     try:
         out = np.full((seg["rows"].size, bank.u.size), -np.inf)
         ln_b = np.full(seg["rows"].size, -np.inf)
-        for j in range(seg["rows"].size):
-            out[j], ln_b[j] = an.ln_lambda(seg, j, n_loud, n_random, seed)
+        for j, (L, lnb) in _segment_results(an, seg, chunk, n_loud, n_random, seed, lse=True):
+            out[j], ln_b[j] = L, lnb
     finally:
         an.close()
     return dict(M=bank.M, q=bank.q, chi=bank.chi, u=bank.u, ln_lambda=out, ln_b=ln_b, rows=seg["rows"])
@@ -264,6 +319,8 @@ def pe_redshift(segments, bank, store, rows=None, chunk=8, zmax=8.0, nz=1600, m1
     spin) on the same segments.  ln B averages Lambda over the prior uniform in M_det = 2 m1 (1+z) on
     mdet_range and proportional to D_L^2 in luminosity distance, p(m1, z) ~ (1+z) D_L^2 dD_L/dz.
 
+    chunk is the number of segments one gather launch reads at most (see _segment_results).
+
     returns dict: m1 (nm1,), z (nz,), ln_lambda (n_segment, nm1, nz), ln_b (n_segment,), rows."""
     if np.any(np.diff(bank.M) <= 0) or np.any(bank.q != 1.0) or np.any(bank.chi != 0.0):
         raise ValueError("pe_redshift needs an ascending equal-mass zero-spin bank")
@@ -277,8 +334,7 @@ def pe_redshift(segments, bank, store, rows=None, chunk=8, zmax=8.0, nz=1600, m1
     out = np.full((seg["rows"].size, m1.size, z.size), -np.inf)
     ln_b = np.empty(seg["rows"].size)
     try:
-        for j in range(seg["rows"].size):
-            lamN = an.node_values(seg, j, n_loud, n_random, seed)
+        for j, lamN in _segment_results(an, seg, chunk, n_loud, n_random, seed, lse=False):
             ln_b[j] = source_frame(lamN, an.geometry[2], bank.M, yq, m1, z, w, out=out[j])
     finally:
         an.close()

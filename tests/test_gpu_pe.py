@@ -99,6 +99,46 @@ def test_pe_redshift_matches_oracle(hip, oracle):
     stm.free(st)
 
 
+def test_segments_sharing_a_launch_match_one_launch_each(hip, monkeypatch):
+    """with chunk > 1 the segments of a bank with few reads per table share gather launches (a SegmentBatch
+    of up to chunk segments); every segment's pe_redshift grid and ln B, and pe_intrinsic ln Lambda and ln B,
+    equal those of one launch per segment (chunk=1) to float64 rounding"""
+    import exmarg_amd.search.batch as bmod
+    from exmarg_amd.search import mass_bank, pe_intrinsic, pe_redshift, store as stm
+    say("shared launches: 3 segments, chunk 2 (pe_redshift) and 3 (pe_intrinsic) against chunk 1")
+    seg, bankrows, s_nodes = small_problem()
+    seg = dict(seg, d_H=np.concatenate([seg["d_H"], seg["d_H"][:1] * 0.5]),
+               d_L=np.concatenate([seg["d_L"], seg["d_L"][:1] * 0.5]))
+    st = stm.synthetic_store(seed=0, s_nodes=s_nodes)
+    made = []
+
+    class Spy(bmod.SegmentBatch):
+        def __init__(self, *a, **k):
+            super().__init__(*a, **k)
+            made.append(self.cap)
+
+    monkeypatch.setattr(bmod, "SegmentBatch", Spy)
+    masses = np.geomspace(20.0, 60.0, 10)
+    rbank = mass_bank(masses, seg, "hip")
+    kw = dict(zmax=3.0, nz=90, m1_range=(5.0, 40.0), nm1=80, mdet_range=(20.0, 60.0), seed=5, resident=True)
+    one = pe_redshift(seg, rbank, st, chunk=1, **kw)
+    assert made == []
+    shared = pe_redshift(seg, rbank, st, chunk=2, **kw)
+    assert made == [2 * masses.size]          # segments 0 and 1 in one launch, segment 2 in the next
+    fin = np.isfinite(one["ln_lambda"])
+    np.testing.assert_array_equal(np.isfinite(shared["ln_lambda"]), fin)
+    np.testing.assert_allclose(shared["ln_lambda"][fin], one["ln_lambda"][fin], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(shared["ln_b"], one["ln_b"], rtol=0, atol=1e-12)
+    ibank = mass_bank(bankrows, seg, "hip")
+    made.clear()
+    one = pe_intrinsic(seg, ibank, st, chunk=1, seed=3, resident=True)
+    shared = pe_intrinsic(seg, ibank, st, chunk=3, seed=3, resident=True)
+    assert made == [3 * bankrows.shape[0]]
+    np.testing.assert_allclose(shared["ln_lambda"], one["ln_lambda"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(shared["ln_b"], one["ln_b"], rtol=0, atol=1e-12)
+    stm.free(st)
+
+
 def test_pe_intrinsic_rejects_templates_outside_the_store(hip):
     from exmarg_amd.search import mass_bank, pe_intrinsic, store as stm
     seg, bankrows, _ = small_problem(n_templates=3)
