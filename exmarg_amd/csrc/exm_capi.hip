@@ -3,7 +3,10 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cmath>
+#include <cstring>
 #include <map>
+#include <thread>
+#include <vector>
 
 #include "exm_common.h"
 
@@ -189,6 +192,47 @@ Times::~Times() {
 Plan::~Plan() {
     jobset_free(&js);
     if (d_w) dev_free(d_w, w_bytes);
+}
+SourceGrid::~SourceGrid() {
+    if (d) dev_free(d, bytes);
+}
+
+// Pinned host staging of source-frame grids (the device-to-host copy at full rate, then a threaded copy to
+// the caller's memory): one block, grown on demand and kept for the process; calls hold the library lock.
+static void* g_pin = nullptr;
+static size_t g_pin_bytes = 0;
+
+static int pinned_staging(size_t bytes, void** p) {
+    if (g_pin_bytes < bytes) {
+        if (g_pin) (void)hipHostFree(g_pin);
+        g_pin = nullptr;
+        g_pin_bytes = 0;
+        EXM_CHECK(hipHostMalloc(&g_pin, bytes, hipHostMallocDefault) == hipSuccess, EXM_ERR_HIP,
+                  "hipHostMalloc of %zu bytes failed", bytes);
+        g_pin_bytes = bytes;
+    }
+    *p = g_pin;
+    return EXM_OK;
+}
+
+// memcpy of a large host block by up to 8 threads (one when a thread cannot be started)
+static void copy_host_parallel(void* dst, const void* src, size_t bytes) {
+    constexpr size_t PIECE = 2u << 20;
+    const size_t n = std::min<size_t>(8, (bytes + PIECE - 1) / PIECE);
+    const size_t per = n ? (bytes + n - 1) / n : 0;
+    std::vector<std::thread> th;
+    size_t done = per;                  // bytes [0, per) are copied by this thread
+    try {
+        for (size_t k = 1; k < n && k * per < bytes; ++k) {
+            const size_t a = k * per, len = std::min(per, bytes - a);
+            th.emplace_back([=] { std::memcpy((char*)dst + a, (const char*)src + a, len); });
+            done = a + len;
+        }
+    } catch (...) {
+    }
+    std::memcpy(dst, src, std::min(per, bytes));
+    for (auto& t : th) t.join();
+    if (done < bytes) std::memcpy((char*)dst + done, (const char*)src + done, bytes - done);
 }
 Batch::~Batch() {
     if (done) {
@@ -1359,9 +1403,90 @@ int exm_source_frame(const double* lamN, const double* yN, int nrow, int nm, con
         EXM_TRY(launch_source_frame(dL, dY, nrow, nm, dM, dyq, dm1, nm1, dz, nz, dw, (double*)(b + o_surf), dout, dlnb,
                                     (double*)(b + o_part)));
     }
-    if (out) EXM_HIP(hipMemcpyAsync(out, dout, 8 * n_grid, hipMemcpyDeviceToHost, s));
+    void* pin = nullptr;
+    if (out) {
+        EXM_TRY(pinned_staging(8 * n_grid, &pin));
+        EXM_HIP(hipMemcpyAsync(pin, dout, 8 * n_grid, hipMemcpyDeviceToHost, s));
+    }
     if (w) EXM_HIP(hipMemcpyAsync(ln_b, dlnb, 8, hipMemcpyDeviceToHost, s));
     EXM_HIP(hipStreamSynchronize(s));
+    if (out) copy_host_parallel(out, pin, 8 * n_grid);
+    return EXM_OK;
+}
+
+int exm_source_grid_create(const double* yN, int nrow, int nm, const double* Mdet, const double* yq, const double* m1,
+                           int nm1, const double* z, int nz, const double* w, exm_handle* out) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    EXM_CHECK(yN && Mdet && yq && m1 && z && out, EXM_ERR_ARG, "NULL argument");
+    EXM_CHECK(nm1 > 0 && nz > 0, EXM_ERR_ARG, "empty (m1, z) grid");
+    EXM_TRY(source_frame_limits(nrow, nm));
+    for (int m = 1; m < nm; ++m)
+        EXM_CHECK(Mdet[m] > Mdet[m - 1], EXM_ERR_ARG, "detector-frame masses must ascend (template %d)", m);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t n_node = (size_t)nrow * nm, n_grid = (size_t)nm1 * nz;
+    // lamN | yN | M | yq | m1 | z | w | surf | out | ln_b | partials
+    const size_t o_y = al(8 * n_node), o_m = o_y + al(8 * n_node), o_yq = o_m + al(8 * (size_t)nm),
+                 o_m1 = o_yq + al(8 * (size_t)nz), o_z = o_m1 + al(8 * (size_t)nm1), o_w = o_z + al(8 * (size_t)nz),
+                 o_surf = o_w + (w ? al(8 * n_grid) : 0), o_out = o_surf + al(8 * (size_t)nz * nm),
+                 o_lnb = o_out + al(8 * n_grid), o_part = o_lnb + 256, total = o_part + al(16 * LOG_MEAN_BLOCKS);
+    SourceGrid* g = new SourceGrid();
+    g->kind = K_GRID;
+    g->nrow = nrow, g->nm = nm, g->nm1 = nm1, g->nz = nz, g->has_w = w != nullptr;
+    int rc = dev_alloc(&g->d, total);
+    if (rc != EXM_OK) {
+        delete g;
+        return rc;
+    }
+    g->bytes = total;
+    char* b = (char*)g->d;
+    g->lamN = (double*)b, g->yN = (double*)(b + o_y), g->M = (double*)(b + o_m), g->yq = (double*)(b + o_yq);
+    g->m1 = (double*)(b + o_m1), g->z = (double*)(b + o_z), g->w = w ? (double*)(b + o_w) : nullptr;
+    g->surf = (double*)(b + o_surf), g->out = (double*)(b + o_out), g->ln_b = (double*)(b + o_lnb);
+    g->part = (double*)(b + o_part);
+    const hipStream_t s = g_ctx.stream;
+    const struct { void* dst; const void* src; size_t n; } up[] = {
+        {g->yN, yN, 8 * n_node}, {g->M, Mdet, 8 * (size_t)nm}, {g->yq, yq, 8 * (size_t)nz},
+        {g->m1, m1, 8 * (size_t)nm1}, {g->z, z, 8 * (size_t)nz}, {g->w, w, w ? 8 * n_grid : 0}};
+    for (const auto& u : up)
+        if (u.n && rc == EXM_OK && hipMemcpyAsync(u.dst, u.src, u.n, hipMemcpyHostToDevice, s) != hipSuccess) {
+            set_error("hipMemcpyAsync of the source-frame grid failed");
+            rc = EXM_ERR_HIP;
+        }
+    if (rc == EXM_OK && hipStreamSynchronize(s) != hipSuccess) {
+        set_error("hipStreamSynchronize failed");
+        rc = EXM_ERR_HIP;
+    }
+    if (rc != EXM_OK) {
+        delete g;
+        return rc;
+    }
+    *out = register_object(g);
+    return EXM_OK;
+}
+
+int exm_source_grid_run(exm_handle hgrid, const double* lamN, double* out, double* ln_b) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    NEED(SourceGrid, g, hgrid, K_GRID);
+    EXM_CHECK(lamN != nullptr, EXM_ERR_ARG, "NULL argument");
+    EXM_CHECK(!g->has_w || ln_b != nullptr, EXM_ERR_ARG, "ln_b is NULL");
+    const hipStream_t s = g_ctx.stream;
+    EXM_HIP(hipMemcpyAsync(g->lamN, lamN, 8 * (size_t)g->nrow * g->nm, hipMemcpyHostToDevice, s));
+    {
+        ProfileScope prof("source_frame");
+        EXM_TRY(launch_source_frame(g->lamN, g->yN, g->nrow, g->nm, g->M, g->yq, g->m1, g->nm1, g->z, g->nz, g->w,
+                                    g->surf, g->out, g->ln_b, g->part));
+    }
+    const size_t grid_bytes = 8 * (size_t)g->nm1 * g->nz;
+    void* pin = nullptr;
+    if (out) {
+        EXM_TRY(pinned_staging(grid_bytes, &pin));
+        EXM_HIP(hipMemcpyAsync(pin, g->out, grid_bytes, hipMemcpyDeviceToHost, s));
+    }
+    if (g->has_w) EXM_HIP(hipMemcpyAsync(ln_b, g->ln_b, 8, hipMemcpyDeviceToHost, s));
+    EXM_HIP(hipStreamSynchronize(s));
+    if (out) copy_host_parallel(out, pin, grid_bytes);
     return EXM_OK;
 }
 

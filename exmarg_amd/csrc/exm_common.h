@@ -59,7 +59,7 @@ const char* last_error();
     } while (0)
 
 // ---------------------------------------------------------------- objects
-enum Kind : uint32_t { K_STORE = 1, K_BANK = 2, K_TRAJ = 3, K_TIMES = 4, K_PLAN = 5, K_BATCH = 6 };
+enum Kind : uint32_t { K_STORE = 1, K_BANK = 2, K_TRAJ = 3, K_TIMES = 4, K_PLAN = 5, K_BATCH = 6, K_GRID = 7 };
 
 struct Object {
     uint32_t kind;
@@ -174,6 +174,19 @@ struct Plan : Object {
     size_t w_bytes = 0;
     LseDevice lse;
     ~Plan();
+};
+
+// The per-call constants of the source-frame step on the device (exm_source_grid_create): node
+// coordinates yN (nrow, nm), masses, yq, m1, z and the prior weights, then per segment its node values,
+// the surface, the grid and ln B, all in one allocation.
+struct SourceGrid : Object {
+    int nrow = 0, nm = 0, nm1 = 0, nz = 0;
+    bool has_w = false;
+    void* d = nullptr;
+    size_t bytes = 0;
+    double *lamN = nullptr, *yN = nullptr, *M = nullptr, *yq = nullptr, *m1 = nullptr, *z = nullptr,
+           *w = nullptr, *surf = nullptr, *out = nullptr, *ln_b = nullptr, *part = nullptr;
+    ~SourceGrid();
 };
 
 // Segments analysed together: walk records of every admitted segment in device buffers, the reads of

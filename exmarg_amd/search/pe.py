@@ -10,6 +10,7 @@ Per segment the device runs the matched filter, the arrival-time selection, one 
 (once per bank), the log-sum-exp over each template's points and the average over the bank.
 """
 
+import functools
 import itertools
 
 import numpy as np
@@ -247,7 +248,8 @@ def _segment_results(an, seg, chunk, n_loud, n_random, seed, lse):
     else the node values lamN (n_node + 1, n_template).  Segments whose bank has few reads per table share
     gather launches (a SegmentBatch of _segments_per_launch segments), so that the table rows one launch
     stages serve the reads of all of them; otherwise, or when the device memory for a shared launch is not
-    free, each segment has a launch of its own."""
+    free, each segment has a launch of its own.  (Two slots, one gathering while the other's results are
+    used, measured no faster: the gather holds every CU, so the source-frame kernels wait for it.)"""
     n = seg["rows"].size
     g = _segments_per_launch(an, n, chunk, n_loud)
     batch = None
@@ -325,20 +327,71 @@ def pe_redshift(segments, bank, store, rows=None, chunk=8, zmax=8.0, nz=1600, m1
     if np.any(np.diff(bank.M) <= 0) or np.any(bank.q != 1.0) or np.any(bank.chi != 0.0):
         raise ValueError("pe_redshift needs an ascending equal-mass zero-spin bank")
     seg = read_segments(segments, rows)
-    z = np.linspace(0.02, zmax, int(nz))
-    m1 = np.geomspace(m1_range[0], m1_range[1], int(nm1))
-    dl = _dl_of_z(z)
-    yq = D0 / dl
-    w = redshift_prior_weights(m1, z, dl, mdet_range)
+    z, m1, yq, w = _redshift_grid(float(zmax), int(nz), float(m1_range[0]), float(m1_range[1]), int(nm1),
+                                  float(mdet_range[0]), float(mdet_range[1]))
     an = _Analysis(bank, store, ymin, resident).start()
-    out = np.full((seg["rows"].size, m1.size, z.size), -np.inf)
+    # every cell of every segment's grid is written by its source-frame step
+    out = np.empty((seg["rows"].size, m1.size, z.size))
     ln_b = np.empty(seg["rows"].size)
     try:
-        for j, lamN in _segment_results(an, seg, chunk, n_loud, n_random, seed, lse=False):
-            ln_b[j] = source_frame(lamN, an.geometry[2], bank.M, yq, m1, z, w, out=out[j])
+        grid = SourceGrid(an.geometry[2], bank.M, yq, m1, z, w)
+        try:
+            for j, lamN in _segment_results(an, seg, chunk, n_loud, n_random, seed, lse=False):
+                ln_b[j] = grid.run(lamN, out=out[j])
+        finally:
+            grid.free()
     finally:
         an.close()
-    return dict(m1=m1, z=z, ln_lambda=out, ln_b=ln_b, rows=seg["rows"])
+    return dict(m1=m1.copy(), z=z.copy(), ln_lambda=out, ln_b=ln_b, rows=seg["rows"])
+
+
+@functools.lru_cache(maxsize=4)
+def _redshift_grid(zmax, nz, m1_lo, m1_hi, nm1, mdet_lo, mdet_hi):
+    """(z, m1, yq = D0 / D_L(z), prior weights w) of a pe_redshift grid (src/exmarg/search/pe.py:236-255),
+    kept for the next call with the same grid (read-only arrays; w is nm1 x nz float64)"""
+    z = np.linspace(0.02, zmax, nz)
+    m1 = np.geomspace(m1_lo, m1_hi, nm1)
+    dl = _dl_of_z(z)
+    out = (z, m1, D0 / dl, redshift_prior_weights(m1, z, dl, (mdet_lo, mdet_hi)))
+    for a in out:
+        a.setflags(write=False)
+    return out
+
+
+class SourceGrid:
+    """the source-frame step of many segments with one bank geometry, grid and prior (one pe_redshift call):
+    the constants stay on the device (exm_source_grid_create), each segment sends its node values only
+    (exm_source_grid_run); the same arithmetic as source_frame"""
+
+    def __init__(self, yN, Mdet, yq, m1, z, w=None):
+        self._keep = [np.ascontiguousarray(a, np.float64) for a in (yN, Mdet, yq, m1, z)]
+        yN_, Md, yq_, m1_, z_ = self._keep
+        if yN_.ndim != 2 or Md.shape != (yN_.shape[1],) or yq_.shape != z_.shape:
+            raise ValueError("yN must be (nrow, nm) with one mass per template, yq one value per z")
+        wv = None if w is None else np.ascontiguousarray(w, np.float64)
+        if wv is not None and wv.shape != (m1_.size, z_.size):
+            raise ValueError("w has shape %s, expected %s" % (wv.shape, (m1_.size, z_.size)))
+        self.shape = (m1_.size, z_.size)
+        self.nodes = yN_.shape
+        self.has_w = wv is not None
+        self.handle = _hip.new_handle("exm_source_grid_create", _hip.ptr(yN_), yN_.shape[0], yN_.shape[1],
+                                      _hip.ptr(Md), _hip.ptr(yq_), _hip.ptr(m1_), m1_.size, _hip.ptr(z_), z_.size,
+                                      _hip.ptr(wv) if wv is not None else None, kind="grid")
+
+    def run(self, lamN, out=None):
+        """ln Lambda of one segment into out (nm1, nz) when given; returns ln B when the grid has w"""
+        lamN = np.ascontiguousarray(lamN, np.float64)
+        if lamN.shape != self.nodes:
+            raise ValueError("lamN has shape %s, expected %s" % (lamN.shape, self.nodes))
+        if out is not None and (out.shape != self.shape or out.dtype != np.float64 or not out.flags.c_contiguous):
+            raise ValueError("out must be a C-contiguous float64 (nm1, nz) array")
+        ln_b = np.zeros(1)
+        _hip.call("exm_source_grid_run", int(self.handle), _hip.ptr(lamN), _hip.ptr(out) if out is not None else None,
+                  _hip.ptr(ln_b))
+        return float(ln_b[0]) if self.has_w else None
+
+    def free(self):
+        self.handle.free()
 
 
 def redshift_prior_weights(m1, z, dl, mdet_range):

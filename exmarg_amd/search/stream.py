@@ -203,13 +203,17 @@ def pe_redshift_stream(groups, masses, store, zmax=10.0, nz=1600, m1_range=(2.0,
     analyses = {}
     done_order = []
 
+    grids = {}              # group -> its source-frame constants on the device
+
     def finish(k):
         lamNs = slots[k].collect()
         for (g, j, an), lamN in zip(members[k], lamNs):
             res = results[g]
             want = keep is not None and keep(g, int(res["rows"][j]))
             out = np.empty((m1.size, z.size)) if want else None
-            res["ln_b"][j] = _pe.source_frame(lamN, an.geometry[2], an.bank.M, yq, m1, z, w, out=out)
+            if g not in grids:
+                grids[g] = _pe.SourceGrid(an.geometry[2], an.bank.M, yq, m1, z, w)
+            res["ln_b"][j] = grids[g].run(lamN, out=out)
             if want:
                 res["kept"][int(res["rows"][j])] = out
             remaining[g] -= 1
@@ -231,6 +235,9 @@ def pe_redshift_stream(groups, masses, store, zmax=10.0, nz=1600, m1_range=(2.0,
         while done_order and remaining[done_order[0]] == 0:
             g = done_order.pop(0)
             an = analyses.pop(g)
+            grid = grids.pop(g, None)
+            if grid is not None:
+                grid.free()
             an.plan.handle.free()
             an.bank.handle.free()
             out.append(results.pop(g))
@@ -264,3 +271,5 @@ def pe_redshift_stream(groups, masses, store, zmax=10.0, nz=1600, m1_range=(2.0,
     finally:
         for s in slots:
             s.free()
+        for grid in grids.values():
+            grid.free()

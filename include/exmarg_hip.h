@@ -297,6 +297,16 @@ int exm_get_gather_tuning(int* kernel, int* piece, int* tile, int* precision);
 int exm_source_frame(const double* lamN, const double* yN, int nrow, int nm, const double* Mdet, const double* yq,
                      const double* m1, int nm1, const double* z, int nz, const double* w, double* out, double* ln_b);
 
+/* The same step for many segments of one pe_redshift call: exm_source_grid_create keeps the call's
+ * constants on the device (yN, Mdet, yq, m1, z and w as for exm_source_frame; the arrays are copied, the
+ * caller may free them) and exm_source_grid_run takes one segment's node values lamN (nrow, nm) and
+ * writes out (nm1, nz, may be NULL) and ln_b (when the grid has w).  Release the grid with exm_free.
+ * Replaces the per-segment loop of src/exmarg/search/pe.py:236-257 (_to_source_frame, pe.py:87-132, and
+ * _log_mean, :135-142, once per segment with the call's grid and prior). */
+int exm_source_grid_create(const double* yN, int nrow, int nm, const double* Mdet, const double* yq, const double* m1,
+                           int nm1, const double* z, int nz, const double* w, exm_handle* out);
+int exm_source_grid_run(exm_handle grid, const double* lamN, double* out, double* ln_b);
+
 /* ---------------------------------------------------------------- time-domain front end
  * The frequency-domain segment of each of n_rows time series x (n_rows, n) float64 sampled at fs:
  * d (n_rows, n_freq) complex128 = rfft(x)[nu0 .. nu0 + n_freq - 1] / fs, the segment convention of

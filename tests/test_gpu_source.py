@@ -74,3 +74,46 @@ def test_source_frame_rejects_unsorted_masses(hip):
     lamN, yN, M, yq, m1, z = case(3)
     with pytest.raises(ValueError, match="ascend"):
         pe.source_frame(lamN, yN, M[::-1].copy(), yq, m1, z)
+
+
+def test_source_grid_equals_source_frame(hip, oracle):
+    """exm_source_grid_create / _run (the call's constants kept on the device) give the grid and ln B of
+    exm_source_frame bit for bit, segment after segment on one grid; a grid without w returns no ln B;
+    the full-size ln B against a numpy sum over the 2.56e6 cells (the two-stage reduction)"""
+    from exmarg_amd.search import pe
+    from exmarg_amd import cosmology
+    say("source grid: three segments on one grid, then the 1600 x 1600 grid")
+    lamN, yN, M, yq, m1, z = case(4)
+    w = pe.redshift_prior_weights(m1, z, cosmology.luminosity_distance(z), (10.0, 500.0))
+    grid = pe.SourceGrid(yN, M, yq, m1, z, w)
+    rng = np.random.default_rng(21)
+    for k in range(3):
+        L = np.where(np.isfinite(lamN), lamN + rng.standard_normal(lamN.shape), -np.inf)
+        a, b = np.empty((m1.size, z.size)), np.empty((m1.size, z.size))
+        lb_a = grid.run(L, out=a)
+        lb_b = pe.source_frame(L, yN, M, yq, m1, z, w, out=b)
+        np.testing.assert_array_equal(a, b)
+        assert lb_a == lb_b
+        ref = oracle.to_source_frame(L, yN, M, yq, m1, z)
+        assert abs(lb_a - oracle.log_mean(ref, w)) < 1e-12
+    with pytest.raises(ValueError, match="shape"):
+        grid.run(lamN[:-1])
+    grid.free()
+    nw = pe.SourceGrid(yN, M, yq, m1, z)
+    assert nw.run(lamN, out=None) is None
+    nw.free()
+    nn, nm = 54, 3517
+    yN = np.sort(rng.uniform(0.05, 20.0, (nn, nm)), axis=0)
+    lamN = np.where(rng.uniform(size=(nn, nm)) < 0.95, rng.standard_normal((nn, nm)), -np.inf)
+    M = np.geomspace(10.0, 500.0, nm)
+    z = np.linspace(0.02, 8.0, 1600)
+    m1 = np.geomspace(3.0, 150.0, 1600)
+    yq = pe.D0 / cosmology.luminosity_distance(z)
+    w = pe.redshift_prior_weights(m1, z, cosmology.luminosity_distance(z), (10.0, 500.0))
+    grid = pe.SourceGrid(yN, M, yq, m1, z, w)
+    out = np.empty((m1.size, z.size))
+    ln_b = grid.run(lamN, out=out)
+    grid.free()
+    fin = np.isfinite(out) & (w > 0)
+    mx = out[fin].max()
+    assert abs(ln_b - (mx + np.log(np.sum(w[fin] * np.exp(out[fin] - mx))))) < 1e-12
