@@ -9,7 +9,9 @@ float32 tables in the reference store layout (39.6 GB, HBM-resident); pe_intrins
 prior and the reference defaults n_loud = n_random = 1024.
 
 A step is one pe_intrinsic call on the segment (matched filter, time selection, every table read, the
-node and bank assembly).  value = segments/s over the timed steps.
+node and bank assembly).  value = segments/s over the timed steps.  After the timed steps the same line
+reports the serving form on the same store ('serving'), the reference's own timed workloads beside its
+published times ('reference_workloads', see published_workloads) and the CPU baseline.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
 """
@@ -43,7 +45,8 @@ def parse():
                         "chirp-mass grid, one node)")
     p.add_argument("--grid", default="20,10,10", help="bank grid (M, q, chi)")
     p.add_argument("--nodes", type=int, default=53)
-    p.add_argument("--cpu-templates", type=int, default=32, help="templates in the CPU-baseline sample")
+    p.add_argument("--cpu-templates", type=int, default=512, help="templates in the CPU-baseline sample")
+    p.add_argument("--cpu-nodes", type=int, default=16, help="nodes (the most used) in the CPU-baseline gather sample")
     p.add_argument("--check-templates", type=int, default=8, help="templates checked against the oracle")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle check")
     p.add_argument("--gather-kernel", type=int, default=0, help="0 automatic, 1 reference-order kernel, 2-4 band kernel (piece, wave, scheduled form)")
@@ -62,7 +65,8 @@ def parse():
                         "batch: 256 segments of uneven length in batched launches (float32 form, autotuned); "
                         "published: the reference's own timed workloads, beside its published times")
     p.add_argument("--published", default="speed,twelve,three",
-                   help="published: which of the reference's timed workloads to run (speed, twelve, three)")
+                   help="the reference's timed workloads to run (speed, twelve, three; '' for none): the whole "
+                        "line of --mode published, a 'reference_workloads' entry of the plain line")
     p.add_argument("--segments", type=int, default=64, help="stream: segments in the timed run")
     p.add_argument("--group", type=int, default=32, help="stream: segments sharing one spectrum estimate")
     p.add_argument("--rate", type=float, default=50.0, help="serve: mean arrival rate (requests/s)")
@@ -169,7 +173,7 @@ def cpu_leg(args, seg, bank, store, res, an_geometry, seed_row):
     # -- CPU baseline in the reference's own CPU arithmetic (src/exmarg/search/bank.py:66-75 complex64
     # numpy filter, single-threaded as numpy's FFT is; gather.py:396-420 float32 score ranked by
     # argpartition; gather.py:32-100 float64 gather with (template, lag) jobs over the threads as the
-    # reference's prange): filter + selection of a template sample, gather rate on 4 nodes in RAM
+    # reference's prange): filter + selection of a template sample, gather rate on the most used nodes in RAM
     samp = np.linspace(0, nm - 1, args.cpu_templates).astype(int)
     t0 = time.perf_counter()
     trs = O.filter_ref(bank.W[samp], bank.lo, nt, d, s2)
@@ -178,7 +182,7 @@ def cpu_leg(args, seg, bank, store, res, an_geometry, seed_row):
     tms = O.select_times_ref(trs, store.tau_idx, K, S, 0)
     t_select = time.perf_counter() - t0
     counts = valid.sum(axis=1)
-    nodes = list(np.argsort(-counts)[:4])
+    nodes = list(np.argsort(-counts)[:max(1, args.cpu_nodes)])
     host_tables = []
     for k in range(nn):
         host_tables.append(O.synth_table(0, k, store.table_shape(), store.tau_idx) if k in nodes else None)
@@ -195,8 +199,10 @@ def cpu_leg(args, seg, bank, store, res, an_geometry, seed_row):
     t_seg = (t_filter + t_select) * nm / samp.size + lookups_all / cpu_lookup_rate
     sample = ("the reference's CPU arithmetic (complex64 numpy filter, float32 score with argpartition, float64 "
               "OpenMP gather over (template, lag) jobs): filter+selection of %d of %d templates (scaled x%.1f) "
-              "and the gather of those templates on the 4 most-used nodes (%d reads x 87 lags x %d times, tables "
-              "in RAM), scaled to all %d reads" % (samp.size, nm, nm / samp.size, ids.size, K + S, n_jobs_all))
+              "and the gather of those templates on the %d most-used nodes (%d reads x 87 lags x %d times, tables "
+              "in RAM), scaled to all %d reads; %.1f s of CPU work timed" % (samp.size, nm, nm / samp.size, len(nodes),
+                                                                           ids.size, K + S, n_jobs_all,
+                                                                           t_filter + t_select + t_gather))
     # the same on one core: selection of a quarter of the sample, gather of it on one node
     O.set_threads(1)
     try:
@@ -692,8 +698,9 @@ def published_entry(name, seconds, n_seg, reads, n_lag, extra):
                 speedup_vs_reference=ref_s / s)
 
 
-def main_published(args):
-    """the reference's own timed workloads through the drop-in, one JSON line beside its published times:
+def published_workloads(st, which):
+    """the reference's own timed workloads through the drop-in on the resident store st, beside its published
+    times (PUBLISHED):
 
     speed   128 simulated 8 s segments (20-512 Hz, equal-mass injections M_det ~ U(20, 200), D_L log-uniform in
             1-15 Gpc), the 120 masses geomspace(20, 200) of its paper, pe_redshift on the 1600 x 1600 grid in
@@ -705,27 +712,20 @@ def main_published(args):
 
     The segments' noise is the curve of the reference's segment files; the tables are the synthetic 53 nodes
     of the same geometry (the work of a lookup does not depend on the table values)."""
-    from exmarg_amd import _hip, synth
-    from exmarg_amd.search import mass_bank, pe_intrinsic, pe_redshift, store as stm
-    _hip.init(0)
-    info = _hip.device_info()
-    st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
+    from exmarg_amd import synth
+    from exmarg_amd.search import mass_bank, pe_intrinsic, pe_redshift
     noise = reference_noise()
     n_lag = st.tau_idx.size
-    which = [w.strip() for w in args.published.split(",") if w.strip()]
-    out = {"metric": "seconds per segment of the reference's own timed workloads (its notebooks), this drop-in "
-                     "on one MI355X beside the reference's published times",
-           "unit": "s/segment", "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
-           "device": info["name"], "num_cu": info["num_cu"], "workloads": {}}
+    out = {}
     fast = dict(block=64, resident=True)
     if "speed" in which:
         draw = lambda rng: (rng.uniform(20.0, 200.0), 1.0, 0.0)                       # noqa: E731
         draw_y = lambda rng: synth.D0 / np.exp(rng.uniform(np.log(1000.0), np.log(15000.0)))   # noqa: E731
         seg, _, snr = synth.simulate(128, 20261020, draw, draw_y, 8.0, 20.0, 512.0, psd=noise)
         bank = mass_bank(np.geomspace(20.0, 200.0, 120), seg, "hip")
-        pe_redshift(seg, bank, st, rows=[0], **fast)
         order = np.argsort(snr)
         batches = [np.sort(order[i:i + 16]) for i in range(0, order.size, 16)]
+        pe_redshift(seg, bank, st, rows=batches[0], chunk=16, **fast)      # warm-up: kernels and buffers
         per = []
         t0 = time.perf_counter()
         for b in batches:
@@ -733,7 +733,7 @@ def main_published(args):
             pe_redshift(seg, bank, st, rows=b, chunk=16, **fast)
             per.append((time.perf_counter() - t1) / b.size)
         wall = time.perf_counter() - t0
-        out["workloads"]["speed"] = published_entry(
+        out["speed"] = published_entry(
             "speed", wall, seg["d_H"].shape[0], reads_of(bank, st), n_lag,
             {"workload": "128 simulated 8 s segments at 20-512 Hz, SNR %.1f-%.1f; 120-mass equal-mass bank "
                          "geomspace(20, 200); pe_redshift 1600 x 1600, batches of 16" % (snr.min(), snr.max()),
@@ -744,11 +744,11 @@ def main_published(args):
             two = {k: z[k] for k in z.files}
         seg = dict(two, d_H=np.tile(two["d_H"], (6, 1)), d_L=np.tile(two["d_L"], (6, 1)))
         bank = mass_bank(os.path.join(ROOT, "exmarg_amd", "data", "bank_2p.npy"), seg, "hip")
-        pe_redshift(seg, bank, st, rows=[0], **fast)
+        pe_redshift(seg, bank, st, rows=[0, 1], chunk=12, **fast)           # warm-up: kernels and buffers
         t0 = time.perf_counter()
         res = pe_redshift(seg, bank, st, chunk=12, **fast)
         wall = time.perf_counter() - t0
-        out["workloads"]["twelve"] = published_entry(
+        out["twelve"] = published_entry(
             "twelve", wall, 12, reads_of(bank, st), n_lag,
             {"workload": "12 segments of 8 s at 20-512 Hz (the reference's two_parameters rows 0 and 1, six times "
                          "each); its 3517-mass bank; pe_redshift 1600 x 1600 in one call",
@@ -766,18 +766,33 @@ def main_published(args):
         banks = [mass_bank(rows[p], seg, "hip") for p in pieces]
         t_bank = time.perf_counter() - t0
         kw = dict(prior="volume", resident=True)
-        pe_intrinsic(seg, banks[0], st, rows=[0], **kw)
+        pe_intrinsic(seg, banks[0], st, rows=[0, 1], chunk=2, **kw)         # warm-up: kernels and buffers
         lnl = np.empty((2, rows.shape[0]))
         t0 = time.perf_counter()
         for p, b in zip(pieces, banks):
             lnl[:, p] = pe_intrinsic(seg, b, st, rows=[0, 1], chunk=2, **kw)["ln_lambda"]
         wall = time.perf_counter() - t0
-        out["workloads"]["three"] = published_entry(
+        out["three"] = published_entry(
             "three", wall, 2, sum(reads_of(b, st) for b in banks), n_lag,
             {"workload": "2 simulated 4 s segments at 20-512 Hz (SNR %.1f, %.1f); 78,000-template (M_det, q, chi) "
                          "bank in 10 pieces; pe_intrinsic 'volume'" % tuple(snr),
              "bank_build_s": t_bank})
         print("three: %.3f s per segment" % (wall / 2), file=sys.stderr, flush=True)
+    return out
+
+
+def main_published(args):
+    """the reference's own timed workloads (published_workloads): one JSON line"""
+    from exmarg_amd import _hip
+    from exmarg_amd.search import store as stm
+    _hip.init(0)
+    info = _hip.device_info()
+    st = stm.synthetic_store(n_nodes=args.nodes, seed=0)
+    which = [w.strip() for w in args.published.split(",") if w.strip()]
+    out = {"metric": "seconds per segment of the reference's own timed workloads (its notebooks), this drop-in "
+                     "on one MI355X beside the reference's published times",
+           "unit": "s/segment", "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+           "device": info["name"], "num_cu": info["num_cu"], "workloads": published_workloads(st, which)}
     print(json.dumps(out), flush=True)
 
 
@@ -911,6 +926,11 @@ def main():
         sv = serve_run(st, args.serve_requests, args.rate, args.seed, args.n_loud, args.n_random)
         sv["workload"] = SERVE_WORKLOAD % args.n_loud
         out["serving"] = sv
+    which = [w.strip() for w in args.published.split(",") if w.strip()]
+    if which and args.config == 2 and args.nodes == 53 and args.precision == "f64" \
+            and args.filter_precision == "f64":
+        # the reference's own timed workloads on the same resident store, beside its published times
+        out["reference_workloads"] = published_workloads(st, which)
     if not args.no_cpu and args.n_loud > 0 and args.config == 2:
         an_geometry = geometry
         cpu, err, nchk = cpu_leg(args, seg, bank, st, res, an_geometry, 0)
