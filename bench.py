@@ -862,6 +862,27 @@ def main():
         f64_check = {"max_abs_dln_lambda": float(np.nanmax(np.abs(res["ln_lambda"] - ref64["ln_lambda"]))),
                      "dln_b": float(abs(res["ln_b"][0] - ref64["ln_b"][0]))}
 
+    precision_form = None
+    if args.precision == "f64" and args.filter_precision == "f64" and args.config == 2:
+        # the precision form on the same segment, timed the same way after one warm-up call: float32 transform
+        # and interpolation, float64 weighting, index arithmetic, log-sum-exp and prior quadrature
+        _hip.call("exm_set_gather_precision", 1)
+        _hip.call("exm_set_filter_precision", 1)
+        try:
+            pe_intrinsic(seg, bank, st, **kw)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                r32 = pe_intrinsic(seg, bank, st, **kw)
+            w32 = time.perf_counter() - t0
+        finally:
+            _hip.call("exm_set_gather_precision", 0)
+            _hip.call("exm_set_filter_precision", 0)
+        precision_form = {
+            "value": args.steps / w32, "unit": "segments/s", "ms_per_step": 1e3 * w32 / args.steps,
+            "dtype": "f32 transform and interpolation; f64 weighting, index arithmetic, log-sum-exp and prior quadrature",
+            "max_abs_dln_lambda_vs_f64": float(np.nanmax(np.abs(r32["ln_lambda"] - res["ln_lambda"]))),
+            "dln_b_vs_f64": float(abs(r32["ln_b"][0] - res["ln_b"][0]))}
+
     geometry = _node_geometry(bank, st, 0.1267)
     valid, bnode = geometry[0], geometry[3]
     n_nodes_used = int(valid.any(axis=1).sum())
@@ -921,6 +942,8 @@ def main():
                                                  "16s_2000tmpl_53nodes.txt)"}
     if f64_check is not None:
         out["float32_form_vs_float64"] = f64_check
+    if precision_form is not None:
+        out["precision_form"] = precision_form
     if args.serve_requests > 0 and args.config == 2 and args.nodes == 53:
         # the serving form on the same resident store (config 4's request mix), reported beside the headline
         sv = serve_run(st, args.serve_requests, args.rate, args.seed, args.n_loud, args.n_random)
