@@ -364,8 +364,7 @@ class SourceGrid:
     (exm_source_grid_run); the same arithmetic as source_frame"""
 
     def __init__(self, yN, Mdet, yq, m1, z, w=None):
-        self._keep = [np.ascontiguousarray(a, np.float64) for a in (yN, Mdet, yq, m1, z)]
-        yN_, Md, yq_, m1_, z_ = self._keep
+        yN_, Md, yq_, m1_, z_ = [np.ascontiguousarray(a, np.float64) for a in (yN, Mdet, yq, m1, z)]
         if yN_.ndim != 2 or Md.shape != (yN_.shape[1],) or yq_.shape != z_.shape:
             raise ValueError("yN must be (nrow, nm) with one mass per template, yq one value per z")
         wv = None if w is None else np.ascontiguousarray(w, np.float64)
