@@ -227,20 +227,24 @@ class _Analysis:
 # Reads per table that one gather launch should reach before segments share it: below it a launch's
 # workgroups have few lookups per staged table row, and staging the rows, not the lookups, sets its time.
 READS_PER_TABLE = 4096
-# template-segments one shared launch may hold (2.14 MB of walk records each at 87 lags x 2048 times)
-SHARED_TEMPLATES = 8192
+# device bytes the batch of one shared launch may hold (walk records and launch work: 2.2 MB per
+# template-segment at 87 lags x 2048 walked times, so 8,000 template-segments)
+SHARED_BYTES = 16 << 30
 
 
-def _segments_per_launch(an, n_seg, chunk, n_loud):
+def _segments_per_launch(an, n_seg, chunk, n_loud, n_random):
     """how many segments share one gather launch: 1 unless the bank's reads per table are few, then up to
     chunk (the reference's segments per launch, src/exmarg/search/pe.py:173-194) and as many as bring the
-    reads per table to READS_PER_TABLE, within SHARED_TEMPLATES template-segments"""
+    reads per table to READS_PER_TABLE, within SHARED_BYTES of batch memory"""
     if n_loud <= 0 or n_seg < 2 or int(chunk) < 2:
         return 1
+    from .batch import template_bytes
     n_tab = max(int(np.asarray(an.store.s).size), 1)
     per_tab = max(an.plan.job_m.size / n_tab, 1.0)
     want = int(np.ceil(READS_PER_TABLE / per_tab))
-    return max(1, min(int(chunk), n_seg, want, SHARED_TEMPLATES // max(an.bank.u.size, 1)))
+    per = template_bytes(int(np.asarray(an.store.tau_idx).size), int(n_loud) + int(n_random), n_tab + 1,
+                         an.store.table_shape()[2]) * max(an.bank.u.size, 1)
+    return max(1, min(int(chunk), n_seg, want, SHARED_BYTES // per))
 
 
 def _segment_results(an, seg, chunk, n_loud, n_random, seed, lse):
@@ -251,7 +255,7 @@ def _segment_results(an, seg, chunk, n_loud, n_random, seed, lse):
     free, each segment has a launch of its own.  (Two slots, one gathering while the other's results are
     used, measured no faster: the gather holds every CU, so the source-frame kernels wait for it.)"""
     n = seg["rows"].size
-    g = _segments_per_launch(an, n, chunk, n_loud)
+    g = _segments_per_launch(an, n, chunk, n_loud, n_random)
     batch = None
     if g > 1:
         from .batch import SegmentBatch
