@@ -79,6 +79,7 @@ SIGNATURES = {
     "exm_source_frame": [P, P, I32, I32, P, P, P, I32, P, I32, P, P, P],
     "exm_source_grid_create": [P, I32, I32, P, P, P, I32, P, I32, P, HP],
     "exm_source_grid_run": [H, P, P, P],
+    "exm_source_grid_run_many": [H, I32, P, P, P],
     "exm_strain_segments": [P, I32, I64, F64, I32, I32, P],
     "exm_strain_psd": [P, I32, I64, F64, I32, I32, F64, P],
 }

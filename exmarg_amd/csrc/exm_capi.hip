@@ -194,6 +194,8 @@ Plan::~Plan() {
     if (d_w) dev_free(d_w, w_bytes);
 }
 SourceGrid::~SourceGrid() {
+    for (hipEvent_t e : done)
+        if (e) (void)hipEventDestroy(e);
     if (d) dev_free(d, bytes);
 }
 
@@ -1444,6 +1446,12 @@ int exm_source_grid_create(const double* yN, int nrow, int nm, const double* Mde
     g->m1 = (double*)(b + o_m1), g->z = (double*)(b + o_z), g->w = w ? (double*)(b + o_w) : nullptr;
     g->surf = (double*)(b + o_surf), g->out = (double*)(b + o_out), g->ln_b = (double*)(b + o_lnb);
     g->part = (double*)(b + o_part);
+    for (hipEvent_t& e : g->done)
+        if (rc == EXM_OK && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            e = nullptr;
+            set_error("hipEventCreate failed");
+            rc = EXM_ERR_HIP;
+        }
     const hipStream_t s = g_ctx.stream;
     const struct { void* dst; const void* src; size_t n; } up[] = {
         {g->yN, yN, 8 * n_node}, {g->M, Mdet, 8 * (size_t)nm}, {g->yq, yq, 8 * (size_t)nz},
@@ -1465,29 +1473,70 @@ int exm_source_grid_create(const double* yN, int nrow, int nm, const double* Mde
     return EXM_OK;
 }
 
-int exm_source_grid_run(exm_handle hgrid, const double* lamN, double* out, double* ln_b) {
+int exm_source_grid_run_many(exm_handle hgrid, int n, const double* lamN, double* out, double* ln_b) {
     LOCK;
     EXM_TRY(ensure_init());
     NEED(SourceGrid, g, hgrid, K_GRID);
-    EXM_CHECK(lamN != nullptr, EXM_ERR_ARG, "NULL argument");
-    EXM_CHECK(!g->has_w || ln_b != nullptr, EXM_ERR_ARG, "ln_b is NULL");
+    EXM_CHECK(n >= 0 && (n == 0 || lamN != nullptr), EXM_ERR_ARG, "NULL argument");
+    EXM_CHECK(!g->has_w || n == 0 || ln_b != nullptr, EXM_ERR_ARG, "ln_b is NULL");
+    if (n == 0) return EXM_OK;
     const hipStream_t s = g_ctx.stream;
-    EXM_HIP(hipMemcpyAsync(g->lamN, lamN, 8 * (size_t)g->nrow * g->nm, hipMemcpyHostToDevice, s));
-    {
-        ProfileScope prof("source_frame");
-        EXM_TRY(launch_source_frame(g->lamN, g->yN, g->nrow, g->nm, g->M, g->yq, g->m1, g->nm1, g->z, g->nz, g->w,
-                                    g->surf, g->out, g->ln_b, g->part));
+    const size_t lam_n = (size_t)g->nrow * g->nm, grid_n = (size_t)g->nm1 * g->nz;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    // pinned staging, two slots: the grid, the node values, ln B
+    const size_t o_lam = 2 * al(8 * grid_n), o_b = o_lam + 2 * al(8 * lam_n);
+    char* pin = nullptr;
+    EXM_TRY(pinned_staging(o_b + 2 * 256, (void**)&pin));
+    auto slot_out = [&](int k) { return pin + (k & 1) * al(8 * grid_n); };
+    auto slot_lam = [&](int k) { return pin + o_lam + (k & 1) * al(8 * lam_n); };
+    auto slot_b = [&](int k) { return (double*)(pin + o_b + (k & 1) * 256); };
+    // segment k is staged, computed and copied back while the host copies segment k - 1's grid out of the
+    // other slot; a slot is reused only after its previous segment's event has been waited for
+    int rc = EXM_OK;
+    for (int k = 0; k <= n && rc == EXM_OK; ++k) {
+        if (k < n) {
+            std::memcpy(slot_lam(k), lamN + (size_t)k * lam_n, 8 * lam_n);
+            if (hipMemcpyAsync(g->lamN, slot_lam(k), 8 * lam_n, hipMemcpyHostToDevice, s) != hipSuccess) {
+                set_error("hipMemcpyAsync of the node values failed");
+                rc = EXM_ERR_HIP;
+            }
+            if (rc == EXM_OK) {
+                ProfileScope prof("source_frame");
+                rc = launch_source_frame(g->lamN, g->yN, g->nrow, g->nm, g->M, g->yq, g->m1, g->nm1, g->z, g->nz,
+                                         g->w, g->surf, g->out, g->ln_b, g->part);
+            }
+            if (rc == EXM_OK && out &&
+                hipMemcpyAsync(slot_out(k), g->out, 8 * grid_n, hipMemcpyDeviceToHost, s) != hipSuccess) {
+                set_error("hipMemcpyAsync of the grid failed");
+                rc = EXM_ERR_HIP;
+            }
+            if (rc == EXM_OK && g->has_w &&
+                hipMemcpyAsync(slot_b(k), g->ln_b, 8, hipMemcpyDeviceToHost, s) != hipSuccess) {
+                set_error("hipMemcpyAsync of ln B failed");
+                rc = EXM_ERR_HIP;
+            }
+            if (rc == EXM_OK && hipEventRecord(g->done[k & 1], s) != hipSuccess) {
+                set_error("hipEventRecord failed");
+                rc = EXM_ERR_HIP;
+            }
+        }
+        if (rc == EXM_OK && k >= 1) {
+            const int p = k - 1;
+            if (hipEventSynchronize(g->done[p & 1]) != hipSuccess) {
+                set_error("hipEventSynchronize failed");
+                rc = EXM_ERR_HIP;
+                break;
+            }
+            if (out) copy_host_parallel(out + (size_t)p * grid_n, slot_out(p), 8 * grid_n);
+            if (g->has_w) ln_b[p] = *slot_b(p);
+        }
     }
-    const size_t grid_bytes = 8 * (size_t)g->nm1 * g->nz;
-    void* pin = nullptr;
-    if (out) {
-        EXM_TRY(pinned_staging(grid_bytes, &pin));
-        EXM_HIP(hipMemcpyAsync(pin, g->out, grid_bytes, hipMemcpyDeviceToHost, s));
-    }
-    if (g->has_w) EXM_HIP(hipMemcpyAsync(ln_b, g->ln_b, 8, hipMemcpyDeviceToHost, s));
-    EXM_HIP(hipStreamSynchronize(s));
-    if (out) copy_host_parallel(out, pin, grid_bytes);
-    return EXM_OK;
+    if (rc != EXM_OK) (void)hipStreamSynchronize(s);   // no copy left in flight into the staging block
+    return rc;
+}
+
+int exm_source_grid_run(exm_handle hgrid, const double* lamN, double* out, double* ln_b) {
+    return exm_source_grid_run_many(hgrid, 1, lamN, out, ln_b);
 }
 
 int exm_copy_calibration(int64_t bytes, double* ms) {

@@ -186,6 +186,7 @@ struct SourceGrid : Object {
     size_t bytes = 0;
     double *lamN = nullptr, *yN = nullptr, *M = nullptr, *yq = nullptr, *m1 = nullptr, *z = nullptr,
            *w = nullptr, *surf = nullptr, *out = nullptr, *ln_b = nullptr, *part = nullptr;
+    hipEvent_t done[2] = {nullptr, nullptr};     // the copies of the two staging slots have landed
     ~SourceGrid();
 };
 

@@ -247,13 +247,14 @@ def _segments_per_launch(an, n_seg, chunk, n_loud, n_random):
     return max(1, min(int(chunk), n_seg, want, SHARED_BYTES // per))
 
 
-def _segment_results(an, seg, chunk, n_loud, n_random, seed, lse):
-    """(j, result) for every segment j of seg, in order: (ln Lambda, ln B) assembled on the device when lse,
-    else the node values lamN (n_node + 1, n_template).  Segments whose bank has few reads per table share
-    gather launches (a SegmentBatch of _segments_per_launch segments), so that the table rows one launch
-    stages serve the reads of all of them; otherwise, or when the device memory for a shared launch is not
-    free, each segment has a launch of its own.  (Two slots, one gathering while the other's results are
-    used, measured no faster: the gather holds every CU, so the source-frame kernels wait for it.)"""
+def _segment_groups(an, seg, chunk, n_loud, n_random, seed, lse):
+    """(js, results) for consecutive groups of the segments of seg, in order: per segment j of the range js,
+    (ln Lambda, ln B) assembled on the device when lse, else the node values lamN (n_node + 1, n_template).
+    Segments whose bank has few reads per table share gather launches (a SegmentBatch of
+    _segments_per_launch segments, one group), so that the table rows one launch stages serve the reads of
+    all of them; otherwise, or when the device memory for a shared launch is not free, each segment has a
+    launch (and a group) of its own.  (Two slots, one gathering while the other's results are used,
+    measured no faster: the gather holds every CU, so the source-frame kernels wait for it.)"""
     n = seg["rows"].size
     g = _segments_per_launch(an, n, chunk, n_loud, n_random)
     batch = None
@@ -265,8 +266,8 @@ def _segment_results(an, seg, chunk, n_loud, n_random, seed, lse):
             batch = None
     if batch is None:
         for j in range(n):
-            yield j, (an.ln_lambda(seg, j, n_loud, n_random, seed) if lse
-                      else an.node_values(seg, j, n_loud, n_random, seed))
+            yield range(j, j + 1), [an.ln_lambda(seg, j, n_loud, n_random, seed) if lse
+                                    else an.node_values(seg, j, n_loud, n_random, seed)]
         return
     try:
         sig2 = {I: seg["sig2_" + I] for I in "HL"}
@@ -277,8 +278,7 @@ def _segment_results(an, seg, chunk, n_loud, n_random, seed, lse):
             batch.launch()
             res = batch.collect_lse() if lse else batch.collect()
             batch.reset()
-            for j, r in zip(js, res):
-                yield j, r
+            yield js, res
     finally:
         batch.free()
 
@@ -297,7 +297,7 @@ def pe_intrinsic(segments, bank, store, rows=None, prior="volume", ymin=0.1267, 
     resident          keep the tables on the device after the call
     n_loud, n_random  arrival times visited per template (select_times); n_loud = 0 visits all of them
     seed              seed of the random draw (plus the segment index)
-    chunk             segments per gather launch at most (see _segment_results); nodes_per_launch and
+    chunk             segments per gather launch at most (see _segment_groups); nodes_per_launch and
                       block are accepted for the reference signature: every node of a segment is read in
                       one launch.
 
@@ -309,8 +309,9 @@ def pe_intrinsic(segments, bank, store, rows=None, prior="volume", ymin=0.1267, 
     try:
         out = np.full((seg["rows"].size, bank.u.size), -np.inf)
         ln_b = np.full(seg["rows"].size, -np.inf)
-        for j, (L, lnb) in _segment_results(an, seg, chunk, n_loud, n_random, seed, lse=True):
-            out[j], ln_b[j] = L, lnb
+        for js, res in _segment_groups(an, seg, chunk, n_loud, n_random, seed, lse=True):
+            for j, (L, lnb) in zip(js, res):
+                out[j], ln_b[j] = L, lnb
     finally:
         an.close()
     return dict(M=bank.M, q=bank.q, chi=bank.chi, u=bank.u, ln_lambda=out, ln_b=ln_b, rows=seg["rows"])
@@ -325,7 +326,7 @@ def pe_redshift(segments, bank, store, rows=None, chunk=8, zmax=8.0, nz=1600, m1
     spin) on the same segments.  ln B averages Lambda over the prior uniform in M_det = 2 m1 (1+z) on
     mdet_range and proportional to D_L^2 in luminosity distance, p(m1, z) ~ (1+z) D_L^2 dD_L/dz.
 
-    chunk is the number of segments one gather launch reads at most (see _segment_results).
+    chunk is the number of segments one gather launch reads at most (see _segment_groups).
 
     returns dict: m1 (nm1,), z (nz,), ln_lambda (n_segment, nm1, nz), ln_b (n_segment,), rows."""
     if np.any(np.diff(bank.M) <= 0) or np.any(bank.q != 1.0) or np.any(bank.chi != 0.0):
@@ -340,8 +341,8 @@ def pe_redshift(segments, bank, store, rows=None, chunk=8, zmax=8.0, nz=1600, m1
     try:
         grid = SourceGrid(an.geometry[2], bank.M, yq, m1, z, w)
         try:
-            for j, lamN in _segment_results(an, seg, chunk, n_loud, n_random, seed, lse=False):
-                ln_b[j] = grid.run(lamN, out=out[j])
+            for js, lamNs in _segment_groups(an, seg, chunk, n_loud, n_random, seed, lse=False):
+                ln_b[js.start:js.stop] = grid.run_many(np.stack(lamNs), out=out[js.start:js.stop])
         finally:
             grid.free()
     finally:
@@ -392,6 +393,21 @@ class SourceGrid:
         _hip.call("exm_source_grid_run", int(self.handle), _hip.ptr(lamN), _hip.ptr(out) if out is not None else None,
                   _hip.ptr(ln_b))
         return float(ln_b[0]) if self.has_w else None
+
+    def run_many(self, lamNs, out=None):
+        """run over n segments in one call: lamNs (n, nrow, nm); out (n, nm1, nz) when given; returns ln B (n,)
+        when the grid has w.  Each grid's copy to the host overlaps the next segment's work on the device."""
+        lamNs = np.ascontiguousarray(lamNs, np.float64)
+        if lamNs.ndim != 3 or lamNs.shape[1:] != self.nodes:
+            raise ValueError("lamNs has shape %s, expected (n,) + %s" % (lamNs.shape, self.nodes))
+        n = lamNs.shape[0]
+        if out is not None and (out.shape != (n,) + self.shape or out.dtype != np.float64
+                                or not out.flags.c_contiguous):
+            raise ValueError("out must be a C-contiguous float64 (%d, nm1, nz) array" % n)
+        ln_b = np.zeros(max(n, 1))
+        _hip.call("exm_source_grid_run_many", int(self.handle), n, _hip.ptr(lamNs),
+                  _hip.ptr(out) if out is not None else None, _hip.ptr(ln_b))
+        return ln_b[:n] if self.has_w else None
 
     def free(self):
         self.handle.free()

@@ -306,6 +306,9 @@ int exm_source_frame(const double* lamN, const double* yN, int nrow, int nm, con
 int exm_source_grid_create(const double* yN, int nrow, int nm, const double* Mdet, const double* yq, const double* m1,
                            int nm1, const double* z, int nz, const double* w, exm_handle* out);
 int exm_source_grid_run(exm_handle grid, const double* lamN, double* out, double* ln_b);
+/* exm_source_grid_run over n segments in one call: lamN (n, nrow, nm), out (n, nm1, nz) or NULL, ln_b (n).
+ * Each grid's copy to the host overlaps the next segment's source-frame step on the device. */
+int exm_source_grid_run_many(exm_handle grid, int n, const double* lamN, double* out, double* ln_b);
 
 /* ---------------------------------------------------------------- time-domain front end
  * The frequency-domain segment of each of n_rows time series x (n_rows, n) float64 sampled at fs:

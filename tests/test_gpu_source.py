@@ -96,8 +96,20 @@ def test_source_grid_equals_source_frame(hip, oracle):
         assert lb_a == lb_b
         ref = oracle.to_source_frame(L, yN, M, yq, m1, z)
         assert abs(lb_a - oracle.log_mean(ref, w)) < 1e-12
+    # run_many over five segments (the staging slots reused twice over) equals run one at a time, bit for bit
+    Ls = np.stack([np.where(np.isfinite(lamN), lamN + rng.standard_normal(lamN.shape), -np.inf) for _ in range(5)])
+    many = np.empty((5, m1.size, z.size))
+    lb_many = grid.run_many(Ls, out=many)
+    for k in range(5):
+        one = np.empty((m1.size, z.size))
+        assert grid.run(Ls[k], out=one) == lb_many[k]
+        np.testing.assert_array_equal(many[k], one)
+    np.testing.assert_array_equal(grid.run_many(Ls[:2]), lb_many[:2])
+    assert grid.run_many(Ls[:0]).size == 0
     with pytest.raises(ValueError, match="shape"):
         grid.run(lamN[:-1])
+    with pytest.raises(ValueError, match="C-contiguous"):
+        grid.run_many(Ls, out=many[:4])
     grid.free()
     nw = pe.SourceGrid(yN, M, yq, m1, z)
     assert nw.run(lamN, out=None) is None
