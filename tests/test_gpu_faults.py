@@ -36,6 +36,9 @@ def test_refused_allocations_in_pe_intrinsic(hip, small):
     from exmarg_amd.search import pe_intrinsic
     seg, bank, st = small
     kw = dict(n_loud=64, n_random=64, seed=2, resident=True)
+    # the two segments share a gather launch; a refused batch allocation falls back to one launch per
+    # segment, whose work memory the library then keeps: run that path once before taking the baseline
+    pe_intrinsic(seg, bank, st, chunk=1, **kw)
     ref = pe_intrinsic(seg, bank, st, **kw)
     base = _in_use(hip)
     failed = 0
@@ -52,6 +55,36 @@ def test_refused_allocations_in_pe_intrinsic(hip, small):
         again = pe_intrinsic(seg, bank, st, **kw)
         np.testing.assert_allclose(again["ln_lambda"], ref["ln_lambda"], rtol=REPEAT_RTOL, atol=0)
     say("pe_intrinsic: %d of 24 injection points hit an allocation" % failed)
+    assert failed > 0
+
+
+def test_refused_allocations_in_pe_redshift(hip, small):
+    """pe_redshift of two segments sharing a gather launch, with the source-frame grid kept on the device"""
+    from exmarg_amd.search import mass_bank, pe_redshift
+    seg, _, st = small
+    rbank = mass_bank(np.geomspace(20.0, 60.0, 6), seg, "hip")
+    kw = dict(n_loud=64, n_random=64, seed=2, resident=True, zmax=3.0, nz=60, m1_range=(5.0, 40.0), nm1=50,
+              mdet_range=(20.0, 60.0))
+    pe_redshift(seg, rbank, st, chunk=1, **kw)          # the fallback path's kept work memory, as above
+    ref = pe_redshift(seg, rbank, st, **kw)
+    base = _in_use(hip)
+    failed = 0
+    for k in range(24):
+        hip.call("exm_debug_fail_alloc", k)
+        try:
+            pe_redshift(seg, rbank, st, **kw)
+        except MemoryError as e:
+            assert "injected" in str(e)
+            failed += 1
+        finally:
+            hip.call("exm_debug_fail_alloc", -1)
+        assert _in_use(hip) == base, "device memory held after a refused allocation (k = %d)" % k
+    again = pe_redshift(seg, rbank, st, **kw)
+    fin = np.isfinite(ref["ln_lambda"])
+    np.testing.assert_array_equal(np.isfinite(again["ln_lambda"]), fin)
+    np.testing.assert_allclose(again["ln_lambda"][fin], ref["ln_lambda"][fin], rtol=REPEAT_RTOL, atol=1e-12)
+    np.testing.assert_allclose(again["ln_b"], ref["ln_b"], rtol=REPEAT_RTOL, atol=1e-12)
+    say("pe_redshift: %d of 24 injection points hit an allocation" % failed)
     assert failed > 0
 
 
