@@ -277,7 +277,7 @@ int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const doub
         st[i] = tab[js->order[i]];
         so[i] = offb[js->order[i]];
     }
-    // tiles: runs of one table split evenly into pieces of at most TILE_JOBS jobs.  Piece q of every
+    // tiles: runs of one table split evenly into pieces of at most SCHED_TILE_JOBS jobs.  Piece q of every
     // table holds about the same templates, so the tiles of one piece read the same walk records:
     // they are queued for one XCD (workgroups go to the 8 XCDs round-robin by workgroup index) and
     // the queues interleaved, so that the workgroups of an XCD share their records in its L2.
@@ -288,7 +288,7 @@ int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const doub
         int64_t b = a;
         while (b < njobs && st[b] == st[a]) ++b;
         const int64_t n = b - a;
-        const int64_t tile = g_ctx.tile ? g_ctx.tile : TILE_JOBS;
+        const int64_t tile = g_ctx.tile ? g_ctx.tile : SCHED_TILE_JOBS;
         const int64_t pieces = (n + tile - 1) / tile;
         run_a.push_back(a);
         run_n.push_back(n);
@@ -324,9 +324,20 @@ int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const doub
             }
     }
     js->ntiles = (int)(tiles.size() / 2);
+    // the same tiles in consecutive parts of at most TILE_JOBS jobs, for the wave and piece forms
+    std::vector<int32_t> small;
+    for (size_t t = 0; t < tiles.size(); t += 2) {
+        const int64_t lo = tiles[t], n = tiles[t + 1], parts = (n + TILE_JOBS - 1) / TILE_JOBS;
+        for (int64_t q = 0; q < parts; ++q) {
+            small.push_back((int32_t)(lo + n * q / parts));
+            small.push_back((int32_t)(n * (q + 1) / parts - n * q / parts));
+        }
+    }
+    js->ntiles_small = (int)(small.size() / 2);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t o_m = 0, o_t = al(4 * (size_t)njobs), o_o = o_t + al(4 * (size_t)njobs),
-                 o_tiles = o_o + al(8 * (size_t)njobs), total = o_tiles + al(4 * tiles.size());
+                 o_tiles = o_o + al(8 * (size_t)njobs), o_small = o_tiles + al(4 * tiles.size()),
+                 total = o_small + al(4 * small.size());
     EXM_TRY(dev_alloc(&js->d_block, total));
     js->bytes = total;
     char* base = (char*)js->d_block;
@@ -334,10 +345,12 @@ int jobset_build(int64_t njobs, const int32_t* m, const int32_t* tab, const doub
     js->d_tab = (int32_t*)(base + o_t);
     js->d_offb = (double*)(base + o_o);
     js->d_tiles = (int32_t*)(base + o_tiles);
+    js->d_tiles_small = (int32_t*)(base + o_small);
     EXM_HIP(hipMemcpyAsync(js->d_m, sm.data(), 4 * (size_t)njobs, hipMemcpyHostToDevice, cur_stream()));
     EXM_HIP(hipMemcpyAsync(js->d_tab, st.data(), 4 * (size_t)njobs, hipMemcpyHostToDevice, cur_stream()));
     EXM_HIP(hipMemcpyAsync(js->d_offb, so.data(), 8 * (size_t)njobs, hipMemcpyHostToDevice, cur_stream()));
     EXM_HIP(hipMemcpyAsync(js->d_tiles, tiles.data(), 4 * tiles.size(), hipMemcpyHostToDevice, cur_stream()));
+    EXM_HIP(hipMemcpyAsync(js->d_tiles_small, small.data(), 4 * small.size(), hipMemcpyHostToDevice, cur_stream()));
     EXM_HIP(hipStreamSynchronize(cur_stream()));
     return EXM_OK;
 }
@@ -1298,8 +1311,8 @@ int exm_set_gather_piece(int n) {
 
 int exm_set_gather_tile(int n) {
     LOCK;
-    EXM_CHECK(n == 0 || n == 256 || n == 512 || n == TILE_JOBS, EXM_ERR_ARG,
-              "reads per workgroup must be 0 (default), 256, 512 or %d", TILE_JOBS);
+    EXM_CHECK(n == 0 || n == 256 || n == 512 || n == TILE_JOBS || n == SCHED_TILE_JOBS, EXM_ERR_ARG,
+              "reads per workgroup must be 0 (default), 256, 512, %d or %d", TILE_JOBS, SCHED_TILE_JOBS);
     g_ctx.tile = n;
     return EXM_OK;
 }

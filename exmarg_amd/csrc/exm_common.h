@@ -116,15 +116,19 @@ struct Times : Object {
 };
 
 // A list of (template, table, ln a) reads grouped by table: jobs sorted by table, tiles of at most
-// TILE_JOBS jobs of one table, and for each sorted job its index in the caller's order.
+// SCHED_TILE_JOBS jobs of one table (the scheduled form of the band kernel), the same tiles split into
+// parts of at most TILE_JOBS jobs (the wave and piece forms), and for each sorted job its index in the
+// caller's order.
 constexpr int TILE_JOBS = 1024;
+constexpr int SCHED_TILE_JOBS = 2048;
 struct JobSet {
     int64_t njobs = 0;
-    int ntiles = 0;
+    int ntiles = 0, ntiles_small = 0;
     int32_t* d_m = nullptr;       // (njobs) template row, table-sorted
     int32_t* d_tab = nullptr;     // (njobs) table index
     double* d_offb = nullptr;     // (njobs) (ln a - beta0) / dbeta
     int32_t* d_tiles = nullptr;   // (ntiles, 2) first job, job count
+    int32_t* d_tiles_small = nullptr;   // (ntiles_small, 2) the tiles in parts of at most TILE_JOBS jobs
     std::vector<int64_t> order;   // sorted position -> caller index
     void* d_block = nullptr;
     size_t bytes = 0;
@@ -241,7 +245,7 @@ struct Context {
     std::unordered_map<uint64_t, hipfftHandle> fft_plans;  // key: (n_time << 20) | batch
     int piece = 0;                                          // walked times per piece (0: default)
     int f32_interp = 0;                                     // band kernel interpolates in float32
-    int tile = 0;                                           // reads per band workgroup (0: TILE_JOBS)
+    int tile = 0;                                           // reads per band workgroup (0: SCHED_TILE_JOBS)
     int64_t filter_bytes = 0;                               // matched-filter workspace per transform (0: 64 MiB)
     int filter_f32 = 0;                                     // matched-filter transform in float32
     int filter_transform = 0;                               // 0: band transform where it fits, 1: hipFFT

@@ -929,11 +929,14 @@ __global__ __launch_bounds__(WAVE_THREADS) void k_gather_wave(
 // the read's values span more than ~700 (an overflow the final step detects, flagging the (read, lag)
 // for k_gather_edge to recompute exactly).
 //
-// Piece descriptor, 32 bits: the read's index in its tile (bits 0-9), the slots used minus one (bits
-// 10-12) and the walk position of its first slot (bits 13-31).
+// Piece descriptor, 32 bits: the read's index in its tile (bits 0-10), the slots used minus one (bits
+// 11-13) and the walk position of its first slot (bits 14-31).
 constexpr int SCHED_THREADS = 1024;
-constexpr int SD_USE_SHIFT = 10, SD_POS_SHIFT = 13;
-constexpr uint32_t SD_READ_MASK = 0x3FFu;
+constexpr int SD_USE_SHIFT = 11, SD_POS_SHIFT = 14;
+constexpr uint32_t SD_READ_MASK = 0x7FFu;
+// reads of a tile per schedule thread: thread t takes reads 2t and 2t + 1, so the read order is kept
+constexpr int SCHED_READS_PER_THREAD = SCHED_TILE_JOBS / SCHED_THREADS;
+static_assert(SCHED_READS_PER_THREAD == 2, "the schedule kernels take two reads per thread");
 
 // pieces a read may hold at most: each row's run ends in at most one partial piece
 __host__ __device__ inline int64_t sched_cap_per_read(int nw, int pn, int nb) { return (nw + pn - 1) / pn + nb; }
@@ -975,10 +978,13 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_sched_count(const int32_t* __
     const int t = blockIdx.x, b = blockIdx.y;
     const int j0 = tiles[2 * t], nj = tiles[2 * t + 1];
     int q = 0;
-    if ((int)threadIdx.x < nj) {
-        int lo, hi;
-        row_run(first_last, bounds, (int64_t)j0 + threadIdx.x, nb, b, lo, hi);
-        q = (hi - lo + PN - 1) / PN;
+    for (int u = 0; u < SCHED_READS_PER_THREAD; ++u) {
+        const int jl = SCHED_READS_PER_THREAD * (int)threadIdx.x + u;
+        if (jl < nj) {
+            int lo, hi;
+            row_run(first_last, bounds, (int64_t)j0 + jl, nb, b, lo, hi);
+            q += (hi - lo + PN - 1) / PN;
+        }
     }
     const int total = block_sum<SCHED_THREADS / 64>(q, red);
     if (threadIdx.x == 0) cnt[(int64_t)t * (nb + 1) + b] = total;
@@ -1001,9 +1007,14 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_sched_emit(const int32_t* __r
     const int32_t* ct = cnt + (int64_t)t * (nb + 1);
     // where row b starts: the pieces of the rows before it (nb < 1024 rows, one per thread)
     const int start = block_sum<SCHED_THREADS / 64>(tid < b ? ct[tid] : 0, red);
-    int lo = 0, hi = 0;
-    if (tid < nj) row_run(first_last, bounds, (int64_t)j0 + tid, nb, b, lo, hi);
-    const int q = (hi - lo + PN - 1) / PN;
+    // this thread's reads 2 tid and 2 tid + 1 (in read order) and their pieces in row b
+    int lo[SCHED_READS_PER_THREAD], hi[SCHED_READS_PER_THREAD], q = 0;
+    for (int u = 0; u < SCHED_READS_PER_THREAD; ++u) {
+        const int jl = SCHED_READS_PER_THREAD * tid + u;
+        lo[u] = hi[u] = 0;
+        if (jl < nj) row_run(first_last, bounds, (int64_t)j0 + jl, nb, b, lo[u], hi[u]);
+        q += (hi[u] - lo[u] + PN - 1) / PN;
+    }
     // exclusive sum of q over the workgroup: within the wave by shuffles, then over the waves
     int x = q;
 #pragma unroll
@@ -1017,10 +1028,12 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_sched_emit(const int32_t* __r
     for (int w = 0; w < wid; ++w) before += wex[w];
     const int ex = before + x - q;
     uint32_t* out = desc + (int64_t)j0 * cap + start + ex;
-    for (int i = 0; i < q; ++i) {
-        const int pa = lo + i * PN;
-        const int use = min(PN, hi - pa);
-        out[i] = (uint32_t)tid | ((uint32_t)(use - 1) << SD_USE_SHIFT) | ((uint32_t)pa << SD_POS_SHIFT);
+    for (int u = 0; u < SCHED_READS_PER_THREAD; ++u) {
+        const uint32_t jl = (uint32_t)(SCHED_READS_PER_THREAD * tid + u);
+        for (int pa = lo[u]; pa < hi[u]; pa += PN) {
+            const int use = min(PN, hi[u] - pa);
+            *out++ = jl | ((uint32_t)(use - 1) << SD_USE_SHIFT) | ((uint32_t)pa << SD_POS_SHIFT);
+        }
     }
     if (tid == 0) {
         rowoff[(int64_t)t * (nb + 1) + b] = start;
@@ -1029,10 +1042,10 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_sched_emit(const int32_t* __r
 }
 
 // bytes of LDS the scheduled kernel uses for a row of rowlen floats: the ring, then per read of the tile
-// its ln-a offset and sum (float64), template and drawn-time weight (int, float) and reference (float),
-// then the chunk counters and the band range
+// its sum (float64) and reference (float), then the chunk counters and the band range.  A read's
+// template, ln-a offset and drawn-time weight are read from memory where a piece needs them.
 __host__ __device__ constexpr size_t sched_lds_bytes(int rowlen) {
-    return piece_res_offset(rowlen) + (size_t)TILE_JOBS * (8 + 8 + 8 + 4) + 8 * sizeof(int);
+    return piece_res_offset(rowlen) + (size_t)SCHED_TILE_JOBS * (8 + 4) + 8 * sizeof(int);
 }
 
 template <int PN, typename V>
@@ -1046,36 +1059,34 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
     extern __shared__ __align__(16) unsigned char lds[];
     const int rowlen = nphif * nd, slot = row_slot(rowlen);
     float* rows = (float*)lds;                                              // 3 ring slots
-    double* s_offb = (double*)(lds + piece_res_offset(rowlen));
-    double* s_acc = s_offb + TILE_JOBS;
-    int2* s_mlw = (int2*)(s_acc + TILE_JOBS);                               // template, drawn-time weight
-    float* s_M = (float*)(s_mlw + TILE_JOBS);
-    int* s_ctr = (int*)(s_M + TILE_JOBS);                                   // [0..2] chunks taken, [4..5] band
+    double* s_acc = (double*)(lds + piece_res_offset(rowlen));              // each read's sum
+    float* s_M = (float*)(s_acc + SCHED_TILE_JOBS);                         // each read's reference
+    int* s_ctr = (int*)(s_M + SCHED_TILE_JOBS);                             // [0..2] chunks taken, [4..5] band
     const int r = blockIdx.y, t = blockIdx.x;
     const int j0 = tiles[2 * t], nj = tiles[2 * t + 1];
     const int tid = threadIdx.x, lane = tid & 63;
     const float* Tr = T + (int64_t)jtab[j0] * tstride + (int64_t)r * nb * rowlen;
+    const int32_t* tm = jm + j0;                                            // the tile's templates
+    const double* toffb = joffb + j0;                                       // and ln-a offsets
     const BandGeom geo{rdb, offd, rdd, rdd / rdb, nb - 1 - 1e-6, nd - 1 - 1e-6, nd};
     if (tid == 0) {
         s_ctr[4] = INT_MAX;
         s_ctr[5] = INT_MIN;
     }
     __syncthreads();
-    if (tid < nj) {
-        const int64_t j = (int64_t)j0 + tid;
+    for (int jl = tid; jl < nj; jl += SCHED_THREADS) {
+        const int64_t j = (int64_t)j0 + jl;
         const int first = first_last[2 * j], last = first_last[2 * j + 1];
-        const int m = jm[j];
-        const double offb = joffb[j];
+        const int m = tm[jl];
+        const double offb = toffb[jl];
         const float lw = lw_g[m];
-        s_offb[tid] = offb;
-        s_acc[tid] = 0.0;
-        s_mlw[tid] = make_int2(m, __float_as_int(lw));
+        s_acc[jl] = 0.0;
         // the reference: the value at the loudest walked time (row `last`, the exact lookup from memory)
         const int64_t e = ((int64_t)m * ntau + r) * nw + nw - 1;
         const float* R0 = Tr + (int64_t)last * rowlen;
         const double M = exact_lookup<V>(R0, R0 + rowlen, geo, (double)last, offb, (double)lw,
                                          wa[(int64_t)m * nw + nw - 1], stream[e], aHs[e]);
-        s_M[tid] = (M > -1e30 && M < 1e30) ? (float)M : 0.0f;
+        s_M[jl] = (M > -1e30 && M < 1e30) ? (float)M : 0.0f;
         atomicMin(&s_ctr[4], first);
         atomicMax(&s_ctr[5], last);
     }
@@ -1122,11 +1133,11 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
             P.nuse = act ? (int)((d >> SD_USE_SHIFT) & 7u) + 1 : 0;
             const int pa = (int)(d >> SD_POS_SHIFT);
             P.pa = pa;
-            const int2 mlw = s_mlw[P.jl];
-            P.offb = s_offb[P.jl];
-            P.lw = __int_as_float(mlw.y);
+            const int m = tm[P.jl];
+            P.offb = toffb[P.jl];
+            P.lw = lw_g[m];
             P.M = s_M[P.jl];
-            load_piece<PN>(wa + (int64_t)mlw.x * nw + pa, stream + ((int64_t)mlw.x * ntau + r) * nw + pa, P.aL, P.ev);
+            load_piece<PN>(wa + (int64_t)m * nw + pa, stream + ((int64_t)m * ntau + r) * nw + pa, P.aL, P.ev);
         };
         // one chunk's lookups: each piece's (max, sum), rescaled to its read's reference and added to the
         // read's sum
@@ -1188,10 +1199,10 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
                     const uint32_t d = dt[ro[b] + i];
                     const int jl = (int)(d & SD_READ_MASK), nuse = (int)((d >> SD_USE_SHIFT) & 7u) + 1;
                     const int pa = (int)(d >> SD_POS_SHIFT);
-                    const int2 mlw = s_mlw[jl];
-                    const double offb = s_offb[jl], lw = (double)__int_as_float(mlw.y);
-                    const int64_t e0 = ((int64_t)mlw.x * ntau + r) * nw + pa;
-                    const float* wp = wa + (int64_t)mlw.x * nw + pa;
+                    const int m = tm[jl];
+                    const double offb = toffb[jl], lw = (double)lw_g[m];
+                    const int64_t e0 = ((int64_t)m * ntau + r) * nw + pa;
+                    const float* wp = wa + (int64_t)m * nw + pa;
                     double mx = -1e300, sm = 0.0;
                     for (int k = 0; k < nuse; ++k) {
                         const double a = exact_lookup<V>(R0, R1, geo, (double)b, offb, lw, wp[k], stream[e0 + k],
@@ -1252,15 +1263,15 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
             __syncthreads();                    // every read's sum complete
         }
     }
-    if (tid < nj) {
-        const int64_t o = ((int64_t)j0 + tid) * ntau + r;
-        const double S = s_acc[tid];
+    for (int jl = tid; jl < nj; jl += SCHED_THREADS) {
+        const int64_t o = ((int64_t)j0 + jl) * ntau + r;
+        const double S = s_acc[jl];
         const bool bad = !(S < INFINITY);       // a term beyond float64 (or a NaN input): recomputed
         // the sum as (M + ln S, 1): the log-sum itself, so that the merges after it (edge rows, slices,
         // lags) rescale terms of comparable size
         double mx = -1e300, sm = 0.0;
         if (!bad && S > 0.0) {
-            mx = (double)s_M[tid] + log(S);
+            mx = (double)s_M[jl] + log(S);
             sm = 1.0;
         }
         if (accumulate) lse_merge_exact(mx, sm, om[o], os[o]);
@@ -1441,8 +1452,9 @@ int launch_piece(Store* st, const uint2* stream, const float* aHs, const float* 
     EXM_CHECK(row_slot(st->nphif * st->nd) - st->nphif * st->nd <= TABLE_PAD_FLOATS, EXM_ERR_ARG,
               "row copy overruns the table padding");
     const BandForm form = band_form();
-    static_assert(TILE_JOBS <= WAVE_THREADS && TILE_JOBS <= BAND_THREADS && TILE_JOBS <= SCHED_THREADS &&
-                      TILE_JOBS <= (int)SD_READ_MASK + 1,
+    static_assert(TILE_JOBS <= WAVE_THREADS && TILE_JOBS <= BAND_THREADS &&
+                      SCHED_TILE_JOBS <= SCHED_READS_PER_THREAD * SCHED_THREADS &&
+                      SCHED_TILE_JOBS <= (int)SD_READ_MASK + 1,
                   "a workgroup holds every read of its tile");
     const int rowlen = st->nphif * st->nd;
     const size_t lds = form == BandForm::Sched  ? sched_lds_bytes(rowlen)
@@ -1461,7 +1473,7 @@ int launch_piece(Store* st, const uint2* stream, const float* aHs, const float* 
         EXM_CHECK(st->nb <= SCHED_THREADS, EXM_ERR_ARG, "the schedule takes at most %d beta rows", SCHED_THREADS);
         EXM_CHECK(nw < (1 << (32 - SD_POS_SHIFT)), EXM_ERR_ARG, "walk of %d times exceeds the piece descriptor", nw);
         const int64_t cap = sched_cap_per_read(nw, pn, st->nb);
-        EXM_CHECK(cap * TILE_JOBS < INT32_MAX, EXM_ERR_ARG, "a tile's pieces exceed the 32-bit row offsets");
+        EXM_CHECK(cap * SCHED_TILE_JOBS < INT32_MAX, EXM_ERR_ARG, "a tile's pieces exceed the 32-bit row offsets");
         flags = (uint8_t*)(wb + pw.flags);
         int32_t* cnt = (int32_t*)(wb + pw.cnt);
         int32_t* rowoff = (int32_t*)(wb + pw.rowoff);
@@ -1504,9 +1516,11 @@ int launch_piece(Store* st, const uint2* stream, const float* aHs, const float* 
                                                                                   : k_gather_piece<8, double>);
         EXM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         ProfileScope prof("gather");
-        hipLaunchKernelGGL(kern, dim3(js.ntiles, st->ntau), dim3(wave ? WAVE_THREADS : BAND_THREADS), lds, cs, stream,
-                           wa, nw, lw_g, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m, js.d_tab,
-                           js.d_offb, js.d_tiles, first_last, bounds, rdb, offd, rdd, om, os, accumulate ? 1 : 0);
+        // tiles of at most TILE_JOBS reads: one per lane of the wave form, one per thread of the piece form
+        hipLaunchKernelGGL(kern, dim3(js.ntiles_small, st->ntau), dim3(wave ? WAVE_THREADS : BAND_THREADS), lds, cs,
+                           stream, wa, nw, lw_g, st->d_T, st->tstride, st->nphif, st->nb, st->nd, st->ntau, js.d_m,
+                           js.d_tab, js.d_offb, js.d_tiles_small, first_last, bounds, rdb, offd, rdd, om, os,
+                           accumulate ? 1 : 0);
         EXM_HIP(hipGetLastError());
     }
     // the piece and wave forms: rows 0 and nb - 2 (beta possibly clamped), merged into the interior rows'

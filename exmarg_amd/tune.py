@@ -4,7 +4,8 @@ Autotuner of the band gather kernel.
 The band kernel has a form and two shape knobs: the form (4: pieces listed once for every lag and
 dealt to the waves in chunks; 3: each wave deals its own reads' pieces; 2: pieces dealt over the whole
 workgroup), the piece length (walked times one lane reads in a row: 4, 6 or 8 in the scheduled and wave
-forms, 4, 8 or 16 in the piece form) and the tile (reads per workgroup, 256, 512 or 1024).  The best choice depends on the walk, the table geometry and how the reads spread over the
+forms, 4, 8 or 16 in the piece form) and the tile (reads per workgroup, 256, 512, 1024 or 2048; the wave
+and piece forms cut tiles of 2048 in two).  The best choice depends on the walk, the table geometry and how the reads spread over the
 beta rows, so it is measured, not modelled: tune_batch times every candidate on the caller's own
 filled batch (the gather is rerun, nothing is consumed) with the kernel's own HIP events, keeps the
 fastest and records it in a JSON cache keyed by device, table geometry, walk length and precision;
@@ -26,7 +27,7 @@ from . import _hip
 
 FORMS = (4, 3, 2)
 PIECES = (4, 6, 8, 16)
-TILES = (256, 512, 1024)
+TILES = (256, 512, 1024, 2048)
 
 
 def cache_path():

@@ -399,11 +399,11 @@ def tile(hip, request):
 
 
 @pytest.mark.parametrize("kernel", [2, 3, 4], indirect=True)
-@pytest.mark.parametrize("tile", [256, 512], indirect=True)
+@pytest.mark.parametrize("tile", [256, 512, 2048], indirect=True)
 def test_band_gather_tiles_over_xcd_queues(hip, oracle, kernel, tile):
-    # two tables cut into 9 and 3 tiles (256 reads per tile) or 5 and 2 (512): the tile list is built
-    # from per-XCD queues, piece q of every table in one queue (>= 8 pieces) or shared by several (< 8);
-    # every read must be gathered exactly once
+    # two tables cut into 9 and 3 tiles (256 reads per tile), 5 and 2 (512) or 2 and 1 (2048, each cut in
+    # two for the wave and piece forms): the tile list is built from per-XCD queues, piece q of every
+    # table in one queue (>= 8 pieces) or shared by several (< 8); every read must be gathered exactly once
     from exmarg_amd.search import gather as g
     say("gather: band kernel %d, tile %d, 2300 + 700 reads of two tables" % (kernel, tile))
     s = _tiny_store("rough", (5, 49, 24, 17), n_nodes=2, seed=6)
