@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--cpu-nodes", type=int, default=16, help="nodes (the most used) in the CPU-baseline gather sample")
     p.add_argument("--check-templates", type=int, default=8, help="templates checked against the oracle")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle check")
+    p.add_argument("--no-precision-form", action="store_true",
+                   help="plain: skip the precision form's run on the same segment after the timed steps")
     p.add_argument("--gather-kernel", type=int, default=0, help="0 automatic, 1 reference-order kernel, 2-4 band kernel (piece, wave, scheduled form)")
     p.add_argument("--piece", type=int, default=0, help="walked times per piece of the band kernel (0 default)")
     p.add_argument("--tile", type=int, default=0, help="reads per band-kernel workgroup (0 default)")
@@ -863,7 +865,7 @@ def main():
                      "dln_b": float(abs(res["ln_b"][0] - ref64["ln_b"][0]))}
 
     precision_form = None
-    if args.precision == "f64" and args.filter_precision == "f64" and args.config == 2:
+    if args.precision == "f64" and args.filter_precision == "f64" and args.config == 2 and not args.no_precision_form:
         # the precision form on the same segment, timed the same way after one warm-up call: float32 transform
         # and interpolation, float64 weighting, index arithmetic, log-sum-exp and prior quadrature
         _hip.call("exm_set_gather_precision", 1)
