@@ -259,9 +259,10 @@ int exm_set_gather_piece(int n);
  * src/exmarg/search/gather.py:143-306).  Applies to the band kernel only. */
 int exm_set_gather_precision(int p);
 
-/* Reads per workgroup of the band kernel: 256, 512 or 1024 (0 restores the default, 1024).  Smaller
- * tiles give more workgroups, each sweeping fewer table rows of its reads' band.  A tuning knob (the
- * autotuner exmarg_amd.tune chooses it); no reference counterpart. */
+/* Reads per workgroup of the band kernel: 256, 512, 1024 or 2048 (0 restores the default, 2048; the wave
+ * and piece forms take tiles of 2048 in two parts of at most 1024).  Smaller tiles give more workgroups,
+ * each sweeping fewer table rows of its reads' band.  A tuning knob (the autotuner exmarg_amd.tune
+ * chooses it); no reference counterpart. */
 int exm_set_gather_tile(int n);
 
 /* Workspace of one batched transform of the matched filter (templates per transform = bytes / (16
