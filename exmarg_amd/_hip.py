@@ -54,6 +54,7 @@ SIGNATURES = {
     "exm_batch_create": [H, I64, I32, HP],
     "exm_batch_add": [H, H, H, P, P, P, P, I32, I32, I32, I32, P, F32, ctypes.POINTER(ctypes.c_int)],
     "exm_batch_add_seeded": [H, H, H, P, P, P, P, I32, I32, I32, I32, P, F32, ctypes.POINTER(ctypes.c_int)],
+    "exm_batch_add_seeded_many": [H, H, H, I32, P, P, P, P, I32, I32, I32, I32, P, F32, ctypes.POINTER(ctypes.c_int)],
     "exm_batch_run": [H, P],
     "exm_batch_launch": [H],
     "exm_reserve": [I32, I64, I32, I32],

@@ -1005,75 +1005,101 @@ int exm_batch_create(exm_handle hstore, int64_t max_templates, int n_walk, exm_h
     return EXM_OK;
 }
 
-static int batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, const double* dH, const double* dL,
-                     const double* s2H, const double* s2L, int lag_lo, int lag_hi, int K, int S, const int64_t* pick,
-                     const uint64_t* rng_state, float lw_rand, int* slot) {
+// n segments of one bank (and plan) admitted together: dH, dL (n, n_freq) complex128, one noise variance
+// s2H, s2L for all of them; the random draw from pick (n, n_tmpl, S) or from rng_state (n, 4), one
+// generator per segment.  Each segment is filtered apart; the selection, the walk and the fills run once
+// over the n x n_tmpl templates.
+static int batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, int n, const double* dH,
+                     const double* dL, const double* s2H, const double* s2L, int lag_lo, int lag_hi, int K, int S,
+                     const int64_t* pick, const uint64_t* rng_state, float lw_rand, int* slot) {
     NEED(Batch, b, hbatch, K_BATCH);
     NEED(Bank, bank, hbank, K_BANK);
     NEED(Plan, plan, hplan, K_PLAN);
     NEED(Store, st, b->store, K_STORE);
     EXM_CHECK(dH && dL && s2H && s2L && (pick || rng_state), EXM_ERR_ARG, "NULL argument");
+    EXM_CHECK(n >= 1, EXM_ERR_ARG, "no segment to admit (n = %d)", n);
     EXM_CHECK(!b->in_flight, EXM_ERR_STATE, "batch in flight; collect it before adding segments");
     EXM_CHECK(K + S == b->nw, EXM_ERR_ARG, "n_loud + n_random = %d, the batch walks %d times", K + S, b->nw);
     EXM_CHECK(plan->n_tmpl == bank->n_tmpl, EXM_ERR_ARG, "plan is for %d templates, bank has %d", plan->n_tmpl,
               bank->n_tmpl);
-    EXM_CHECK(b->used + bank->n_tmpl <= b->cap, EXM_ERR_NOMEM, "batch full: %lld + %d templates exceed %lld",
-              (long long)b->used, bank->n_tmpl, (long long)b->cap);
+    const int nm = bank->n_tmpl;
+    const int64_t ntot = (int64_t)n * nm;
+    EXM_CHECK(ntot <= INT32_MAX && b->used + ntot <= b->cap, EXM_ERR_NOMEM,
+              "batch full: %lld + %d x %d templates exceed %lld", (long long)b->used, n, nm, (long long)b->cap);
     EXM_CHECK(lag_lo <= lag_hi && (int64_t)(lag_hi - lag_lo) < bank->n_time, EXM_ERR_ARG, "bad lag range");
     EXM_CHECK(K > 0 && K < bank->n_time && S > 0, EXM_ERR_ARG, "n_loud %d / n_random %d invalid for n_time %lld", K,
               S, (long long)bank->n_time);
     const int64_t nrest = bank->n_time - K;
     if (pick)
-        for (int64_t i = 0; i < (int64_t)bank->n_tmpl * S; ++i)
+        for (int64_t i = 0; i < ntot * S; ++i)
             EXM_CHECK(pick[i] >= 0 && pick[i] < nrest, EXM_ERR_ARG, "pick[%lld] outside [0, %lld)", (long long)i,
                       (long long)nrest);
-    // transient trajectories and times in scratch slot 7
+    // transient trajectories and times of the n segments in scratch slot 7
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t tr_bytes = al(sizeof(float) * 4 * (size_t)bank->n_tmpl * bank->n_time);
-    const size_t sel_bytes = al(sizeof(int32_t) * (size_t)bank->n_tmpl * b->nw);
+    const size_t tr_bytes = al(sizeof(float) * 4 * (size_t)ntot * bank->n_time);
+    const size_t sel_bytes = al(sizeof(int32_t) * (size_t)ntot * b->nw);
     void* tbuf = nullptr;
     EXM_TRY(scratch(7, tr_bytes + 2 * sel_bytes, &tbuf));
     Traj tr;
-    tr.n_tmpl = bank->n_tmpl;
+    tr.n_tmpl = (int)ntot;
     tr.n_time = bank->n_time;
     tr.d_buf = (float*)tbuf;
     Times tm;
-    tm.n_tmpl = bank->n_tmpl;
+    tm.n_tmpl = (int)ntot;
     tm.n_walk = b->nw;
     tm.lw_rand = lw_rand;
     tm.d_sel = (int32_t*)((char*)tbuf + tr_bytes);
     tm.d_lw = (float*)((char*)tbuf + tr_bytes + sel_bytes);
-    int rc = launch_filter(bank, dH, dL, s2H, s2L, &tr);
+    int rc = EXM_OK;
+    for (int s = 0; s < n && rc == EXM_OK; ++s) {
+        // segment s writes template rows s nm .. s nm + nm - 1 of each of the four arrays: a view with the
+        // whole set's row count and its first row moved
+        Traj view;
+        view.n_tmpl = (int)ntot;
+        view.n_time = bank->n_time;
+        view.d_buf = tr.d_buf + (int64_t)s * nm * bank->n_time;
+        rc = launch_filter(bank, dH + 2 * (size_t)s * bank->n_freq, dL + 2 * (size_t)s * bank->n_freq, s2H, s2L, &view);
+        view.d_buf = nullptr;
+    }
     int64_t* d_pick = nullptr;
-    if (rc == EXM_OK && !pick) rc = device_draw(rng_state, (int64_t)bank->n_tmpl * S, nrest, &d_pick);
+    if (rc == EXM_OK && !pick) {
+        void* pk = nullptr;
+        rc = scratch(2, sizeof(int64_t) * (size_t)std::max<int64_t>(ntot * S, 1), &pk);
+        d_pick = (int64_t*)pk;
+        for (int s = 0; s < n && rc == EXM_OK; ++s)
+            rc = launch_draw(rng_state + 4 * (size_t)s, (int64_t)nm * S, nrest, d_pick + (int64_t)s * nm * S);
+    }
     if (rc == EXM_OK) rc = launch_select(&tr, lag_lo, lag_hi, K, S, pick, d_pick, lw_rand, &tm);
     if (rc == EXM_OK)
         rc = launch_walk(&tr, &tm, st, b->d_wa + b->used * b->nw, b->d_stream + b->used * b->ntau * b->nw,
                          b->d_aH + b->used * b->ntau * b->nw);
-    if (rc == EXM_OK) rc = launch_fill(b->d_lw + b->used, bank->n_tmpl, lw_rand);
-    if (rc == EXM_OK) rc = launch_fill(b->d_lnnt + b->used, bank->n_tmpl, std::log((double)bank->n_time));
+    if (rc == EXM_OK) rc = launch_fill(b->d_lw + b->used, ntot, lw_rand);
+    if (rc == EXM_OK) rc = launch_fill(b->d_lnnt + b->used, ntot, std::log((double)bank->n_time));
     // the host-side trajectory and times objects only borrow scratch memory
     tr.d_buf = nullptr;
     tm.d_sel = nullptr;
     tm.d_lw = nullptr;
     if (rc != EXM_OK) return rc;
     EXM_HIP(hipStreamSynchronize(ctx().stream));
-    for (size_t e = 0; e < plan->h_m.size(); ++e) {
-        b->h_m.push_back((int32_t)(plan->h_m[e] + b->used));
-        b->h_tab.push_back(plan->h_tab[e]);
-        b->h_offb.push_back(plan->h_offb[e]);
+    const int first = (int)b->seg_reads.size();
+    for (int s = 0; s < n; ++s) {
+        for (size_t e = 0; e < plan->h_m.size(); ++e) {
+            b->h_m.push_back((int32_t)(plan->h_m[e] + b->used));
+            b->h_tab.push_back(plan->h_tab[e]);
+            b->h_offb.push_back(plan->h_offb[e]);
+        }
+        // the plan's node weights go with its reads (a segment admitted without them leaves the batch
+        // without a device assembly: exm_batch_collect still gives the read values)
+        b->weights = b->weights && plan->has_weights;
+        if (b->weights) {
+            b->h_lnw.insert(b->h_lnw.end(), plan->h_lnw.begin(), plan->h_lnw.end());
+            b->h_cellw.insert(b->h_cellw.end(), plan->h_cellw.begin(), plan->h_cellw.end());
+        }
+        b->seg_tmpl.push_back((int32_t)b->used);
+        b->seg_reads.push_back((int64_t)plan->h_m.size());
+        b->used += nm;
     }
-    // the plan's node weights go with its reads (a segment admitted without them leaves the batch
-    // without a device assembly: exm_batch_collect still gives the read values)
-    b->weights = b->weights && plan->has_weights;
-    if (b->weights) {
-        b->h_lnw.insert(b->h_lnw.end(), plan->h_lnw.begin(), plan->h_lnw.end());
-        b->h_cellw.insert(b->h_cellw.end(), plan->h_cellw.begin(), plan->h_cellw.end());
-    }
-    b->seg_tmpl.push_back((int32_t)b->used);
-    b->seg_reads.push_back((int64_t)plan->h_m.size());
-    b->used += bank->n_tmpl;
-    if (slot) *slot = (int)b->seg_reads.size() - 1;
+    if (slot) *slot = first;
     return EXM_OK;
 }
 
@@ -1084,7 +1110,7 @@ int exm_batch_add(exm_handle hbatch, exm_handle hbank, exm_handle hplan, const d
     LOCK;
     EXM_TRY(ensure_init());
     EXM_CHECK(pick != nullptr, EXM_ERR_ARG, "pick is NULL");
-    return batch_add(hbatch, hbank, hplan, dH, dL, s2H, s2L, lag_lo, lag_hi, K, S, pick, nullptr, lw_rand, slot);
+    return batch_add(hbatch, hbank, hplan, 1, dH, dL, s2H, s2L, lag_lo, lag_hi, K, S, pick, nullptr, lw_rand, slot);
 }
 
 int exm_batch_add_seeded(exm_handle hbatch, exm_handle hbank, exm_handle hplan, const double* dH, const double* dL,
@@ -1092,7 +1118,17 @@ int exm_batch_add_seeded(exm_handle hbatch, exm_handle hbank, exm_handle hplan, 
                          const uint64_t* rng_state, float lw_rand, int* slot) {
     LOCK;
     EXM_TRY(ensure_init());
-    return batch_add(hbatch, hbank, hplan, dH, dL, s2H, s2L, lag_lo, lag_hi, K, S, nullptr, rng_state, lw_rand, slot);
+    return batch_add(hbatch, hbank, hplan, 1, dH, dL, s2H, s2L, lag_lo, lag_hi, K, S, nullptr, rng_state, lw_rand,
+                     slot);
+}
+
+int exm_batch_add_seeded_many(exm_handle hbatch, exm_handle hbank, exm_handle hplan, int n, const double* dH,
+                              const double* dL, const double* s2H, const double* s2L, int lag_lo, int lag_hi, int K,
+                              int S, const uint64_t* rng_states, float lw_rand, int* first_slot) {
+    LOCK;
+    EXM_TRY(ensure_init());
+    return batch_add(hbatch, hbank, hplan, n, dH, dL, s2H, s2L, lag_lo, lag_hi, K, S, nullptr, rng_states, lw_rand,
+                     first_slot);
 }
 
 static void batch_release_jobs(Batch* b) {

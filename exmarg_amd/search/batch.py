@@ -78,6 +78,31 @@ class SegmentBatch:
         self.entries.append(analysis)
         return slot.value
 
+    def add_many(self, analysis, dH, dL, sig2, seeds):
+        """admit len(seeds) segments of one bank at once: dH, dL (n, n_freq) data, sig2 (dict 'H', 'L') the
+        noise variance they share, seeds one per segment; the same as add() of each in turn, with the
+        selection and the walk launched once for all of them.  Returns the first segment's slot."""
+        bank = analysis.bank
+        nm, nt = bank.u.size, bank.n_time
+        n = len(seeds)
+        if n == 0:
+            return None
+        if not 0 < self.K < nt:
+            raise ValueError("n_loud must lie in (0, n_time), got %d" % self.K)
+        states = np.stack([rng_state(s) for s in seeds])
+        lw = np.float32(np.log((nt - self.K) / self.S))
+        arrs = [_hip.contiguous(dH, np.complex128), _hip.contiguous(dL, np.complex128),
+                _hip.contiguous(sig2["H"], np.float64), _hip.contiguous(sig2["L"], np.float64)]
+        if arrs[0].shape != (n, bank.nu.size) or arrs[1].shape != arrs[0].shape:
+            raise ValueError("dH and dL must be (%d, %d) arrays" % (n, bank.nu.size))
+        slot = _hip.ctypes.c_int()
+        _hip.call("exm_batch_add_seeded_many", int(self.handle), int(bank.handle), int(analysis.plan.handle), n,
+                  *[_hip.ptr(a) for a in arrs], self.lag_lo, self.lag_hi, self.K, self.S,
+                  _hip.ptr(_hip.contiguous(states, np.uint64)), float(lw), _hip.ctypes.byref(slot))
+        self.used += n * nm
+        self.entries.extend([analysis] * n)
+        return slot.value
+
     def run(self):
         """lamN (n_node + 1, n_template) of every admitted segment, in admission order"""
         self.launch()

@@ -273,8 +273,8 @@ def _segment_groups(an, seg, chunk, n_loud, n_random, seed, lse):
         sig2 = {I: seg["sig2_" + I] for I in "HL"}
         for j0 in range(0, n, g):
             js = range(j0, min(n, j0 + g))
-            for j in js:
-                batch.add(an, {I: seg["d_" + I][j] for I in "HL"}, sig2, seed + int(seg["rows"][j]))
+            batch.add_many(an, seg["d_H"][js.start:js.stop], seg["d_L"][js.start:js.stop], sig2,
+                           [seed + int(seg["rows"][j]) for j in js])
             batch.launch()
             res = batch.collect_lse() if lse else batch.collect()
             batch.reset()

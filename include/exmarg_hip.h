@@ -212,6 +212,14 @@ int exm_batch_add(exm_handle batch, exm_handle bank, exm_handle plan, const doub
 int exm_batch_add_seeded(exm_handle batch, exm_handle bank, exm_handle plan, const double* dH, const double* dL,
                          const double* s2H, const double* s2L, int lag_lo, int lag_hi, int K, int S,
                          const uint64_t* rng_state, float lw_rand, int* slot);
+/* n segments of one bank sharing one noise variance (s2H, s2L) admitted together: dH, dL (n, n_freq),
+ * rng_states (n, 4), one generator state per segment; each segment is filtered apart, the selection and
+ * the walk run once over the n x n_tmpl templates.  The segments take slots *first_slot ..
+ * *first_slot + n - 1.  The same results as n calls of exm_batch_add_seeded; the reference's chunk of
+ * segments per call (src/exmarg/search/pe.py:173-194). */
+int exm_batch_add_seeded_many(exm_handle batch, exm_handle bank, exm_handle plan, int n, const double* dH,
+                              const double* dL, const double* s2H, const double* s2L, int lag_lo, int lag_hi, int K,
+                              int S, const uint64_t* rng_states, float lw_rand, int* first_slot);
 
 /* ln Lambda of every read of the batch: out holds, segment after segment, the reads of each segment's
  * plan in plan order (float64). */

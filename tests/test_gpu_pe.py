@@ -139,6 +139,41 @@ def test_segments_sharing_a_launch_match_one_launch_each(hip, monkeypatch):
     stm.free(st)
 
 
+def test_batch_add_many_equals_one_add_each(hip):
+    """SegmentBatch.add_many (exm_batch_add_seeded_many: the segments of one bank filtered one by one, selected
+    and walked in one launch) gives the node values of add() per segment, to float64 rounding"""
+    from exmarg_amd.search import mass_bank, read_segments, store as stm
+    from exmarg_amd.search.batch import SegmentBatch
+    from exmarg_amd.search.pe import _Analysis
+    seg, bankrows, s_nodes = small_problem()
+    seg = dict(seg, d_H=np.concatenate([seg["d_H"], seg["d_H"][:1] * 0.5]),
+               d_L=np.concatenate([seg["d_L"], seg["d_L"][:1] * 0.5]))
+    say("batch: 3 segments admitted one by one and at once")
+    st = stm.synthetic_store(seed=0, s_nodes=s_nodes)
+    bank = mass_bank(bankrows, seg, "hip")
+    an = _Analysis(bank, st, 0.1267, True).start()
+    rseg = read_segments(seg)
+    sig2 = {I: rseg["sig2_" + I] for I in "HL"}
+    b = SegmentBatch(st, 128, 128, max_templates=4 * bank.u.size)
+    try:
+        b.add(an, {I: rseg["d_" + I][0] for I in "HL"}, sig2, 5)      # a segment ahead of the group
+        for j in range(3):
+            b.add(an, {I: rseg["d_" + I][j] for I in "HL"}, sig2, 7 + j)
+        one = b.run()
+        b.reset()
+        b.add(an, {I: rseg["d_" + I][0] for I in "HL"}, sig2, 5)
+        assert b.add_many(an, rseg["d_H"], rseg["d_L"], sig2, [7, 8, 9]) == 1
+        many = b.run()
+    finally:
+        b.free()
+    assert len(one) == len(many) == 4
+    for x, y in zip(one, many):
+        fin = np.isfinite(x)
+        np.testing.assert_array_equal(np.isfinite(y), fin)
+        np.testing.assert_allclose(y[fin], x[fin], rtol=REPEAT_RTOL, atol=0)
+    stm.free(st)
+
+
 def test_pe_intrinsic_rejects_templates_outside_the_store(hip):
     from exmarg_amd.search import mass_bank, pe_intrinsic, store as stm
     seg, bankrows, _ = small_problem(n_templates=3)
