@@ -1042,10 +1042,11 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_sched_emit(const int32_t* __r
 }
 
 // bytes of LDS the scheduled kernel uses for a row of rowlen floats: the ring, then per read of the tile
-// its sum (float64) and reference (float), then the chunk counters and the band range.  A read's
-// template, ln-a offset and drawn-time weight are read from memory where a piece needs them.
+// its sum (float64), reference (float) and template (int), then the chunk counters and the band range.
+// A read's ln-a offset and drawn-time weight are read from memory where a piece needs them (at compute
+// time, a chunk after the fetch); the template, which the piece's record addresses need, from LDS.
 __host__ __device__ constexpr size_t sched_lds_bytes(int rowlen) {
-    return piece_res_offset(rowlen) + (size_t)SCHED_TILE_JOBS * (8 + 4) + 8 * sizeof(int);
+    return piece_res_offset(rowlen) + (size_t)SCHED_TILE_JOBS * (8 + 4 + 4) + 8 * sizeof(int);
 }
 
 template <int PN, typename V>
@@ -1061,7 +1062,8 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
     float* rows = (float*)lds;                                              // 3 ring slots
     double* s_acc = (double*)(lds + piece_res_offset(rowlen));              // each read's sum
     float* s_M = (float*)(s_acc + SCHED_TILE_JOBS);                         // each read's reference
-    int* s_ctr = (int*)(s_M + SCHED_TILE_JOBS);                             // [0..2] chunks taken, [4..5] band
+    int* s_m = (int*)(s_M + SCHED_TILE_JOBS);                               // each read's template
+    int* s_ctr = s_m + SCHED_TILE_JOBS;                                     // [0..2] chunks taken, [4..5] band
     const int r = blockIdx.y, t = blockIdx.x;
     const int j0 = tiles[2 * t], nj = tiles[2 * t + 1];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -1081,6 +1083,7 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
         const double offb = toffb[jl];
         const float lw = lw_g[m];
         s_acc[jl] = 0.0;
+        s_m[jl] = m;
         // the reference: the value at the loudest walked time (row `last`, the exact lookup from memory)
         const int64_t e = ((int64_t)m * ntau + r) * nw + nw - 1;
         const float* R0 = Tr + (int64_t)last * rowlen;
@@ -1133,7 +1136,7 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
             P.nuse = act ? (int)((d >> SD_USE_SHIFT) & 7u) + 1 : 0;
             const int pa = (int)(d >> SD_POS_SHIFT);
             P.pa = pa;
-            const int m = tm[P.jl];
+            const int m = s_m[P.jl];
             P.offb = toffb[P.jl];
             P.lw = lw_g[m];
             P.M = s_M[P.jl];
@@ -1199,7 +1202,7 @@ __global__ __launch_bounds__(SCHED_THREADS) void k_gather_sched(
                     const uint32_t d = dt[ro[b] + i];
                     const int jl = (int)(d & SD_READ_MASK), nuse = (int)((d >> SD_USE_SHIFT) & 7u) + 1;
                     const int pa = (int)(d >> SD_POS_SHIFT);
-                    const int m = tm[jl];
+                    const int m = s_m[jl];
                     const double offb = toffb[jl], lw = (double)lw_g[m];
                     const int64_t e0 = ((int64_t)m * ntau + r) * nw + pa;
                     const float* wp = wa + (int64_t)m * nw + pa;
