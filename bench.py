@@ -954,8 +954,12 @@ def main():
     which = [w.strip() for w in args.published.split(",") if w.strip()]
     if which and args.config == 2 and args.nodes == 53 and args.precision == "f64" \
             and args.filter_precision == "f64":
-        # the reference's own timed workloads on the same resident store, beside its published times
-        out["reference_workloads"] = published_workloads(st, which)
+        # the reference's own timed workloads on the same resident store, beside its published times (a
+        # failure there is reported in the line, which still carries the headline)
+        try:
+            out["reference_workloads"] = published_workloads(st, which)
+        except Exception as e:                   # noqa: BLE001
+            out["reference_workloads"] = {"error": "%s: %s" % (type(e).__name__, e)}
     if not args.no_cpu and args.n_loud > 0 and args.config == 2:
         an_geometry = geometry
         cpu, err, nchk = cpu_leg(args, seg, bank, st, res, an_geometry, 0)
